@@ -1,0 +1,320 @@
+"""ctypes access to the CPU oracle (liboracle.so) and, where it was built,
+to the reference's own BIEF (oracle/_ref/libbief_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libbief_ref.so")
+
+OPS = {"X=AY": 0, "X=CAY": 1, "X=-AY": 2, "X=X+AY": 3, "X=X-AY": 4, "X=X+CAY": 5,
+       "X=TAY": 6, "X=-TAY": 7, "X=X+TAY": 8, "X=X-TAY": 9, "X=X+CTAY": 10}
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_C = ctypes.c_char
+
+
+class OrcInfo(ctypes.Structure):
+    _fields_ = [("niter", ctypes.c_int), ("relres", ctypes.c_double),
+                ("relat", ctypes.c_int), ("converged", ctypes.c_int),
+                ("zero_rhs", ctypes.c_int)]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_dot.restype = _D
+        L.orc_dot.argtypes = [_I, _P, _P]
+        L.orc_matvec.argtypes = [_I, _P, _P, _C, _P, _C, _P, _D, _P, _I, _I, _I, _I, _P, _P]
+        L.orc_mt01aa.argtypes = [_P, _P, _D, _P, _I, _I]
+        L.orc_mt02aa.argtypes = [_I, _P, _P, _D, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I]
+        L.orc_mt01pp.argtypes = [_P, _P, _D, _P, _P, _P, _I, _I]
+        L.orc_assemble_diag.argtypes = [_P, _P, _P, _I, _I, _I, _I]
+        L.orc_om_dmd.argtypes = [_P, _P, _P, _C, _P, _P, _I, _I, _I, _I]
+        L.orc_precd1.argtypes = [_P, _P, _P, _P, _C, _P, _P, _P, _I, _I, _I, _I, _I]
+        L.orc_solve.restype = _I
+        L.orc_solve.argtypes = [_I, _I, _I, _I, _D, _P, _P, _P, _P, _C, _P, _P,
+                                _I, _I, _I, _I, ctypes.POINTER(OrcInfo)]
+        L.orc_dirich.argtypes = [_P, _P, _C, _P, _C, _P, _P, _P, _I, _P, _I, _I, _I, _I]
+        L.orc_lump.argtypes = [_P, _P, _C, _P, _C, _D, _P, _I, _I, _I, _I]
+        _lib = L
+    return _lib
+
+
+class Matrix:
+    """An EBE matrix as BIEF stores it: diagonal DA, off-diagonal XA
+    (ncoef, nelem), TYPDIA, TYPEXT."""
+
+    def __init__(self, da, xa, typdia="Q", typext="S"):
+        self.da = _f64(da)
+        self.xa = _f64(xa)
+        self.typdia = typdia
+        self.typext = typext
+
+    def copy(self):
+        return Matrix(self.da.copy(), self.xa.copy(), self.typdia, self.typext)
+
+
+# --------------------------------------------------------------------------
+# oracle (C restatement)
+# --------------------------------------------------------------------------
+
+def dot(x, y):
+    x = _f64(x)
+    y = _f64(y)
+    return lib().orc_dot(x.size, _ptr(x), _ptr(y))
+
+
+def matvec(op, A: Matrix, ikle, y, x=None, c=0.0):
+    ikle = _i32(ikle)
+    ndp, nelem = ikle.shape
+    y = _f64(y)
+    n = y.size
+    x = np.zeros(n) if x is None else _f64(x).copy()
+    w = np.empty(ndp * nelem)
+    t = np.empty(n)
+    lib().orc_matvec(OPS[op], _ptr(x), _ptr(A.da), A.typdia.encode(), _ptr(A.xa),
+                     A.typext.encode(), _ptr(y), c, _ptr(ikle), n, nelem, nelem, ndp,
+                     _ptr(w), _ptr(t))
+    return x
+
+
+def mt01aa(surfac, xmul=1.0):
+    s = _f64(surfac)
+    ne = s.size
+    t = np.empty((3, ne))
+    xm = np.empty((3, ne))
+    lib().orc_mt01aa(_ptr(t), _ptr(xm), xmul, _ptr(s), ne, ne)
+    return t, xm
+
+
+def mt02aa(kind, xel, yel, surfac, ikle, u, v=None, xmul=1.0, mon=False, ldu=0):
+    s = _f64(surfac)
+    ne = s.size
+    t = np.empty((3, ne))
+    xm = np.empty((3, ne))
+    u = _f64(u)
+    v = _f64(v) if v is not None else u
+    lib().orc_mt02aa(kind, _ptr(t), _ptr(xm), xmul, _ptr(u), _ptr(v), ldu,
+                     _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(s), _ptr(_i32(ikle)), ne, ne,
+                     1 if mon else 0)
+    return t, xm
+
+
+def mt01pp(surfac, z, ikle, xmul=1.0):
+    s = _f64(surfac)
+    ne = s.size
+    t = np.empty((6, ne))
+    xm = np.empty((15, ne))
+    lib().orc_mt01pp(_ptr(t), _ptr(xm), xmul, _ptr(s), _ptr(_f64(z)), _ptr(_i32(ikle)), ne, ne)
+    return t, xm
+
+
+def assemble_diag(t, ikle, npoin):
+    ikle = _i32(ikle)
+    ndp, ne = ikle.shape
+    da = np.empty(npoin)
+    lib().orc_assemble_diag(_ptr(da), _ptr(_f64(t)), _ptr(ikle), npoin, ne, ne, ndp)
+    return da
+
+
+def solve(A: Matrix, ikle, b, x0=None, slv=1, precon=2, krylov=5, nitmax=1000, eps=1e-8):
+    """Oracle SOLVE. A and b are consumed as the reference does (copies are
+    made here); returns x, info, and the preconditioned A, b."""
+    ikle = _i32(ikle)
+    ndp, ne = ikle.shape
+    A2 = A.copy()
+    b2 = _f64(b).copy()
+    n = b2.size
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    td = ctypes.c_char(A2.typdia.encode())
+    info = OrcInfo()
+    rc = lib().orc_solve(slv, precon, krylov, nitmax, eps, _ptr(x), _ptr(A2.da),
+                         ctypes.byref(td), _ptr(A2.xa), A2.typext.encode(), _ptr(b2),
+                         _ptr(ikle), n, ne, ne, ndp, ctypes.byref(info))
+    if rc != 0:
+        raise ValueError(f"orc_solve returned {rc}")
+    A2.typdia = td.value.decode()
+    return x, info, A2, b2
+
+
+def dirich(A: Matrix, ikle, sm, fbor, limdir, kdir=5, f=None):
+    ikle = _i32(ikle)
+    ndp, ne = ikle.shape
+    A2 = A.copy()
+    sm = _f64(sm).copy()
+    n = sm.size
+    f = np.zeros(n) if f is None else _f64(f).copy()
+    lib().orc_dirich(_ptr(f), _ptr(A2.da), A2.typdia.encode(), _ptr(A2.xa),
+                     A2.typext.encode(), _ptr(sm), _ptr(_f64(fbor)), _ptr(_i32(limdir)),
+                     kdir, _ptr(ikle), n, ne, ne, ndp)
+    return f, A2, sm
+
+
+def lump(A: Matrix, ikle, xmul=1.0):
+    ikle = _i32(ikle)
+    ndp, ne = ikle.shape
+    n = A.da.size
+    d = np.empty(n)
+    lib().orc_lump(_ptr(d), _ptr(A.da), A.typdia.encode(), _ptr(A.xa), A.typext.encode(),
+                   xmul, _ptr(ikle), n, ne, ne, ndp)
+    return d
+
+
+# --------------------------------------------------------------------------
+# the reference's own BIEF (flang build), where present
+# --------------------------------------------------------------------------
+
+_ref = None
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref`")
+        L = ctypes.CDLL(REF_SO)
+        L.ref_mesh_init.argtypes = [_I, _I, _I, _I, _P]
+        L.ref_matvec.argtypes = [_I, _C, _C, _P, _P, _P, _D, _P]
+        L.ref_solve.argtypes = [_I, _I, _I, _I, _D, ctypes.POINTER(ctypes.c_char), _C,
+                                _P, _P, _P, _P, ctypes.c_char_p, _I]
+        L.ref_dirich.argtypes = [_C, _C, _P, _P, _P, _P, _P, _I, _P]
+        L.ref_lump.argtypes = [_C, _C, _P, _P, _D, _P]
+        L.ref_mt01aa.argtypes = [_I, _D, _P, _P, _P]
+        L.ref_mt02aa.argtypes = [_I, _I, _I, _D, _P, _P, _P, _P, _P, _P, _I, _P, _P]
+        _ref = L
+    return _ref
+
+
+class RefMesh:
+    """Binds a mesh to the reference driver's BIEF_MESH (one at a time)."""
+    _current = None
+
+    def __init__(self, ikle, npoin2, nelem2, nplan=1):
+        self.ikle = _i32(ikle)
+        self.ndp = self.ikle.shape[0]
+        self.args = (npoin2, nelem2, nplan)
+
+    def bind(self):
+        if RefMesh._current is not self:
+            ref().ref_mesh_init(self.ndp, *self.args, _ptr(self.ikle))
+            RefMesh._current = self
+
+
+def ref_matvec(rm: RefMesh, op, A: Matrix, y, x=None, c=0.0):
+    rm.bind()
+    y = _f64(y)
+    x = np.zeros(y.size) if x is None else _f64(x).copy()
+    ref().ref_matvec(OPS[op], A.typdia.encode(), A.typext.encode(), _ptr(A.da), _ptr(A.xa),
+                     _ptr(y), c, _ptr(x))
+    return x
+
+
+def _parse_solver_log(path):
+    """Iteration count and precision from the reference's listing line
+    (gracjg.F:426-435, cgstab.f, gmres.f formats)."""
+    last = None
+    with open(path) as fh:
+        for line in fh:
+            if "(BIEF)" in line:
+                last = line.strip()
+    if last is None:
+        return None, None, last
+    import re
+    m = re.search(r":\s+(\d+)\s+ITERATIONS", last) or re.search(r"ITERATIONS:\s+(\d+)", last)
+    niter = int(m.group(1)) if m else None
+    m2 = re.search(r"PRECISION:\s*([-+0-9.Ee]+)", last)
+    prec = float(m2.group(1)) if m2 else None
+    return niter, prec, last
+
+
+def ref_solve(rm: RefMesh, A: Matrix, b, x0=None, slv=1, precon=2, krylov=5, nitmax=1000,
+              eps=1e-8):
+    rm.bind()
+    A2 = A.copy()
+    b2 = _f64(b).copy()
+    x = np.zeros(b2.size) if x0 is None else _f64(x0).copy()
+    td = ctypes.create_string_buffer(A2.typdia.encode(), 1)
+    fd, path = tempfile.mkstemp(suffix=".log")
+    os.close(fd)
+    try:
+        p = path.encode()
+        ref().ref_solve(slv, precon, krylov, nitmax, eps, td, A2.typext.encode(), _ptr(A2.da),
+                        _ptr(A2.xa), _ptr(b2), _ptr(x), p, len(p))
+        niter, prec, line = _parse_solver_log(path)
+    finally:
+        os.unlink(path)
+    A2.typdia = td.raw[:1].decode()
+    return x, {"niter": niter, "relres": prec, "line": line}, A2, b2
+
+
+def ref_dirich(rm: RefMesh, A: Matrix, sm, fbor, limdir, kdir=5, f=None):
+    rm.bind()
+    A2 = A.copy()
+    sm = _f64(sm).copy()
+    f = np.zeros(sm.size) if f is None else _f64(f).copy()
+    ref().ref_dirich(A2.typdia.encode(), A2.typext.encode(), _ptr(A2.da), _ptr(A2.xa), _ptr(sm),
+                     _ptr(_f64(fbor)), _ptr(_i32(limdir)), kdir, _ptr(f))
+    return f, A2, sm
+
+
+def ref_lump(rm: RefMesh, A: Matrix, xmul=1.0):
+    rm.bind()
+    d = np.empty(A.da.size)
+    ref().ref_lump(A.typdia.encode(), A.typext.encode(), _ptr(A.da), _ptr(A.xa), xmul, _ptr(d))
+    return d
+
+
+def ref_mt01aa(surfac, xmul=1.0):
+    s = _f64(surfac)
+    ne = s.size
+    t = np.empty((3, ne))
+    xm = np.empty((3, ne))
+    ref().ref_mt01aa(ne, xmul, _ptr(s), _ptr(t), _ptr(xm))
+    return t, xm
+
+
+def ref_mt02aa(kind, xel, yel, surfac, ikle, u, v=None, xmul=1.0, mon=False, npoin=None):
+    s = _f64(surfac)
+    ne = s.size
+    u = _f64(u)
+    v = _f64(v) if v is not None else u
+    t = np.empty((3, ne))
+    xm = np.empty((3, ne))
+    npoin = npoin if npoin is not None else (u.size // (3 if kind == 4 else 1))
+    ref().ref_mt02aa(kind, ne, npoin, xmul, _ptr(u), _ptr(v), _ptr(_f64(xel)), _ptr(_f64(yel)),
+                     _ptr(s), _ptr(_i32(ikle)), 1 if mon else 0, _ptr(t), _ptr(xm))
+    return t, xm
