@@ -1,0 +1,183 @@
+/*
+ * libbiefhip -- C ABI of the MI355X (gfx950) implementation of TELEMAC's
+ * BIEF linear-system path: element-by-element (EBE) matrix-vector products,
+ * diagonal preconditioning and the CG / BiCGSTAB / GMRES loops behind SOLVE.
+ *
+ * Plain C types only (no torch, no HIP types in the signatures). Arrays are
+ * BIEF's own layouts:
+ *   ikle  : int32, structure of arrays, 0-based: ikle[k*nelem + e] is local
+ *           node k of element e (BIEF IKLE(NELMAX,NDP) minus one);
+ *   da    : float64[npoin], the assembled diagonal (BIEF A%D%R);
+ *   xa    : float64[ncoef*nelem], off-diagonal element terms, column c at
+ *           xa[c*nelem] (BIEF A%X%R with NELMAX = nelem). Triangles:
+ *           symmetric 'S' (12,13,23), non-symmetric 'Q' (12,13,23,21,31,32);
+ *           prisms: 15 upper pairs row-major, 'Q' adds the 15 transposed
+ *           (sources/utils/bief/matvct.f:152-169, mv0606.f).
+ * Vectors are in the caller's node numbering; the library renumbers
+ * internally and translates at its boundary.
+ *
+ * Entry points with a `_dev` suffix take device pointers (HBM-resident data,
+ * the caller's numbering) and run on the context's stream; the others take
+ * host pointers and are synchronous, as BIEF's Fortran routines are.
+ *
+ * Every function returns a bhip_status. The library never aborts the process
+ * (the reference's PLANTE/STOP, sources/utils/special/plante.F, becomes
+ * BHIP_EBADARG / BHIP_EDEVICE, which the Fortran shim turns back into PLANTE).
+ */
+#ifndef BIEF_HIP_H
+#define BIEF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  BHIP_OK = 0,
+  BHIP_MAXITER = 1,   /* NITMAX reached (the reference only prints a message) */
+  BHIP_EBADARG = 2,   /* invalid argument / unsupported configuration         */
+  BHIP_ZERO_RHS = 3,  /* ||B|| < 1e-15: X = 0 (gracjg.F:189-202)              */
+  BHIP_EDEVICE = 4    /* HIP runtime error                                    */
+} bhip_status;
+
+/* MATVEC operation codes: the OP strings of sources/utils/bief/matvec.f:98-103
+ * ('X=AY    ', 'X=CAY   ', ... 'X=X+CTAY'), in this order. */
+typedef enum {
+  BHIP_OP_AY = 0, BHIP_OP_CAY = 1, BHIP_OP_NAY = 2, BHIP_OP_XPAY = 3,
+  BHIP_OP_XMAY = 4, BHIP_OP_XPCAY = 5, BHIP_OP_TAY = 6, BHIP_OP_NTAY = 7,
+  BHIP_OP_XPTAY = 8, BHIP_OP_XMTAY = 9, BHIP_OP_XPCTAY = 10
+} bhip_op;
+
+/* precision of a solve */
+enum {
+  BHIP_PREC_FP64 = 0,   /* FP64 everywhere: the reference's arithmetic       */
+  BHIP_PREC_MIXED = 1   /* FP32 EBE mat-vec + FP32 inner CG inside FP64
+                           iterative refinement; FP64 residual, dots, update */
+};
+
+typedef struct bhip_ctx bhip_ctx;
+
+/* Solver configuration: BIEF's SLVCFG (sources/utils/bief/bief_def.f:506-532)
+ * plus the precision mode. */
+typedef struct {
+  int slv;        /* 1 CG (GRACJG), 6 BiCGSTAB (CGSTAB), 7 GMRES            */
+  int precon;     /* 0/1 none, 2 diagonal, 5 |diagonal| (products of primes
+                     3, 7, 11, 13, 17 are rejected)                          */
+  int krylov;     /* GMRES restart dimension, 1..20                          */
+  int nitmax;     /* CG/CGSTAB: iterations; GMRES: restart cycles            */
+  double eps;     /* stop: ||r'||^2 <= max(||b'||^2,1) eps^2 (CG, CGSTAB);
+                     ||r'|| <= max(||b'||,1) eps (GMRES)                    */
+  int precision;  /* BHIP_PREC_FP64 or BHIP_PREC_MIXED (CG only)           */
+  double inner_eps;  /* mixed: relative reduction per inner FP32 solve       */
+  int inner_nitmax;  /* mixed: iteration cap of one inner solve              */
+} bhip_solver_cfg;
+
+/* Outcome of a solve (BIEF SLVCFG%NIT / OK are never set by the reference;
+ * its listing line carries the same numbers, gracjg.F:426-435). */
+typedef struct {
+  int niter;        /* iterations M (GMRES: restart cycles)                  */
+  int inner_iters;  /* mixed: total FP32 inner CG iterations                 */
+  int outer_iters;  /* mixed: FP64 refinement sweeps                         */
+  int relat;        /* 1: relative precision, 0: absolute (||b'|| < 1)       */
+  double relres;    /* sqrt(RMRM/XL) on the preconditioned system            */
+  int converged;
+} bhip_solve_info;
+
+/* ---- mesh context ------------------------------------------------------ */
+
+/* Creates the per-mesh context (replaces what SOLVE/MATVEC read from
+ * BIEF_MESH, sources/utils/bief/bief_def.f:205-498: NPOIN, NELEM, IKLE).
+ * ndp = 3 (P1 triangles, IELM 11) or 6 (P1 prisms, IELM 41).
+ * x, y (may be NULL) are node coordinates used only to order nodes for
+ * locality; nplan > 1 marks a prism mesh built by plane-major extrusion of a
+ * 2D mesh of npoin/nplan nodes (node p*npoin2 + i), for column-wise blocks.
+ * block_nodes: nodes owned per workgroup (0 = default 1024). */
+int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle,
+                    const double* x, const double* y, int nplan,
+                    int block_nodes, bhip_ctx** out);
+int bhip_ctx_destroy(bhip_ctx* ctx);
+/* the HIP stream (hipStream_t) all work of the context runs on */
+void* bhip_ctx_stream(bhip_ctx* ctx);
+/* layout statistics: nblocks, max colours per block, element instances
+ * (duplicated halo elements included), halo node count */
+int bhip_ctx_stats(const bhip_ctx* ctx, int* nblocks, int* max_colors,
+                   int64_t* elem_instances, int64_t* halo_nodes);
+
+/* Host-only: builds and validates the block layout bhip_ctx_create would use
+ * (node order, blocks, halo, per-block colouring with no two elements of a
+ * colour sharing an owned node, 16-bit local ids), without touching a GPU.
+ * stats[7] (may be NULL): nblocks, max colours, element instances, halo
+ * nodes, max local nodes, max owned nodes, LDS bytes per FP64 block.
+ * No counterpart in the reference (its vector-mode renumbering check is
+ * VECLE3, sources/utils/bief/vecle3.f). */
+int bhip_layout_check(int npoin, int nelem, int ndp, const int32_t* ikle,
+                      const double* x, const double* y, int nplan,
+                      int block_nodes, int64_t* stats);
+
+/* ---- matrix ------------------------------------------------------------ */
+
+/* Loads an EBE matrix (replaces the A%D / A%X / A%TYPDIA / A%TYPEXT fields
+ * BIEF_OBJ hands to MATVEC/SOLVE, bief_def.f:74-196). typdia 'Q','I','0';
+ * typext 'S','Q','0'. The host variant copies from host memory. */
+int bhip_matrix_set(bhip_ctx* ctx, char typdia, char typext, const double* da,
+                    const double* xa);
+int bhip_matrix_set_dev(bhip_ctx* ctx, char typdia, char typext,
+                        const double* da_dev, const double* xa_dev);
+/* Reads the context's (possibly preconditioned) matrix back in BIEF layout;
+ * *typdia receives the current TYPDIA. */
+int bhip_matrix_get(bhip_ctx* ctx, char* typdia, double* da, double* xa);
+
+/* ---- MATVEC ------------------------------------------------------------ */
+
+/* X = op(A, Y, C): sources/utils/bief/matvec.f:116-401 (EBE storage, LEGO
+ * assembly) -> matvct.f -> mv0303.f / mv0606.f -> assvec.f. */
+int bhip_matvec(bhip_ctx* ctx, int op, double c, const double* y, double* x);
+int bhip_matvec_dev(bhip_ctx* ctx, int op, double c, const double* y_dev,
+                    double* x_dev);
+
+/* ---- SOLVE ------------------------------------------------------------- */
+
+/* SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX) for one matrix
+ * (sources/utils/bief/solve.f:2-654): diagonal preconditioning PRECDT/PRECD1
+ * (precd1.f:90-149), then GRACJG (gracjg.F) / CGSTAB (cgstab.f) /
+ * GMRES (gmres.f), then X <- X*D. As in the reference, the context's matrix
+ * is left preconditioned (D*A*D, TYPDIA 'I') and b is overwritten by D*b.
+ * Returns BHIP_OK, BHIP_MAXITER, BHIP_ZERO_RHS, or an error. */
+int bhip_solve(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x,
+               double* b, bhip_solve_info* info);
+int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
+                   double* b_dev, bhip_solve_info* info);
+
+/* ---- operand preparation before SOLVE ----------------------------------- */
+
+/* DIRICH with domain-sized FBOR / LIMDIR (sources/utils/bief/dirich.f:83 ->
+ * diri01.f:94-139 -> diraux.f): SM -= A*FBOR; at nodes with
+ * limdir == kdir: SM = DA*FBOR, F = FBOR; rows and columns of A at those
+ * nodes zeroed (diagonal kept). */
+int bhip_dirich(bhip_ctx* ctx, double* f, double* sm, const double* fbor,
+                const int32_t* limdir, int kdir);
+
+/* LUMP: DIAG = A * (XMUL everywhere) (sources/utils/bief/lump.f) */
+int bhip_lump(bhip_ctx* ctx, double xmul, double* diag);
+
+/* Element matrices on the device, assembled into the context's matrix as
+ * BIEF's MATRIX does (matrix.f: element matrix, diagonal by ASSVEC, then
+ * OM 'M=N' or 'M=M+N'). accumulate = 0: 'M=N'; 1: 'M=M+N'.
+ *  - MATMAS, MT01AA (mt01aa.f:83-106): mass, xmul*S/12 (off-diag) and
+ *    xmul*S/6 (diagonal); surfac[nelem]
+ *  - MATDIFUV, MT02AA 'UV' (mt02aa.f:200-240): diffusion with the P1
+ *    product coefficient U*V; xel, yel [3*nelem] local coordinates,
+ *    u, v [npoin]
+ *  - MT02AA P1 isotropic (mt02aa.f:247-290): u [npoin], v ignored. */
+int bhip_matrix_mt01aa(bhip_ctx* ctx, double xmul, const double* surfac,
+                       int accumulate);
+int bhip_matrix_mt02aa(bhip_ctx* ctx, int kind, double xmul,
+                       const double* xel, const double* yel,
+                       const double* surfac, const double* u, const double* v,
+                       int accumulate);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BIEF_HIP_H */

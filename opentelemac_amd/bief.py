@@ -1,0 +1,257 @@
+"""Python host layer over libbiefhip (the C ABI in include/bief_hip.h).
+
+It mirrors BIEF's interface for the linear-system path:
+
+  * ``Context(mesh)``                 -- what SOLVE/MATVEC read from BIEF_MESH
+  * ``Context.set_matrix(...)``       -- an EBE BIEF_OBJ matrix (D, X, TYPDIA, TYPEXT)
+  * ``Context.matvec(op, x, y, c)``   -- MATVEC(OP, X, A, Y, C, MESH)
+                                         (sources/utils/bief/matvec.f:98-103)
+  * ``Context.solve(x, b, cfg)``      -- SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX)
+                                         (sources/utils/bief/solve.f:2-5)
+  * ``Context.dirich``, ``Context.lump`` -- DIRICH (dirich.f), LUMP (lump.f)
+
+Vectors are either host numpy arrays (synchronous copies, as BIEF's
+routines are) or torch CUDA tensors (device resident, work queued on the
+context's stream). Errors the reference would PLANTE on raise ``BiefError``;
+non-convergence is reported in the returned info, as the reference only
+prints it (gracjg.F:411-418).
+
+The library must be built (``__graft_entry__.build()``); there is no CPU
+fallback: a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbiefhip.so")
+
+OPS = {"X=AY": 0, "X=CAY": 1, "X=-AY": 2, "X=X+AY": 3, "X=X-AY": 4, "X=X+CAY": 5,
+       "X=TAY": 6, "X=-TAY": 7, "X=X+TAY": 8, "X=X-TAY": 9, "X=X+CTAY": 10}
+
+STATUS = {0: "ok", 1: "maxiter", 2: "bad argument", 3: "zero rhs", 4: "device error"}
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class BiefError(RuntimeError):
+    pass
+
+
+class SolverCfg(ctypes.Structure):
+    """SLVCFG (sources/utils/bief/bief_def.f:506-532) + precision mode."""
+    _fields_ = [("slv", ctypes.c_int), ("precon", ctypes.c_int), ("krylov", ctypes.c_int),
+                ("nitmax", ctypes.c_int), ("eps", ctypes.c_double), ("precision", ctypes.c_int),
+                ("inner_eps", ctypes.c_double), ("inner_nitmax", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("niter", ctypes.c_int), ("inner_iters", ctypes.c_int),
+                ("outer_iters", ctypes.c_int), ("relat", ctypes.c_int),
+                ("relres", ctypes.c_double), ("converged", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_C = ctypes.c_char
+
+_lib = None
+
+
+def lib():
+    """Loads libbiefhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BiefError(f"{LIB_PATH} not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.bhip_ctx_create.argtypes = [_I, _I, _I, _P, _P, _P, _I, _I, ctypes.POINTER(_P)]
+        L.bhip_ctx_destroy.argtypes = [_P]
+        L.bhip_ctx_stream.argtypes = [_P]
+        L.bhip_ctx_stream.restype = _P
+        L.bhip_ctx_stats.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                     ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64)]
+        L.bhip_matrix_set.argtypes = [_P, _C, _C, _P, _P]
+        L.bhip_matrix_set_dev.argtypes = [_P, _C, _C, _P, _P]
+        L.bhip_matrix_get.argtypes = [_P, ctypes.c_char_p, _P, _P]
+        L.bhip_matvec.argtypes = [_P, _I, _D, _P, _P]
+        L.bhip_matvec_dev.argtypes = [_P, _I, _D, _P, _P]
+        L.bhip_solve.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P, ctypes.POINTER(SolveInfo)]
+        L.bhip_solve_dev.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
+                                     ctypes.POINTER(SolveInfo)]
+        for name, args in (("bhip_dirich", [_P, _P, _P, _P, _P, _I]),
+                           ("bhip_lump", [_P, _D, _P]),
+                           ("bhip_matrix_mt01aa", [_P, _D, _P, _I]),
+                           ("bhip_matrix_mt02aa", [_P, _I, _D, _P, _P, _P, _P, _P, _I])):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc in (0, 1, 3):
+        return rc
+    raise BiefError(f"{what}: {STATUS.get(rc, rc)} (status {rc})")
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _host_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+@dataclass
+class MatrixInfo:
+    typdia: str
+    typext: str
+
+
+class Context:
+    """Per-mesh device context (libbiefhip bhip_ctx)."""
+
+    def __init__(self, mesh, block_nodes: int = 0, use_coords: bool = True):
+        self.mesh = mesh
+        self.npoin = mesh.npoin
+        self.nelem = mesh.nelem
+        self.ndp = mesh.ndp
+        self._ikle = np.ascontiguousarray(mesh.ikle, dtype=np.int32)
+        x = np.ascontiguousarray(mesh.x, dtype=np.float64) if use_coords else None
+        y = np.ascontiguousarray(mesh.y, dtype=np.float64) if use_coords else None
+        h = _P()
+        rc = lib().bhip_ctx_create(self.npoin, self.nelem, self.ndp, self._ikle.ctypes.data,
+                                   x.ctypes.data if x is not None else None,
+                                   y.ctypes.data if y is not None else None,
+                                   int(getattr(mesh, "nplan", 1)), int(block_nodes),
+                                   ctypes.byref(h))
+        _check(rc, "bhip_ctx_create")
+        self._h = h
+        self.typdia = "0"
+        self.typext = "0"
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bhip_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream_ptr(self) -> int:
+        return lib().bhip_ctx_stream(self._h)
+
+    def torch_stream(self):
+        import torch
+        return torch.cuda.ExternalStream(self.stream_ptr())
+
+    def stats(self):
+        nb, mc = _I(), _I()
+        ei, hn = ctypes.c_int64(), ctypes.c_int64()
+        lib().bhip_ctx_stats(self._h, ctypes.byref(nb), ctypes.byref(mc), ctypes.byref(ei),
+                             ctypes.byref(hn))
+        return {"nblocks": nb.value, "max_colors": mc.value, "elem_instances": ei.value,
+                "halo_nodes": hn.value}
+
+    def synchronize(self):
+        import torch
+        self.torch_stream().synchronize()
+
+    # ---- matrix ---------------------------------------------------------
+    def set_matrix(self, typdia: str, typext: str, da, xa):
+        """A%D, A%X (ncoef x nelem), A%TYPDIA, A%TYPEXT."""
+        if _is_torch(da):
+            self._sync_torch()
+            rc = lib().bhip_matrix_set_dev(self._h, typdia.encode(), typext.encode(),
+                                           da.data_ptr(), xa.data_ptr() if xa is not None else None)
+        else:
+            da = _host_f64(da)
+            xa = _host_f64(xa) if xa is not None else None
+            rc = lib().bhip_matrix_set(self._h, typdia.encode(), typext.encode(), da.ctypes.data,
+                                       xa.ctypes.data if xa is not None else None)
+        _check(rc, "bhip_matrix_set")
+        self.typdia, self.typext = typdia, typext
+
+    def get_matrix(self):
+        npair = self.ndp * (self.ndp - 1) // 2
+        ncoef = {"Q": 2 * npair, "S": npair, "0": 0}[self.typext]
+        da = np.empty(self.npoin)
+        xa = np.empty((max(ncoef, 1), self.nelem))
+        td = ctypes.create_string_buffer(1)
+        _check(lib().bhip_matrix_get(self._h, td, da.ctypes.data, xa.ctypes.data if ncoef else None),
+               "bhip_matrix_get")
+        self.typdia = td.raw[:1].decode()
+        return self.typdia, da, xa[:ncoef]
+
+    # ---- MATVEC ---------------------------------------------------------
+    def _sync_torch(self):
+        import torch
+        torch.cuda.current_stream().synchronize()
+
+    def matvec(self, op: str, y, x=None, c: float = 0.0):
+        """X = op(A, Y, C) as BIEF MATVEC. Host arrays: returns a new array.
+        Torch CUDA tensors: x is updated in place on the context stream."""
+        code = OPS[op]
+        if _is_torch(y):
+            import torch
+            if x is None:
+                x = torch.zeros_like(y)
+            self._sync_torch()
+            _check(lib().bhip_matvec_dev(self._h, code, float(c), y.data_ptr(), x.data_ptr()),
+                   "bhip_matvec_dev")
+            return x
+        y = _host_f64(y)
+        x = np.zeros(self.npoin) if x is None else _host_f64(x).copy()
+        _check(lib().bhip_matvec(self._h, code, float(c), y.ctypes.data, x.ctypes.data),
+               "bhip_matvec")
+        return x
+
+    # ---- SOLVE ----------------------------------------------------------
+    def solve(self, x, b, slv=1, precon=2, krylov=5, nitmax=1000, eps=1e-8,
+              precision=PREC_FP64, inner_eps=1e-4, inner_nitmax=0):
+        """SOLVE: x (initial guess, overwritten), b (overwritten by D*b as in
+        the reference); the context's matrix is left preconditioned.
+        Returns (x, info dict, status)."""
+        cfg = SolverCfg(slv, precon, krylov, nitmax, eps, precision, inner_eps, inner_nitmax)
+        info = SolveInfo()
+        if _is_torch(x):
+            self._sync_torch()
+            rc = lib().bhip_solve_dev(self._h, ctypes.byref(cfg), x.data_ptr(), b.data_ptr(),
+                                      ctypes.byref(info))
+            _check(rc, "bhip_solve_dev")
+            self.synchronize()
+        else:
+            xh = _host_f64(x)
+            bh = _host_f64(b)
+            if not xh.flags.writeable or xh is x:
+                xh = xh.copy()
+            rc = lib().bhip_solve(self._h, ctypes.byref(cfg), xh.ctypes.data, bh.ctypes.data,
+                                  ctypes.byref(info))
+            _check(rc, "bhip_solve")
+            if isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.writeable:
+                x[...] = xh
+            if isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.writeable:
+                b[...] = bh
+            x = xh
+        self.typdia = "I" if precon in (2,) else self.typdia
+        d = info.as_dict()
+        d["status"] = STATUS[rc]
+        return x, d

@@ -1,0 +1,87 @@
+"""Build recipes for the native parts (run from __graft_entry__.build()).
+
+  * libbiefhip.so   -- HIP kernels + C ABI, gfx950 only (hipcc)
+  * libbiefmesh.so  -- host-only synthetic mesh helper (g++)
+  * libbiefshim.so  -- the Fortran BIEF-signature host layer over the C ABI
+                       (flang, ISO_C_BINDING), when flang is present
+Everything is built in-tree so the libraries travel with the sources.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+FSRC = os.path.join(HERE, "fortran")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# packed FP32 instructions are kept out of the device code
+HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed ({' '.join(cmd)}):\n{r.stdout[-6000:]}")
+    return r.stdout
+
+
+def build_hip(verbose=False):
+    out = os.path.join(HERE, "libbiefhip.so")
+    srcs = [os.path.join(CSRC, f) for f in ("bief_hip.hip", "bief_krylov.hip", "bief_elem.hip",
+                                            "layout.cpp")]
+    srcs = [s for s in srcs if os.path.exists(s)]
+    tmp = out + ".tmp"
+    log = _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    if verbose:
+        print(log)
+    return out
+
+
+def build_mesh():
+    out = os.path.join(HERE, "libbiefmesh.so")
+    _run(["g++", "-O2", "-fPIC", "-shared", os.path.join(CSRC, "meshgen.cpp"), "-o", out])
+    return out
+
+
+def have_flang():
+    return shutil.which("flang") is not None or os.path.exists("/opt/rocm/bin/amdflang")
+
+
+def build_shim():
+    """Fortran host layer: BIEF-shaped types and SOLVE/MATVEC/MATRBL with the
+    reference's signatures, calling libbiefhip through ISO_C_BINDING."""
+    if not os.path.isdir(FSRC) or not have_flang():
+        return None
+    fc = shutil.which("flang") or "/opt/rocm/bin/amdflang"
+    out = os.path.join(HERE, "libbiefshim.so")
+    mod = os.path.join(HERE, "_fmod")
+    os.makedirs(mod, exist_ok=True)
+    srcs = [os.path.join(FSRC, f) for f in ("bief_hip_def.f90", "bief_hip_capi.f90",
+                                            "bief_hip_shim.f90")]
+    srcs = [s for s in srcs if os.path.exists(s)]
+    if not srcs:
+        return None
+    _run([fc, "-O2", "-fPIC", "-shared", "-module-dir", mod, *srcs,
+          "-L", HERE, "-lbiefhip", f"-Wl,-rpath,{HERE}", "-o", out])
+    return out
+
+
+def build_oracle():
+    _run(["make", "-C", os.path.join(ROOT, "oracle")])
+    if os.path.isdir("/root/reference/sources/utils/bief") and have_flang():
+        _run(["make", "-C", ROOT, "-f", "oracle/ref.mk", "-j8"])
+
+
+def build_all(verbose=False):
+    build_mesh()
+    build_hip(verbose)
+    build_shim()
+    build_oracle()

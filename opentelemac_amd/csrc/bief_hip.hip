@@ -1,0 +1,1234 @@
+// libbiefhip: MI355X (gfx950) implementation of TELEMAC's BIEF linear-system
+// path (EBE mat-vec, diagonal preconditioning, CG / BiCGSTAB / GMRES).
+// C ABI in include/bief_hip.h.
+//
+// Design (HBM-bound integer/FP streaming, no MFMA):
+//  * nodes renumbered (Hilbert) and cut into blocks of consecutive ids; one
+//    workgroup per block owns its nodes (layout.cpp);
+//  * a workgroup stages the vector it multiplies into LDS for its owned and
+//    halo nodes, walks its elements colour by colour (no two elements of a
+//    colour share an owned node) accumulating in LDS, then writes its owned
+//    nodes once -- deterministic, atomic-free scatter-add;
+//  * element data are structure of arrays in slot order (coalesced): 16-bit
+//    local node ids and the off-diagonal coefficients;
+//  * CG keeps its scalars on the device: each kernel writes one partial sum
+//    per workgroup and the last workgroup to finish (ticket counter) reduces
+//    them in a fixed order and advances the solver state, so iterations run
+//    back to back without host round trips and results are bitwise
+//    reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bief_hip.h"
+#include "layout.h"
+
+namespace bhip {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+#define HIP_OK(expr)                                              \
+  do {                                                            \
+    hipError_t e_ = (expr);                                       \
+    if (e_ != hipSuccess) {                                       \
+      fprintf(stderr, "libbiefhip: %s failed: %s (%s:%d)\n", #expr, \
+              hipGetErrorString(e_), __FILE__, __LINE__);         \
+      return BHIP_EDEVICE;                                        \
+    }                                                             \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// solver state (device resident)
+// ---------------------------------------------------------------------------
+enum : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FINAL = 2, ST_MAXITER = 3 };
+
+struct SolveState {
+  double xl;       // max(||b'||^2, 1) (gracjg.F:189-209)
+  double eps2xl;   // stop threshold on ||r||^2
+  double rr;       // RMRM
+  double rr_prev;  // RM1GM1
+  double rho;      // RO
+  double beta;     // BETA
+  double dad;      // D.AD
+  double rr0;      // initial ||r||^2 (mixed inner solves)
+  int m;           // iteration counter M
+  int state;
+  int nitmax;
+  int pad;
+  unsigned ticket[4];
+};
+
+// ---------------------------------------------------------------------------
+// deterministic reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// fixed-order block sum; result valid in thread 0
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+  }
+  __syncthreads();
+  return s;
+}
+
+// Writes this block's partial, then returns true (in every thread) for the
+// last block to arrive; that block sees every partial.
+__device__ inline bool publish_partial(double* partial, int slot, double v,
+                                       unsigned* ticket, int nblocks, int* sh_flag) {
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partial[slot], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    *sh_flag = (t == (unsigned)(nblocks - 1));
+  }
+  __syncthreads();
+  bool last = *sh_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// fixed-order sum of n partials by one block; result valid in thread 0
+__device__ inline double sum_partials(double* partial, int n, double* sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(s, sh);
+}
+
+// ---------------------------------------------------------------------------
+// EBE kernel
+// ---------------------------------------------------------------------------
+enum : int { DIAG_ZERO = 0, DIAG_ONE = 1, DIAG_Q = 2 };
+// how the multiplied vector is obtained for each local node
+enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2 };
+// what happens at owned nodes after the element loop
+enum : int { EPI_OP = 0, EPI_RESID = 1, EPI_CGAD = 2, EPI_RESID_MIXED = 3 };
+// what the last workgroup does with the reduced sum
+enum : int { FIN_NONE = 0, FIN_RESID_INIT = 1, FIN_CGAD = 2, FIN_MIXED_OUTER = 3 };
+
+template <typename T, typename TA>
+struct EbeArgs {
+  // layout
+  const int32_t* node_start;
+  const int32_t* halo_start;
+  const int32_t* halo;
+  const int32_t* color_off;
+  const uint16_t* lid;
+  int nslot_pad;
+  int nblocks;
+  // matrix
+  const TA* xa;   // [ncoef][nslot_pad]
+  const TA* da;   // [npoin] (DIAG_Q)
+  // stage inputs
+  const T* y;                 // PLAIN: multiplied vector
+  const T* r;                 // CGDIR: residual
+  const T* dold;              // CGDIR: previous direction
+  T* dnew;                    // CGDIR: new direction (owned nodes written)
+  const double* xd;           // XUPD: x (FP64)
+  const float* delta;         // XUPD: inner correction
+  double* xnew;               // XUPD: x - delta (owned nodes written)
+  // epilogue
+  T* out;
+  const T* xin;
+  double c;
+  int op;
+  const double* b;            // RESID: right-hand side (FP64)
+  float* r32;                 // RESID_MIXED: inner residual -(A x - b)
+  float* zero32;              // RESID_MIXED: inner solution reset
+  // reductions / state
+  double* partial;
+  SolveState* st;
+  int ticket_id;
+  int check_state;            // kernels of a running solve exit when done
+};
+
+// column of the coefficient array holding A(i, j), i != j
+template <int NDP, bool SYMQ>
+__host__ __device__ constexpr int coef_col(int i, int j) {
+  constexpr int npair = NDP * (NDP - 1) / 2;
+  int a = i < j ? i : j, b = i < j ? j : i;
+  // row-major upper pair index
+  int idx = a * NDP - a * (a + 1) / 2 + (b - a - 1);
+  return (i < j || !SYMQ) ? idx : npair + idx;
+}
+
+template <typename T, typename TA, int NDP, bool SYMQ, bool TRANS, int DIAG, int STAGE,
+          int EPI, int FIN>
+__global__ void __launch_bounds__(kThreads)
+k_ebe(EbeArgs<T, TA> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double sh_red[kWaves];
+  __shared__ int sh_flag;
+  if (a.check_state && a.st->state != ST_RUNNING) return;
+
+  // XCD-aware mapping: workgroup g is dispatched to XCD g % 8; give each XCD
+  // a contiguous range of node blocks so blocks sharing halo nodes share an
+  // L2. Bijective for any block count.
+  const int nb = a.nblocks;
+  int b;
+  {
+    const int g = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = g & 7, k = g >> 3;
+    b = xcd * q + (xcd < r ? xcd : r) + k;
+  }
+  const int n0 = a.node_start[b], n1 = a.node_start[b + 1];
+  const int nown = n1 - n0;
+  const int h0 = a.halo_start[b], h1 = a.halo_start[b + 1];
+  const int nloc = nown + (h1 - h0);
+
+  T* ys = reinterpret_cast<T*>(smem_raw);
+  T* acc = ys + ((nloc + 1) & ~1);
+
+  // ---- stage the multiplied vector (owned + halo) into LDS ---------------
+  for (int l = threadIdx.x; l < nloc; l += blockDim.x) {
+    const bool owned = l < nown;
+    const int g = owned ? n0 + l : a.halo[h0 + l - nown];
+    T v;
+    if constexpr (STAGE == STAGE_PLAIN) {
+      v = a.y[g];
+    } else if constexpr (STAGE == STAGE_CGDIR) {
+      const T beta = (T)a.st->beta;
+      v = a.r[g] + beta * a.dold[g];
+      if (owned) a.dnew[g] = v;
+    } else {
+      const double xv = a.xd[g] - (double)a.delta[g];
+      v = (T)xv;
+      if (owned) a.xnew[g] = xv;
+    }
+    ys[l] = v;
+  }
+  __syncthreads();
+  // ---- diagonal term ------------------------------------------------------
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+    T v;
+    if constexpr (DIAG == DIAG_Q) v = (T)a.da[n0 + l] * ys[l];
+    else if constexpr (DIAG == DIAG_ONE) v = ys[l];
+    else v = (T)0;
+    acc[l] = v;
+  }
+  __syncthreads();
+  // ---- elements, colour by colour ----------------------------------------
+  const int32_t* co = a.color_off + (size_t)b * (kMaxColors + 1);
+  const int send = co[kMaxColors];
+  int c = 0;
+  while (true) {
+    const int s0 = co[c];
+    if (s0 >= send) break;
+    int s1 = co[c + 1];
+    for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+      int l[NDP];
+      T yv[NDP];
+#pragma unroll
+      for (int k = 0; k < NDP; ++k) {
+        l[k] = a.lid[(size_t)k * a.nslot_pad + s];
+        yv[k] = ys[l[k]];
+      }
+      constexpr int npair = NDP * (NDP - 1) / 2;
+      constexpr int ncoef = SYMQ ? 2 * npair : npair;
+      T cf[ncoef];
+#pragma unroll
+      for (int q = 0; q < ncoef; ++q) cf[q] = (T)a.xa[(size_t)q * a.nslot_pad + s];
+#pragma unroll
+      for (int k = 0; k < NDP; ++k) {
+        T w = (T)0;
+        bool first = true;
+#pragma unroll
+        for (int j = 0; j < NDP; ++j) {
+          if (j == k) continue;
+          const int q = TRANS ? coef_col<NDP, SYMQ>(j, k) : coef_col<NDP, SYMQ>(k, j);
+          const T t = cf[q] * yv[j];
+          w = first ? t : w + t;
+          first = false;
+        }
+        if (l[k] < nown) acc[l[k]] += w;
+      }
+    }
+    __syncthreads();
+    ++c;
+    if (c >= kMaxColors) break;
+  }
+  // ---- epilogue at owned nodes -------------------------------------------
+  double part = 0.0;
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+    const int g = n0 + l;
+    const T v = acc[l];
+    if constexpr (EPI == EPI_OP) {
+      T o;
+      switch (a.op) {
+        case BHIP_OP_AY: case BHIP_OP_TAY: o = v; break;
+        case BHIP_OP_CAY: o = (T)a.c * v; break;
+        case BHIP_OP_NAY: case BHIP_OP_NTAY: o = -v; break;
+        case BHIP_OP_XPAY: case BHIP_OP_XPTAY: o = a.xin[g] + v; break;
+        case BHIP_OP_XMAY: case BHIP_OP_XMTAY: o = a.xin[g] - v; break;
+        default: o = a.xin[g] + (T)a.c * v; break;  // X+CAY, X+CTAY
+      }
+      a.out[g] = o;
+    } else if constexpr (EPI == EPI_RESID) {
+      const double rv = (double)v - a.b[g];
+      a.out[g] = (T)rv;
+      part += rv * rv;
+    } else if constexpr (EPI == EPI_CGAD) {
+      a.out[g] = v;
+      part += (double)ys[l] * (double)v;
+    } else {  // EPI_RESID_MIXED: FP64 residual, inner RHS in FP32
+      const double rv = (double)v - a.b[g];
+      a.r32[g] = (float)(-rv);
+      a.zero32[g] = 0.0f;
+      part += rv * rv;
+    }
+  }
+  if constexpr (FIN != FIN_NONE) {
+    double s = block_sum(part, sh_red);
+    if (publish_partial(a.partial, b, s, &a.st->ticket[a.ticket_id], nb, &sh_flag)) {
+      double tot = sum_partials(a.partial, nb, sh_red);
+      if (threadIdx.x == 0) {
+        SolveState* st = a.st;
+        if constexpr (FIN == FIN_RESID_INIT) {
+          st->rr = tot;
+          st->rr0 = tot;
+          st->beta = 0.0;
+          st->m = 0;
+          st->state = (tot < st->eps2xl) ? ST_CONVERGED : ST_RUNNING;
+        } else if constexpr (FIN == FIN_CGAD) {
+          st->dad = tot;
+          st->rho = st->rr / tot;
+          if (st->m >= st->nitmax) st->state = ST_FINAL;
+        } else if constexpr (FIN == FIN_MIXED_OUTER) {
+          st->rr = tot;
+          st->rr0 = tot;
+          st->state = (tot <= st->eps2xl) ? ST_CONVERGED : ST_RUNNING;
+        }
+        st->ticket[a.ticket_id] = 0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CG vector update: X -= RO D ; R -= RO AD ; RMRM = R.R (gracjg.F:284-296,
+// 387-388), with the state transitions of the loop test (gracjg.F:306-324)
+// ---------------------------------------------------------------------------
+template <typename T, typename TX>
+__global__ void __launch_bounds__(kThreads)
+k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
+            const T* __restrict__ ad, int n, double* partial, SolveState* st, int ticket_id,
+            int use_rr0) {
+  __shared__ double sh_red[kWaves];
+  __shared__ int sh_flag;
+  const int state = st->state;
+  if (state != ST_RUNNING && state != ST_FINAL) return;
+  const double rho = st->rho;
+  const T rhot = (T)rho;
+  double part = 0.0;
+  const int stride = gridDim.x * blockDim.x;
+  if (state == ST_RUNNING) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      x[i] = x[i] - (TX)rhot * (TX)d[i];
+      const T rv = r[i] - rhot * ad[i];
+      r[i] = rv;
+      part += (double)rv * (double)rv;
+    }
+  } else {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+      x[i] = x[i] - (TX)rhot * (TX)d[i];
+  }
+  double s = block_sum(part, sh_red);
+  if (publish_partial(partial, blockIdx.x, s, &st->ticket[ticket_id], gridDim.x, &sh_flag)) {
+    double tot = sum_partials(partial, gridDim.x, sh_red);
+    if (threadIdx.x == 0) {
+      if (state == ST_FINAL) {
+        st->state = ST_MAXITER;
+      } else {
+        st->m = st->m + 1;
+        st->rr_prev = st->rr;
+        st->rr = tot;
+        const double thr = use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
+        if (tot <= thr) {
+          st->state = ST_CONVERGED;
+        } else {
+          st->beta = tot / st->rr_prev;
+        }
+      }
+      st->ticket[ticket_id] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_dot_into(const T* x, const T* y, int n, double* partial, SolveState* st,
+                           int ticket_id, double* dst) {
+  __shared__ double sh_red[kWaves];
+  __shared__ int sh_flag;
+  double part = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    part += (double)x[i] * (double)y[i];
+  double s = block_sum(part, sh_red);
+  if (publish_partial(partial, blockIdx.x, s, &st->ticket[ticket_id], gridDim.x, &sh_flag)) {
+    double tot = sum_partials(partial, gridDim.x, sh_red);
+    if (threadIdx.x == 0) {
+      *dst = tot;
+      st->ticket[ticket_id] = 0;
+    }
+  }
+}
+
+// out[i] = in[idx[i]]
+template <typename T>
+__global__ void k_gather(T* __restrict__ out, const T* __restrict__ in,
+                         const int32_t* __restrict__ idx, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = in[idx[i]];
+}
+// out[idx[i]] = in[i]
+template <typename T>
+__global__ void k_scatter(T* __restrict__ out, const T* __restrict__ in,
+                          const int32_t* __restrict__ idx, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[idx[i]] = in[i];
+}
+// out[idx[i]] = in[i] * s[i]  (undo of the variable change, solve.f:637-641)
+__global__ void k_scatter_scaled(double* __restrict__ out, const double* __restrict__ in,
+                                 const double* __restrict__ s, const int32_t* __restrict__ idx,
+                                 int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[idx[i]] = in[i] * s[i];
+}
+template <typename T>
+__global__ void k_fill(T* __restrict__ x, T v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void k_to_f32(float* __restrict__ out, const double* __restrict__ in, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
+}
+
+// PRECD1 node part (precd1.f:90-149): D = sqrt(DA) (|DA| for PRECON 5);
+// X = X*D; D = 1/D guarded (|D| > 1e-10, else 1: ovd.f:92-102); B = B*D;
+// DA = DA*D*D ('Q') or D*D ('I') (om1111.f:559-575).
+__global__ void k_precd1_nodes(double* __restrict__ x, double* __restrict__ b,
+                               double* __restrict__ da, double* __restrict__ dout, int n,
+                               int precon, int typdia_q, int typdia_i) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double d = da[i];
+    if (precon == 5) d = fabs(d);
+    d = sqrt(d);
+    x[i] = x[i] * d;
+    d = (fabs(d) > 1e-10) ? 1.0 / d : 1.0;
+    dout[i] = d;
+    if (typdia_q) {
+      double v = da[i] * d;
+      da[i] = v * d;
+    } else if (typdia_i) {
+      da[i] = d * d;
+    }
+    b[i] = b[i] * d;
+  }
+}
+
+// OM 'M=DMD' on element coefficients: XM(e,c) = XM(e,c)*D(IKLE(e,j))*D(IKLE(e,i))
+// for the pair (i<j) of column c (om1111.f:583-600, om4141.f), per slot.
+template <int NDP>
+__global__ void k_dmd_slots(double* __restrict__ xa, int ncoef, int nslot_pad,
+                            const uint16_t* __restrict__ lid, const double* __restrict__ d,
+                            const int32_t* __restrict__ node_start,
+                            const int32_t* __restrict__ halo_start,
+                            const int32_t* __restrict__ halo,
+                            const int32_t* __restrict__ color_off, int nblocks) {
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int n0 = node_start[b], nown = node_start[b + 1] - n0;
+  const int h0 = halo_start[b];
+  const int s0 = color_off[(size_t)b * (kMaxColors + 1)];
+  const int s1 = color_off[(size_t)b * (kMaxColors + 1) + kMaxColors];
+  constexpr int npair = NDP * (NDP - 1) / 2;
+  for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+    double dv[NDP];
+#pragma unroll
+    for (int k = 0; k < NDP; ++k) {
+      int l = lid[(size_t)k * nslot_pad + s];
+      int g = l < nown ? n0 + l : halo[h0 + l - nown];
+      dv[k] = d[g];
+    }
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < NDP; ++i)
+#pragma unroll
+      for (int j = i + 1; j < NDP; ++j) {
+        double* p = xa + (size_t)q * nslot_pad + s;
+        *p = (*p) * dv[j] * dv[i];
+        if (ncoef > npair) {
+          double* p2 = xa + (size_t)(q + npair) * nslot_pad + s;
+          *p2 = (*p2) * dv[j] * dv[i];
+        }
+        ++q;
+      }
+  }
+}
+
+// xa_slots[c][s] = xa_caller[c][slot_elem[s]]
+__global__ void k_matrix_to_slots(double* __restrict__ dst, const double* __restrict__ src,
+                                  const int32_t* __restrict__ slot_elem, int nslot,
+                                  int nslot_pad, int nelem, int ncoef) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)nslot * ncoef;
+       t += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(t / nslot), s = (int)(t % nslot);
+    dst[(size_t)c * nslot_pad + s] = src[(size_t)c * nelem + slot_elem[s]];
+  }
+}
+// caller layout back from slots: first instance of each element wins (all
+// instances of an element hold identical values)
+__global__ void k_matrix_from_slots(double* __restrict__ dst, const double* __restrict__ src,
+                                    const int32_t* __restrict__ first_slot, int nelem,
+                                    int nslot_pad, int ncoef) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)nelem * ncoef;
+       t += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(t / nelem), e = (int)(t % nelem);
+    dst[(size_t)c * nelem + e] = src[(size_t)c * nslot_pad + first_slot[e]];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side: context
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    if (p && n >= count) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace bhip
+
+struct bhip_ctx {
+  bhip::Layout L;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int grid_nodes = 0;  // grid of the node-wise kernels
+  // layout on device
+  bhip::DevBuf<int32_t> node_start, halo_start, halo, color_off, slot_elem, perm, first_slot;
+  bhip::DevBuf<uint16_t> lid;
+  // matrix
+  char typdia = '0', typext = '0';
+  int ncoef = 0;
+  bool have_matrix = false;
+  bhip::DevBuf<double> da, xa;       // internal node order / slot order
+  bhip::DevBuf<float> da32, xa32;
+  // work vectors (internal order)
+  bhip::DevBuf<double> vx, vb, vr, vd0, vd1, vad, vdz, vdinv, vio0, vio1;
+  bhip::DevBuf<float> fr, fd0, fd1, fad, fx0, fx1, fdz;
+  bhip::DevBuf<double> partial;
+  bhip::DevBuf<bhip::SolveState> st;
+  bhip::SolveState* st_host = nullptr;  // pinned
+  size_t lds_f64 = 0, lds_f32 = 0;
+};
+
+namespace bhip {
+
+static size_t lds_bytes(const Layout& L, size_t elt) {
+  size_t nl = ((size_t)L.max_local + 1) & ~(size_t)1;
+  return (nl + (size_t)L.max_owned) * elt;
+}
+
+static int grid_for(int n) {
+  int g = (n + kThreads * 4 - 1) / (kThreads * 4);
+  return std::max(1, std::min(g, 2048));
+}
+
+template <typename T, typename TA>
+static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
+  EbeArgs<T, TA> a;
+  std::memset(&a, 0, sizeof(a));
+  a.node_start = c->node_start.p;
+  a.halo_start = c->halo_start.p;
+  a.halo = c->halo.p;
+  a.color_off = c->color_off.p;
+  a.lid = c->lid.p;
+  a.nslot_pad = c->L.nslot_pad;
+  a.nblocks = c->L.nblocks;
+  a.xa = xa;
+  a.da = da;
+  a.partial = c->partial.p;
+  a.st = c->st.p;
+  return a;
+}
+
+// dispatch over (ndp, symmetry, transpose, diagonal)
+template <typename T, typename TA, int STAGE, int EPI, int FIN>
+static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typdia,
+                             char typext) {
+  const size_t lds = lds_bytes(c->L, sizeof(T));
+  dim3 grid(c->L.nblocks), block(kThreads);
+  const int diag = typdia == 'Q' ? DIAG_Q : (typdia == 'I' ? DIAG_ONE : DIAG_ZERO);
+  // TYPEXT '0' is stored as zero symmetric coefficients
+  const bool symq = typext == 'Q';
+#define BHIP_LAUNCH(NDP, SQ, TR, DG)                                                    \
+  hipLaunchKernelGGL((k_ebe<T, TA, NDP, SQ, TR, DG, STAGE, EPI, FIN>), grid, block, lds, \
+                     c->stream, a)
+#define BHIP_DIAG(NDP, SQ, TR)            \
+  do {                                    \
+    if (diag == DIAG_Q)                   \
+      BHIP_LAUNCH(NDP, SQ, TR, DIAG_Q);   \
+    else if (diag == DIAG_ONE)            \
+      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ONE); \
+    else                                  \
+      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ZERO); \
+  } while (0)
+  if (c->L.ndp == 3) {
+    if (!symq) BHIP_DIAG(3, false, false);
+    else if (trans) BHIP_DIAG(3, true, true);
+    else BHIP_DIAG(3, true, false);
+  } else {
+    if (!symq) BHIP_DIAG(6, false, false);
+    else if (trans) BHIP_DIAG(6, true, true);
+    else BHIP_DIAG(6, true, false);
+  }
+#undef BHIP_DIAG
+#undef BHIP_LAUNCH
+  return hipGetLastError();
+}
+
+}  // namespace bhip
+
+using namespace bhip;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const double* x,
+                    const double* y, int nplan, int block_nodes, bhip_ctx** out) {
+  if (!out || !ikle) return BHIP_EBADARG;
+  *out = nullptr;
+  std::unique_ptr<bhip_ctx> c(new bhip_ctx());
+  std::string err = build_layout(npoin, nelem, ndp, ikle, x, y, nplan, block_nodes, c->L);
+  if (!err.empty()) {
+    fprintf(stderr, "libbiefhip: bhip_ctx_create: %s\n", err.c_str());
+    return BHIP_EBADARG;
+  }
+  const Layout& L = c->L;
+  c->lds_f64 = lds_bytes(L, sizeof(double));
+  c->lds_f32 = lds_bytes(L, sizeof(float));
+  if (c->lds_f64 > 64 * 1024) {
+    fprintf(stderr, "libbiefhip: block needs %zu B of LDS; use smaller blocks\n", c->lds_f64);
+    return BHIP_EBADARG;
+  }
+  HIP_OK(hipGetDevice(&c->device));
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // the element kernels need more than the default dynamic LDS limit only
+  // for large blocks; raise it for every instantiation lazily is not
+  // possible in HIP, so blocks are sized to stay within 64 KiB above.
+  const int n = L.npoin;
+  c->grid_nodes = grid_for(n);
+  HIP_OK(c->node_start.alloc(L.node_start.size()));
+  HIP_OK(c->halo_start.alloc(L.halo_start.size()));
+  HIP_OK(c->halo.alloc(std::max<size_t>(1, L.halo.size())));
+  HIP_OK(c->color_off.alloc(L.color_off.size()));
+  HIP_OK(c->lid.alloc(L.lid.size()));
+  HIP_OK(c->slot_elem.alloc(L.slot_elem.size()));
+  HIP_OK(c->perm.alloc(L.perm.size()));
+  HIP_OK(c->first_slot.alloc(L.nelem));
+  std::vector<int32_t> first(L.nelem, -1);
+  for (int s = 0; s < L.nslot; ++s)
+    if (first[L.slot_elem[s]] < 0) first[L.slot_elem[s]] = s;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  HIP_OK(up(c->node_start.p, L.node_start.data(), L.node_start.size() * 4));
+  HIP_OK(up(c->halo_start.p, L.halo_start.data(), L.halo_start.size() * 4));
+  if (!L.halo.empty()) HIP_OK(up(c->halo.p, L.halo.data(), L.halo.size() * 4));
+  HIP_OK(up(c->color_off.p, L.color_off.data(), L.color_off.size() * 4));
+  HIP_OK(up(c->lid.p, L.lid.data(), L.lid.size() * 2));
+  HIP_OK(up(c->slot_elem.p, L.slot_elem.data(), L.slot_elem.size() * 4));
+  HIP_OK(up(c->perm.p, L.perm.data(), L.perm.size() * 4));
+  HIP_OK(up(c->first_slot.p, first.data(), first.size() * 4));
+  // work memory
+  for (auto* v : {&c->vx, &c->vb, &c->vr, &c->vd0, &c->vd1, &c->vad, &c->vdz, &c->vdinv,
+                  &c->vio0, &c->vio1})
+    HIP_OK(v->alloc(n));
+  const int maxgrid = std::max(L.nblocks, 2048);
+  HIP_OK(c->partial.alloc(maxgrid));
+  HIP_OK(c->st.alloc(1));
+  HIP_OK(hipHostMalloc((void**)&c->st_host, sizeof(SolveState), hipHostMallocDefault));
+  std::memset(c->st_host, 0, sizeof(SolveState));
+  HIP_OK(up(c->st.p, c->st_host, sizeof(SolveState)));
+  hipLaunchKernelGGL(k_fill<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                     c->vdz.p, 0.0, (long)n);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  *out = c.release();
+  return BHIP_OK;
+}
+
+int bhip_ctx_destroy(bhip_ctx* c) {
+  if (!c) return BHIP_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto* v : {&c->node_start, &c->halo_start, &c->halo, &c->color_off, &c->slot_elem,
+                  &c->perm, &c->first_slot})
+    v->release();
+  c->lid.release();
+  for (auto* v : {&c->da, &c->xa, &c->vx, &c->vb, &c->vr, &c->vd0, &c->vd1, &c->vad, &c->vdz,
+                  &c->vdinv, &c->vio0, &c->vio1, &c->partial})
+    v->release();
+  for (auto* v : {&c->da32, &c->xa32, &c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1,
+                  &c->fdz})
+    v->release();
+  c->st.release();
+  if (c->st_host) (void)hipHostFree(c->st_host);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BHIP_OK;
+}
+
+void* bhip_ctx_stream(bhip_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int bhip_layout_check(int npoin, int nelem, int ndp, const int32_t* ikle, const double* x,
+                      const double* y, int nplan, int block_nodes, int64_t* stats) {
+  Layout L;
+  std::string err = build_layout(npoin, nelem, ndp, ikle, x, y, nplan, block_nodes, L);
+  if (!err.empty()) {
+    fprintf(stderr, "libbiefhip: bhip_layout_check: %s\n", err.c_str());
+    return BHIP_EBADARG;
+  }
+  if (stats) {
+    stats[0] = L.nblocks;
+    stats[1] = L.max_colors;
+    stats[2] = L.nslot;
+    stats[3] = L.halo_total;
+    stats[4] = L.max_local;
+    stats[5] = L.max_owned;
+    stats[6] = (int64_t)lds_bytes(L, sizeof(double));
+  }
+  return BHIP_OK;
+}
+
+int bhip_ctx_stats(const bhip_ctx* c, int* nblocks, int* max_colors, int64_t* elem_instances,
+                   int64_t* halo_nodes) {
+  if (!c) return BHIP_EBADARG;
+  if (nblocks) *nblocks = c->L.nblocks;
+  if (max_colors) *max_colors = c->L.max_colors;
+  if (elem_instances) *elem_instances = c->L.nslot;
+  if (halo_nodes) *halo_nodes = c->L.halo_total;
+  return BHIP_OK;
+}
+
+static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double* da,
+                             const double* xa, hipMemcpyKind kind) {
+  if (!c) return BHIP_EBADARG;
+  if (typdia != 'Q' && typdia != 'I' && typdia != '0') return BHIP_EBADARG;
+  if (typext != 'Q' && typext != 'S' && typext != '0') return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int npair = L.ndp * (L.ndp - 1) / 2;
+  const int ncoef = typext == 'Q' ? 2 * npair : (typext == 'S' ? npair : 0);
+  HIP_OK(c->da.alloc(L.npoin));
+  // coefficient storage always holds the non-symmetric width, so a 'S'
+  // matrix can be turned 'Q' later without reallocation
+  HIP_OK(c->xa.alloc((size_t)2 * npair * L.nslot_pad));
+  if (da) {
+    // DA in internal order
+    double* tmp = c->vio0.p;
+    HIP_OK(hipMemcpyAsync(tmp, da, sizeof(double) * L.npoin, kind, c->stream));
+    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                       c->da.p, tmp, c->perm.p, L.npoin);
+  } else {
+    hipLaunchKernelGGL(k_fill<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                       c->da.p, 0.0, (long)L.npoin);
+  }
+  HIP_OK(hipGetLastError());
+  if (ncoef > 0) {
+    double* src = nullptr;
+    bhip::DevBuf<double> staging;
+    if (kind == hipMemcpyHostToDevice) {
+      HIP_OK(staging.alloc((size_t)ncoef * L.nelem));
+      HIP_OK(hipMemcpyAsync(staging.p, xa, sizeof(double) * ncoef * L.nelem, kind, c->stream));
+      src = staging.p;
+    } else {
+      src = const_cast<double*>(xa);
+    }
+    hipLaunchKernelGGL(k_matrix_to_slots, dim3(grid_for(L.nslot * ncoef)), dim3(kThreads), 0,
+                       c->stream, c->xa.p, src, c->slot_elem.p, L.nslot, L.nslot_pad, L.nelem,
+                       ncoef);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    staging.release();
+  } else {
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(L.nslot_pad)), dim3(kThreads), 0,
+                       c->stream, c->xa.p, 0.0, (long)npair * L.nslot_pad);
+    HIP_OK(hipGetLastError());
+  }
+  c->typdia = typdia;
+  c->typext = typext;
+  c->ncoef = ncoef;
+  c->have_matrix = true;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+int bhip_matrix_set(bhip_ctx* c, char typdia, char typext, const double* da, const double* xa) {
+  return matrix_set_common(c, typdia, typext, da, xa, hipMemcpyHostToDevice);
+}
+int bhip_matrix_set_dev(bhip_ctx* c, char typdia, char typext, const double* da,
+                        const double* xa) {
+  return matrix_set_common(c, typdia, typext, da, xa, hipMemcpyDeviceToDevice);
+}
+
+int bhip_matrix_get(bhip_ctx* c, char* typdia, double* da, double* xa) {
+  if (!c || !c->have_matrix) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  if (typdia) *typdia = c->typdia;
+  if (da) {
+    hipLaunchKernelGGL(k_scatter<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                       c->vio0.p, c->da.p, c->perm.p, L.npoin);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(da, c->vio0.p, sizeof(double) * L.npoin, hipMemcpyDeviceToHost,
+                          c->stream));
+  }
+  if (xa && c->ncoef > 0) {
+    bhip::DevBuf<double> tmp;
+    HIP_OK(tmp.alloc((size_t)c->ncoef * L.nelem));
+    hipLaunchKernelGGL(k_matrix_from_slots, dim3(grid_for(L.nelem * c->ncoef)), dim3(kThreads),
+                       0, c->stream, tmp.p, c->xa.p, c->first_slot.p, L.nelem, L.nslot_pad,
+                       c->ncoef);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(xa, tmp.p, sizeof(double) * c->ncoef * L.nelem,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    tmp.release();
+  }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+// y (caller order, device) -> vio0 (internal); result vio1 (internal) ->
+// x (caller order, device)
+static int matvec_internal(bhip_ctx* c, int op, double cst, const double* y_in,
+                           const double* x_in, double* x_out) {
+  EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+  a.y = y_in;
+  a.xin = x_in;
+  a.out = x_out;
+  a.op = op;
+  a.c = cst;
+  const bool trans = op >= BHIP_OP_TAY;
+  HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_OP, FIN_NONE>(c, a, trans, c->typdia,
+                                                                     c->typext)));
+  return BHIP_OK;
+}
+
+int bhip_matvec_dev(bhip_ctx* c, int op, double cst, const double* y, double* x) {
+  if (!c || !c->have_matrix || op < 0 || op > 10 || !y || !x) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int g = c->grid_nodes;
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio0.p, y,
+                     c->perm.p, L.npoin);
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p, x,
+                     c->perm.p, L.npoin);
+  HIP_OK(hipGetLastError());
+  int rc = matvec_internal(c, op, cst, c->vio0.p, c->vio1.p, c->vr.p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, x, c->vr.p,
+                     c->perm.p, L.npoin);
+  HIP_OK(hipGetLastError());
+  return BHIP_OK;
+}
+
+int bhip_matvec(bhip_ctx* c, int op, double cst, const double* y, double* x) {
+  if (!c || !c->have_matrix || !y || !x) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  DevBuf<double> dy, dx;
+  HIP_OK(dy.alloc(L.npoin));
+  HIP_OK(dx.alloc(L.npoin));
+  HIP_OK(hipMemcpyAsync(dy.p, y, sizeof(double) * L.npoin, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(dx.p, x, sizeof(double) * L.npoin, hipMemcpyHostToDevice, c->stream));
+  int rc = bhip_matvec_dev(c, op, cst, dy.p, dx.p);
+  if (rc == BHIP_OK) {
+    HIP_OK(hipMemcpyAsync(x, dx.p, sizeof(double) * L.npoin, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dy.release();
+  dx.release();
+  return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// SOLVE
+// ===========================================================================
+namespace bhip {
+
+static int read_state(bhip_ctx* c) {
+  HIP_OK(hipMemcpyAsync(c->st_host, c->st.p, sizeof(SolveState), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+static int write_state(bhip_ctx* c) {
+  HIP_OK(hipMemcpyAsync(c->st.p, c->st_host, sizeof(SolveState), hipMemcpyHostToDevice,
+                        c->stream));
+  return BHIP_OK;
+}
+
+// GRACJG iterations on the device-resident preconditioned system, FP64.
+// x, b internal order; returns after the state leaves RUNNING/FINAL.
+static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
+                   bhip_solve_info* info) {
+  const Layout& L = c->L;
+  const int n = L.npoin;
+  const int g = c->grid_nodes;
+  SolveState* sh = c->st_host;
+  // XL = B.B (gracjg.F:189)
+  std::memset(sh, 0, sizeof(SolveState));
+  if (write_state(c)) return BHIP_EDEVICE;
+  hipLaunchKernelGGL(k_dot_into<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b, n,
+                     c->partial.p, c->st.p, 0, &c->st.p->xl);
+  HIP_OK(hipGetLastError());
+  if (read_state(c)) return BHIP_EDEVICE;
+  double xl = sh->xl;
+  info->relat = 1;
+  if (std::sqrt(xl) < 1e-15) {
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
+    HIP_OK(hipGetLastError());
+    info->niter = 0;
+    info->relres = 0.0;
+    info->converged = 1;
+    return BHIP_ZERO_RHS;
+  }
+  if (xl < 1.0) {
+    xl = 1.0;
+    info->relat = 0;
+  }
+  sh->xl = xl;
+  sh->eps2xl = eps * eps * xl;
+  sh->nitmax = nitmax;
+  sh->state = ST_RUNNING;
+  if (write_state(c)) return BHIP_EDEVICE;
+  // R = A X - B ; RMRM (gracjg.F:213-219)
+  {
+    EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+    a.y = x;
+    a.b = b;
+    a.out = c->vr.p;
+    a.ticket_id = 1;
+    HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_RESID_INIT>(
+        c, a, false, c->typdia, c->typext)));
+  }
+  if (read_state(c)) return BHIP_EDEVICE;
+  if (sh->state == ST_CONVERGED) {
+    info->niter = 0;
+    info->relres = std::sqrt(sh->rr / sh->xl);
+    info->converged = 1;
+    return BHIP_OK;
+  }
+  // D = R ; AD = A D ; RO = RMRM / D.AD ; (X -= RO D deferred) (gracjg.F:248-268)
+  auto launch_dir = [&](const double* dold, double* dnew) {
+    EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+    a.r = c->vr.p;
+    a.dold = dold;
+    a.dnew = dnew;
+    a.out = c->vad.p;
+    a.ticket_id = 2;
+    a.check_state = 1;
+    return launch_ebe<double, double, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
+                                                                        c->typext);
+  };
+  HIP_OK(launch_dir(c->vdz.p, c->vd0.p));
+  auto launch_upd = [&](const double* d) {
+    hipLaunchKernelGGL((k_cg_update<double, double>), dim3(g), dim3(kThreads), 0, c->stream, x,
+                       c->vr.p, d, c->vad.p, n, c->partial.p, c->st.p, 3, 0);
+    return hipGetLastError();
+  };
+  // iterations in batches; kernels exit early once the state leaves RUNNING
+  int batch = 8;
+  for (;;) {
+    for (int k = 0; k < batch; ++k) {
+      HIP_OK(launch_upd(c->vd0.p));
+      HIP_OK(launch_dir(c->vd0.p, c->vd1.p));
+      HIP_OK(launch_upd(c->vd1.p));
+      HIP_OK(launch_dir(c->vd1.p, c->vd0.p));
+    }
+    if (read_state(c)) return BHIP_EDEVICE;
+    if (sh->state == ST_CONVERGED || sh->state == ST_MAXITER) break;
+    batch = std::min(batch * 2, 64);
+  }
+  info->niter = sh->m;
+  info->relres = std::sqrt(sh->rr / sh->xl);
+  info->converged = sh->state == ST_CONVERGED;
+  return sh->state == ST_CONVERGED ? BHIP_OK : BHIP_MAXITER;
+}
+
+static int ensure_f32(bhip_ctx* c) {
+  const Layout& L = c->L;
+  const int n = L.npoin;
+  for (auto* v : {&c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1, &c->fdz})
+    HIP_OK(v->alloc(n));
+  HIP_OK(c->da32.alloc(n));
+  const int npair = L.ndp * (L.ndp - 1) / 2;
+  HIP_OK(c->xa32.alloc((size_t)2 * npair * L.nslot_pad));
+  hipLaunchKernelGGL(k_fill<float>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                     c->fdz.p, 0.0f, (long)n);
+  HIP_OK(hipGetLastError());
+  return BHIP_OK;
+}
+
+// Mixed precision CG: FP64 true residual r = A'x - b' and FP64 update
+// x <- x - delta, with delta from an FP32 EBE CG on the FP32 copy of A'.
+// Same stop rule as GRACJG on the preconditioned system, evaluated with the
+// FP64 residual.
+static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
+                    double inner_eps, int inner_nitmax, bhip_solve_info* info) {
+  const Layout& L = c->L;
+  const int n = L.npoin;
+  const int g = c->grid_nodes;
+  SolveState* sh = c->st_host;
+  if (ensure_f32(c)) return BHIP_EDEVICE;
+  const int npair = L.ndp * (L.ndp - 1) / 2;
+  const long ncf = (long)(c->typext == 'Q' ? 2 * npair : npair) * L.nslot_pad;
+  if (c->typext != '0') {
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for((int)std::min<long>(ncf, 1 << 30))),
+                       dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p, ncf);
+  }
+  hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
+                     (long)n);
+  HIP_OK(hipGetLastError());
+  std::memset(sh, 0, sizeof(SolveState));
+  if (write_state(c)) return BHIP_EDEVICE;
+  hipLaunchKernelGGL(k_dot_into<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b, n,
+                     c->partial.p, c->st.p, 0, &c->st.p->xl);
+  HIP_OK(hipGetLastError());
+  if (read_state(c)) return BHIP_EDEVICE;
+  double xl = sh->xl;
+  info->relat = 1;
+  if (std::sqrt(xl) < 1e-15) {
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
+    HIP_OK(hipGetLastError());
+    info->converged = 1;
+    return BHIP_ZERO_RHS;
+  }
+  if (xl < 1.0) {
+    xl = 1.0;
+    info->relat = 0;
+  }
+  const double eps2xl = eps * eps * xl;
+  // delta buffers: fx0 holds zeros for the first outer residual
+  hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, c->fx0.p, 0.0f,
+                     (long)n);
+  HIP_OK(hipGetLastError());
+  double* xa_ = x;
+  double* xb_ = c->vio1.p;
+  float* dl_prev = c->fx0.p;
+  float* dl_next = c->fx1.p;
+  int total_inner = 0, outer = 0;
+  int status = BHIP_MAXITER;
+  for (;;) {
+    // outer: x_new = x - delta_prev ; r = A x_new - b ; inner RHS = -r ; delta_next = 0
+    sh->xl = xl;
+    sh->eps2xl = eps2xl;
+    sh->state = ST_RUNNING;
+    sh->m = 0;
+    if (write_state(c)) return BHIP_EDEVICE;
+    {
+      EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+      a.xd = xa_;
+      a.delta = dl_prev;
+      a.xnew = xb_;
+      a.b = b;
+      a.r32 = c->fr.p;
+      a.zero32 = dl_next;
+      a.ticket_id = 1;
+      HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_MIXED_OUTER>(
+          c, a, false, c->typdia, c->typext)));
+    }
+    std::swap(xa_, xb_);
+    if (read_state(c)) return BHIP_EDEVICE;
+    const double rr = sh->rr;
+    if (sh->state == ST_CONVERGED) {
+      status = BHIP_OK;
+      info->relres = std::sqrt(rr / xl);
+      break;
+    }
+    if (total_inner >= nitmax) {
+      info->relres = std::sqrt(rr / xl);
+      break;
+    }
+    ++outer;
+    // inner FP32 CG: A32 delta = r, from delta = 0, R0 = -r, stop when
+    // ||R||^2 <= inner_eps^2 ||R0||^2
+    sh->eps2xl = inner_eps * inner_eps;
+    sh->rr0 = rr;
+    sh->rr = rr;
+    sh->beta = 0.0;
+    sh->m = 0;
+    sh->nitmax = std::min(inner_nitmax, nitmax - total_inner);
+    sh->state = ST_RUNNING;
+    if (write_state(c)) return BHIP_EDEVICE;
+    auto launch_dir = [&](const float* dold, float* dnew) {
+      EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
+      a.r = c->fr.p;
+      a.dold = dold;
+      a.dnew = dnew;
+      a.out = c->fad.p;
+      a.ticket_id = 2;
+      a.check_state = 1;
+      return launch_ebe<float, float, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
+                                                                        c->typext);
+    };
+    auto launch_upd = [&](const float* d) {
+      hipLaunchKernelGGL((k_cg_update<float, float>), dim3(g), dim3(kThreads), 0, c->stream,
+                         dl_next, c->fr.p, d, c->fad.p, n, c->partial.p, c->st.p, 3, 1);
+      return hipGetLastError();
+    };
+    HIP_OK(launch_dir(c->fdz.p, c->fd0.p));
+    int batch = 8;
+    for (;;) {
+      for (int k = 0; k < batch; ++k) {
+        HIP_OK(launch_upd(c->fd0.p));
+        HIP_OK(launch_dir(c->fd0.p, c->fd1.p));
+        HIP_OK(launch_upd(c->fd1.p));
+        HIP_OK(launch_dir(c->fd1.p, c->fd0.p));
+      }
+      if (read_state(c)) return BHIP_EDEVICE;
+      if (sh->state == ST_CONVERGED || sh->state == ST_MAXITER) break;
+      batch = std::min(batch * 2, 64);
+    }
+    total_inner += sh->m;
+    std::swap(dl_prev, dl_next);
+    if (outer > 100) break;
+  }
+  // the latest x is in xa_; make sure it lands in x
+  if (xa_ != x) HIP_OK(hipMemcpyAsync(x, xa_, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                      c->stream));
+  info->niter = total_inner;
+  info->inner_iters = total_inner;
+  info->outer_iters = outer;
+  info->converged = status == BHIP_OK;
+  return status;
+}
+
+}  // namespace bhip
+
+extern "C" {
+
+int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, double* b_dev,
+                   bhip_solve_info* info) {
+  if (!c || !cfg || !x_dev || !b_dev || !c->have_matrix) return BHIP_EBADARG;
+  bhip_solve_info dummy;
+  if (!info) info = &dummy;
+  std::memset(info, 0, sizeof(*info));
+  int precon = cfg->precon == 0 ? 1 : cfg->precon;
+  if (precon % 3 == 0 || precon % 7 == 0 || precon % 11 == 0 || precon % 13 == 0 ||
+      precon % 17 == 0)
+    return BHIP_EBADARG;
+  if (cfg->slv != 1) return BHIP_EBADARG;  // CGSTAB / GMRES: bhip_krylov.hip
+  const Layout& L = c->L;
+  const int n = L.npoin;
+  const int g = c->grid_nodes;
+  // caller order -> internal order
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, x_dev,
+                     c->perm.p, n);
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vb.p, b_dev,
+                     c->perm.p, n);
+  HIP_OK(hipGetLastError());
+  const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
+  if (diag) {
+    // PRECD1 (precd1.f:90-149)
+    hipLaunchKernelGGL(k_precd1_nodes, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, c->vb.p,
+                       c->da.p, c->vdinv.p, n, precon, c->typdia == 'Q', c->typdia == 'I');
+    HIP_OK(hipGetLastError());
+    if (c->typdia == 'I') c->typdia = 'Q';
+    if (c->ncoef > 0) {
+      if (L.ndp == 3)
+        hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), 0, c->stream,
+                           c->xa.p, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
+                           c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
+                           L.nblocks);
+      else
+        hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), 0, c->stream,
+                           c->xa.p, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
+                           c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
+                           L.nblocks);
+      HIP_OK(hipGetLastError());
+    }
+    if (precon % 2 == 0 || precon % 3 == 0) c->typdia = 'I';
+  }
+  int rc;
+  if (cfg->precision == BHIP_PREC_MIXED) {
+    double ie = cfg->inner_eps > 0 ? cfg->inner_eps : 1e-4;
+    int inn = cfg->inner_nitmax > 0 ? cfg->inner_nitmax : std::max(cfg->nitmax, 1);
+    rc = cg_mixed(c, c->vx.p, c->vb.p, cfg->eps, std::max(cfg->nitmax, 1), ie, inn, info);
+  } else {
+    rc = cg_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
+  }
+  if (rc != BHIP_OK && rc != BHIP_MAXITER && rc != BHIP_ZERO_RHS) return rc;
+  // X <- X*D (solve.f:637-641), back to caller order; B <- D*B as the reference leaves it
+  if (diag)
+    hipLaunchKernelGGL(k_scatter_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
+                       c->vdinv.p, c->perm.p, n);
+  else
+    hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
+                       c->perm.p, n);
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, b_dev, c->vb.p,
+                     c->perm.p, n);
+  HIP_OK(hipGetLastError());
+  return rc;
+}
+
+int bhip_solve(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x, double* b,
+               bhip_solve_info* info) {
+  if (!c || !x || !b) return BHIP_EBADARG;
+  const int n = c->L.npoin;
+  DevBuf<double> dx, db;
+  HIP_OK(dx.alloc(n));
+  HIP_OK(db.alloc(n));
+  HIP_OK(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  int rc = bhip_solve_dev(c, cfg, dx.p, db.p, info);
+  if (rc == BHIP_OK || rc == BHIP_MAXITER || rc == BHIP_ZERO_RHS) {
+    HIP_OK(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(b, db.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dx.release();
+  db.release();
+  return rc;
+}
+
+}  // extern "C"

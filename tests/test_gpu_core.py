@@ -1,0 +1,101 @@
+"""GPU parity of the EBE mat-vec and SOLVE against the CPU oracle.
+
+FP64 bar: relative L2 <= 1e-10 against the oracle's FP64 SOLVE (the
+reference arithmetic), matvec entries within 1e-13 relative of the row
+scale; mixed precision: the FP64 residual test of the solve itself.
+"""
+import numpy as np
+import pytest
+
+from opentelemac_amd import bief, mesh as M, system as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_matrix(m, typdia, typext, seed):
+    rng = np.random.default_rng(seed)
+    npair = m.ndp * (m.ndp - 1) // 2
+    nc = 2 * npair if typext == "Q" else npair
+    return rng.uniform(1.0, 3.0, m.npoin), rng.uniform(-1, 1, (nc, m.nelem))
+
+
+@pytest.mark.parametrize("block_nodes", [0, 16])
+@pytest.mark.parametrize("typext", ["S", "Q"])
+@pytest.mark.parametrize("typdia", ["Q", "I", "0"])
+def test_matvec_all_ops(gpu, block_nodes, typext, typdia):
+    m = M.random_small_mesh(11, 9, seed=5)
+    print(f"matvec mesh={m.npoin} bn={block_nodes} {typdia}{typext}", flush=True)
+    ctx = bief.Context(m, block_nodes=block_nodes)
+    da, xa = _rand_matrix(m, typdia, typext, 1)
+    ctx.set_matrix(typdia, typext, da, xa)
+    A = O.Matrix(da, xa, typdia, typext)
+    rng = np.random.default_rng(2)
+    for op in bief.OPS:
+        y = rng.normal(size=m.npoin)
+        x0 = rng.normal(size=m.npoin)
+        got = ctx.matvec(op, y, x0, c=0.37)
+        ref = O.matvec(op, A, m.ikle, y, x0, c=0.37)
+        scale = np.abs(ref).max() + 1.0
+        assert np.abs(got - ref).max() <= 1e-13 * scale, op
+
+
+def test_matvec_deterministic(gpu):
+    m = M.channel_mesh()
+    ctx = bief.Context(m, block_nodes=512)
+    sysd = S.wave_system(m, M.channel_depth(m))
+    ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+    y = np.random.default_rng(0).normal(size=m.npoin)
+    a = ctx.matvec("X=AY", y)
+    b = ctx.matvec("X=AY", y)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("block_nodes", [0, 64])
+@pytest.mark.parametrize("precon", [0, 2, 5])
+def test_solve_cg_small(gpu, block_nodes, precon):
+    m = M.random_small_mesh(13, 10, seed=7)
+    depth = 1.0 + np.random.default_rng(1).uniform(0, 4, m.npoin)
+    sysd = S.wave_system(m, depth, dt=2.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    x_ref, info_ref, A2, b2 = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=precon,
+                                      eps=1e-10, nitmax=2000)
+    ctx = bief.Context(m, block_nodes=block_nodes)
+    ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+    x = sysd["x0"].copy()
+    b = sysd["b"].copy()
+    x, info = ctx.solve(x, b, slv=1, precon=precon, eps=1e-10, nitmax=2000)
+    print("solve", precon, info, info_ref.niter, flush=True)
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-10
+    assert abs(info["niter"] - info_ref.niter) <= 1
+    # b and A are left preconditioned exactly as the reference leaves them
+    assert np.allclose(b, b2, rtol=1e-14, atol=0)
+    td, da2, xa2 = ctx.get_matrix()
+    assert td == A2.typdia
+    assert np.allclose(da2, A2.da, rtol=1e-14, atol=1e-300)
+    assert np.allclose(xa2, A2.xa, rtol=1e-14, atol=1e-300)
+
+
+def test_solve_channel_fp64_and_mixed(gpu):
+    m = M.channel_mesh()
+    sysd = S.wave_system(m, M.channel_depth(m), dt=1.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    x_ref, info_ref, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2,
+                                    eps=1e-8, nitmax=5000)
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ctx = bief.Context(m)
+        ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+        x, info = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-8,
+                            nitmax=5000, precision=prec)
+        rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+        print("channel", prec, info, "oracle iters", info_ref.niter, "rel", rel, flush=True)
+        assert info["converged"]
+        if prec == bief.PREC_FP64:
+            assert rel <= 1e-10
+            assert abs(info["niter"] - info_ref.niter) <= 1
+        else:
+            # both satisfy ||r'|| <= 1e-8 ||b'||: solutions agree to the
+            # conditioning-scaled tolerance
+            assert info["relres"] <= 1e-8
+            assert rel <= 1e-6
