@@ -149,6 +149,16 @@ int bhip_solve(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x,
 int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
                    double* b_dev, bhip_solve_info* info);
 
+/* Measurement: average duration (ms, HIP events on the context stream) of
+ * the two kernels of one CG iteration -- the fused direction update + EBE
+ * mat-vec + D.AD kernel and the X/R update + R.R kernel -- on the context's
+ * current (preconditioned) matrix, FP64 or the mixed path's FP32 inner
+ * kernels (the latter after a mixed solve). Scratch vectors are reused, so
+ * call it after the solves whose results are needed. No counterpart in the
+ * reference (it has no timers below the whole run, homere_telemac2d.F). */
+int bhip_time_cg_kernels(bhip_ctx* ctx, int precision, int reps,
+                         double* ms_dir, double* ms_upd);
+
 /* ---- operand preparation before SOLVE ----------------------------------- */
 
 /* DIRICH with domain-sized FBOR / LIMDIR (sources/utils/bief/dirich.f:83 ->

@@ -89,6 +89,8 @@ def lib():
         L.bhip_solve.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P, ctypes.POINTER(SolveInfo)]
         L.bhip_solve_dev.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
                                      ctypes.POINTER(SolveInfo)]
+        L.bhip_time_cg_kernels.argtypes = [_P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]
+        L.bhip_layout_check.argtypes = [_I, _I, _I, _P, _P, _P, _I, _I, _P]
         for name, args in (("bhip_dirich", [_P, _P, _P, _P, _P, _I]),
                            ("bhip_lump", [_P, _D, _P]),
                            ("bhip_matrix_mt01aa", [_P, _D, _P, _I]),
@@ -111,6 +113,21 @@ def _is_torch(a):
 
 def _host_f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def layout_check(mesh, block_nodes=0):
+    """Host-only build + validation of the block layout (no GPU needed).
+    Returns the statistics dict; raises BiefError if the layout is invalid."""
+    st = np.zeros(7, dtype=np.int64)
+    ik = np.ascontiguousarray(mesh.ikle, dtype=np.int32)
+    x = np.ascontiguousarray(mesh.x, dtype=np.float64)
+    y = np.ascontiguousarray(mesh.y, dtype=np.float64)
+    _check(lib().bhip_layout_check(mesh.npoin, mesh.nelem, mesh.ndp, ik.ctypes.data,
+                                   x.ctypes.data, y.ctypes.data, int(getattr(mesh, "nplan", 1)),
+                                   int(block_nodes), st.ctypes.data), "bhip_layout_check")
+    keys = ["nblocks", "max_colors", "elem_instances", "halo_nodes", "max_local", "max_owned",
+            "lds_bytes_f64"]
+    return dict(zip(keys, (int(v) for v in st)))
 
 
 @dataclass
@@ -223,6 +240,14 @@ class Context:
         _check(lib().bhip_matvec(self._h, code, float(c), y.ctypes.data, x.ctypes.data),
                "bhip_matvec")
         return x
+
+    def time_cg_kernels(self, precision=PREC_FP64, reps=50):
+        """Average ms per launch of the CG direction/mat-vec kernel and the
+        X/R update kernel (HIP events on the context stream)."""
+        a, b = _D(), _D()
+        _check(lib().bhip_time_cg_kernels(self._h, precision, reps, ctypes.byref(a),
+                                          ctypes.byref(b)), "bhip_time_cg_kernels")
+        return a.value, b.value
 
     # ---- SOLVE ----------------------------------------------------------
     def solve(self, x, b, slv=1, precon=2, krylov=5, nitmax=1000, eps=1e-8,

@@ -1211,6 +1211,74 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
   return rc;
 }
 
+int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
+                         double* ms_upd) {
+  if (!c || !c->have_matrix || reps < 1) return BHIP_EBADARG;
+  if (precision == BHIP_PREC_MIXED && !c->fr.p) return BHIP_EBADARG;
+  const int n = c->L.npoin;
+  const int g = c->grid_nodes;
+  SolveState* sh = c->st_host;
+  std::memset(sh, 0, sizeof(SolveState));
+  sh->state = ST_RUNNING;
+  sh->nitmax = 0x7fffffff;
+  sh->eps2xl = -1.0;  // never converges: every launch does its full work
+  sh->rr = 1.0;
+  if (write_state(c)) return BHIP_EDEVICE;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  float t_dir = 0.f, t_upd = 0.f;
+  if (precision == BHIP_PREC_MIXED) {
+    EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
+    a.r = c->fr.p;
+    a.dold = c->fd0.p;
+    a.dnew = c->fd1.p;
+    a.out = c->fad.p;
+    a.ticket_id = 2;
+    a.check_state = 1;
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i)
+      HIP_OK((launch_ebe<float, float, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
+                                                                         c->typext)));
+    HIP_OK(hipEventRecord(e1, c->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(&t_dir, e0, e1));
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i)
+      hipLaunchKernelGGL((k_cg_update<float, float>), dim3(g), dim3(kThreads), 0, c->stream,
+                         c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, n, c->partial.p, c->st.p, 3, 0);
+    HIP_OK(hipEventRecord(e1, c->stream));
+  } else {
+    EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+    a.r = c->vr.p;
+    a.dold = c->vd0.p;
+    a.dnew = c->vd1.p;
+    a.out = c->vad.p;
+    a.ticket_id = 2;
+    a.check_state = 1;
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i)
+      HIP_OK((launch_ebe<double, double, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
+                                                                           c->typext)));
+    HIP_OK(hipEventRecord(e1, c->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(&t_dir, e0, e1));
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i)
+      hipLaunchKernelGGL((k_cg_update<double, double>), dim3(g), dim3(kThreads), 0, c->stream,
+                         c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, n, c->partial.p, c->st.p, 3, 0);
+    HIP_OK(hipEventRecord(e1, c->stream));
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventSynchronize(e1));
+  HIP_OK(hipEventElapsedTime(&t_upd, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_dir) *ms_dir = t_dir / reps;
+  if (ms_upd) *ms_upd = t_upd / reps;
+  return BHIP_OK;
+}
+
 int bhip_solve(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x, double* b,
                bhip_solve_info* info) {
   if (!c || !x || !b) return BHIP_EBADARG;

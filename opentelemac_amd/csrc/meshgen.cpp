@@ -41,9 +41,12 @@ extern "C" {
 // Returns the number of points written (at most max_points) or -1 when the
 // buffer was too small. fx is a (gny, gnx) row-major grid of spacing factors
 // over the polygon's bounding box.
+// n_init fixed points (e.g. samples of the outline) are inserted first,
+// without distance test; sampling then grows from them.
 long meshgen_poisson_disk(const double* px, const double* py, int npoly,
                           double r0, const double* fgrid, int gnx, int gny,
-                          uint64_t seed, int k_attempts, double* out_x,
+                          uint64_t seed, int k_attempts, const double* init_x,
+                          const double* init_y, long n_init, double* out_x,
                           double* out_y, long max_points) {
   double xmin = px[0], xmax = px[0], ymin = py[0], ymax = py[0];
   for (int i = 1; i < npoly; ++i) {
@@ -73,6 +76,38 @@ long meshgen_poisson_disk(const double* px, const double* py, int npoly,
                tx * ty * fgrid[(long)(iy + 1) * gnx + ix + 1];
     return r0 * f;
   };
+  // coarse inside/outside raster: cells farther than their diagonal from
+  // every polygon edge take the status of their centre, the others are
+  // tested exactly
+  const int mnx = 1024, mny = 1024;
+  const double mdx = (xmax - xmin) / mnx, mdy = (ymax - ymin) / mny;
+  std::vector<signed char> mask((size_t)mnx * mny);
+  {
+    const double diag = std::sqrt(mdx * mdx + mdy * mdy);
+    for (int j = 0; j < mny; ++j)
+      for (int i = 0; i < mnx; ++i) {
+        double cx = xmin + (i + 0.5) * mdx, cy = ymin + (j + 0.5) * mdy;
+        double dmin = 1e300;
+        for (int a = 0, b = npoly - 1; a < npoly; b = a++) {
+          double ex = px[a] - px[b], ey = py[a] - py[b];
+          double l2 = ex * ex + ey * ey;
+          double t = l2 > 0 ? ((cx - px[b]) * ex + (cy - py[b]) * ey) / l2 : 0.0;
+          t = std::fmin(1.0, std::fmax(0.0, t));
+          double qx = px[b] + t * ex - cx, qy = py[b] + t * ey - cy;
+          dmin = std::fmin(dmin, qx * qx + qy * qy);
+        }
+        signed char v = 2;  // 2: test exactly
+        if (std::sqrt(dmin) > diag) v = inside(px, py, npoly, cx, cy) ? 1 : 0;
+        mask[(size_t)j * mnx + i] = v;
+      }
+  }
+  auto in_domain = [&](double x, double y) {
+    int i = (int)((x - xmin) / mdx), j = (int)((y - ymin) / mdy);
+    if (i < 0 || j < 0 || i >= mnx || j >= mny) return false;
+    signed char v = mask[(size_t)j * mnx + i];
+    if (v != 2) return v == 1;
+    return inside(px, py, npoly, x, y);
+  };
   const double cell = rmin / std::sqrt(2.0);
   const long ncx = (long)std::ceil((xmax - xmin) / cell) + 1;
   const long ncy = (long)std::ceil((ymax - ymin) / cell) + 1;
@@ -81,11 +116,16 @@ long meshgen_poisson_disk(const double* px, const double* py, int npoly,
   Rng rng(seed);
   long n = 0;
   std::vector<long> active;
+  std::vector<double> rad_of;
+  rad_of.reserve((size_t)max_points);
   auto add = [&](double x, double y) -> bool {
     if (n >= max_points) return false;
     out_x[n] = x;
     out_y[n] = y;
+    rad_of.push_back(radius(x, y));
     long cx = (long)((x - xmin) / cell), cy = (long)((y - ymin) / cell);
+    cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
     grid[(size_t)(cy * ncx + cx)].push_back((int)n);
     active.push_back(n);
     ++n;
@@ -99,18 +139,20 @@ long meshgen_poisson_disk(const double* px, const double* py, int npoly,
         if (i < 0 || i >= ncx) continue;
         for (int q : grid[(size_t)(j * ncx + i)]) {
           double dx = out_x[q] - x, dy = out_y[q] - y;
-          double rq = std::fmax(r, radius(out_x[q], out_y[q]));
+          double rq = std::fmax(r, rad_of[q]);
           if (dx * dx + dy * dy < rq * rq) return false;
         }
       }
     }
     return true;
   };
+  for (long i = 0; i < n_init; ++i)
+    if (!add(init_x[i], init_y[i])) return -1;
   // seed point: first random point inside the polygon
-  for (int tries = 0; tries < 100000; ++tries) {
+  for (int tries = 0; tries < 100000 && n_init == 0; ++tries) {
     double x = xmin + rng.uniform() * (xmax - xmin);
     double y = ymin + rng.uniform() * (ymax - ymin);
-    if (inside(px, py, npoly, x, y)) {
+    if (in_domain(x, y)) {
       add(x, y);
       break;
     }
@@ -120,14 +162,14 @@ long meshgen_poisson_disk(const double* px, const double* py, int npoly,
   while (head < active.size()) {
     long p = active[head];
     double x0 = out_x[p], y0 = out_y[p];
-    double r = radius(x0, y0);
+    double r = rad_of[p];
     bool found = false;
     for (int k = 0; k < k_attempts; ++k) {
       double ang = 2.0 * M_PI * rng.uniform();
       double rad = r * (1.0 + rng.uniform());
       double x = x0 + rad * std::cos(ang), y = y0 + rad * std::sin(ang);
       if (x < xmin || x > xmax || y < ymin || y > ymax) continue;
-      if (!inside(px, py, npoly, x, y)) continue;
+      if (!in_domain(x, y)) continue;
       double rx = radius(x, y);
       if (ok(x, y, rx)) {
         if (!add(x, y)) return -1;

@@ -163,6 +163,7 @@ def _meshgen_lib():
     lib.meshgen_poisson_disk.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
     return lib
 
@@ -228,21 +229,44 @@ def estuary_mesh(npoin_target: int = 2_000_000, seed: int = 0, lx: float = 50000
     ins = points_in_polygon(GX.ravel(), GY.ravel(), px, py)
     cell_area = (gx[1] - gx[0]) * (gy[1] - gy[0])
     eff_area = float(np.sum(cell_area / fgrid.ravel()[ins] ** 2))
-    # Bridson fills about 0.68 points per r^2 of (effective) area
-    r0 = math.sqrt(0.68 * eff_area / npoin_target)
+    # this sampler fills about 0.641 points per r^2 of effective area
+    r0 = math.sqrt(0.6412 * eff_area / npoin_target)
     pxc = np.ascontiguousarray(px)
     pyc = np.ascontiguousarray(py)
+
+    def spacing(xq, yq):
+        ix = np.clip(((xq - gx[0]) / (gx[-1] - gx[0]) * (gnx - 1)).astype(int), 0, gnx - 1)
+        iy = np.clip(((yq - gy[0]) / (gy[-1] - gy[0]) * (gny - 1)).astype(int), 0, gny - 1)
+        return fgrid[iy, ix]
+
+    def outline(r0):
+        # outline samples at the local spacing, vertices included once
+        bx, by = [], []
+        for i in range(px.size):
+            x0, y0, x1, y1 = px[i], py[i], px[(i + 1) % px.size], py[(i + 1) % px.size]
+            length = math.hypot(x1 - x0, y1 - y0)
+            if length == 0:
+                continue
+            h = r0 * float(spacing(np.array([0.5 * (x0 + x1)]), np.array([0.5 * (y0 + y1)]))[0])
+            k = max(1, int(math.ceil(length / (1.05 * h))))
+            t = np.arange(k) / k
+            bx.append(x0 + t * (x1 - x0))
+            by.append(y0 + t * (y1 - y0))
+        return np.ascontiguousarray(np.concatenate(bx)), np.ascontiguousarray(np.concatenate(by))
+
     for _ in range(3):
         cap = int(npoin_target * 1.3) + 1000
         ox = np.empty(cap)
         oy = np.empty(cap)
+        ix, iy = outline(r0)
         n = lib.meshgen_poisson_disk(pxc.ctypes.data, pyc.ctypes.data, px.size, r0,
                                      fgrid.ctypes.data, gnx, gny, seed, 30,
+                                     ix.ctypes.data, iy.ctypes.data, ix.size,
                                      ox.ctypes.data, oy.ctypes.data, cap)
         if n < 0:
             r0 *= 1.1
             continue
-        if abs(n - npoin_target) <= 0.01 * npoin_target:
+        if abs(n - npoin_target) <= 0.005 * npoin_target:
             break
         r0 *= math.sqrt(n / npoin_target)
     x = ox[:n].copy()
