@@ -218,6 +218,46 @@ class Context:
         self.typdia = td.raw[:1].decode()
         return self.typdia, da, xa[:ncoef]
 
+    # ---- operand preparation --------------------------------------------
+    def dirich(self, f, sm, fbor, limdir, kdir=5):
+        """DIRICH with domain-sized FBOR/LIMDIR (dirich.f -> diri01.f ->
+        diraux.f). Returns (f, sm); the context's matrix gets the rows and
+        columns of prescribed nodes zeroed."""
+        f = _host_f64(f).copy()
+        sm = _host_f64(sm).copy()
+        fb = _host_f64(fbor)
+        ld = np.ascontiguousarray(limdir, dtype=np.int32)
+        _check(lib().bhip_dirich(self._h, f.ctypes.data, sm.ctypes.data, fb.ctypes.data,
+                                 ld.ctypes.data, int(kdir)), "bhip_dirich")
+        return f, sm
+
+    def lump(self, xmul=1.0):
+        """LUMP: A * (XMUL everywhere) (lump.f)."""
+        d = np.empty(self.npoin)
+        _check(lib().bhip_lump(self._h, float(xmul), d.ctypes.data), "bhip_lump")
+        return d
+
+    def matrix_mt01aa(self, xmul, surfac, accumulate=False):
+        """MATRIX(... 'MATMAS' ...) with OM 'M=N' / 'M=M+N' (mt01aa.f)."""
+        s = _host_f64(surfac)
+        _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), s.ctypes.data, int(accumulate)),
+               "bhip_matrix_mt01aa")
+        self.typdia, self.typext = "Q", "S"
+
+    def matrix_mt02aa(self, kind, xmul, xel, yel, surfac, u, v=None, accumulate=False):
+        """MATRIX(... 'MATDIF' / 'MATDIFUV' ...): kind 1 P1 isotropic
+        viscosity U, kind 2 coefficient U*V (mt02aa.f)."""
+        xel = _host_f64(xel)
+        yel = _host_f64(yel)
+        s = _host_f64(surfac)
+        u = _host_f64(u)
+        v = _host_f64(v) if v is not None else None
+        _check(lib().bhip_matrix_mt02aa(self._h, int(kind), float(xmul), xel.ctypes.data,
+                                        yel.ctypes.data, s.ctypes.data, u.ctypes.data,
+                                        v.ctypes.data if v is not None else None,
+                                        int(accumulate)), "bhip_matrix_mt02aa")
+        self.typdia, self.typext = "Q", "S"
+
     # ---- MATVEC ---------------------------------------------------------
     def _sync_torch(self):
         import torch

@@ -11,11 +11,14 @@
 //    nodes once -- deterministic, atomic-free scatter-add;
 //  * element data are structure of arrays in slot order (coalesced): 16-bit
 //    local node ids and the off-diagonal coefficients;
-//  * CG keeps its scalars on the device: each kernel writes one partial sum
-//    per workgroup and the last workgroup to finish (ticket counter) reduces
-//    them in a fixed order and advances the solver state, so iterations run
-//    back to back without host round trips and results are bitwise
-//    reproducible run to run.
+//  * CG keeps its scalars on the device and never synchronises workgroups
+//    inside a kernel (on multi-XCD MI355X an agent-scope fence is an L2
+//    write-back): each kernel writes one partial sum per workgroup, and every
+//    workgroup of the next kernel reduces those partials itself in the same
+//    fixed order, so all of them derive identical scalars (rho, beta, the
+//    stop test). Scalars written for the host are double-buffered by
+//    iteration parity so no kernel reads a slot it writes. Iterations run
+//    back to back without host round trips and are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,20 +52,25 @@ constexpr int kWaves = kThreads / 64;
 // ---------------------------------------------------------------------------
 enum : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FINAL = 2, ST_MAXITER = 3 };
 
+// Iteration m of a CG solve runs U_m (vector update, parity p = m & 1) then
+// E_m (direction + EBE mat-vec). Slot [p] is written by iteration m and read
+// by iteration m + 1.
 struct SolveState {
-  double xl;       // max(||b'||^2, 1) (gracjg.F:189-209)
-  double eps2xl;   // stop threshold on ||r||^2
-  double rr;       // RMRM
-  double rr_prev;  // RM1GM1
-  double rho;      // RO
-  double beta;     // BETA
-  double dad;      // D.AD
-  double rr0;      // initial ||r||^2 (mixed inner solves)
-  int m;           // iteration counter M
-  int state;
+  double xl;        // max(||b'||^2, 1) (gracjg.F:189-209)
+  double eps2xl;    // stop threshold on ||r||^2 (inner solves: relative)
+  double rr0;       // ||r_0||^2 of the solve (inner solves scale eps2xl by it)
+  double dot;       // result of k_reduce_into
+  double rr[2];     // RMRM of iteration m, by parity
+  double rho[2];    // RO of iteration m (informational)
+  int m[2];         // iteration number M, by parity
+  int status[2];    // ST_RUNNING or ST_FINAL (NITMAX reached), by parity
+  int done;         // sticky: 0, ST_CONVERGED or ST_MAXITER
   int nitmax;
-  int pad;
-  unsigned ticket[4];
+  int final_m;      // M at exit
+  int use_rr0;      // threshold = eps2xl * rr0
+  double final_rr;  // RMRM at exit
+  int final_flag;   // set by the update kernel that applied the last X update
+  int pad2;
 };
 
 // ---------------------------------------------------------------------------
@@ -87,28 +95,25 @@ __device__ inline double block_sum(double v, double* sh) {
   return s;
 }
 
-// Writes this block's partial, then returns true (in every thread) for the
-// last block to arrive; that block sees every partial.
-__device__ inline bool publish_partial(double* partial, int slot, double v,
-                                       unsigned* ticket, int nblocks, int* sh_flag) {
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partial[slot], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    *sh_flag = (t == (unsigned)(nblocks - 1));
-  }
+// Fixed-order sum of n partials computed by every workgroup that calls it
+// (identical result in all of them); valid in all threads.
+__device__ inline double reduce_partials(const double* __restrict__ partial, int n,
+                                         double* sh, double* bcast) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) *bcast = s;
   __syncthreads();
-  bool last = *sh_flag != 0;
-  if (last) __threadfence();
-  return last;
+  s = *bcast;
+  __syncthreads();
+  return s;
 }
 
-// fixed-order sum of n partials by one block; result valid in thread 0
-__device__ inline double sum_partials(double* partial, int n, double* sh) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x)
-    s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return block_sum(s, sh);
+__device__ inline int load_relaxed(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_relaxed(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,8 +124,8 @@ enum : int { DIAG_ZERO = 0, DIAG_ONE = 1, DIAG_Q = 2 };
 enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2 };
 // what happens at owned nodes after the element loop
 enum : int { EPI_OP = 0, EPI_RESID = 1, EPI_CGAD = 2, EPI_RESID_MIXED = 3 };
-// what the last workgroup does with the reduced sum
-enum : int { FIN_NONE = 0, FIN_RESID_INIT = 1, FIN_CGAD = 2, FIN_MIXED_OUTER = 3 };
+// whether the kernel writes a partial sum per workgroup / runs the CG control
+enum : int { FIN_NONE = 0, FIN_PARTIAL = 1, FIN_CG = 2 };
 
 template <typename T, typename TA>
 struct EbeArgs {
@@ -152,10 +157,11 @@ struct EbeArgs {
   float* r32;                 // RESID_MIXED: inner residual -(A x - b)
   float* zero32;              // RESID_MIXED: inner solution reset
   // reductions / state
-  double* partial;
+  double* partial;            // this kernel's partials (one per workgroup)
+  const double* part_in;      // FIN_CG: partials of the preceding update kernel
+  int n_part_in;
   SolveState* st;
-  int ticket_id;
-  int check_state;            // kernels of a running solve exit when done
+  int parity;                 // FIN_CG: parity of the iteration
 };
 
 // column of the coefficient array holding A(i, j), i != j
@@ -174,8 +180,38 @@ __global__ void __launch_bounds__(kThreads)
 k_ebe(EbeArgs<T, TA> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh_red[kWaves];
-  __shared__ int sh_flag;
-  if (a.check_state && a.st->state != ST_RUNNING) return;
+  __shared__ double sh_b;
+  double cg_beta = 0.0;
+  if constexpr (FIN == FIN_CG) {
+    // E_m: RMRM from the update kernel's partials, stop test (gracjg.F:306-324,
+    // and the strict test of the initial residual, gracjg.F:219), BETA
+    SolveState* st = a.st;
+    if (load_relaxed(&st->done)) return;
+    if (load_relaxed(&st->final_flag)) {
+      // the update kernel before this one applied the last X update
+      if (blockIdx.x == 0 && threadIdx.x == 0) store_relaxed(&st->done, ST_MAXITER);
+      return;
+    }
+    const int p = a.parity;
+    const double rr = reduce_partials(a.part_in, a.n_part_in, sh_red, &sh_b);
+    const int m = st->m[1 - p] + 1;
+    const double thr = st->use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
+    const bool conv = (m == 0) ? (rr < thr) : (rr <= thr);
+    if (conv) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->final_m = m;
+        st->final_rr = rr;
+        store_relaxed(&st->done, ST_CONVERGED);
+      }
+      return;
+    }
+    cg_beta = (m == 0) ? 0.0 : rr / st->rr[1 - p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr[p] = rr;
+      st->m[p] = m;
+      st->status[p] = (m >= st->nitmax) ? ST_FINAL : ST_RUNNING;
+    }
+  }
 
   // XCD-aware mapping: workgroup g is dispatched to XCD g % 8; give each XCD
   // a contiguous range of node blocks so blocks sharing halo nodes share an
@@ -204,7 +240,7 @@ k_ebe(EbeArgs<T, TA> a) {
     if constexpr (STAGE == STAGE_PLAIN) {
       v = a.y[g];
     } else if constexpr (STAGE == STAGE_CGDIR) {
-      const T beta = (T)a.st->beta;
+      const T beta = (T)cg_beta;
       v = a.r[g] + beta * a.dold[g];
       if (owned) a.dnew[g] = v;
     } else {
@@ -296,28 +332,7 @@ k_ebe(EbeArgs<T, TA> a) {
   }
   if constexpr (FIN != FIN_NONE) {
     double s = block_sum(part, sh_red);
-    if (publish_partial(a.partial, b, s, &a.st->ticket[a.ticket_id], nb, &sh_flag)) {
-      double tot = sum_partials(a.partial, nb, sh_red);
-      if (threadIdx.x == 0) {
-        SolveState* st = a.st;
-        if constexpr (FIN == FIN_RESID_INIT) {
-          st->rr = tot;
-          st->rr0 = tot;
-          st->beta = 0.0;
-          st->m = 0;
-          st->state = (tot < st->eps2xl) ? ST_CONVERGED : ST_RUNNING;
-        } else if constexpr (FIN == FIN_CGAD) {
-          st->dad = tot;
-          st->rho = st->rr / tot;
-          if (st->m >= st->nitmax) st->state = ST_FINAL;
-        } else if constexpr (FIN == FIN_MIXED_OUTER) {
-          st->rr = tot;
-          st->rr0 = tot;
-          st->state = (tot <= st->eps2xl) ? ST_CONVERGED : ST_RUNNING;
-        }
-        st->ticket[a.ticket_id] = 0;
-      }
-    }
+    if (threadIdx.x == 0) a.partial[b] = s;
   }
 }
 
@@ -328,68 +343,63 @@ k_ebe(EbeArgs<T, TA> a) {
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kThreads)
 k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
-            const T* __restrict__ ad, int n, double* partial, SolveState* st, int ticket_id,
-            int use_rr0) {
+            const T* __restrict__ ad, int n, const double* __restrict__ part_e, int n_part_e,
+            double* __restrict__ part_u, SolveState* st, int parity) {
   __shared__ double sh_red[kWaves];
-  __shared__ int sh_flag;
-  const int state = st->state;
-  if (state != ST_RUNNING && state != ST_FINAL) return;
-  const double rho = st->rho;
+  __shared__ double sh_b;
+  if (load_relaxed(&st->done)) return;
+  const int q = 1 - parity;  // slot of iteration m - 1
+  const int status = st->status[q];
+  // RO = RMRM / D.AD of iteration m - 1 (gracjg.F:380-383)
+  const double dad = reduce_partials(part_e, n_part_e, sh_red, &sh_b);
+  const double rho = st->rr[q] / dad;
   const T rhot = (T)rho;
-  double part = 0.0;
   const int stride = gridDim.x * blockDim.x;
-  if (state == ST_RUNNING) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      x[i] = x[i] - (TX)rhot * (TX)d[i];
-      const T rv = r[i] - rhot * ad[i];
-      r[i] = rv;
-      part += (double)rv * (double)rv;
-    }
-  } else {
+  if (status == ST_FINAL) {
+    // NITMAX reached: the last X = X - RO D, then stop (gracjg.F:387-409)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
       x[i] = x[i] - (TX)rhot * (TX)d[i];
+    // every workgroup runs this branch (status[q] was written by an earlier
+    // kernel); the next direction kernel turns final_flag into done
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->final_m = st->m[q];
+      st->final_rr = st->rr[q];
+      st->rho[q] = rho;
+      store_relaxed(&st->final_flag, 1);
+    }
+    return;
+  }
+  double part = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    x[i] = x[i] - (TX)rhot * (TX)d[i];
+    const T rv = r[i] - rhot * ad[i];
+    r[i] = rv;
+    part += (double)rv * (double)rv;
   }
   double s = block_sum(part, sh_red);
-  if (publish_partial(partial, blockIdx.x, s, &st->ticket[ticket_id], gridDim.x, &sh_flag)) {
-    double tot = sum_partials(partial, gridDim.x, sh_red);
-    if (threadIdx.x == 0) {
-      if (state == ST_FINAL) {
-        st->state = ST_MAXITER;
-      } else {
-        st->m = st->m + 1;
-        st->rr_prev = st->rr;
-        st->rr = tot;
-        const double thr = use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
-        if (tot <= thr) {
-          st->state = ST_CONVERGED;
-        } else {
-          st->beta = tot / st->rr_prev;
-        }
-      }
-      st->ticket[ticket_id] = 0;
-    }
-  }
+  if (threadIdx.x == 0) part_u[blockIdx.x] = s;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rho[q] = rho;
 }
 
 // ---------------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_dot_into(const T* x, const T* y, int n, double* partial, SolveState* st,
-                           int ticket_id, double* dst) {
+__global__ void k_dot_partials(const T* x, const T* y, int n, double* partial) {
   __shared__ double sh_red[kWaves];
-  __shared__ int sh_flag;
   double part = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     part += (double)x[i] * (double)y[i];
   double s = block_sum(part, sh_red);
-  if (publish_partial(partial, blockIdx.x, s, &st->ticket[ticket_id], gridDim.x, &sh_flag)) {
-    double tot = sum_partials(partial, gridDim.x, sh_red);
-    if (threadIdx.x == 0) {
-      *dst = tot;
-      st->ticket[ticket_id] = 0;
-    }
-  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// one workgroup: *dst = fixed-order sum of n partials
+__global__ void k_reduce_into(const double* __restrict__ partial, int n, double* dst) {
+  __shared__ double sh_red[kWaves];
+  __shared__ double sh_b;
+  double s = reduce_partials(partial, n, sh_red, &sh_b);
+  if (threadIdx.x == 0) *dst = s;
 }
 
 // out[i] = in[idx[i]]
@@ -552,8 +562,17 @@ struct bhip_ctx {
   // work vectors (internal order)
   bhip::DevBuf<double> vx, vb, vr, vd0, vd1, vad, vdz, vdinv, vio0, vio1;
   bhip::DevBuf<float> fr, fd0, fd1, fad, fx0, fx1, fdz;
-  bhip::DevBuf<double> partial;
+  bhip::DevBuf<double> partial;  // 4 arrays of max_part: U[0], U[1], E[0], E[1]
+  int max_part = 0;
   bhip::DevBuf<bhip::SolveState> st;
+  // captured CG iteration batches (key: precision, solution buffer)
+  struct Graph {
+    const void* key = nullptr;
+    int prec = -1;
+    hipGraphExec_t exec = nullptr;
+  };
+  Graph graphs[4];
+  bool use_graphs = true;
   bhip::SolveState* st_host = nullptr;  // pinned
   size_t lds_f64 = 0, lds_f32 = 0;
 };
@@ -566,8 +585,8 @@ static size_t lds_bytes(const Layout& L, size_t elt) {
 }
 
 static int grid_for(int n) {
-  int g = (n + kThreads * 4 - 1) / (kThreads * 4);
-  return std::max(1, std::min(g, 2048));
+  int g = (n + kThreads * 8 - 1) / (kThreads * 8);
+  return std::max(1, std::min(g, 1024));
 }
 
 template <typename T, typename TA>
@@ -682,8 +701,12 @@ int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const do
   for (auto* v : {&c->vx, &c->vb, &c->vr, &c->vd0, &c->vd1, &c->vad, &c->vdz, &c->vdinv,
                   &c->vio0, &c->vio1})
     HIP_OK(v->alloc(n));
-  const int maxgrid = std::max(L.nblocks, 2048);
-  HIP_OK(c->partial.alloc(maxgrid));
+  c->max_part = std::max(L.nblocks, c->grid_nodes) + 64;
+  HIP_OK(c->partial.alloc((size_t)4 * c->max_part));
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(4 * c->max_part)), dim3(kThreads), 0,
+                     c->stream, c->partial.p, 0.0, (long)4 * c->max_part);
+  HIP_OK(hipGetLastError());
+  if (const char* ev = getenv("BHIP_NO_GRAPH")) c->use_graphs = !(ev[0] == '1');
   HIP_OK(c->st.alloc(1));
   HIP_OK(hipHostMalloc((void**)&c->st_host, sizeof(SolveState), hipHostMallocDefault));
   std::memset(c->st_host, 0, sizeof(SolveState));
@@ -710,6 +733,8 @@ int bhip_ctx_destroy(bhip_ctx* c) {
                   &c->fdz})
     v->release();
   c->st.release();
+  for (auto& gr : c->graphs)
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -907,22 +932,142 @@ static int write_state(bhip_ctx* c) {
   return BHIP_OK;
 }
 
-// GRACJG iterations on the device-resident preconditioned system, FP64.
-// x, b internal order; returns after the state leaves RUNNING/FINAL.
-static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
-                   bhip_solve_info* info) {
-  const Layout& L = c->L;
-  const int n = L.npoin;
+static double* part_u(bhip_ctx* c, int p) { return c->partial.p + (size_t)p * c->max_part; }
+static double* part_e(bhip_ctx* c, int p) { return c->partial.p + (size_t)(2 + p) * c->max_part; }
+
+// host copy of the state -> device (the device never writes a field the
+// host sets here while no kernel is in flight)
+static void init_cg_state(SolveState* sh, double eps2xl, int use_rr0, int nitmax) {
+  sh->eps2xl = eps2xl;
+  sh->use_rr0 = use_rr0;
+  // the reference always runs iteration 1 before testing NITMAX (gracjg.F:406)
+  sh->nitmax = std::max(nitmax, 1);
+  sh->m[0] = sh->m[1] = -1;
+  sh->status[0] = sh->status[1] = ST_RUNNING;
+  sh->rr[0] = sh->rr[1] = 1.0;
+  sh->done = 0;
+  sh->final_flag = 0;
+  sh->final_m = 0;
+  sh->final_rr = 0.0;
+}
+
+// One batch of `iters` CG iterations (each U then E; an even count keeps the
+// direction ping-pong aligned). All kernels exit at once when the solve is
+// done, so a batch can run past convergence at no cost but launches.
+template <typename T, typename TA, typename TX>
+static hipError_t enqueue_cg_batch(bhip_ctx* c, int iters, TX* x, T* r, T* d0, T* d1, T* ad,
+                                   const TA* xa, const TA* da) {
+  const int n = c->L.npoin;
   const int g = c->grid_nodes;
+  for (int k = 0; k < iters; ++k) {
+    const int p = (k & 1) ? 0 : 1;  // iterations 1, 2, 3, ... of the batch
+    T* dprev = (p == 1) ? d0 : d1;
+    T* dnext = (p == 1) ? d1 : d0;
+    hipLaunchKernelGGL((k_cg_update<T, TX>), dim3(g), dim3(kThreads), 0, c->stream, x, r,
+                       dprev, ad, n, part_e(c, 1 - p), c->L.nblocks, part_u(c, p), c->st.p, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    EbeArgs<T, TA> a = base_args<T, TA>(c, xa, da);
+    a.r = r;
+    a.dold = dprev;
+    a.dnew = dnext;
+    a.out = ad;
+    a.partial = part_e(c, p);
+    a.part_in = part_u(c, p);
+    a.n_part_in = g;
+    a.parity = p;
+    e = launch_ebe<T, TA, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia, c->typext);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// GRACJG from the state after the initial residual: R holds A X - B and
+// part_u[0] its nblocks partial sums of ||R||^2. Runs E_0 (D = R, AD, RO)
+// then batches of iterations until the device reports done.
+template <typename T, typename TA, typename TX>
+static int run_cg(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad, const T* dz,
+                  const TA* xa, const TA* da) {
   SolveState* sh = c->st_host;
-  // XL = B.B (gracjg.F:189)
-  std::memset(sh, 0, sizeof(SolveState));
-  if (write_state(c)) return BHIP_EDEVICE;
-  hipLaunchKernelGGL(k_dot_into<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b, n,
-                     c->partial.p, c->st.p, 0, &c->st.p->xl);
+  {
+    EbeArgs<T, TA> a = base_args<T, TA>(c, xa, da);
+    a.r = r;
+    a.dold = dz;
+    a.dnew = d0;
+    a.out = ad;
+    a.partial = part_e(c, 0);
+    a.part_in = part_u(c, 0);
+    a.n_part_in = c->L.nblocks;
+    a.parity = 0;
+    HIP_OK((launch_ebe<T, TA, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia,
+                                                              c->typext)));
+  }
+  constexpr int kBatch = 8;
+  hipGraphExec_t exec = nullptr;
+  if (c->use_graphs) {
+    int slot = -1;
+    for (int i = 0; i < 4; ++i)
+      if (c->graphs[i].exec && c->graphs[i].key == (const void*)x && c->graphs[i].prec == prec)
+        slot = i;
+    if (slot < 0) {
+      for (int i = 0; i < 4 && slot < 0; ++i)
+        if (!c->graphs[i].exec) slot = i;
+      if (slot < 0) {
+        (void)hipGraphExecDestroy(c->graphs[3].exec);
+        c->graphs[3].exec = nullptr;
+        slot = 3;
+      }
+      hipGraph_t graph;
+      HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da);
+      hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+      HIP_OK(e);
+      HIP_OK(e2);
+      HIP_OK(hipGraphInstantiate(&c->graphs[slot].exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      c->graphs[slot].key = (const void*)x;
+      c->graphs[slot].prec = prec;
+    }
+    exec = c->graphs[slot].exec;
+  }
+  int nb = 1;
+  for (;;) {
+    for (int k = 0; k < nb; ++k) {
+      if (exec)
+        HIP_OK(hipGraphLaunch(exec, c->stream));
+      else
+        HIP_OK((enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da)));
+    }
+    if (read_state(c)) return BHIP_EDEVICE;
+    if (sh->done) break;
+    nb = std::min(nb * 2, 8);
+  }
+  return BHIP_OK;
+}
+
+// ||b||^2 into st->dot (and the host copy)
+static int dot_b(bhip_ctx* c, const double* b, double* out) {
+  const int g = c->grid_nodes;
+  hipLaunchKernelGGL(k_dot_partials<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b,
+                     c->L.npoin, part_u(c, 0));
+  hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0), g,
+                     &c->st.p->dot);
   HIP_OK(hipGetLastError());
   if (read_state(c)) return BHIP_EDEVICE;
-  double xl = sh->xl;
+  *out = c->st_host->dot;
+  return BHIP_OK;
+}
+
+// GRACJG (gracjg.F:184-420) on the device-resident preconditioned system, FP64.
+static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
+                   bhip_solve_info* info) {
+  const int n = c->L.npoin;
+  const int g = c->grid_nodes;
+  SolveState* sh = c->st_host;
+  std::memset(sh, 0, sizeof(SolveState));
+  if (write_state(c)) return BHIP_EDEVICE;
+  double xl;
+  if (dot_b(c, b, &xl)) return BHIP_EDEVICE;
   info->relat = 1;
   if (std::sqrt(xl) < 1e-15) {
     hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
@@ -937,62 +1082,25 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
     info->relat = 0;
   }
   sh->xl = xl;
-  sh->eps2xl = eps * eps * xl;
-  sh->nitmax = nitmax;
-  sh->state = ST_RUNNING;
+  init_cg_state(sh, eps * eps * xl, 0, nitmax);
   if (write_state(c)) return BHIP_EDEVICE;
-  // R = A X - B ; RMRM (gracjg.F:213-219)
+  // R = A X - B (gracjg.F:213-215), partial sums of RMRM
   {
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.y = x;
     a.b = b;
     a.out = c->vr.p;
-    a.ticket_id = 1;
-    HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_RESID_INIT>(
+    a.partial = part_u(c, 0);
+    HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
   }
-  if (read_state(c)) return BHIP_EDEVICE;
-  if (sh->state == ST_CONVERGED) {
-    info->niter = 0;
-    info->relres = std::sqrt(sh->rr / sh->xl);
-    info->converged = 1;
-    return BHIP_OK;
-  }
-  // D = R ; AD = A D ; RO = RMRM / D.AD ; (X -= RO D deferred) (gracjg.F:248-268)
-  auto launch_dir = [&](const double* dold, double* dnew) {
-    EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
-    a.r = c->vr.p;
-    a.dold = dold;
-    a.dnew = dnew;
-    a.out = c->vad.p;
-    a.ticket_id = 2;
-    a.check_state = 1;
-    return launch_ebe<double, double, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
-                                                                        c->typext);
-  };
-  HIP_OK(launch_dir(c->vdz.p, c->vd0.p));
-  auto launch_upd = [&](const double* d) {
-    hipLaunchKernelGGL((k_cg_update<double, double>), dim3(g), dim3(kThreads), 0, c->stream, x,
-                       c->vr.p, d, c->vad.p, n, c->partial.p, c->st.p, 3, 0);
-    return hipGetLastError();
-  };
-  // iterations in batches; kernels exit early once the state leaves RUNNING
-  int batch = 8;
-  for (;;) {
-    for (int k = 0; k < batch; ++k) {
-      HIP_OK(launch_upd(c->vd0.p));
-      HIP_OK(launch_dir(c->vd0.p, c->vd1.p));
-      HIP_OK(launch_upd(c->vd1.p));
-      HIP_OK(launch_dir(c->vd1.p, c->vd0.p));
-    }
-    if (read_state(c)) return BHIP_EDEVICE;
-    if (sh->state == ST_CONVERGED || sh->state == ST_MAXITER) break;
-    batch = std::min(batch * 2, 64);
-  }
-  info->niter = sh->m;
-  info->relres = std::sqrt(sh->rr / sh->xl);
-  info->converged = sh->state == ST_CONVERGED;
-  return sh->state == ST_CONVERGED ? BHIP_OK : BHIP_MAXITER;
+  int rc = run_cg<double, double, double>(c, BHIP_PREC_FP64, x, c->vr.p, c->vd0.p, c->vd1.p,
+                                          c->vad.p, c->vdz.p, c->xa.p, c->da.p);
+  if (rc) return rc;
+  info->niter = sh->final_m;
+  info->relres = std::sqrt(sh->final_rr / xl);
+  info->converged = sh->done == ST_CONVERGED;
+  return sh->done == ST_CONVERGED ? BHIP_OK : BHIP_MAXITER;
 }
 
 static int ensure_f32(bhip_ctx* c) {
@@ -1022,20 +1130,15 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   if (ensure_f32(c)) return BHIP_EDEVICE;
   const int npair = L.ndp * (L.ndp - 1) / 2;
   const long ncf = (long)(c->typext == 'Q' ? 2 * npair : npair) * L.nslot_pad;
-  if (c->typext != '0') {
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for((int)std::min<long>(ncf, 1 << 30))),
-                       dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p, ncf);
-  }
+  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
+                     ncf);
   hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
                      (long)n);
   HIP_OK(hipGetLastError());
   std::memset(sh, 0, sizeof(SolveState));
   if (write_state(c)) return BHIP_EDEVICE;
-  hipLaunchKernelGGL(k_dot_into<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b, n,
-                     c->partial.p, c->st.p, 0, &c->st.p->xl);
-  HIP_OK(hipGetLastError());
-  if (read_state(c)) return BHIP_EDEVICE;
-  double xl = sh->xl;
+  double xl;
+  if (dot_b(c, b, &xl)) return BHIP_EDEVICE;
   info->relat = 1;
   if (std::sqrt(xl) < 1e-15) {
     hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
@@ -1048,7 +1151,6 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     info->relat = 0;
   }
   const double eps2xl = eps * eps * xl;
-  // delta buffers: fx0 holds zeros for the first outer residual
   hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, c->fx0.p, 0.0f,
                      (long)n);
   HIP_OK(hipGetLastError());
@@ -1059,12 +1161,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   int total_inner = 0, outer = 0;
   int status = BHIP_MAXITER;
   for (;;) {
-    // outer: x_new = x - delta_prev ; r = A x_new - b ; inner RHS = -r ; delta_next = 0
-    sh->xl = xl;
-    sh->eps2xl = eps2xl;
-    sh->state = ST_RUNNING;
-    sh->m = 0;
-    if (write_state(c)) return BHIP_EDEVICE;
+    // outer: x_new = x - delta_prev ; r = A x_new - b ; inner RHS -r ; delta_next = 0
     {
       EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
       a.xd = xa_;
@@ -1073,69 +1170,35 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
       a.b = b;
       a.r32 = c->fr.p;
       a.zero32 = dl_next;
-      a.ticket_id = 1;
-      HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_MIXED_OUTER>(
+      a.partial = part_u(c, 0);
+      HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_PARTIAL>(
           c, a, false, c->typdia, c->typext)));
     }
+    hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0),
+                       L.nblocks, &c->st.p->rr0);
+    HIP_OK(hipGetLastError());
     std::swap(xa_, xb_);
     if (read_state(c)) return BHIP_EDEVICE;
-    const double rr = sh->rr;
-    if (sh->state == ST_CONVERGED) {
+    const double rr = sh->rr0;
+    info->relres = std::sqrt(rr / xl);
+    if (rr <= eps2xl) {
       status = BHIP_OK;
-      info->relres = std::sqrt(rr / xl);
       break;
     }
-    if (total_inner >= nitmax) {
-      info->relres = std::sqrt(rr / xl);
-      break;
-    }
+    if (total_inner >= nitmax || outer >= 100) break;
     ++outer;
-    // inner FP32 CG: A32 delta = r, from delta = 0, R0 = -r, stop when
+    // inner FP32 CG: A32 delta = r from delta = 0 (R0 = -r), stop when
     // ||R||^2 <= inner_eps^2 ||R0||^2
-    sh->eps2xl = inner_eps * inner_eps;
-    sh->rr0 = rr;
-    sh->rr = rr;
-    sh->beta = 0.0;
-    sh->m = 0;
-    sh->nitmax = std::min(inner_nitmax, nitmax - total_inner);
-    sh->state = ST_RUNNING;
+    init_cg_state(sh, inner_eps * inner_eps, 1, std::min(inner_nitmax, nitmax - total_inner));
     if (write_state(c)) return BHIP_EDEVICE;
-    auto launch_dir = [&](const float* dold, float* dnew) {
-      EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
-      a.r = c->fr.p;
-      a.dold = dold;
-      a.dnew = dnew;
-      a.out = c->fad.p;
-      a.ticket_id = 2;
-      a.check_state = 1;
-      return launch_ebe<float, float, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
-                                                                        c->typext);
-    };
-    auto launch_upd = [&](const float* d) {
-      hipLaunchKernelGGL((k_cg_update<float, float>), dim3(g), dim3(kThreads), 0, c->stream,
-                         dl_next, c->fr.p, d, c->fad.p, n, c->partial.p, c->st.p, 3, 1);
-      return hipGetLastError();
-    };
-    HIP_OK(launch_dir(c->fdz.p, c->fd0.p));
-    int batch = 8;
-    for (;;) {
-      for (int k = 0; k < batch; ++k) {
-        HIP_OK(launch_upd(c->fd0.p));
-        HIP_OK(launch_dir(c->fd0.p, c->fd1.p));
-        HIP_OK(launch_upd(c->fd1.p));
-        HIP_OK(launch_dir(c->fd1.p, c->fd0.p));
-      }
-      if (read_state(c)) return BHIP_EDEVICE;
-      if (sh->state == ST_CONVERGED || sh->state == ST_MAXITER) break;
-      batch = std::min(batch * 2, 64);
-    }
-    total_inner += sh->m;
+    int rc = run_cg<float, float, float>(c, BHIP_PREC_MIXED, dl_next, c->fr.p, c->fd0.p,
+                                         c->fd1.p, c->fad.p, c->fdz.p, c->xa32.p, c->da32.p);
+    if (rc) return rc;
+    total_inner += sh->final_m;
     std::swap(dl_prev, dl_next);
-    if (outer > 100) break;
   }
-  // the latest x is in xa_; make sure it lands in x
-  if (xa_ != x) HIP_OK(hipMemcpyAsync(x, xa_, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                      c->stream));
+  if (xa_ != x)
+    HIP_OK(hipMemcpyAsync(x, xa_, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   info->niter = total_inner;
   info->inner_iters = total_inner;
   info->outer_iters = outer;
@@ -1219,59 +1282,68 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
   const int g = c->grid_nodes;
   SolveState* sh = c->st_host;
   std::memset(sh, 0, sizeof(SolveState));
-  sh->state = ST_RUNNING;
-  sh->nitmax = 0x7fffffff;
-  sh->eps2xl = -1.0;  // never converges: every launch does its full work
-  sh->rr = 1.0;
+  // never converges, never reaches NITMAX: every launch does its full work
+  init_cg_state(sh, -1.0, 0, 0x7fffffff);
+  sh->m[0] = sh->m[1] = 1;
   if (write_state(c)) return BHIP_EDEVICE;
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
   float t_dir = 0.f, t_upd = 0.f;
+  auto time_it = [&](auto&& launch, float* out) -> int {
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) HIP_OK(launch());
+    HIP_OK(hipEventRecord(e1, c->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(out, e0, e1));
+    return BHIP_OK;
+  };
+  int rc;
   if (precision == BHIP_PREC_MIXED) {
     EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
     a.r = c->fr.p;
     a.dold = c->fd0.p;
     a.dnew = c->fd1.p;
     a.out = c->fad.p;
-    a.ticket_id = 2;
-    a.check_state = 1;
-    HIP_OK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; ++i)
-      HIP_OK((launch_ebe<float, float, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
-                                                                         c->typext)));
-    HIP_OK(hipEventRecord(e1, c->stream));
-    HIP_OK(hipEventSynchronize(e1));
-    HIP_OK(hipEventElapsedTime(&t_dir, e0, e1));
-    HIP_OK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; ++i)
+    a.partial = part_e(c, 0);
+    a.part_in = part_u(c, 0);
+    a.n_part_in = g;
+    a.parity = 0;
+    rc = time_it([&]() {
+      return launch_ebe<float, float, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia,
+                                                                      c->typext);
+    }, &t_dir);
+    if (rc) return rc;
+    rc = time_it([&]() {
       hipLaunchKernelGGL((k_cg_update<float, float>), dim3(g), dim3(kThreads), 0, c->stream,
-                         c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, n, c->partial.p, c->st.p, 3, 0);
-    HIP_OK(hipEventRecord(e1, c->stream));
+                         c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, n, part_e(c, 0), c->L.nblocks,
+                         part_u(c, 1), c->st.p, 1);
+      return hipGetLastError();
+    }, &t_upd);
+    if (rc) return rc;
   } else {
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.r = c->vr.p;
     a.dold = c->vd0.p;
     a.dnew = c->vd1.p;
     a.out = c->vad.p;
-    a.ticket_id = 2;
-    a.check_state = 1;
-    HIP_OK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; ++i)
-      HIP_OK((launch_ebe<double, double, STAGE_CGDIR, EPI_CGAD, FIN_CGAD>(c, a, false, c->typdia,
-                                                                           c->typext)));
-    HIP_OK(hipEventRecord(e1, c->stream));
-    HIP_OK(hipEventSynchronize(e1));
-    HIP_OK(hipEventElapsedTime(&t_dir, e0, e1));
-    HIP_OK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; ++i)
+    a.partial = part_e(c, 0);
+    a.part_in = part_u(c, 0);
+    a.n_part_in = g;
+    a.parity = 0;
+    rc = time_it([&]() {
+      return launch_ebe<double, double, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia,
+                                                                        c->typext);
+    }, &t_dir);
+    if (rc) return rc;
+    rc = time_it([&]() {
       hipLaunchKernelGGL((k_cg_update<double, double>), dim3(g), dim3(kThreads), 0, c->stream,
-                         c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, n, c->partial.p, c->st.p, 3, 0);
-    HIP_OK(hipEventRecord(e1, c->stream));
+                         c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, n, part_e(c, 0), c->L.nblocks,
+                         part_u(c, 1), c->st.p, 1);
+      return hipGetLastError();
+    }, &t_upd);
+    if (rc) return rc;
   }
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipEventSynchronize(e1));
-  HIP_OK(hipEventElapsedTime(&t_upd, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (ms_dir) *ms_dir = t_dir / reps;
@@ -1297,6 +1369,271 @@ int bhip_solve(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x, double* b,
   dx.release();
   db.release();
   return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// operand preparation: DIRICH, LUMP, element matrices
+// ===========================================================================
+namespace bhip {
+
+// DIRAUX (diraux.f) after SM -= A FBOR (diri01.f:94-116): at prescribed
+// nodes SM = DA*FBOR, W = 0, F = FBOR; elsewhere W = 1.
+__global__ void k_diraux(double* __restrict__ sm, const double* __restrict__ ax,
+                         const double* __restrict__ da, const double* __restrict__ fbor,
+                         const int32_t* __restrict__ limdir, int kdir, double* __restrict__ w,
+                         double* __restrict__ f, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = sm[i] - ax[i];
+    double wi = 1.0;
+    if (limdir[i] == kdir) {
+      s = da[i] * fbor[i];
+      wi = 0.0;
+      f[i] = fbor[i];
+    }
+    sm[i] = s;
+    w[i] = wi;
+  }
+}
+
+template <typename T>
+__global__ void k_gather_any(T* __restrict__ out, const T* __restrict__ in,
+                             const int32_t* __restrict__ idx, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = in[idx[i]];
+}
+
+// Element matrices per slot (each element instance computes the same values)
+// and the assembly of their diagonal terms onto owned nodes in LDS, colour
+// by colour (same deterministic scheme as the mat-vec).
+//   kind 100: MT01AA mass (mt01aa.f:83-106)
+//   kind 1  : MT02AA P1 isotropic viscosity (mt02aa.f:247-290)
+//   kind 2  : MT02AA 'UV' (mt02aa.f:200-240)
+// accumulate: OM 'M=M+N' (else 'M=N').
+__global__ void __launch_bounds__(kThreads)
+k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
+           const double* __restrict__ xel, const double* __restrict__ yel, int nelem,
+           const double* __restrict__ u_int, const double* __restrict__ v_int,
+           const int32_t* __restrict__ slot_elem, const uint16_t* __restrict__ lid,
+           int nslot_pad, const int32_t* __restrict__ node_start,
+           const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
+           const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
+           double* __restrict__ da, int accumulate) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* acc = reinterpret_cast<double*>(smem_raw);
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int n0 = node_start[b], nown = node_start[b + 1] - n0;
+  const int h0 = halo_start[b];
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) acc[l] = 0.0;
+  __syncthreads();
+  const int32_t* co = color_off + (size_t)b * (kMaxColors + 1);
+  const int send = co[kMaxColors];
+  for (int c = 0; c < kMaxColors; ++c) {
+    const int s0 = co[c];
+    if (s0 >= send) break;
+    const int s1 = co[c + 1];
+    for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+      const int e = slot_elem[s];
+      int l[3], g[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        l[k] = lid[(size_t)k * nslot_pad + s];
+        g[k] = l[k] < nown ? n0 + l[k] : halo[h0 + l[k] - nown];
+      }
+      double a12, a13, a23, t1, t2, t3;
+      if (kind == 100) {
+        const double det = surfac[e] * (xmul / 12.0);
+        a12 = a13 = a23 = det;
+        t1 = t2 = t3 = det + det;
+      } else {
+        const double x2 = xel[(size_t)nelem + e], x3 = xel[2 * (size_t)nelem + e];
+        const double y2 = yel[(size_t)nelem + e], y3 = yel[2 * (size_t)nelem + e];
+        if (kind == 2) {
+          const double g1 = v_int[g[0]], g2 = v_int[g[1]], g3 = v_int[g[2]];
+          const double coef1 = (xmul / 48.0) / surfac[e];
+          const double g123 = g1 + g2 + g3;
+          const double coef2 = u_int[g[0]] * (g1 + g123) + u_int[g[1]] * (g2 + g123) +
+                               u_int[g[2]] * (g3 + g123);
+          a12 = (y3 * (y2 - y3) + x3 * (x2 - x3)) * coef2 * coef1;
+          a13 = -(y2 * (y2 - y3) + x2 * (x2 - x3)) * coef2 * coef1;
+          a23 = -(y2 * y3 + x2 * x3) * coef2 * coef1;
+        } else {
+          const double somvx = (u_int[g[0]] + u_int[g[1]] + u_int[g[2]]) * (xmul / 12.0) /
+                               surfac[e];
+          const double x2x3 = x2 * x3 + y2 * y3;
+          const double x2aux = x2 * (-x2 + x3) + y2 * (-y2 + y3);
+          const double x3aux = x3 * (-x2 + x3) + y3 * (-y2 + y3);
+          a12 = -somvx * x3aux;
+          a13 = somvx * x2aux;
+          a23 = -somvx * x2x3;
+        }
+        t1 = -a12 - a13;
+        t2 = -a12 - a23;
+        t3 = -a13 - a23;
+      }
+      double* p12 = xa + s;
+      double* p13 = xa + (size_t)nslot_pad + s;
+      double* p23 = xa + 2 * (size_t)nslot_pad + s;
+      if (accumulate) {
+        *p12 += a12;
+        *p13 += a13;
+        *p23 += a23;
+      } else {
+        *p12 = a12;
+        *p13 = a13;
+        *p23 = a23;
+      }
+      const double tk[3] = {t1, t2, t3};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (l[k] < nown) acc[l[k]] += tk[k];
+    }
+    __syncthreads();
+  }
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+    if (accumulate)
+      da[n0 + l] += acc[l];
+    else
+      da[n0 + l] = acc[l];
+  }
+}
+
+}  // namespace bhip
+
+extern "C" {
+
+int bhip_dirich(bhip_ctx* c, double* f, double* sm, const double* fbor, const int32_t* limdir,
+                int kdir) {
+  if (!c || !c->have_matrix || !f || !sm || !fbor || !limdir) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int n = L.npoin;
+  const int g = c->grid_nodes;
+  DevBuf<double> hf, hs, hb;
+  DevBuf<int32_t> hl, il;
+  HIP_OK(hf.alloc(n));
+  HIP_OK(hs.alloc(n));
+  HIP_OK(hb.alloc(n));
+  HIP_OK(hl.alloc(n));
+  HIP_OK(il.alloc(n));
+  HIP_OK(hipMemcpyAsync(hf.p, f, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(hs.p, sm, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(hb.p, fbor, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(hl.p, limdir, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  // internal order: vio0 = FBOR, vd0 = SM, vd1 = F, il = LIMDIR
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio0.p, hb.p,
+                     c->perm.p, n);
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vd0.p, hs.p,
+                     c->perm.p, n);
+  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vd1.p, hf.p,
+                     c->perm.p, n);
+  hipLaunchKernelGGL(k_gather_any<int32_t>, dim3(g), dim3(kThreads), 0, c->stream, il.p, hl.p,
+                     c->perm.p, n);
+  HIP_OK(hipGetLastError());
+  // WORK2 = A FBOR (diri01.f:103-110), SM -= WORK2, DIRAUX
+  int rc = matvec_internal(c, BHIP_OP_AY, 0.0, c->vio0.p, c->vio0.p, c->vr.p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_diraux, dim3(g), dim3(kThreads), 0, c->stream, c->vd0.p, c->vr.p,
+                     c->da.p, c->vio0.p, il.p, kdir, c->vdinv.p, c->vd1.p, n);
+  HIP_OK(hipGetLastError());
+  // OM 'M=DMD' with W and the diagonal declared '0' (diri01.f:131-136)
+  if (c->ncoef > 0) {
+    if (L.ndp == 3)
+      hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->xa.p,
+                         c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
+                         c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
+    else
+      hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->xa.p,
+                         c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
+                         c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
+    HIP_OK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, hs.p, c->vd0.p,
+                     c->perm.p, n);
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, hf.p, c->vd1.p,
+                     c->perm.p, n);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(sm, hs.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(f, hf.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+int bhip_lump(bhip_ctx* c, double xmul, double* diag) {
+  if (!c || !c->have_matrix || !diag) return BHIP_EBADARG;
+  const int n = c->L.npoin;
+  const int g = c->grid_nodes;
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio0.p, xmul,
+                     (long)n);
+  HIP_OK(hipGetLastError());
+  int rc = matvec_internal(c, BHIP_OP_AY, 0.0, c->vio0.p, c->vio0.p, c->vr.p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p,
+                     c->vr.p, c->perm.p, n);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(diag, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
+                       const double* surfac, const double* u, const double* v,
+                       int accumulate) {
+  if (!c || !surfac || c->L.ndp != 3) return BHIP_EBADARG;
+  if (kind != 100 && (!xel || !yel || !u)) return BHIP_EBADARG;
+  if (kind == 2 && !v) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int n = L.npoin, ne = L.nelem;
+  if (accumulate && (!c->have_matrix || c->typext == 'Q')) return BHIP_EBADARG;
+  if (!c->have_matrix) {
+    HIP_OK(c->da.alloc(n));
+    HIP_OK(c->xa.alloc((size_t)6 * L.nslot_pad));
+  }
+  DevBuf<double> dS, dX, dY, dU, dV;
+  HIP_OK(dS.alloc(ne));
+  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  if (kind != 100) {
+    HIP_OK(dX.alloc((size_t)3 * ne));
+    HIP_OK(dY.alloc((size_t)3 * ne));
+    HIP_OK(dU.alloc(n));
+    HIP_OK(dV.alloc(n));
+    HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->vio0.p, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dU.p,
+                       c->vio0.p, c->perm.p, n);
+    if (v) {
+      HIP_OK(hipMemcpyAsync(c->vio1.p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                         dV.p, c->vio1.p, c->perm.p, n);
+    }
+    HIP_OK(hipGetLastError());
+  }
+  const size_t lds = (size_t)L.max_owned * sizeof(double);
+  hipLaunchKernelGGL(k_elem_tri, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
+                     dS.p, dX.p, dY.p, ne, dU.p, v ? dV.p : dU.p, c->slot_elem.p, c->lid.p,
+                     L.nslot_pad, c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
+                     L.nblocks, c->xa.p, c->da.p, accumulate);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  // MATRIY sets TYPDIA 'Q', TYPEXT 'S' (matriy.f:375-376); 'M=M+N' of two
+  // symmetric matrices stays symmetric
+  c->typdia = 'Q';
+  c->typext = 'S';
+  c->ncoef = 3;
+  c->have_matrix = true;
+  return BHIP_OK;
+}
+
+int bhip_matrix_mt01aa(bhip_ctx* c, double xmul, const double* surfac, int accumulate) {
+  return elem_common(c, 100, xmul, nullptr, nullptr, surfac, nullptr, nullptr, accumulate);
+}
+
+int bhip_matrix_mt02aa(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
+                       const double* surfac, const double* u, const double* v, int accumulate) {
+  if (kind != 1 && kind != 2) return BHIP_EBADARG;
+  return elem_common(c, kind, xmul, xel, yel, surfac, u, v, accumulate);
 }
 
 }  // extern "C"

@@ -1,0 +1,102 @@
+"""Writes the golden vectors of the reference's own BIEF (oracle/_ref, the
+flang build of sources/utils/bief) for the CPU oracle's tests.
+
+Run in the development container, where /root/reference is present:
+    make -C oracle ref && python tests/golden/make_golden.py
+Inputs are regenerated from their seeds by the tests; the files hold the
+reference's outputs (and the inputs, for self-containedness).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from opentelemac_amd import mesh as M  # noqa: E402
+from opentelemac_amd import system as S  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def small_case():
+    m = M.random_small_mesh(9, 7, seed=3)
+    ik, n, ne = m.ikle, m.npoin, m.nelem
+    rm = O.RefMesh(ik, n, ne)
+    rng = np.random.default_rng(20)
+    out = {"x": m.x, "y": m.y, "ikle": ik}
+    xe, ye = m.xel_yel()
+    s = m.surfac()
+    u = rng.uniform(1, 2, n)
+    v = rng.uniform(1, 3, n)
+    ue = rng.uniform(1, 2, ne)
+    out["u"], out["v"], out["ue"] = u, v, ue
+    out["mt01aa_t"], out["mt01aa_xm"] = O.ref_mt01aa(s, 0.7)
+    for kind, uu in ((0, ue), (1, u), (2, u)):
+        for mon in (0, 1):
+            t, xm = O.ref_mt02aa(kind, xe, ye, s, ik, uu, v, xmul=-1.3, mon=bool(mon), npoin=n)
+            out[f"mt02aa_{kind}_{mon}_t"] = t
+            out[f"mt02aa_{kind}_{mon}_xm"] = xm
+    da = rng.uniform(2, 3, n)
+    out["da"] = da
+    for typext, nc in (("S", 3), ("Q", 6)):
+        xa = rng.uniform(-1, 1, (nc, ne))
+        out[f"xa_{typext}"] = xa
+        for typdia in "QI0":
+            A = O.Matrix(da, xa, typdia, typext)
+            for op, code in O.OPS.items():
+                y = rng.normal(size=n)
+                x0 = rng.normal(size=n)
+                key = f"mv_{typdia}{typext}_{code}"
+                out[key + "_y"] = y
+                out[key + "_x0"] = x0
+                out[key] = O.ref_matvec(rm, op, A, y, x0, c=0.37)
+    # SPD system: mass + diffusion
+    t1, x1 = O.mt01aa(s, 1.0)
+    t2, x2 = O.mt02aa(1, xe, ye, s, ik, u, xmul=2.0)
+    Ada = O.assemble_diag(t1 + t2, ik, n)
+    Axa = x1 + x2
+    b = rng.normal(size=n)
+    out["spd_da"], out["spd_xa"], out["spd_b"] = Ada, Axa, b
+    A = O.Matrix(Ada, Axa, "Q", "S")
+    for slv in (1, 6, 7):
+        for precon in (0, 2, 5):
+            x, info, A2, b2 = O.ref_solve(rm, A, b, slv=slv, precon=precon, krylov=5,
+                                          eps=1e-10, nitmax=500)
+            key = f"solve_{slv}_{precon}"
+            out[key + "_x"] = x
+            out[key + "_niter"] = np.array(info["niter"])
+            out[key + "_relres"] = np.array(info["relres"])
+            out[key + "_da"] = A2.da
+            out[key + "_xa"] = A2.xa
+            out[key + "_b"] = b2
+            out[key + "_typdia"] = np.array(ord(A2.typdia))
+    limdir = np.where(rng.uniform(size=n) < 0.2, 5, 2).astype(np.int32)
+    fbor = np.where(limdir == 5, rng.normal(size=n), 0.0)
+    out["dir_limdir"], out["dir_fbor"] = limdir, fbor
+    f, A3, sm = O.ref_dirich(rm, A, b, fbor, limdir)
+    out["dir_f"], out["dir_da"], out["dir_xa"], out["dir_sm"] = f, A3.da, A3.xa, sm
+    out["lump"] = O.ref_lump(rm, A, 0.3)
+    return out
+
+
+def channel_case():
+    """The 20,451-node channel wave-equation step, CG + PRECON 2, eps 1e-8."""
+    m = M.channel_mesh()
+    sysd = S.wave_system(m, M.channel_depth(m), dt=1.0)
+    rm = O.RefMesh(m.ikle, m.npoin, m.nelem)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    x, info, _, _ = O.ref_solve(rm, A, sysd["b"], sysd["x0"], slv=1, precon=2, eps=1e-8,
+                                nitmax=5000)
+    return {"x": x, "niter": np.array(info["niter"]), "relres": np.array(info["relres"])}
+
+
+def main():
+    np.savez_compressed(os.path.join(HERE, "ref_small.npz"), **small_case())
+    np.savez_compressed(os.path.join(HERE, "ref_channel.npz"), **channel_case())
+    print("golden vectors written")
+
+
+if __name__ == "__main__":
+    main()

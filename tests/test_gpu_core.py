@@ -99,3 +99,48 @@ def test_solve_channel_fp64_and_mixed(gpu):
             # conditioning-scaled tolerance
             assert info["relres"] <= 1e-8
             assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("nitmax", [1, 2, 3, 7])
+def test_solve_nitmax_matches_oracle(gpu, nitmax):
+    """Non-convergence (NITMAX reached): same iterate as the reference's
+    GRACJG after exactly NITMAX iterations (gracjg.F:406-418)."""
+    m = M.random_small_mesh(12, 9, seed=11)
+    sysd = S.wave_system(m, 2.0 + m.x / 10.0, dt=5.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    x_ref, info_ref, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2,
+                                    eps=1e-14, nitmax=nitmax)
+    assert not info_ref.converged
+    ctx = bief.Context(m, block_nodes=32)
+    ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+    x, info = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-14,
+                        nitmax=nitmax)
+    assert info["status"] == "maxiter" and not info["converged"]
+    assert info["niter"] == info_ref.niter == max(nitmax, 1)
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-12
+    assert abs(info["relres"] - info_ref.relres) <= 1e-9 * info_ref.relres
+
+
+def test_solve_zero_rhs(gpu):
+    m = M.random_small_mesh(6, 5, seed=2)
+    sysd = S.wave_system(m, np.full(m.npoin, 3.0))
+    ctx = bief.Context(m)
+    ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+    x, info = ctx.solve(np.ones(m.npoin), np.zeros(m.npoin), slv=1, precon=2)
+    assert info["status"] == "zero rhs"
+    assert np.all(x == 0.0)
+
+
+def test_solve_repeatable_bitwise(gpu):
+    """Same inputs, same bits (fixed-order reductions, no atomics)."""
+    m = M.channel_mesh(100, 20, 500.0, 100.0)
+    sysd = S.wave_system(m, 1.0 + m.x / 100.0, dt=3.0)
+    outs = []
+    for _ in range(2):
+        ctx = bief.Context(m, block_nodes=128)
+        ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+        x, info = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-10)
+        outs.append((x, info["niter"]))
+    assert outs[0][1] == outs[1][1]
+    assert np.array_equal(outs[0][0], outs[1][0])

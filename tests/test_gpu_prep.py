@@ -1,0 +1,60 @@
+"""GPU parity of the operand preparation before SOLVE: DIRICH, LUMP and the
+device element matrices MT01AA / MT02AA assembled by the context."""
+import numpy as np
+import pytest
+
+from opentelemac_amd import bief, mesh as M, system as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("bn", [0, 24])
+def test_dirich_and_lump(gpu, bn):
+    m = M.random_small_mesh(12, 10, seed=21)
+    da, xa, td, te = S.wave_matrix(m, 1.0 + m.y / 3.0, dt=2.0)
+    rng = np.random.default_rng(4)
+    sm = rng.normal(size=m.npoin)
+    f0 = rng.normal(size=m.npoin)
+    limdir = np.where(rng.uniform(size=m.npoin) < 0.25, 5, 2).astype(np.int32)
+    fbor = np.where(limdir == 5, rng.normal(size=m.npoin), 0.0)
+    A = O.Matrix(da, xa, td, te)
+    fo, Ao, smo = O.dirich(A, m.ikle, sm, fbor, limdir, f=f0)
+    ctx = bief.Context(m, block_nodes=bn)
+    ctx.set_matrix(td, te, da, xa)
+    f, sm2 = ctx.dirich(f0, sm, fbor, limdir)
+    assert np.array_equal(f, fo)
+    assert np.allclose(sm2, smo, rtol=1e-13, atol=1e-13 * np.abs(smo).max())
+    _, da2, xa2 = ctx.get_matrix()
+    assert np.array_equal(da2, Ao.da) and np.array_equal(xa2, Ao.xa)
+    ctx.set_matrix(td, te, da, xa)
+    lo = O.lump(A, m.ikle, 0.3)
+    assert np.allclose(ctx.lump(0.3), lo, rtol=1e-13, atol=1e-13 * np.abs(lo).max())
+
+
+@pytest.mark.parametrize("bn", [0, 40])
+def test_device_element_matrices(gpu, bn):
+    m = M.random_small_mesh(14, 9, seed=22)
+    xe, ye = m.xel_yel()
+    s = m.surfac()
+    rng = np.random.default_rng(5)
+    u = rng.uniform(-3, -1, m.npoin)
+    v = rng.uniform(0.5, 4, m.npoin)
+    ctx = bief.Context(m, block_nodes=bn)
+    # A = M/dt (MATMAS) then A += MATDIFUV(XMUL=-1, U, V) as PROPAG builds it
+    ctx.matrix_mt01aa(0.5, s)
+    ctx.matrix_mt02aa(2, -1.0, xe, ye, s, u, v, accumulate=True)
+    td, da, xa = ctx.get_matrix()
+    t1, x1 = O.mt01aa(s, 0.5)
+    t2, x2 = O.mt02aa(2, xe, ye, s, m.ikle, u, v, xmul=-1.0)
+    da_o = O.assemble_diag(t1, m.ikle, m.npoin) + O.assemble_diag(t2, m.ikle, m.npoin)
+    assert td == "Q"
+    assert np.allclose(xa, x1 + x2, rtol=1e-14, atol=1e-14 * np.abs(x1 + x2).max())
+    assert np.allclose(da, da_o, rtol=1e-13, atol=1e-13 * np.abs(da_o).max())
+    # P1 isotropic viscosity, 'M=N'
+    ctx.matrix_mt02aa(1, 0.7, xe, ye, s, v)
+    _, da, xa = ctx.get_matrix()
+    t3, x3 = O.mt02aa(1, xe, ye, s, m.ikle, v, xmul=0.7)
+    assert np.allclose(xa, x3, rtol=1e-14, atol=1e-14 * np.abs(x3).max())
+    d3 = O.assemble_diag(t3, m.ikle, m.npoin)
+    assert np.allclose(da, d3, rtol=1e-12, atol=1e-12 * np.abs(d3).max())

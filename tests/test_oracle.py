@@ -1,0 +1,192 @@
+"""The CPU oracle pinned against the reference and against mathematics.
+
+1. Golden vectors of the reference's own BIEF (flang build of
+   sources/utils/bief, written by tests/golden/make_golden.py): the oracle
+   must reproduce them bit for bit (MATVEC all ops, MT01AA, MT02AA, SOLVE
+   with CG/CGSTAB/GMRES x PRECON 0/2/5 incl. iteration counts, DIRICH, LUMP).
+2. Closed forms and invariants that a plausible mistake would break.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from opentelemac_amd import mesh as M
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def g():
+    return np.load(os.path.join(GOLD, "ref_small.npz"))
+
+
+def test_golden_mesh_is_regenerated(g):
+    m = M.random_small_mesh(9, 7, seed=3)
+    assert np.array_equal(m.ikle, g["ikle"]) and np.array_equal(m.x, g["x"])
+
+
+def test_golden_element_matrices(g):
+    m = M.random_small_mesh(9, 7, seed=3)
+    xe, ye = m.xel_yel()
+    s = m.surfac()
+    t, xm = O.mt01aa(s, 0.7)
+    assert np.array_equal(t, g["mt01aa_t"]) and np.array_equal(xm, g["mt01aa_xm"])
+    for kind, uu in ((0, g["ue"]), (1, g["u"]), (2, g["u"])):
+        for mon in (0, 1):
+            t, xm = O.mt02aa(kind, xe, ye, s, m.ikle, uu, g["v"], xmul=-1.3, mon=bool(mon))
+            assert np.array_equal(t, g[f"mt02aa_{kind}_{mon}_t"]), (kind, mon)
+            assert np.array_equal(xm, g[f"mt02aa_{kind}_{mon}_xm"]), (kind, mon)
+
+
+def test_golden_matvec_all_ops(g):
+    ik = g["ikle"]
+    for typext in "SQ":
+        for typdia in "QI0":
+            A = O.Matrix(g["da"], g[f"xa_{typext}"], typdia, typext)
+            for op, code in O.OPS.items():
+                key = f"mv_{typdia}{typext}_{code}"
+                got = O.matvec(op, A, ik, g[key + "_y"], g[key + "_x0"], c=0.37)
+                assert np.array_equal(got, g[key]), key
+
+
+@pytest.mark.parametrize("slv", [1, 6, 7])
+@pytest.mark.parametrize("precon", [0, 2, 5])
+def test_golden_solve(g, slv, precon):
+    A = O.Matrix(g["spd_da"], g["spd_xa"], "Q", "S")
+    x, info, A2, b2 = O.solve(A, g["ikle"], g["spd_b"], slv=slv, precon=precon, krylov=5,
+                              eps=1e-10, nitmax=500)
+    key = f"solve_{slv}_{precon}"
+    assert info.niter == int(g[key + "_niter"])
+    assert np.array_equal(x, g[key + "_x"])
+    assert np.array_equal(A2.da, g[key + "_da"]) and np.array_equal(A2.xa, g[key + "_xa"])
+    assert np.array_equal(b2, g[key + "_b"])
+    assert ord(A2.typdia) == int(g[key + "_typdia"])
+    # the listing prints G16.7: 7 significant digits
+    assert abs(info.relres - float(g[key + "_relres"])) <= 1e-6 * float(g[key + "_relres"])
+
+
+def test_golden_dirich_lump(g):
+    A = O.Matrix(g["spd_da"], g["spd_xa"], "Q", "S")
+    f, A3, sm = O.dirich(A, g["ikle"], g["spd_b"], g["dir_fbor"], g["dir_limdir"])
+    assert np.array_equal(f, g["dir_f"]) and np.array_equal(sm, g["dir_sm"])
+    assert np.array_equal(A3.da, g["dir_da"]) and np.array_equal(A3.xa, g["dir_xa"])
+    assert np.array_equal(O.lump(A, g["ikle"], 0.3), g["lump"])
+
+
+def test_golden_channel_solve():
+    from opentelemac_amd import system as S
+    z = np.load(os.path.join(GOLD, "ref_channel.npz"))
+    m = M.channel_mesh()
+    sysd = S.wave_system(m, M.channel_depth(m), dt=1.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    x, info, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2, eps=1e-8,
+                            nitmax=5000)
+    assert info.niter == int(z["niter"])
+    assert np.array_equal(x, z["x"])
+
+
+# ---- mathematics ---------------------------------------------------------
+
+def _dense(da, xa, ikle, typext):
+    n = da.size
+    D = np.diag(da.astype(float))
+    nc = xa.shape[0]
+    for e in range(ikle.shape[1]):
+        i1, i2, i3 = ikle[:, e]
+        pairs = [(i1, i2), (i1, i3), (i2, i3)]
+        for c, (a, b) in enumerate(pairs):
+            D[a, b] += xa[c, e]
+            D[b, a] += xa[c + 3, e] if (typext == "Q" and nc == 6) else xa[c, e]
+    return D
+
+
+@pytest.mark.parametrize("typext", ["S", "Q"])
+def test_matvec_equals_dense_assembly(typext):
+    m = M.random_small_mesh(7, 6, seed=9)
+    rng = np.random.default_rng(3)
+    nc = 6 if typext == "Q" else 3
+    da = rng.uniform(1, 2, m.npoin)
+    xa = rng.normal(size=(nc, m.nelem))
+    A = O.Matrix(da, xa, "Q", typext)
+    D = _dense(da, xa, m.ikle, typext)
+    y = rng.normal(size=m.npoin)
+    assert np.allclose(O.matvec("X=AY", A, m.ikle, y), D @ y, rtol=1e-13, atol=1e-13)
+    assert np.allclose(O.matvec("X=TAY", A, m.ikle, y), D.T @ y, rtol=1e-13, atol=1e-13)
+
+
+def test_mass_matrix_closed_form():
+    m = M.random_small_mesh(6, 6, seed=4)
+    s = m.surfac()
+    t, xm = O.mt01aa(s, 1.0)
+    # S/12 off-diagonal, S/6 diagonal; sum of all entries = total area
+    assert np.allclose(xm, s / 12.0, rtol=1e-15) and np.allclose(t, s / 6.0, rtol=1e-15)
+    total = t.sum() + 2 * xm.sum()
+    assert np.isclose(total, s.sum(), rtol=1e-13)
+
+
+def test_stiffness_textbook_and_invariants():
+    m = M.random_small_mesh(6, 5, seed=8)
+    xe, ye = m.xel_yel()
+    s = m.surfac()
+    u = np.full(m.npoin, 1.0)
+    t, xm = O.mt02aa(1, xe, ye, s, m.ikle, u, xmul=1.0)
+    # textbook P1 stiffness K_ij = (b_i b_j + c_i c_j) / (4 S)
+    x = m.x[m.ikle]
+    y = m.y[m.ikle]
+    b = np.stack([y[1] - y[2], y[2] - y[0], y[0] - y[1]])
+    c = np.stack([x[2] - x[1], x[0] - x[2], x[1] - x[0]])
+    K = lambda i, j: (b[i] * b[j] + c[i] * c[j]) / (4 * s)  # noqa: E731
+    assert np.allclose(xm[0], K(0, 1), rtol=1e-12, atol=1e-12)
+    assert np.allclose(xm[1], K(0, 2), rtol=1e-12, atol=1e-12)
+    assert np.allclose(xm[2], K(1, 2), rtol=1e-12, atol=1e-12)
+    assert np.allclose(t[0], K(0, 0), rtol=1e-12)
+    # row sums vanish: constants are in the kernel
+    assert np.allclose(t[0] + xm[0] + xm[1], 0, atol=1e-12)
+    # 'UV' with U=1 and V=1: (sum U_k G_k + sum U sum G)/12 = (3 + 9)/12 = 1
+    t2, xm2 = O.mt02aa(2, xe, ye, s, m.ikle, u, u, xmul=1.0)
+    assert np.allclose(xm2, xm, rtol=1e-13, atol=1e-14)
+
+
+def test_solve_matches_direct_solve():
+    from opentelemac_amd import system as S
+    m = M.random_small_mesh(10, 8, seed=6)
+    sysd = S.wave_system(m, 1.0 + m.x / 5.0, dt=3.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    D = _dense(sysd["da"], sysd["xa"], m.ikle, "S")
+    xd = np.linalg.solve(D, sysd["b"])
+    for slv in (1, 6, 7):
+        x, info, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=slv, precon=2,
+                                krylov=5, eps=1e-13, nitmax=2000)
+        assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd), slv
+
+
+def test_dmd_scaling_unit_diagonal():
+    from opentelemac_amd import system as S
+    m = M.random_small_mesh(8, 6, seed=12)
+    sysd = S.wave_system(m, 2.0 + m.y, dt=2.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    x, info, A2, b2 = O.solve(A, m.ikle, sysd["b"], slv=1, precon=5, eps=1e-8)
+    # PRECON 5 keeps TYPDIA 'Q' with D A D: the scaled diagonal is exactly +-1
+    assert A2.typdia == "Q"
+    assert np.allclose(np.abs(A2.da), 1.0, rtol=1e-15)
+    D0 = _dense(sysd["da"], sysd["xa"], m.ikle, "S")
+    d = 1 / np.sqrt(np.abs(sysd["da"]))
+    D2 = _dense(A2.da, A2.xa, m.ikle, "S")
+    assert np.allclose(D2, d[:, None] * D0 * d[None, :], rtol=1e-13, atol=1e-15)
+    assert np.allclose(b2, d * sysd["b"], rtol=1e-15)
+
+
+def test_planted_mistake_is_caught():
+    """A wrong coefficient map (21 <-> 12 swapped) must fail the dense check:
+    the transposed product of a non-symmetric matrix differs."""
+    m = M.random_small_mesh(5, 5, seed=1)
+    rng = np.random.default_rng(0)
+    xa = rng.normal(size=(6, m.nelem))
+    A = O.Matrix(np.ones(m.npoin), xa, "Q", "Q")
+    bad = O.Matrix(np.ones(m.npoin), xa[[3, 4, 5, 0, 1, 2]], "Q", "Q")
+    y = rng.normal(size=m.npoin)
+    D = _dense(A.da, A.xa, m.ikle, "Q")
+    assert np.allclose(O.matvec("X=AY", A, m.ikle, y), D @ y, atol=1e-12)
+    assert not np.allclose(O.matvec("X=AY", bad, m.ikle, y), D @ y, atol=1e-6)
