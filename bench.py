@@ -9,8 +9,9 @@ from a seeded sum of 32 Gaussians; one telemac2d wave-equation step
 symmetric system solved by diagonally preconditioned CG to the 1e-8
 relative residual of the preconditioned system (GRACJG's stop rule).
 
-A step = load the assembled EBE matrix into the solver (device to device:
-SOLVE consumes it, as BIEF's does) + one SOLVE, data resident in HBM.
+A step = restore the assembled EBE matrix from its device copy (SOLVE
+consumes it, as BIEF's does; telemac2d keeps a copy, propag.F:1643) + one
+SOLVE, data resident in HBM.
 `value` is mixed-precision solves/s (FP32 EBE CG inside FP64 iterative
 refinement); the FP64 path is timed the same way and reported beside it.
 
@@ -132,8 +133,13 @@ def main():
     x = torch.empty_like(x00)
     b = torch.empty_like(b0)
 
+    ctx.set_matrix("Q", "S", da0, xa0)
+    ctx.save_matrix()
+
     def step(precision):
-        ctx.set_matrix("Q", "S", da0, xa0)
+        # SOLVE consumes A (D A D in place): restore the assembled copy first,
+        # as telemac2d copies AM1 before its SOLVE (propag.F:1643)
+        ctx.restore_matrix()
         x.copy_(x00)
         b.copy_(b0)
         _, info = ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=precision,

@@ -124,6 +124,12 @@ int bhip_matrix_set(bhip_ctx* ctx, char typdia, char typext, const double* da,
                     const double* xa);
 int bhip_matrix_set_dev(bhip_ctx* ctx, char typdia, char typext,
                         const double* da_dev, const double* xa_dev);
+/* OM 'M=N' between the context's matrix and one saved copy in the same
+ * device layout (sources/utils/bief/om.f; telemac2d copies AM1 before SOLVE
+ * overwrites it, propag.F:1643): save copies the current matrix aside,
+ * restore copies it back (device to device, queued on the context stream). */
+int bhip_matrix_save(bhip_ctx* ctx);
+int bhip_matrix_restore(bhip_ctx* ctx);
 /* Reads the context's (possibly preconditioned) matrix back in BIEF layout;
  * *typdia receives the current TYPDIA. */
 int bhip_matrix_get(bhip_ctx* ctx, char* typdia, double* da, double* xa);

@@ -89,6 +89,8 @@ def lib():
         L.bhip_solve.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P, ctypes.POINTER(SolveInfo)]
         L.bhip_solve_dev.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
                                      ctypes.POINTER(SolveInfo)]
+        L.bhip_matrix_save.argtypes = [_P]
+        L.bhip_matrix_restore.argtypes = [_P]
         L.bhip_time_cg_kernels.argtypes = [_P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]
         L.bhip_layout_check.argtypes = [_I, _I, _I, _P, _P, _P, _I, _I, _P]
         for name, args in (("bhip_dirich", [_P, _P, _P, _P, _P, _I]),
@@ -206,6 +208,14 @@ class Context:
                                        xa.ctypes.data if xa is not None else None)
         _check(rc, "bhip_matrix_set")
         self.typdia, self.typext = typdia, typext
+
+    def save_matrix(self):
+        """OM 'M=N' into the context's saved copy (device to device)."""
+        _check(lib().bhip_matrix_save(self._h), "bhip_matrix_save")
+
+    def restore_matrix(self):
+        """OM 'M=N' back from the saved copy (queued on the context stream)."""
+        _check(lib().bhip_matrix_restore(self._h), "bhip_matrix_restore")
 
     def get_matrix(self):
         npair = self.ndp * (self.ndp - 1) // 2

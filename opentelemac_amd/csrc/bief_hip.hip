@@ -70,7 +70,18 @@ struct SolveState {
   int use_rr0;      // threshold = eps2xl * rr0
   double final_rr;  // RMRM at exit
   int final_flag;   // set by the update kernel that applied the last X update
-  int pad2;
+  int zero_rhs;     // ||b'|| < 1e-15 (gracjg.F:194-202)
+  double xl_raw;    // ||b'||^2
+  // mixed precision outer loop (device-side control)
+  double eps2_outer;   // outer stop threshold on the FP64 ||r'||^2
+  double rr_outer;     // last FP64 ||r'||^2
+  double inner_eps2;   // inner relative threshold
+  int inner_nitmax;
+  int outer_done;      // 1 when the FP64 residual met eps2_outer
+  int stalled;         // an outer test found its inner solve unfinished
+  int outer_count;     // inner solves started
+  int inner_total;     // inner iterations of finished inner solves
+  int pad3;
 };
 
 // ---------------------------------------------------------------------------
@@ -123,7 +134,8 @@ enum : int { DIAG_ZERO = 0, DIAG_ONE = 1, DIAG_Q = 2 };
 // how the multiplied vector is obtained for each local node
 enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2 };
 // what happens at owned nodes after the element loop
-enum : int { EPI_OP = 0, EPI_RESID = 1, EPI_CGAD = 2, EPI_RESID_MIXED = 3 };
+enum : int { EPI_OP = 0, EPI_RESID = 1, EPI_CGAD = 2, EPI_RESID_MIXED = 3, EPI_DOTW = 4,
+             EPI_DOT2 = 5 };
 // whether the kernel writes a partial sum per workgroup / runs the CG control
 enum : int { FIN_NONE = 0, FIN_PARTIAL = 1, FIN_CG = 2 };
 
@@ -154,6 +166,8 @@ struct EbeArgs {
   double c;
   int op;
   const double* b;            // RESID: right-hand side (FP64)
+  const T* w;                 // DOTW: out = A y, partial w.out
+  double* partial2;           // DOT2: out = A y, partials out.y and out.out
   float* r32;                 // RESID_MIXED: inner residual -(A x - b)
   float* zero32;              // RESID_MIXED: inner solution reset
   // reductions / state
@@ -162,6 +176,9 @@ struct EbeArgs {
   int n_part_in;
   SolveState* st;
   int parity;                 // FIN_CG: parity of the iteration
+  int skip_if_stalled;        // first direction kernel of a scheduled inner solve
+  int skip_if_done;           // kernels of a running solve after it finished
+  int skip_if_outer_done;     // mixed outer residual: skip once the solve is done
 };
 
 // column of the coefficient array holding A(i, j), i != j
@@ -174,18 +191,101 @@ __host__ __device__ constexpr int coef_col(int i, int j) {
   return (i < j || !SYMQ) ? idx : npair + idx;
 }
 
+// element contributions W_k = sum_{j != k} A(k, j) Y(i_j), j increasing
+// (mv0303.f:123-130, mv0606.f:117-200), scattered into the owned nodes' LDS
+// accumulators
+template <typename T, int NDP, bool SYMQ, bool TRANS>
+__device__ inline void slot_apply(const int (&l)[NDP], const T* cf, const T* ys, T* acc,
+                                  int nown) {
+  T yv[NDP];
+#pragma unroll
+  for (int k = 0; k < NDP; ++k) yv[k] = ys[l[k]];
+#pragma unroll
+  for (int k = 0; k < NDP; ++k) {
+    T w = (T)0;
+    bool first = true;
+#pragma unroll
+    for (int j = 0; j < NDP; ++j) {
+      if (j == k) continue;
+      const int q = TRANS ? coef_col<NDP, SYMQ>(j, k) : coef_col<NDP, SYMQ>(k, j);
+      const T t = cf[q] * yv[j];
+      w = first ? t : w + t;
+      first = false;
+    }
+    if (l[k] < nown) acc[l[k]] += w;
+  }
+}
+
+// Element data of one colour phase for one thread (its first slot of the
+// phase; phases larger than the workgroup are finished by a loop).
+template <typename T, int NDP, int NC>
+struct PhaseSlot {
+  int l[NDP];
+  T cf[NC];
+  bool valid;
+};
+
+template <typename T, typename TA, int NDP, int NC>
+__device__ inline void load_phase(PhaseSlot<T, NDP, NC>& ps, int c, const int* sh_co, int se,
+                                  const uint16_t* __restrict__ lid,
+                                  const TA* __restrict__ xa, int nslot_pad) {
+  const int s = (c < kMaxColors) ? sh_co[c] + (int)threadIdx.x : se;
+  ps.valid = (c < kMaxColors) && s < sh_co[c + 1] && s < se;
+  if (ps.valid) {
+#pragma unroll
+    for (int k = 0; k < NDP; ++k) ps.l[k] = lid[(size_t)k * nslot_pad + s];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) ps.cf[q] = (T)xa[(size_t)q * nslot_pad + s];
+  }
+}
+
+// PF > 0: the colour phases are software pipelined: each thread keeps the
+// element data of the next PF phases in flight in registers, so a phase
+// waits on loads issued PF phases earlier instead of paying a full HBM round
+// trip behind every barrier. PF = 0: loads inside each phase.
 template <typename T, typename TA, int NDP, bool SYMQ, bool TRANS, int DIAG, int STAGE,
-          int EPI, int FIN>
+          int EPI, int FIN, int PF>
 __global__ void __launch_bounds__(kThreads)
 k_ebe(EbeArgs<T, TA> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
+  __shared__ int sh_co[kMaxColors + 1];
+  constexpr int npair = NDP * (NDP - 1) / 2;
+  constexpr int ncoef = SYMQ ? 2 * npair : npair;
+
+  // XCD-aware mapping: workgroup g is dispatched to XCD g % 8; give each XCD
+  // a contiguous range of node blocks so blocks sharing halo nodes share an
+  // L2. Bijective for any block count.
+  const int nb = a.nblocks;
+  int b;
+  {
+    const int g = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = g & 7, k = g >> 3;
+    b = xcd * q + (xcd < r ? xcd : r) + k;
+  }
+  if (a.skip_if_outer_done && outer_blocked(a.st)) return;
+  if (a.skip_if_done && load_relaxed(&a.st->done)) return;
+  const int32_t* co = a.color_off + (size_t)b * (kMaxColors + 1);
+  if (threadIdx.x <= kMaxColors) sh_co[threadIdx.x] = co[threadIdx.x];
+  __syncthreads();
+  const int se = sh_co[kMaxColors];
+
+  // first PF phases in flight before the vector staging
+  PhaseSlot<T, NDP, ncoef> pipe[PF > 0 ? PF : 1];
+  if constexpr (PF > 0) {
+#pragma unroll
+    for (int q = 0; q < PF; ++q)
+      load_phase<T, TA, NDP, ncoef>(pipe[q], q, sh_co, se, a.lid, a.xa, a.nslot_pad);
+  }
+
   double cg_beta = 0.0;
   if constexpr (FIN == FIN_CG) {
     // E_m: RMRM from the update kernel's partials, stop test (gracjg.F:306-324,
     // and the strict test of the initial residual, gracjg.F:219), BETA
     SolveState* st = a.st;
+    if (a.skip_if_stalled && st->stalled) return;
     if (load_relaxed(&st->done)) return;
     if (load_relaxed(&st->final_flag)) {
       // the update kernel before this one applied the last X update
@@ -213,17 +313,6 @@ k_ebe(EbeArgs<T, TA> a) {
     }
   }
 
-  // XCD-aware mapping: workgroup g is dispatched to XCD g % 8; give each XCD
-  // a contiguous range of node blocks so blocks sharing halo nodes share an
-  // L2. Bijective for any block count.
-  const int nb = a.nblocks;
-  int b;
-  {
-    const int g = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = g & 7, k = g >> 3;
-    b = xcd * q + (xcd < r ? xcd : r) + k;
-  }
   const int n0 = a.node_start[b], n1 = a.node_start[b + 1];
   const int nown = n1 - n0;
   const int h0 = a.halo_start[b], h1 = a.halo_start[b + 1];
@@ -249,59 +338,60 @@ k_ebe(EbeArgs<T, TA> a) {
       if (owned) a.xnew[g] = xv;
     }
     ys[l] = v;
-  }
-  __syncthreads();
-  // ---- diagonal term ------------------------------------------------------
-  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
-    T v;
-    if constexpr (DIAG == DIAG_Q) v = (T)a.da[n0 + l] * ys[l];
-    else if constexpr (DIAG == DIAG_ONE) v = ys[l];
-    else v = (T)0;
-    acc[l] = v;
+    // diagonal term (mv0303.f:150-160)
+    if (owned) {
+      T dv;
+      if constexpr (DIAG == DIAG_Q) dv = (T)a.da[g] * v;
+      else if constexpr (DIAG == DIAG_ONE) dv = v;
+      else dv = (T)0;
+      acc[l] = dv;
+    }
   }
   __syncthreads();
   // ---- elements, colour by colour ----------------------------------------
-  const int32_t* co = a.color_off + (size_t)b * (kMaxColors + 1);
-  const int send = co[kMaxColors];
-  int c = 0;
-  while (true) {
-    const int s0 = co[c];
-    if (s0 >= send) break;
-    int s1 = co[c + 1];
-    for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-      int l[NDP];
-      T yv[NDP];
+  if constexpr (PF > 0) {
+    for (int c0 = 0; c0 < kMaxColors; c0 += PF) {
+      if (sh_co[c0] >= se) break;
 #pragma unroll
-      for (int k = 0; k < NDP; ++k) {
-        l[k] = a.lid[(size_t)k * a.nslot_pad + s];
-        yv[k] = ys[l[k]];
-      }
-      constexpr int npair = NDP * (NDP - 1) / 2;
-      constexpr int ncoef = SYMQ ? 2 * npair : npair;
-      T cf[ncoef];
+      for (int q = 0; q < PF; ++q) {
+        const int c = c0 + q;
+        if (c < kMaxColors && sh_co[c] < se) {
+          if (pipe[q].valid) slot_apply<T, NDP, SYMQ, TRANS>(pipe[q].l, pipe[q].cf, ys, acc, nown);
+          // slots beyond the first kThreads of a large phase
+          for (int s = sh_co[c] + (int)threadIdx.x + kThreads; s < sh_co[c + 1];
+               s += kThreads) {
+            int l[NDP];
+            T cf[ncoef];
 #pragma unroll
-      for (int q = 0; q < ncoef; ++q) cf[q] = (T)a.xa[(size_t)q * a.nslot_pad + s];
+            for (int k = 0; k < NDP; ++k) l[k] = a.lid[(size_t)k * a.nslot_pad + s];
 #pragma unroll
-      for (int k = 0; k < NDP; ++k) {
-        T w = (T)0;
-        bool first = true;
-#pragma unroll
-        for (int j = 0; j < NDP; ++j) {
-          if (j == k) continue;
-          const int q = TRANS ? coef_col<NDP, SYMQ>(j, k) : coef_col<NDP, SYMQ>(k, j);
-          const T t = cf[q] * yv[j];
-          w = first ? t : w + t;
-          first = false;
+            for (int qq = 0; qq < ncoef; ++qq) cf[qq] = (T)a.xa[(size_t)qq * a.nslot_pad + s];
+            slot_apply<T, NDP, SYMQ, TRANS>(l, cf, ys, acc, nown);
+          }
+          // refill this pipeline stage with phase c + PF before the barrier
+          load_phase<T, TA, NDP, ncoef>(pipe[q], c + PF, sh_co, se, a.lid, a.xa, a.nslot_pad);
         }
-        if (l[k] < nown) acc[l[k]] += w;
+        __syncthreads();
       }
     }
-    __syncthreads();
-    ++c;
-    if (c >= kMaxColors) break;
+  } else {
+    for (int c = 0; c < kMaxColors; ++c) {
+      const int c0 = sh_co[c], c1 = sh_co[c + 1];
+      if (c0 >= se) break;
+      for (int s = c0 + threadIdx.x; s < c1; s += blockDim.x) {
+        int l[NDP];
+        T cf[ncoef];
+#pragma unroll
+        for (int k = 0; k < NDP; ++k) l[k] = a.lid[(size_t)k * a.nslot_pad + s];
+#pragma unroll
+        for (int q = 0; q < ncoef; ++q) cf[q] = (T)a.xa[(size_t)q * a.nslot_pad + s];
+        slot_apply<T, NDP, SYMQ, TRANS>(l, cf, ys, acc, nown);
+      }
+      __syncthreads();
+    }
   }
   // ---- epilogue at owned nodes -------------------------------------------
-  double part = 0.0;
+  double part = 0.0, part2 = 0.0;
   for (int l = threadIdx.x; l < nown; l += blockDim.x) {
     const int g = n0 + l;
     const T v = acc[l];
@@ -323,6 +413,13 @@ k_ebe(EbeArgs<T, TA> a) {
     } else if constexpr (EPI == EPI_CGAD) {
       a.out[g] = v;
       part += (double)ys[l] * (double)v;
+    } else if constexpr (EPI == EPI_DOTW) {
+      a.out[g] = v;
+      part += (double)a.w[g] * (double)v;
+    } else if constexpr (EPI == EPI_DOT2) {
+      a.out[g] = v;
+      part += (double)v * (double)ys[l];
+      part2 += (double)v * (double)v;
     } else {  // EPI_RESID_MIXED: FP64 residual, inner RHS in FP32
       const double rv = (double)v - a.b[g];
       a.r32[g] = (float)(-rv);
@@ -333,6 +430,10 @@ k_ebe(EbeArgs<T, TA> a) {
   if constexpr (FIN != FIN_NONE) {
     double s = block_sum(part, sh_red);
     if (threadIdx.x == 0) a.partial[b] = s;
+    if constexpr (EPI == EPI_DOT2) {
+      double s2 = block_sum(part2, sh_red);
+      if (threadIdx.x == 0) a.partial2[b] = s2;
+    }
   }
 }
 
@@ -402,6 +503,92 @@ __global__ void k_reduce_into(const double* __restrict__ partial, int n, double*
   if (threadIdx.x == 0) *dst = s;
 }
 
+// the mixed outer step must wait: solved, stalled, or its inner solve still running
+__device__ inline bool outer_blocked(const SolveState* st) {
+  return st->outer_done || st->stalled || (st->outer_count > 0 && st->done == 0);
+}
+
+__device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int nitmax) {
+  st->eps2xl = eps2xl;
+  st->use_rr0 = use_rr0;
+  st->nitmax = nitmax < 1 ? 1 : nitmax;
+  st->m[0] = st->m[1] = -1;
+  st->status[0] = st->status[1] = ST_RUNNING;
+  st->rr[0] = st->rr[1] = 1.0;
+  st->final_flag = 0;
+  st->final_m = 0;
+  st->final_rr = 0.0;
+  st->done = 0;
+}
+
+// GRACJG set-up (gracjg.F:189-209) from the partial sums of ||b'||^2: XL,
+// the zero right-hand-side exit, the relative/absolute switch; then the
+// state of a fresh CG solve (FP64) or of the mixed outer loop.
+__global__ void k_cg_setup(const double* __restrict__ partial, int n, double eps, int nitmax,
+                           int mixed, double inner_eps, int inner_nitmax, SolveState* st) {
+  __shared__ double sh_red[kWaves];
+  __shared__ double sh_b;
+  const double xl_raw = reduce_partials(partial, n, sh_red, &sh_b);
+  if (threadIdx.x != 0) return;
+  const double xl = xl_raw < 1.0 ? 1.0 : xl_raw;
+  st->xl_raw = xl_raw;
+  st->xl = xl;
+  st->zero_rhs = sqrt(xl_raw) < 1e-15;
+  st->outer_done = 0;
+  st->stalled = 0;
+  st->outer_count = 0;
+  st->inner_total = 0;
+  st->rr_outer = 0.0;
+  if (!mixed) {
+    reset_cg(st, eps * eps * xl, 0, nitmax);
+  } else {
+    st->eps2_outer = eps * eps * xl;
+    st->inner_eps2 = inner_eps * inner_eps;
+    st->inner_nitmax = inner_nitmax;
+    reset_cg(st, inner_eps * inner_eps, 1, inner_nitmax);
+    st->done = ST_CONVERGED;  // no inner solve until an outer test starts one
+  }
+  if (st->zero_rhs) {
+    st->done = ST_CONVERGED;
+    st->outer_done = 1;
+    st->final_m = 0;
+    st->final_rr = 0.0;
+  }
+}
+
+// Mixed precision outer test: FP64 ||r'||^2 from the residual kernel's
+// partials; converged -> stop everything, else start the next inner solve
+// (its state reset on the device, rr0 = ||r'||^2).
+__global__ void k_outer_check(const double* __restrict__ partial, int n, SolveState* st) {
+  __shared__ double sh_red[kWaves];
+  __shared__ double sh_b;
+  if (st->outer_done || st->stalled) return;
+  if (st->outer_count > 0 && st->done == 0) {
+    if (threadIdx.x == 0) st->stalled = 1;  // inner solve still running
+    return;
+  }
+  const double rr = reduce_partials(partial, n, sh_red, &sh_b);
+  if (threadIdx.x != 0) return;
+  if (st->outer_count > 0) st->inner_total += st->final_m;
+  st->rr_outer = rr;
+  if (rr <= st->eps2_outer) {
+    st->outer_done = 1;
+    st->done = ST_CONVERGED;
+    return;
+  }
+  st->outer_count += 1;
+  reset_cg(st, st->inner_eps2, 1, st->inner_nitmax);
+  st->rr0 = rr;
+}
+
+// mixed outer update x <- x - delta (owned nodes, in place), skipped once done
+__global__ void k_outer_xupd(double* __restrict__ x, const float* __restrict__ delta, int n,
+                             const SolveState* st) {
+  if (outer_blocked(st)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] = x[i] - (double)delta[i];
+}
+
 // out[i] = in[idx[i]]
 template <typename T>
 __global__ void k_gather(T* __restrict__ out, const T* __restrict__ in,
@@ -461,7 +648,8 @@ __global__ void k_precd1_nodes(double* __restrict__ x, double* __restrict__ b,
 // OM 'M=DMD' on element coefficients: XM(e,c) = XM(e,c)*D(IKLE(e,j))*D(IKLE(e,i))
 // for the pair (i<j) of column c (om1111.f:583-600, om4141.f), per slot.
 template <int NDP>
-__global__ void k_dmd_slots(double* __restrict__ xa, int ncoef, int nslot_pad,
+__global__ void k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, int ncoef,
+                            int nslot_pad,
                             const uint16_t* __restrict__ lid, const double* __restrict__ d,
                             const int32_t* __restrict__ node_start,
                             const int32_t* __restrict__ halo_start,
@@ -488,10 +676,14 @@ __global__ void k_dmd_slots(double* __restrict__ xa, int ncoef, int nslot_pad,
 #pragma unroll
       for (int j = i + 1; j < NDP; ++j) {
         double* p = xa + (size_t)q * nslot_pad + s;
-        *p = (*p) * dv[j] * dv[i];
+        const double v1 = (*p) * dv[j] * dv[i];
+        *p = v1;
+        if (xa32) xa32[(size_t)q * nslot_pad + s] = (float)v1;
         if (ncoef > npair) {
           double* p2 = xa + (size_t)(q + npair) * nslot_pad + s;
-          *p2 = (*p2) * dv[j] * dv[i];
+          const double v2 = (*p2) * dv[j] * dv[i];
+          *p2 = v2;
+          if (xa32) xa32[(size_t)(q + npair) * nslot_pad + s] = (float)v2;
         }
         ++q;
       }
@@ -569,10 +761,20 @@ struct bhip_ctx {
   struct Graph {
     const void* key = nullptr;
     int prec = -1;
+    int tag = 0;  // TYPDIA, TYPEXT the kernels were specialised for
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[4];
   bool use_graphs = true;
+  int prefetch_j = 0;  // register prefetch depth of the element kernels
+  int iter_hint[4] = {0, 0, 0, 0};  // FP64 CG, inner CG, inner solves, CGSTAB
+  // BiCGSTAB scalars and extra partial sums; GMRES basis and scalars
+  bhip::DevBuf<double> bicg_sc, bicg_part, gm, gm_scal;
+  // a saved copy of the matrix (OM 'M=N' target), slot layout
+  bhip::DevBuf<double> da_saved, xa_saved;
+  char typdia_saved = '0', typext_saved = '0';
+  int ncoef_saved = 0;
+  bool have_saved = false;
   bhip::SolveState* st_host = nullptr;  // pinned
   size_t lds_f64 = 0, lds_f32 = 0;
 };
@@ -607,7 +809,7 @@ static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
   return a;
 }
 
-// dispatch over (ndp, symmetry, transpose, diagonal)
+// dispatch over (ndp, symmetry, transpose, diagonal, register prefetch depth)
 template <typename T, typename TA, int STAGE, int EPI, int FIN>
 static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typdia,
                              char typext) {
@@ -616,28 +818,52 @@ static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typ
   const int diag = typdia == 'Q' ? DIAG_Q : (typdia == 'I' ? DIAG_ONE : DIAG_ZERO);
   // TYPEXT '0' is stored as zero symmetric coefficients
   const bool symq = typext == 'Q';
-#define BHIP_LAUNCH(NDP, SQ, TR, DG)                                                    \
-  hipLaunchKernelGGL((k_ebe<T, TA, NDP, SQ, TR, DG, STAGE, EPI, FIN>), grid, block, lds, \
+  const int mj = c->prefetch_j;  // pipeline depth 0..3 (triangles)
+#define BHIP_LAUNCH(NDP, SQ, TR, DG, MJ)                                                      \
+  hipLaunchKernelGGL((k_ebe<T, TA, NDP, SQ, TR, DG, STAGE, EPI, FIN, MJ>), grid, block, lds, \
                      c->stream, a)
-#define BHIP_DIAG(NDP, SQ, TR)            \
-  do {                                    \
-    if (diag == DIAG_Q)                   \
-      BHIP_LAUNCH(NDP, SQ, TR, DIAG_Q);   \
-    else if (diag == DIAG_ONE)            \
-      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ONE); \
-    else                                  \
-      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ZERO); \
+#define BHIP_MJ(NDP, SQ, TR, DG)            \
+  do {                                      \
+    if (mj == 1)                            \
+      BHIP_LAUNCH(NDP, SQ, TR, DG, 1);      \
+    else if (mj == 2)                       \
+      BHIP_LAUNCH(NDP, SQ, TR, DG, 2);      \
+    else if (mj == 3)                       \
+      BHIP_LAUNCH(NDP, SQ, TR, DG, 3);      \
+    else                                    \
+      BHIP_LAUNCH(NDP, SQ, TR, DG, 0);      \
+  } while (0)
+#define BHIP_DIAG(NDP, SQ, TR)              \
+  do {                                      \
+    if (diag == DIAG_Q)                     \
+      BHIP_MJ(NDP, SQ, TR, DIAG_Q);         \
+    else if (diag == DIAG_ONE)              \
+      BHIP_MJ(NDP, SQ, TR, DIAG_ONE);       \
+    else                                    \
+      BHIP_MJ(NDP, SQ, TR, DIAG_ZERO);      \
+  } while (0)
+#define BHIP_DIAG0(NDP, SQ, TR)              \
+  do {                                       \
+    if (diag == DIAG_Q)                      \
+      BHIP_LAUNCH(NDP, SQ, TR, DIAG_Q, 0);   \
+    else if (diag == DIAG_ONE)               \
+      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ONE, 0); \
+    else                                     \
+      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ZERO, 0); \
   } while (0)
   if (c->L.ndp == 3) {
     if (!symq) BHIP_DIAG(3, false, false);
     else if (trans) BHIP_DIAG(3, true, true);
     else BHIP_DIAG(3, true, false);
   } else {
-    if (!symq) BHIP_DIAG(6, false, false);
-    else if (trans) BHIP_DIAG(6, true, true);
-    else BHIP_DIAG(6, true, false);
+    // prisms: 15-30 coefficients per element exceed the register budget
+    if (!symq) BHIP_DIAG0(6, false, false);
+    else if (trans) BHIP_DIAG0(6, true, true);
+    else BHIP_DIAG0(6, true, false);
   }
+#undef BHIP_DIAG0
 #undef BHIP_DIAG
+#undef BHIP_MJ
 #undef BHIP_LAUNCH
   return hipGetLastError();
 }
@@ -707,6 +933,11 @@ int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const do
                      c->stream, c->partial.p, 0.0, (long)4 * c->max_part);
   HIP_OK(hipGetLastError());
   if (const char* ev = getenv("BHIP_NO_GRAPH")) c->use_graphs = !(ev[0] == '1');
+  c->prefetch_j = 2;
+  if (const char* ev = getenv("BHIP_PREFETCH")) {
+    int v = atoi(ev);
+    if (v >= 0 && v <= 3) c->prefetch_j = v;
+  }
   HIP_OK(c->st.alloc(1));
   HIP_OK(hipHostMalloc((void**)&c->st_host, sizeof(SolveState), hipHostMallocDefault));
   std::memset(c->st_host, 0, sizeof(SolveState));
@@ -726,7 +957,8 @@ int bhip_ctx_destroy(bhip_ctx* c) {
                   &c->perm, &c->first_slot})
     v->release();
   c->lid.release();
-  for (auto* v : {&c->da, &c->xa, &c->vx, &c->vb, &c->vr, &c->vd0, &c->vd1, &c->vad, &c->vdz,
+  for (auto* v : {&c->bicg_sc, &c->bicg_part, &c->gm, &c->gm_scal, &c->da_saved, &c->xa_saved,
+                  &c->da, &c->xa, &c->vx, &c->vb, &c->vr, &c->vd0, &c->vd1, &c->vad, &c->vdz,
                   &c->vdinv, &c->vio0, &c->vio1, &c->partial})
     v->release();
   for (auto* v : {&c->da32, &c->xa32, &c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1,
@@ -831,6 +1063,41 @@ int bhip_matrix_set(bhip_ctx* c, char typdia, char typext, const double* da, con
 int bhip_matrix_set_dev(bhip_ctx* c, char typdia, char typext, const double* da,
                         const double* xa) {
   return matrix_set_common(c, typdia, typext, da, xa, hipMemcpyDeviceToDevice);
+}
+
+int bhip_matrix_save(bhip_ctx* c) {
+  if (!c || !c->have_matrix) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int npair = L.ndp * (L.ndp - 1) / 2;
+  HIP_OK(c->da_saved.alloc(L.npoin));
+  HIP_OK(c->xa_saved.alloc((size_t)2 * npair * L.nslot_pad));
+  HIP_OK(hipMemcpyAsync(c->da_saved.p, c->da.p, sizeof(double) * L.npoin,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->xa_saved.p, c->xa.p,
+                        sizeof(double) * (size_t)std::max(c->ncoef, npair) * L.nslot_pad,
+                        hipMemcpyDeviceToDevice, c->stream));
+  c->typdia_saved = c->typdia;
+  c->typext_saved = c->typext;
+  c->ncoef_saved = c->ncoef;
+  c->have_saved = true;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+int bhip_matrix_restore(bhip_ctx* c) {
+  if (!c || !c->have_saved) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int npair = L.ndp * (L.ndp - 1) / 2;
+  HIP_OK(hipMemcpyAsync(c->da.p, c->da_saved.p, sizeof(double) * L.npoin,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->xa.p, c->xa_saved.p,
+                        sizeof(double) * (size_t)std::max(c->ncoef_saved, npair) * L.nslot_pad,
+                        hipMemcpyDeviceToDevice, c->stream));
+  c->typdia = c->typdia_saved;
+  c->typext = c->typext_saved;
+  c->ncoef = c->ncoef_saved;
+  c->have_matrix = true;
+  return BHIP_OK;
 }
 
 int bhip_matrix_get(bhip_ctx* c, char* typdia, double* da, double* xa) {
@@ -982,110 +1249,96 @@ static hipError_t enqueue_cg_batch(bhip_ctx* c, int iters, TX* x, T* r, T* d0, T
   return hipSuccess;
 }
 
-// GRACJG from the state after the initial residual: R holds A X - B and
-// part_u[0] its nblocks partial sums of ||R||^2. Runs E_0 (D = R, AD, RO)
-// then batches of iterations until the device reports done.
+// The graph (or plain launches) of one batch of kBatch CG iterations.
+constexpr int kBatch = 8;
+
 template <typename T, typename TA, typename TX>
-static int run_cg(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad, const T* dz,
-                  const TA* xa, const TA* da) {
-  SolveState* sh = c->st_host;
-  {
-    EbeArgs<T, TA> a = base_args<T, TA>(c, xa, da);
-    a.r = r;
-    a.dold = dz;
-    a.dnew = d0;
-    a.out = ad;
-    a.partial = part_e(c, 0);
-    a.part_in = part_u(c, 0);
-    a.n_part_in = c->L.nblocks;
-    a.parity = 0;
-    HIP_OK((launch_ebe<T, TA, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia,
-                                                              c->typext)));
-  }
-  constexpr int kBatch = 8;
-  hipGraphExec_t exec = nullptr;
-  if (c->use_graphs) {
-    int slot = -1;
-    for (int i = 0; i < 4; ++i)
-      if (c->graphs[i].exec && c->graphs[i].key == (const void*)x && c->graphs[i].prec == prec)
-        slot = i;
+static int cg_batch_exec(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad,
+                         const TA* xa, const TA* da, hipGraphExec_t* out) {
+  *out = nullptr;
+  if (!c->use_graphs) return BHIP_OK;
+  int slot = -1;
+  const int tag = ((int)(unsigned char)c->typdia << 8) | (int)(unsigned char)c->typext;
+  for (int i = 0; i < 4; ++i)
+    if (c->graphs[i].exec && c->graphs[i].key == (const void*)x && c->graphs[i].prec == prec &&
+        c->graphs[i].tag == tag)
+      slot = i;
+  if (slot < 0) {
+    for (int i = 0; i < 4 && slot < 0; ++i)
+      if (!c->graphs[i].exec) slot = i;
     if (slot < 0) {
-      for (int i = 0; i < 4 && slot < 0; ++i)
-        if (!c->graphs[i].exec) slot = i;
-      if (slot < 0) {
-        (void)hipGraphExecDestroy(c->graphs[3].exec);
-        c->graphs[3].exec = nullptr;
-        slot = 3;
-      }
-      hipGraph_t graph;
-      HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t e = enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da);
-      hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-      HIP_OK(e);
-      HIP_OK(e2);
-      HIP_OK(hipGraphInstantiate(&c->graphs[slot].exec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      c->graphs[slot].key = (const void*)x;
-      c->graphs[slot].prec = prec;
+      (void)hipGraphExecDestroy(c->graphs[3].exec);
+      c->graphs[3].exec = nullptr;
+      slot = 3;
     }
-    exec = c->graphs[slot].exec;
+    hipGraph_t graph;
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    HIP_OK(e);
+    HIP_OK(e2);
+    HIP_OK(hipGraphInstantiate(&c->graphs[slot].exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    c->graphs[slot].key = (const void*)x;
+    c->graphs[slot].prec = prec;
+    c->graphs[slot].tag = tag;
   }
-  int nb = 1;
-  for (;;) {
-    for (int k = 0; k < nb; ++k) {
-      if (exec)
-        HIP_OK(hipGraphLaunch(exec, c->stream));
-      else
-        HIP_OK((enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da)));
-    }
-    if (read_state(c)) return BHIP_EDEVICE;
-    if (sh->done) break;
-    nb = std::min(nb * 2, 8);
+  *out = c->graphs[slot].exec;
+  return BHIP_OK;
+}
+
+template <typename T, typename TA, typename TX>
+static int enqueue_batches(bhip_ctx* c, hipGraphExec_t exec, int nb, TX* x, T* r, T* d0, T* d1,
+                           T* ad, const TA* xa, const TA* da) {
+  for (int k = 0; k < nb; ++k) {
+    if (exec)
+      HIP_OK(hipGraphLaunch(exec, c->stream));
+    else
+      HIP_OK((enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da)));
   }
   return BHIP_OK;
 }
 
-// ||b||^2 into st->dot (and the host copy)
-static int dot_b(bhip_ctx* c, const double* b, double* out) {
+// E_0 of a CG solve: D = R, AD, RO (gracjg.F:248-268); R's partial sums of
+// ||R||^2 are in part_u[0] (n_in of them)
+template <typename T, typename TA>
+static hipError_t enqueue_cg_start(bhip_ctx* c, T* r, T* d0, T* ad, const T* dz, const TA* xa,
+                                   const TA* da, int n_in, int skip_if_stalled) {
+  EbeArgs<T, TA> a = base_args<T, TA>(c, xa, da);
+  a.r = r;
+  a.dold = dz;
+  a.dnew = d0;
+  a.out = ad;
+  a.partial = part_e(c, 0);
+  a.part_in = part_u(c, 0);
+  a.n_part_in = n_in;
+  a.parity = 0;
+  a.skip_if_stalled = skip_if_stalled;
+  return launch_ebe<T, TA, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia, c->typext);
+}
+
+static hipError_t enqueue_bb_setup(bhip_ctx* c, const double* b, double eps, int nitmax,
+                                   int mixed, double inner_eps, int inner_nitmax) {
   const int g = c->grid_nodes;
   hipLaunchKernelGGL(k_dot_partials<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b,
-                     c->L.npoin, part_u(c, 0));
-  hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0), g,
-                     &c->st.p->dot);
-  HIP_OK(hipGetLastError());
-  if (read_state(c)) return BHIP_EDEVICE;
-  *out = c->st_host->dot;
-  return BHIP_OK;
+                     c->L.npoin, part_u(c, 1));
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 1), g, eps,
+                     nitmax, mixed, inner_eps, inner_nitmax, c->st.p);
+  return hipGetLastError();
 }
 
-// GRACJG (gracjg.F:184-420) on the device-resident preconditioned system, FP64.
+// GRACJG (gracjg.F:184-420) on the device-resident preconditioned system,
+// FP64. The whole solve is queued (set-up, R = A X - B, E_0, as many
+// iteration batches as the previous solve needed); the host then waits once
+// and only queues more batches if the solve is not finished.
 static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
                    bhip_solve_info* info) {
   const int n = c->L.npoin;
   const int g = c->grid_nodes;
   SolveState* sh = c->st_host;
-  std::memset(sh, 0, sizeof(SolveState));
-  if (write_state(c)) return BHIP_EDEVICE;
-  double xl;
-  if (dot_b(c, b, &xl)) return BHIP_EDEVICE;
-  info->relat = 1;
-  if (std::sqrt(xl) < 1e-15) {
-    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
-    HIP_OK(hipGetLastError());
-    info->niter = 0;
-    info->relres = 0.0;
-    info->converged = 1;
-    return BHIP_ZERO_RHS;
-  }
-  if (xl < 1.0) {
-    xl = 1.0;
-    info->relat = 0;
-  }
-  sh->xl = xl;
-  init_cg_state(sh, eps * eps * xl, 0, nitmax);
-  if (write_state(c)) return BHIP_EDEVICE;
-  // R = A X - B (gracjg.F:213-215), partial sums of RMRM
+  HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 0, 0.0, 0));
   {
+    // R = A X - B (gracjg.F:213-215), partial sums of RMRM
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.y = x;
     a.b = b;
@@ -1094,11 +1347,34 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
     HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
   }
-  int rc = run_cg<double, double, double>(c, BHIP_PREC_FP64, x, c->vr.p, c->vd0.p, c->vd1.p,
-                                          c->vad.p, c->vdz.p, c->xa.p, c->da.p);
+  HIP_OK((enqueue_cg_start<double, double>(c, c->vr.p, c->vd0.p, c->vad.p, c->vdz.p, c->xa.p,
+                                           c->da.p, c->L.nblocks, 0)));
+  hipGraphExec_t exec;
+  int rc = cg_batch_exec<double, double, double>(c, BHIP_PREC_FP64, x, c->vr.p, c->vd0.p,
+                                                 c->vd1.p, c->vad.p, c->xa.p, c->da.p, &exec);
   if (rc) return rc;
+  int& hint = c->iter_hint[0];
+  int nb = std::max(1, (hint + 1 + kBatch - 1) / kBatch);
+  for (;;) {
+    rc = enqueue_batches<double, double, double>(c, exec, nb, x, c->vr.p, c->vd0.p, c->vd1.p,
+                                                 c->vad.p, c->xa.p, c->da.p);
+    if (rc) return rc;
+    if (read_state(c)) return BHIP_EDEVICE;
+    if (sh->done) break;
+    nb = std::min(nb * 2, 8);
+  }
+  info->relat = sh->xl_raw >= 1.0;
+  if (sh->zero_rhs) {
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
+    HIP_OK(hipGetLastError());
+    info->niter = 0;
+    info->relres = 0.0;
+    info->converged = 1;
+    return BHIP_ZERO_RHS;
+  }
+  hint = sh->final_m;
   info->niter = sh->final_m;
-  info->relres = std::sqrt(sh->final_rr / xl);
+  info->relres = std::sqrt(sh->final_rr / sh->xl);
   info->converged = sh->done == ST_CONVERGED;
   return sh->done == ST_CONVERGED ? BHIP_OK : BHIP_MAXITER;
 }
@@ -1118,11 +1394,14 @@ static int ensure_f32(bhip_ctx* c) {
 }
 
 // Mixed precision CG: FP64 true residual r = A'x - b' and FP64 update
-// x <- x - delta, with delta from an FP32 EBE CG on the FP32 copy of A'.
-// Same stop rule as GRACJG on the preconditioned system, evaluated with the
-// FP64 residual.
+// x <- x - delta, with delta from an FP32 EBE CG on the FP32 copy of A'
+// (inner stop ||R||^2 <= inner_eps^2 ||r||^2). Same stop rule as GRACJG on the
+// preconditioned system, evaluated with the FP64 residual. The outer loop is
+// driven on the device (k_outer_check), so the expected number of outer and
+// inner iterations is queued at once and the host waits once.
 static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
-                    double inner_eps, int inner_nitmax, bhip_solve_info* info) {
+                    double inner_eps, int inner_nitmax, bool xa32_ready,
+                    bhip_solve_info* info) {
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
@@ -1130,83 +1409,100 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   if (ensure_f32(c)) return BHIP_EDEVICE;
   const int npair = L.ndp * (L.ndp - 1) / 2;
   const long ncf = (long)(c->typext == 'Q' ? 2 * npair : npair) * L.nslot_pad;
-  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
-                     ncf);
+  if (!xa32_ready)
+    hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
+                       ncf);
   hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
                      (long)n);
   HIP_OK(hipGetLastError());
-  std::memset(sh, 0, sizeof(SolveState));
-  if (write_state(c)) return BHIP_EDEVICE;
-  double xl;
-  if (dot_b(c, b, &xl)) return BHIP_EDEVICE;
-  info->relat = 1;
-  if (std::sqrt(xl) < 1e-15) {
+  float* delta = c->fx0.p;
+  hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, delta, 0.0f, (long)n);
+  HIP_OK(hipGetLastError());
+  HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax));
+  hipGraphExec_t exec;
+  int rc = cg_batch_exec<float, float, float>(c, BHIP_PREC_MIXED, delta, c->fr.p, c->fd0.p,
+                                              c->fd1.p, c->fad.p, c->xa32.p, c->da32.p, &exec);
+  if (rc) return rc;
+  // outer step: x -= delta ; r = A x - b, inner RHS -r, delta = 0 ; test
+  auto outer_step = [&]() -> int {
+    hipLaunchKernelGGL(k_outer_xupd, dim3(g), dim3(kThreads), 0, c->stream, x, delta, n,
+                       c->st.p);
+    HIP_OK(hipGetLastError());
+    EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+    a.y = x;
+    a.b = b;
+    a.r32 = c->fr.p;
+    a.zero32 = delta;
+    a.partial = part_u(c, 0);
+    a.skip_if_outer_done = 1;
+    HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID_MIXED, FIN_PARTIAL>(
+        c, a, false, c->typdia, c->typext)));
+    hipLaunchKernelGGL(k_outer_check, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0),
+                       L.nblocks, c->st.p);
+    HIP_OK(hipGetLastError());
+    return BHIP_OK;
+  };
+  auto inner_solve = [&](int nbat) -> int {
+    HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
+                                           c->da32.p, L.nblocks, 1)));
+    return enqueue_batches<float, float, float>(c, exec, nbat, delta, c->fr.p, c->fd0.p,
+                                                c->fd1.p, c->fad.p, c->xa32.p, c->da32.p);
+  };
+  int& hint_in = c->iter_hint[1];   // iterations of one inner solve
+  int& hint_out = c->iter_hint[2];  // inner solves of one mixed solve
+  const int nbat = std::max(1, (hint_in + 1 + kBatch - 1) / kBatch);
+  rc = outer_step();
+  if (rc) return rc;
+  for (int k = 0; k < std::max(hint_out, 1); ++k) {
+    if ((rc = inner_solve(nbat))) return rc;
+    if ((rc = outer_step())) return rc;
+  }
+  int guard = 0;
+  for (;;) {
+    if (read_state(c)) return BHIP_EDEVICE;
+    if (sh->outer_done) break;
+    if (sh->inner_total >= nitmax || sh->outer_count > 100 || ++guard > 10000) break;
+    if (sh->stalled) {
+      // finish the running inner solve, then resume the outer loop
+      if (!sh->done) {
+        rc = enqueue_batches<float, float, float>(c, exec, 1, delta, c->fr.p, c->fd0.p,
+                                                  c->fd1.p, c->fad.p, c->xa32.p, c->da32.p);
+        if (rc) return rc;
+        continue;
+      }
+      sh->stalled = 0;
+      if (write_state(c)) return BHIP_EDEVICE;
+      if ((rc = outer_step())) return rc;
+      continue;
+    }
+    // the last outer test started an inner solve that was not queued
+    if ((rc = inner_solve(nbat))) return rc;
+    if ((rc = outer_step())) return rc;
+  }
+  info->relat = sh->xl_raw >= 1.0;
+  if (sh->zero_rhs) {
     hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
     HIP_OK(hipGetLastError());
     info->converged = 1;
     return BHIP_ZERO_RHS;
   }
-  if (xl < 1.0) {
-    xl = 1.0;
-    info->relat = 0;
-  }
-  const double eps2xl = eps * eps * xl;
-  hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, c->fx0.p, 0.0f,
-                     (long)n);
-  HIP_OK(hipGetLastError());
-  double* xa_ = x;
-  double* xb_ = c->vio1.p;
-  float* dl_prev = c->fx0.p;
-  float* dl_next = c->fx1.p;
-  int total_inner = 0, outer = 0;
-  int status = BHIP_MAXITER;
-  for (;;) {
-    // outer: x_new = x - delta_prev ; r = A x_new - b ; inner RHS -r ; delta_next = 0
-    {
-      EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
-      a.xd = xa_;
-      a.delta = dl_prev;
-      a.xnew = xb_;
-      a.b = b;
-      a.r32 = c->fr.p;
-      a.zero32 = dl_next;
-      a.partial = part_u(c, 0);
-      HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_PARTIAL>(
-          c, a, false, c->typdia, c->typext)));
-    }
-    hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0),
-                       L.nblocks, &c->st.p->rr0);
-    HIP_OK(hipGetLastError());
-    std::swap(xa_, xb_);
-    if (read_state(c)) return BHIP_EDEVICE;
-    const double rr = sh->rr0;
-    info->relres = std::sqrt(rr / xl);
-    if (rr <= eps2xl) {
-      status = BHIP_OK;
-      break;
-    }
-    if (total_inner >= nitmax || outer >= 100) break;
-    ++outer;
-    // inner FP32 CG: A32 delta = r from delta = 0 (R0 = -r), stop when
-    // ||R||^2 <= inner_eps^2 ||R0||^2
-    init_cg_state(sh, inner_eps * inner_eps, 1, std::min(inner_nitmax, nitmax - total_inner));
-    if (write_state(c)) return BHIP_EDEVICE;
-    int rc = run_cg<float, float, float>(c, BHIP_PREC_MIXED, dl_next, c->fr.p, c->fd0.p,
-                                         c->fd1.p, c->fad.p, c->fdz.p, c->xa32.p, c->da32.p);
-    if (rc) return rc;
-    total_inner += sh->final_m;
-    std::swap(dl_prev, dl_next);
-  }
-  if (xa_ != x)
-    HIP_OK(hipMemcpyAsync(x, xa_, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-  info->niter = total_inner;
-  info->inner_iters = total_inner;
-  info->outer_iters = outer;
-  info->converged = status == BHIP_OK;
-  return status;
+  if (sh->outer_count > 0) hint_in = (sh->inner_total + sh->outer_count - 1) / sh->outer_count;
+  hint_out = sh->outer_count;
+  info->niter = sh->inner_total;
+  info->inner_iters = sh->inner_total;
+  info->outer_iters = sh->outer_count;
+  info->relres = std::sqrt(sh->rr_outer / sh->xl);
+  info->converged = sh->outer_done;
+  return sh->outer_done ? BHIP_OK : BHIP_MAXITER;
 }
 
 }  // namespace bhip
+
+extern "C" {
+
+}  // extern "C"
+
+#include "krylov.inl"
 
 extern "C" {
 
@@ -1220,7 +1516,9 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
   if (precon % 3 == 0 || precon % 7 == 0 || precon % 11 == 0 || precon % 13 == 0 ||
       precon % 17 == 0)
     return BHIP_EBADARG;
-  if (cfg->slv != 1) return BHIP_EBADARG;  // CGSTAB / GMRES: bhip_krylov.hip
+  if (cfg->slv != 1 && cfg->slv != 6 && cfg->slv != 7) return BHIP_EBADARG;
+  if (cfg->slv == 7 && (cfg->krylov < 1 || cfg->krylov > 20)) return BHIP_EBADARG;
+  if (cfg->slv != 1 && cfg->precision == BHIP_PREC_MIXED) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
@@ -1231,6 +1529,9 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
                      c->perm.p, n);
   HIP_OK(hipGetLastError());
   const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
+  const bool mixed = cfg->precision == BHIP_PREC_MIXED;
+  if (mixed && ensure_f32(c)) return BHIP_EDEVICE;
+  bool xa32_ready = false;
   if (diag) {
     // PRECD1 (precd1.f:90-149)
     hipLaunchKernelGGL(k_precd1_nodes, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, c->vb.p,
@@ -1238,25 +1539,33 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
     HIP_OK(hipGetLastError());
     if (c->typdia == 'I') c->typdia = 'Q';
     if (c->ncoef > 0) {
+      // the mixed path takes its FP32 copy of D A D in the same pass
+      float* x32 = mixed ? c->xa32.p : nullptr;
       if (L.ndp == 3)
         hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), 0, c->stream,
-                           c->xa.p, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
+                           c->xa.p, x32, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
                            c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
                            L.nblocks);
       else
         hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), 0, c->stream,
-                           c->xa.p, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
+                           c->xa.p, x32, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
                            c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
                            L.nblocks);
       HIP_OK(hipGetLastError());
+      xa32_ready = mixed;
     }
     if (precon % 2 == 0 || precon % 3 == 0) c->typdia = 'I';
   }
   int rc;
-  if (cfg->precision == BHIP_PREC_MIXED) {
+  if (cfg->slv == 6) {
+    rc = bhip::cgstab_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
+  } else if (cfg->slv == 7) {
+    rc = bhip::gmres_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, cfg->krylov, info);
+  } else if (cfg->precision == BHIP_PREC_MIXED) {
     double ie = cfg->inner_eps > 0 ? cfg->inner_eps : 1e-4;
     int inn = cfg->inner_nitmax > 0 ? cfg->inner_nitmax : std::max(cfg->nitmax, 1);
-    rc = cg_mixed(c, c->vx.p, c->vb.p, cfg->eps, std::max(cfg->nitmax, 1), ie, inn, info);
+    rc = cg_mixed(c, c->vx.p, c->vb.p, cfg->eps, std::max(cfg->nitmax, 1), ie, inn, xa32_ready,
+                  info);
   } else {
     rc = cg_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
   }
@@ -1541,11 +1850,11 @@ int bhip_dirich(bhip_ctx* c, double* f, double* sm, const double* fbor, const in
   if (c->ncoef > 0) {
     if (L.ndp == 3)
       hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->xa.p,
-                         c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
+                         (float*)nullptr, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
                          c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
     else
       hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->xa.p,
-                         c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
+                         (float*)nullptr, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
                          c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
     HIP_OK(hipGetLastError());
   }

@@ -144,3 +144,91 @@ def test_solve_repeatable_bitwise(gpu):
         outs.append((x, info["niter"]))
     assert outs[0][1] == outs[1][1]
     assert np.array_equal(outs[0][0], outs[1][0])
+
+
+def test_context_reuse_across_preconditioners(gpu):
+    """One context, successive solves with PRECON 0, 2, 5 (different diagonal
+    kinds of the preconditioned matrix), each against the oracle."""
+    m = M.random_small_mesh(12, 12, seed=31)
+    sysd = S.wave_system(m, 1.0 + 0.2 * m.x, dt=2.0)
+    ctx = bief.Context(m, block_nodes=48)
+    for precon in (0, 2, 5, 2, 0):
+        A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+        x_ref, info_ref, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1,
+                                        precon=precon, eps=1e-10, nitmax=3000)
+        ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+        for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+            ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+            x, info = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=precon,
+                                eps=1e-10, nitmax=3000, precision=prec)
+            rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+            assert info["converged"], (precon, prec, info)
+            assert rel <= (1e-10 if prec == bief.PREC_FP64 else 1e-7), (precon, prec, rel)
+
+
+def test_matrix_save_restore(gpu):
+    m = M.random_small_mesh(10, 8, seed=41)
+    sysd = S.wave_system(m, 1.0 + m.x / 4.0, dt=2.0)
+    ctx = bief.Context(m)
+    ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+    ctx.save_matrix()
+    x1, i1 = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), eps=1e-10)
+    ctx.restore_matrix()
+    td, da, xa = ctx.get_matrix()
+    assert td == "Q" and np.array_equal(da, sysd["da"]) and np.array_equal(xa, sysd["xa"])
+    x2, i2 = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), eps=1e-10)
+    assert np.array_equal(x1, x2) and i1["niter"] == i2["niter"]
+
+
+def _nonsym_system(m, seed):
+    """Diagonally dominant non-symmetric EBE matrix ('Q' storage: 12,13,23,
+    21,31,32), an advection-diffusion-like operator."""
+    rng = np.random.default_rng(seed)
+    da, xa, _, _ = S.wave_matrix(m, 1.0 + m.x / 4.0, dt=2.0)
+    skew = rng.uniform(-0.3, 0.3, (3, m.nelem)) * np.abs(xa).mean()
+    xq = np.concatenate([xa + skew, xa - skew])
+    return da, xq
+
+
+@pytest.mark.parametrize("slv", [6, 7])
+@pytest.mark.parametrize("precon", [0, 2, 5])
+@pytest.mark.parametrize("typext", ["S", "Q"])
+def test_cgstab_gmres_match_oracle(gpu, slv, precon, typext):
+    m = M.random_small_mesh(13, 11, seed=51)
+    if typext == "S":
+        sysd = S.wave_system(m, 1.0 + m.x / 4.0, dt=2.0)
+        da, xa, b, x0 = sysd["da"], sysd["xa"], sysd["b"], sysd["x0"]
+    else:
+        da, xa = _nonsym_system(m, 3)
+        rng = np.random.default_rng(4)
+        b = rng.normal(size=m.npoin)
+        x0 = np.zeros(m.npoin)
+    A = O.Matrix(da, xa, "Q", typext)
+    x_ref, info_ref, _, _ = O.solve(A, m.ikle, b, x0, slv=slv, precon=precon, krylov=5,
+                                    eps=1e-10, nitmax=500)
+    ctx = bief.Context(m, block_nodes=40)
+    ctx.set_matrix("Q", typext, da, xa)
+    x, info = ctx.solve(x0.copy(), b.copy(), slv=slv, precon=precon, krylov=5, eps=1e-10,
+                        nitmax=500)
+    print(slv, precon, typext, info, info_ref.niter, flush=True)
+    assert info["converged"] and info_ref.converged
+    assert abs(info["niter"] - info_ref.niter) <= 1
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-9
+
+
+@pytest.mark.parametrize("slv", [6, 7])
+def test_cgstab_gmres_nitmax(gpu, slv):
+    m = M.random_small_mesh(10, 9, seed=52)
+    da, xa = _nonsym_system(m, 7)
+    b = np.random.default_rng(8).normal(size=m.npoin)
+    A = O.Matrix(da, xa, "Q", "Q")
+    x_ref, info_ref, _, _ = O.solve(A, m.ikle, b, slv=slv, precon=2, krylov=3, eps=1e-15,
+                                    nitmax=3)
+    ctx = bief.Context(m)
+    ctx.set_matrix("Q", "Q", da, xa)
+    x, info = ctx.solve(np.zeros(m.npoin), b.copy(), slv=slv, precon=2, krylov=3, eps=1e-15,
+                        nitmax=3)
+    assert info["status"] == "maxiter" and info["niter"] == info_ref.niter == 3
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-11
