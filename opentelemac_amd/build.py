@@ -55,22 +55,42 @@ def have_flang():
     return shutil.which("flang") is not None or os.path.exists("/opt/rocm/bin/amdflang")
 
 
+def _fc():
+    return shutil.which("flang") or "/opt/rocm/bin/amdflang"
+
+
 def build_shim():
-    """Fortran host layer: BIEF-shaped types and SOLVE/MATVEC/MATRBL with the
-    reference's signatures, calling libbiefhip through ISO_C_BINDING."""
-    if not os.path.isdir(FSRC) or not have_flang():
+    """Fortran host layer with BIEF's SOLVE/MATVEC/MATRBL signatures over the
+    C ABI (ISO_C_BINDING), stand-alone types: libbiefshim.so."""
+    if not have_flang():
         return None
-    fc = shutil.which("flang") or "/opt/rocm/bin/amdflang"
     out = os.path.join(HERE, "libbiefshim.so")
-    mod = os.path.join(HERE, "_fmod")
+    mod = os.path.join(HERE, "_build", "fmod")
     os.makedirs(mod, exist_ok=True)
-    srcs = [os.path.join(FSRC, f) for f in ("bief_hip_def.f90", "bief_hip_capi.f90",
-                                            "bief_hip_shim.f90")]
-    srcs = [s for s in srcs if os.path.exists(s)]
-    if not srcs:
+    srcs = [os.path.join(FSRC, f) for f in ("bief_hip_types.f90", "bief_hip_capi.f90",
+                                            "bief_hip_shim.F90")]
+    _run([_fc(), "-O2", "-cpp", "-DBHIP_OWN_TYPES", "-fPIC", "-shared", "-module-dir", mod,
+          *srcs, "-L", HERE, "-lbiefhip", "-Wl,-rpath,$ORIGIN", "-o", out])
+    return out
+
+
+def build_dropin():
+    """The same host layer compiled against the reference's own BIEF_DEF and
+    linked in front of the reference BIEF archive (oracle/_ref), so that the
+    reference's callers reach libbiefhip through SOLVE/MATVEC: the drop-in
+    test library oracle/_ref/libbief_dropin.so (test infrastructure)."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    if not (have_flang() and os.path.exists(os.path.join(ref, "libbief_ref.a"))):
         return None
-    _run([fc, "-O2", "-fPIC", "-shared", "-module-dir", mod, *srcs,
-          "-L", HERE, "-lbiefhip", f"-Wl,-rpath,{HERE}", "-o", out])
+    out = os.path.join(ref, "libbief_dropin.so")
+    mod = os.path.join(ref, "dropin_mod")
+    os.makedirs(mod, exist_ok=True)
+    srcs = [os.path.join(FSRC, "bief_hip_capi.f90"), os.path.join(FSRC, "bief_hip_shim.F90"),
+            os.path.join(ROOT, "oracle", "ref_driver.f90")]
+    _run([_fc(), "-O2", "-cpp", "-ffixed-line-length-132", "-fPIC", "-shared",
+          "-I", os.path.join(ref, "mod"), "-module-dir", mod, *srcs,
+          os.path.join(ref, "libbief_ref.a"), "-L", HERE, "-lbiefhip",
+          "-Wl,-rpath,$ORIGIN/../../opentelemac_amd", "-o", out])
     return out
 
 
@@ -85,3 +105,4 @@ def build_all(verbose=False):
     build_hip(verbose)
     build_shim()
     build_oracle()
+    build_dropin()

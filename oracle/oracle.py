@@ -195,10 +195,40 @@ def lump(A: Matrix, ikle, xmul=1.0):
 # --------------------------------------------------------------------------
 
 _ref = None
+_dropin = None
+DROPIN_SO = os.path.join(HERE, "_ref", "libbief_dropin.so")
 
 
 def have_ref():
     return os.path.exists(REF_SO)
+
+
+def have_dropin():
+    return os.path.exists(DROPIN_SO)
+
+
+def dropin():
+    """The reference's BIEF with SOLVE / MATVEC / MATRBL replaced by the
+    Fortran host layer over libbiefhip (same driver entry points as ref())."""
+    global _dropin
+    if _dropin is None:
+        if not have_dropin():
+            raise RuntimeError(f"{DROPIN_SO} missing: run __graft_entry__.build()")
+        _dropin = _bind_driver(ctypes.CDLL(DROPIN_SO))
+    return _dropin
+
+
+def _bind_driver(L):
+    L.ref_mesh_init.argtypes = [_I, _I, _I, _I, _P]
+    L.ref_matvec.argtypes = [_I, _C, _C, _P, _P, _P, _D, _P]
+    L.ref_solve.argtypes = [_I, _I, _I, _I, _D, ctypes.POINTER(ctypes.c_char), _C,
+                            _P, _P, _P, _P, ctypes.c_char_p, _I]
+    L.ref_dirich.argtypes = [_C, _C, _P, _P, _P, _P, _P, _I, _P]
+    L.ref_lump.argtypes = [_C, _C, _P, _P, _D, _P]
+    L.ref_mt01aa.argtypes = [_I, _D, _P, _P, _P]
+    L.ref_mt02aa.argtypes = [_I, _I, _I, _D, _P, _P, _P, _P, _P, _P, _I, _P, _P]
+    L.ref_mt01pp.argtypes = [_I, _D, _P, _P, _P, _P, _P]
+    return L
 
 
 def ref():
@@ -215,31 +245,40 @@ def ref():
         L.ref_lump.argtypes = [_C, _C, _P, _P, _D, _P]
         L.ref_mt01aa.argtypes = [_I, _D, _P, _P, _P]
         L.ref_mt02aa.argtypes = [_I, _I, _I, _D, _P, _P, _P, _P, _P, _P, _I, _P, _P]
+        L.ref_mt01pp.argtypes = [_I, _D, _P, _P, _P, _P, _P]
         _ref = L
     return _ref
 
 
 class RefMesh:
-    """Binds a mesh to the reference driver's BIEF_MESH (one at a time)."""
-    _current = None
+    """Binds a mesh to the driver's BIEF_MESH (one at a time per library);
+    loader = ref (the reference's BIEF) or dropin (its SOLVE/MATVEC
+    replaced by the HIP host layer)."""
+    _current = {}
 
-    def __init__(self, ikle, npoin2, nelem2, nplan=1):
+    def __init__(self, ikle, npoin2, nelem2, nplan=1, loader=None):
         self.ikle = _i32(ikle)
         self.ndp = self.ikle.shape[0]
         self.args = (npoin2, nelem2, nplan)
+        self.loader = loader or ref
+
+    @property
+    def lib(self):
+        return self.loader()
 
     def bind(self):
-        if RefMesh._current is not self:
-            ref().ref_mesh_init(self.ndp, *self.args, _ptr(self.ikle))
-            RefMesh._current = self
+        key = self.loader.__name__
+        if RefMesh._current.get(key) is not self:
+            self.lib.ref_mesh_init(self.ndp, *self.args, _ptr(self.ikle))
+            RefMesh._current[key] = self
 
 
 def ref_matvec(rm: RefMesh, op, A: Matrix, y, x=None, c=0.0):
     rm.bind()
     y = _f64(y)
     x = np.zeros(y.size) if x is None else _f64(x).copy()
-    ref().ref_matvec(OPS[op], A.typdia.encode(), A.typext.encode(), _ptr(A.da), _ptr(A.xa),
-                     _ptr(y), c, _ptr(x))
+    rm.lib.ref_matvec(OPS[op], A.typdia.encode(), A.typext.encode(), _ptr(A.da), _ptr(A.xa),
+                      _ptr(y), c, _ptr(x))
     return x
 
 
@@ -272,8 +311,8 @@ def ref_solve(rm: RefMesh, A: Matrix, b, x0=None, slv=1, precon=2, krylov=5, nit
     os.close(fd)
     try:
         p = path.encode()
-        ref().ref_solve(slv, precon, krylov, nitmax, eps, td, A2.typext.encode(), _ptr(A2.da),
-                        _ptr(A2.xa), _ptr(b2), _ptr(x), p, len(p))
+        rm.lib.ref_solve(slv, precon, krylov, nitmax, eps, td, A2.typext.encode(),
+                         _ptr(A2.da), _ptr(A2.xa), _ptr(b2), _ptr(x), p, len(p))
         niter, prec, line = _parse_solver_log(path)
     finally:
         os.unlink(path)
@@ -286,7 +325,7 @@ def ref_dirich(rm: RefMesh, A: Matrix, sm, fbor, limdir, kdir=5, f=None):
     A2 = A.copy()
     sm = _f64(sm).copy()
     f = np.zeros(sm.size) if f is None else _f64(f).copy()
-    ref().ref_dirich(A2.typdia.encode(), A2.typext.encode(), _ptr(A2.da), _ptr(A2.xa), _ptr(sm),
+    rm.lib.ref_dirich(A2.typdia.encode(), A2.typext.encode(), _ptr(A2.da), _ptr(A2.xa), _ptr(sm),
                      _ptr(_f64(fbor)), _ptr(_i32(limdir)), kdir, _ptr(f))
     return f, A2, sm
 
@@ -294,7 +333,7 @@ def ref_dirich(rm: RefMesh, A: Matrix, sm, fbor, limdir, kdir=5, f=None):
 def ref_lump(rm: RefMesh, A: Matrix, xmul=1.0):
     rm.bind()
     d = np.empty(A.da.size)
-    ref().ref_lump(A.typdia.encode(), A.typext.encode(), _ptr(A.da), _ptr(A.xa), xmul, _ptr(d))
+    rm.lib.ref_lump(A.typdia.encode(), A.typext.encode(), _ptr(A.da), _ptr(A.xa), xmul, _ptr(d))
     return d
 
 
@@ -318,3 +357,12 @@ def ref_mt02aa(kind, xel, yel, surfac, ikle, u, v=None, xmul=1.0, mon=False, npo
     ref().ref_mt02aa(kind, ne, npoin, xmul, _ptr(u), _ptr(v), _ptr(_f64(xel)), _ptr(_f64(yel)),
                      _ptr(s), _ptr(_i32(ikle)), 1 if mon else 0, _ptr(t), _ptr(xm))
     return t, xm
+
+
+def ref_mt01pp(surfac, z, ikle, xmul=1.0):
+    s = _f64(surfac)
+    ne = s.size
+    t = np.empty((6, ne))
+    xm = np.empty((30, ne))
+    ref().ref_mt01pp(ne, xmul, _ptr(_f64(z)), _ptr(s), _ptr(_i32(ikle)), _ptr(t), _ptr(xm))
+    return t, xm[:15]

@@ -125,6 +125,8 @@ contains
     cfgm(1) = 1
     cfgm(2) = 1
     call bief_allmat(ma, 'A     ', ielm_s, ielm_s, cfgm, 'Q', 'Q', mesh)
+    ! XA(NELMAX,*) layout of the off-diagonal terms (matvct.f:472)
+    ma%stox = 1
     ready = .true.
   end subroutine
 
@@ -285,6 +287,19 @@ contains
     if (kind == 4) su%dim2 = 3
     call mt02aa(t(1,1), xm(1,1), xm(1,2), t(1,2), xm(1,3), t(1,3), xmul, su, u, sv, v, &
          xel, yel, surfac, ik(1,1), ik(1,2), ik(1,3), nelem, nelem, formul)
+    deallocate(ik)
+  end subroutine
+
+  subroutine ref_mt01pp(nelem, xmul, z, surfac, ikle0, t, xm) bind(c, name='ref_mt01pp')
+    integer(c_int), value :: nelem
+    real(c_double), value :: xmul
+    real(c_double), intent(in) :: z(*), surfac(*)
+    integer(c_int), intent(in) :: ikle0(nelem, 6)
+    real(c_double), intent(out) :: t(nelem, 6), xm(nelem, 30)
+    integer, allocatable :: ik(:,:)
+    allocate(ik(nelem, 6))
+    ik = ikle0 + 1
+    call mt01pp(t, xm, xmul, z, surfac, ik, nelem, nelem)
     deallocate(ik)
   end subroutine
 

@@ -92,7 +92,42 @@ def channel_case():
     return {"x": x, "niter": np.array(info["niter"]), "relres": np.array(info["relres"])}
 
 
+def prism_case():
+    """Prisms (MV0606, MT01PP): 4 planes over a small base mesh."""
+    base = M.random_small_mesh(5, 4, seed=3)
+    rng = np.random.default_rng(30)
+    bot = -rng.uniform(2, 5, base.npoin)
+    p = M.extrude_prisms(base, 4, bot, np.zeros(base.npoin))
+    s = np.tile(base.surfac(), p.nplan - 1)
+    rm = O.RefMesh(p.ikle, base.npoin, base.nelem, p.nplan)
+    out = {"z": p.z, "ikle": p.ikle}
+    out["mt01pp_t"], out["mt01pp_xm"] = O.ref_mt01pp(s, p.z, p.ikle, 0.7)
+    da = rng.uniform(1, 2, p.npoin)
+    out["da"] = da
+    for typext, nc in (("S", 15), ("Q", 30)):
+        xa = rng.normal(size=(nc, p.nelem))
+        out[f"xa_{typext}"] = xa
+        for typdia in "QI":
+            A = O.Matrix(da, xa, typdia, typext)
+            for op, code in O.OPS.items():
+                y = rng.normal(size=p.npoin)
+                x0 = rng.normal(size=p.npoin)
+                key = f"mv_{typdia}{typext}_{code}"
+                out[key + "_y"], out[key + "_x0"] = y, x0
+                out[key] = O.ref_matvec(rm, op, A, y, x0, c=0.3)
+    tm, xm = O.mt01pp(s, p.z, p.ikle, 1.0)
+    A = O.Matrix(O.assemble_diag(tm, p.ikle, p.npoin), xm, "Q", "S")
+    b = rng.normal(size=p.npoin)
+    out["spd_da"], out["spd_xa"], out["spd_b"] = A.da, A.xa, b
+    for slv in (1, 6, 7):
+        x, info, _, _ = O.ref_solve(rm, A, b, slv=slv, precon=2, eps=1e-10)
+        out[f"solve_{slv}_x"] = x
+        out[f"solve_{slv}_niter"] = np.array(info["niter"])
+    return out
+
+
 def main():
+    np.savez_compressed(os.path.join(HERE, "ref_prism.npz"), **prism_case())
     np.savez_compressed(os.path.join(HERE, "ref_small.npz"), **small_case())
     np.savez_compressed(os.path.join(HERE, "ref_channel.npz"), **channel_case())
     print("golden vectors written")

@@ -190,3 +190,42 @@ def test_planted_mistake_is_caught():
     D = _dense(A.da, A.xa, m.ikle, "Q")
     assert np.allclose(O.matvec("X=AY", A, m.ikle, y), D @ y, atol=1e-12)
     assert not np.allclose(O.matvec("X=AY", bad, m.ikle, y), D @ y, atol=1e-6)
+
+
+# ---- prisms (MV0606, MT01PP) ---------------------------------------------
+
+def _prism_mesh():
+    base = M.random_small_mesh(5, 4, seed=3)
+    rng = np.random.default_rng(30)
+    bot = -rng.uniform(2, 5, base.npoin)
+    return base, M.extrude_prisms(base, 4, bot, np.zeros(base.npoin))
+
+
+def test_golden_prisms():
+    z = np.load(os.path.join(GOLD, "ref_prism.npz"))
+    base, p = _prism_mesh()
+    assert np.array_equal(p.ikle, z["ikle"]) and np.array_equal(p.z, z["z"])
+    s = np.tile(base.surfac(), p.nplan - 1)
+    t, xm = O.mt01pp(s, p.z, p.ikle, 0.7)
+    assert np.array_equal(t, z["mt01pp_t"]) and np.array_equal(xm, z["mt01pp_xm"])
+    for typext in "SQ":
+        for typdia in "QI":
+            A = O.Matrix(z["da"], z[f"xa_{typext}"], typdia, typext)
+            for op, code in O.OPS.items():
+                key = f"mv_{typdia}{typext}_{code}"
+                got = O.matvec(op, A, p.ikle, z[key + "_y"], z[key + "_x0"], c=0.3)
+                assert np.array_equal(got, z[key]), key
+    A = O.Matrix(z["spd_da"], z["spd_xa"], "Q", "S")
+    for slv in (1, 6, 7):
+        x, info, _, _ = O.solve(A, p.ikle, z["spd_b"], slv=slv, precon=2, eps=1e-10)
+        assert info.niter == int(z[f"solve_{slv}_niter"])
+        assert np.array_equal(x, z[f"solve_{slv}_x"])
+
+
+def test_prism_mass_integrates_volume():
+    """Sum of all MT01PP entries = volume of the prism column (linear in z)."""
+    base, p = _prism_mesh()
+    s = np.tile(base.surfac(), p.nplan - 1)
+    t, xm = O.mt01pp(s, p.z, p.ikle, 1.0)
+    h = np.mean([p.z[p.ikle[k + 3]] - p.z[p.ikle[k]] for k in range(3)], axis=0)
+    assert np.allclose(t.sum(0) + 2 * xm.sum(0), s * h, rtol=1e-12)
