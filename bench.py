@@ -62,28 +62,38 @@ def build_workload(npoin, seed, cache=None):
     return m, sysd
 
 
-def cpu_baseline(m, sysd, budget_s=60.0):
+def cpu_baseline(m, sysd, budget_s=12.0):
     """Reference SOLVE (flang-built BIEF, sequential, 1 core) on the same
-    system; falls back to the C restatement if the reference build is absent."""
+    system, repeated from the same inputs until about budget_s of CPU time;
+    falls back to the C restatement if the reference build is absent."""
     from oracle import oracle as O
     A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
     if O.have_ref():
         rm = O.RefMesh(m.ikle, m.npoin, m.nelem)
-        t = time.perf_counter()
-        x, info, _, _ = O.ref_solve(rm, A, sysd["b"], sysd["x0"], slv=1, precon=2, eps=1e-8,
-                                    nitmax=5000)
-        dt = time.perf_counter() - t
-        kind, niter = "reference", info["niter"]
+
+        def one():
+            x, info, _, _ = O.ref_solve(rm, A, sysd["b"], sysd["x0"], slv=1, precon=2,
+                                        eps=1e-8, nitmax=5000)
+            return x, info["niter"]
+        kind = "reference"
     else:
-        t = time.perf_counter()
-        x, info, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2, eps=1e-8,
-                                nitmax=5000)
-        dt = time.perf_counter() - t
-        kind, niter = "port", info.niter
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": kind,
-            "sample": f"one full FP64 SOLVE (CG, PRECON 2, eps 1e-8) of the same "
-                      f"{m.npoin}-node system, host-resident, {niter} iterations, "
-                      f"{dt:.2f} s", "iterations": niter}, x
+        def one():
+            x, info, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2,
+                                    eps=1e-8, nitmax=5000)
+            return x, info.niter
+        kind = "port"
+    n, t0 = 0, time.perf_counter()
+    while True:
+        x, niter = one()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            break
+    return {"value": n / dt, "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": f"{n} full FP64 SOLVEs (CG, PRECON 2, eps 1e-8) of the same "
+                      f"{m.npoin}-node system from the same inputs, host-resident, "
+                      f"{niter} iterations each, {dt:.2f} s in total",
+            "iterations": niter}, x
 
 
 def main():
@@ -194,7 +204,7 @@ def main():
         traffic = tj.get("mixed_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CGAD> (mixed inner CG)",
+                "kernel": "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG)",
                 "algorithmic_bytes_per_launch": bytes_f32,
                 "avg_launch_ms": results["mixed"]["ms_dir_kernel"],
                 "fp64_kernel": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),

@@ -204,7 +204,7 @@ subroutine bhip_load_matrix(a, mesh, h, ndp, npoin, nelem)
   type(bief_mesh), intent(inout), target :: mesh
   type(c_ptr), intent(out) :: h
   integer, intent(out) :: ndp, npoin, nelem
-  integer :: nelmax, ncoef, rc, nplan
+  integer :: nelmax, ncoef, rc
   double precision, allocatable, target :: xa(:,:)
   double precision, target :: dummy(1)
   logical :: have_xy
@@ -219,8 +219,8 @@ subroutine bhip_load_matrix(a, mesh, h, ndp, npoin, nelem)
   npoin = a%d%dim1
   nelem = mesh%nelem
   nelmax = mesh%nelmax
-  nplan = 1
-  if (ndp == 6) nplan = max(1, mesh%npoin / max(1, npoin))
+  ! prisms: the nodes of one vertical column share (x, y), so the
+  ! coordinate order keeps columns contiguous without the plane count
   have_xy = associated(mesh%x) .and. associated(mesh%y)
   if (have_xy) have_xy = associated(mesh%x%r) .and. associated(mesh%y%r)
   if (have_xy) then
