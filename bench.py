@@ -109,6 +109,10 @@ def main():
     ap.add_argument("--cache", default=None, help="npz file to reuse the synthetic workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (profiler counter scale)")
+    ap.add_argument("--workload", default="estuary", choices=["estuary", "prisms"],
+                    help="estuary: the 2M-node wave-equation CG (the headline line); prisms: "
+                         "the 5M-node sigma-mesh diffusion and Poisson systems (parity "
+                         "configuration, measured the same way)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -131,6 +135,8 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": src.numel() * 8,
                           "launches": 5}), flush=True)
         return
+    if args.workload == "prisms":
+        return bench_prisms(args, dev)
     m, sysd = build_workload(args.npoin, args.seed, args.cache)
     t = time.time()
     ctx = bief.Context(m, block_nodes=args.block_nodes)
@@ -189,11 +195,15 @@ def main():
                               / np.linalg.norm(sols["fp64"]))
 
     ne, nprime = m.nelem, m.npoin
-    # algorithmic bytes of the fused CG direction + EBE mat-vec kernel per
-    # launch: coefficients (3 per element) + int32 connectivity (3 per
-    # element) + R and D_old read, D_new and AD written (4 node vectors)
+    # algorithmic bytes per launch of the dominant kernels (int32
+    # connectivity, 3 coefficients per element, node vectors):
+    #  * FP64 CG direction + EBE mat-vec + D.AD: R, D_old read; D, AD written
+    #  * mixed inner CG, one kernel per iteration (ms_upd == 0): R, D, AD, X
+    #    read and written (the vector updates are fused into the mat-vec)
+    #  * mixed inner CG, two kernels: as FP64, in FP32
     bytes_f64 = ne * (3 * 8 + 12) + nprime * 4 * 8
-    bytes_f32 = ne * (3 * 4 + 12) + nprime * 4 * 4
+    fused = results["mixed"]["ms_upd_kernel"] == 0.0
+    bytes_f32 = ne * (3 * 4 + 12) + nprime * (8 if fused else 4) * 4
     ach32 = bytes_f32 / (results["mixed"]["ms_dir_kernel"] * 1e-3) / 1e9
     ach64 = bytes_f64 / (results["fp64"]["ms_dir_kernel"] * 1e-3) / 1e9
     traffic = None
@@ -202,9 +212,11 @@ def main():
         with open(tpath) as fh:
             tj = json.load(fh)
         traffic = tj.get("mixed_bytes_per_launch")
+    kname = ("k_ebe<float,...,STAGE_CG1,EPI_CG1,FIN_CG1> (mixed inner CG iteration)" if fused
+             else "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG direction)")
     roofline = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG)",
+                "kernel": kname,
                 "algorithmic_bytes_per_launch": bytes_f32,
                 "avg_launch_ms": results["mixed"]["ms_dir_kernel"],
                 "fp64_kernel": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
@@ -251,6 +263,90 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_prisms(args, dev):
+    """5,014,910-node prism mesh: a 501,491-node estuary base (the 2M
+    generator at 500k points) extruded to 10 sigma planes over the seeded
+    depth; 8,996,184 prisms. Two systems, assembled on the device:
+      * diffusion: MT01PP / dt + MT02PP with nu = 10^(-4 + 2 sigma) m^2/s in
+        all three directions (1e-4 at the bed to 1e-2 at the surface), dt = 2 s;
+      * Poisson: MT02PP with unit coefficients, zero Dirichlet values on the
+        surface plane (DIRICH).
+    Each step restores the assembled matrix and solves with CG + diagonal
+    preconditioning to eps = 1e-8, FP64 and mixed."""
+    import torch
+    from opentelemac_amd import bief, mesh as M
+    t = time.time()
+    base = M.estuary_mesh(500_000, seed=args.seed)
+    depth = M.smooth_random_depth(base, seed=args.seed)
+    nplan = 10
+    p = M.extrude_prisms(base, nplan, -depth, np.zeros(base.npoin))
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    xel, yel = np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb])
+    s = base.surfac()[eb]
+    sig = np.repeat(np.linspace(0.0, 1.0, nplan), base.npoin)
+    nu = 10.0 ** (-4.0 + 2.0 * sig)
+    log(f"prisms {p.npoin} nodes {p.nelem} elements, built in {time.time() - t:.1f}s")
+    t = time.time()
+    ctx = bief.Context(p, block_nodes=args.block_nodes)
+    st = ctx.stats()
+    log(f"context in {time.time() - t:.1f}s: {st}")
+    rng = np.random.default_rng(args.seed)
+    out = {"metric": "EBE-CG solves/s, 5M-node prisms (MT01PP/dt + MT02PP; MT02PP Poisson)",
+           "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "sigma-mesh prisms, 10 planes x 501,491 nodes",
+                      "npoin": p.npoin, "nelem": p.nelem, "dt_s": 2.0, "eps": 1e-8},
+           "layout": st, "systems": {}}
+    top = np.zeros(p.npoin, dtype=np.int32)
+    top[(nplan - 1) * base.npoin:] = 5
+    for name in ("diffusion", "poisson"):
+        if name == "diffusion":
+            ctx.matrix_mt01pp(1.0 / 2.0, s, p.z)
+            ctx.matrix_mt02pp(1.0, xel, yel, s, p.z, nu, nu, nu, base.npoin, accumulate=True)
+            b0 = rng.normal(size=p.npoin)
+        else:
+            one = np.ones(p.npoin)
+            ctx.matrix_mt02pp(1.0, xel, yel, s, p.z, one, one, one, base.npoin)
+            _, b0 = ctx.dirich(np.zeros(p.npoin), rng.normal(size=p.npoin), np.zeros(p.npoin),
+                               top)
+        ctx.save_matrix()
+        bd = torch.from_numpy(b0).to(dev)
+        x0 = torch.zeros(p.npoin, dtype=torch.float64, device=dev)
+        x = torch.empty_like(x0)
+        b = torch.empty_like(bd)
+        res = {}
+        for pname, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
+            def step():
+                ctx.restore_matrix()
+                x.copy_(x0)
+                b.copy_(bd)
+                return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=20000, precision=prec,
+                                 inner_eps=args.inner_eps)[1]
+            for _ in range(args.warmup):
+                step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            infos = [step() for _ in range(args.steps)]
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            ms_dir, ms_upd = ctx.time_cg_kernels(prec, reps=20)
+            res[pname] = {"ms_per_solve": 1e3 * el / args.steps,
+                          "solves_per_s": args.steps / el, "iterations": infos[-1]["niter"],
+                          "outer": infos[-1]["outer_iters"],
+                          "converged": bool(infos[-1]["converged"]),
+                          "ms_dir_kernel": ms_dir, "ms_upd_kernel": ms_upd}
+            log(name, pname, json.dumps(res[pname]))
+        # algorithmic bytes of the FP64 CG direction + mat-vec kernel: 15
+        # coefficients and 6 int32 node ids per prism, 4 node vectors
+        b64 = p.nelem * (15 * 8 + 24) + p.npoin * 4 * 8
+        res["fp64_kernel_GBps"] = round(b64 / (res["fp64"]["ms_dir_kernel"] * 1e-3) / 1e9, 1)
+        res["mixed_speedup_vs_fp64"] = round(res["fp64"]["ms_per_solve"] /
+                                             res["mixed"]["ms_per_solve"], 3)
+        out["systems"][name] = res
+    print(json.dumps(out), flush=True)
 
 
 def _cpu_model():

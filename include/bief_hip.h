@@ -156,10 +156,11 @@ int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
                    double* b_dev, bhip_solve_info* info);
 
 /* Measurement: average duration (ms, HIP events on the context stream) of
- * the two kernels of one CG iteration -- the fused direction update + EBE
- * mat-vec + D.AD kernel and the X/R update + R.R kernel -- on the context's
- * current (preconditioned) matrix, FP64 or the mixed path's FP32 inner
- * kernels (the latter after a mixed solve). Scratch vectors are reused, so
+ * the kernels of one CG iteration on the context's current (preconditioned)
+ * matrix. FP64: the fused direction update + EBE mat-vec + D.AD kernel
+ * (ms_dir) and the X/R update + R.R kernel (ms_upd). Mixed (after a mixed
+ * solve): the FP32 inner CG's single kernel per iteration (ms_dir; ms_upd is
+ * 0), or its two kernels when BHIP_INNER_CG=2 selects the two-kernel form. Scratch vectors are reused, so
  * call it after the solves whose results are needed. No counterpart in the
  * reference (it has no timers below the whole run, homere_telemac2d.F). */
 int bhip_time_cg_kernels(bhip_ctx* ctx, int precision, int reps,
@@ -191,6 +192,25 @@ int bhip_matrix_mt01aa(bhip_ctx* ctx, double xmul, const double* surfac,
 int bhip_matrix_mt02aa(bhip_ctx* ctx, int kind, double xmul,
                        const double* xel, const double* yel,
                        const double* surfac, const double* u, const double* v,
+                       int accumulate);
+
+/* Prism (IELM 41) element matrices on the device, assembled as above
+ * (TYPDIA 'Q', TYPEXT 'S', 15 coefficients in the prism pair order).
+ *  - MT01PP (mt01pp.f): mass, XMUL*SURFAC/360 combinations of the column
+ *    heights z(top) - z(bottom) floored at 1e-3; surfac[nelem] is the area of
+ *    each prism's base triangle, z[npoin] the node elevations.
+ *  - MT02PP (mt02pp.f): diffusion with the diagonal tensor (f, g, h) [npoin]
+ *    of P1 coefficients; xel, yel [3*nelem] the base triangle's local
+ *    coordinates (node 1 at the origin); h_zero = 1 cancels the vertical
+ *    term (H of type '0', mt02pp.f:165-167); npou0 = NPOIN2 for a P1
+ *    vertical viscosity or 0 for P0 on the vertical (mt02pp.f:200-215);
+ *    inchyd: the hydrostatic inconsistency test; mon: 'MON' clipping. */
+int bhip_matrix_mt01pp(bhip_ctx* ctx, double xmul, const double* surfac,
+                       const double* z, int accumulate);
+int bhip_matrix_mt02pp(bhip_ctx* ctx, double xmul, const double* xel,
+                       const double* yel, const double* surfac, const double* z,
+                       const double* f, const double* g, const double* h,
+                       int h_zero, int npou0, int inchyd, int mon,
                        int accumulate);
 
 #ifdef __cplusplus

@@ -93,12 +93,12 @@ def lib():
         L.bhip_matrix_restore.argtypes = [_P]
         L.bhip_time_cg_kernels.argtypes = [_P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]
         L.bhip_layout_check.argtypes = [_I, _I, _I, _P, _P, _P, _I, _I, _P]
-        for name, args in (("bhip_dirich", [_P, _P, _P, _P, _P, _I]),
-                           ("bhip_lump", [_P, _D, _P]),
-                           ("bhip_matrix_mt01aa", [_P, _D, _P, _I]),
-                           ("bhip_matrix_mt02aa", [_P, _I, _D, _P, _P, _P, _P, _P, _I])):
-            if hasattr(L, name):
-                getattr(L, name).argtypes = args
+        L.bhip_dirich.argtypes = [_P, _P, _P, _P, _P, _I]
+        L.bhip_lump.argtypes = [_P, _D, _P]
+        L.bhip_matrix_mt01aa.argtypes = [_P, _D, _P, _I]
+        L.bhip_matrix_mt02aa.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _I]
+        L.bhip_matrix_mt01pp.argtypes = [_P, _D, _P, _P, _I]
+        L.bhip_matrix_mt02pp.argtypes = [_P, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I]
         _lib = L
     return _lib
 
@@ -268,10 +268,41 @@ class Context:
                                         int(accumulate)), "bhip_matrix_mt02aa")
         self.typdia, self.typext = "Q", "S"
 
+    def matrix_mt01pp(self, xmul, surfac, z, accumulate=False):
+        """MATRIX(... 'MATMAS' ...) on prisms (mt01pp.f): surfac is the base
+        triangle area of each prism, z the node elevations."""
+        s = _host_f64(surfac)
+        z = _host_f64(z)
+        _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), s.ctypes.data, z.ctypes.data,
+                                        int(accumulate)), "bhip_matrix_mt01pp")
+        self.typdia, self.typext = "Q", "S"
+
+    def matrix_mt02pp(self, xmul, xel, yel, surfac, z, f, g, h, npou0, h_zero=False,
+                      inchyd=False, mon=False, accumulate=False):
+        """MATRIX(... 'MATDIF' ...) on prisms (mt02pp.f): diagonal diffusion
+        tensor (f, g, h) per node; npou0 = NPOIN2 (P1 vertical viscosity) or
+        0 (P0 on the vertical)."""
+        arrs = [_host_f64(a) for a in (xel, yel, surfac, z, f, g, h)]
+        _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[a.ctypes.data for a in arrs],
+                                        int(h_zero), int(npou0), int(inchyd), int(mon),
+                                        int(accumulate)), "bhip_matrix_mt02pp")
+        self.typdia, self.typext = "Q", "S"
+
     # ---- MATVEC ---------------------------------------------------------
     def _sync_torch(self):
+        """Orders the context stream after the work queued so far on torch's
+        current stream (an event wait on the device, no host round trip)."""
         import torch
-        torch.cuda.current_stream().synchronize()
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream())
+        self.torch_stream().wait_event(ev)
+
+    def _join_torch(self):
+        """Orders torch's current stream after the context stream's work."""
+        import torch
+        ev = torch.cuda.Event()
+        ev.record(self.torch_stream())
+        torch.cuda.current_stream().wait_event(ev)
 
     def matvec(self, op: str, y, x=None, c: float = 0.0):
         """X = op(A, Y, C) as BIEF MATVEC. Host arrays: returns a new array.
@@ -284,6 +315,7 @@ class Context:
             self._sync_torch()
             _check(lib().bhip_matvec_dev(self._h, code, float(c), y.data_ptr(), x.data_ptr()),
                    "bhip_matvec_dev")
+            self._join_torch()
             return x
         y = _host_f64(y)
         x = np.zeros(self.npoin) if x is None else _host_f64(x).copy()
@@ -312,7 +344,7 @@ class Context:
             rc = lib().bhip_solve_dev(self._h, ctypes.byref(cfg), x.data_ptr(), b.data_ptr(),
                                       ctypes.byref(info))
             _check(rc, "bhip_solve_dev")
-            self.synchronize()
+            self._join_torch()
         else:
             xh = _host_f64(x)
             bh = _host_f64(b)

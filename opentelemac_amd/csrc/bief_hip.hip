@@ -81,7 +81,11 @@ struct SolveState {
   int stalled;         // an outer test found its inner solve unfinished
   int outer_count;     // inner solves started
   int inner_total;     // inner iterations of finished inner solves
-  int pad3;
+  int done1;           // inner solve finished: (M << 2) | status, set by a kernel
+                       // after the one that applied the last update
+  int stop[2];         // single-kernel inner CG: set by the kernel of iteration M
+                       // that applied the last update, (M << 2) | status, by parity
+  int pad4;
 };
 
 // ---------------------------------------------------------------------------
@@ -104,6 +108,51 @@ __device__ inline double block_sum(double v, double* sh) {
   }
   __syncthreads();
   return s;
+}
+
+// four fixed-order block sums at once; results valid in thread 0
+__device__ inline void block_sum4(double (&v)[4], double* sh /* [kWaves * 4] */) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[w * 4 + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i * 4 + k];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Fixed-order sums of n partial quadruples ([n][4], 32-byte aligned) by every
+// workgroup that calls it (identical results everywhere); valid in all threads.
+__device__ inline void reduce_partials4(const double* __restrict__ part, int n, double* sh,
+                                        double* bcast /* [4] */, double (&out)[4]) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const double2* p2 = reinterpret_cast<const double2*>(part);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double2 lo = p2[2 * i], hi = p2[2 * i + 1];
+    v[0] += lo.x;
+    v[1] += lo.y;
+    v[2] += hi.x;
+    v[3] += hi.y;
+  }
+  block_sum4(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bcast[k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[k] = bcast[k];
+  __syncthreads();
 }
 
 // Fixed-order sum of n partials computed by every workgroup that calls it
@@ -132,12 +181,12 @@ __device__ inline void store_relaxed(int* p, int v) {
 // ---------------------------------------------------------------------------
 enum : int { DIAG_ZERO = 0, DIAG_ONE = 1, DIAG_Q = 2 };
 // how the multiplied vector is obtained for each local node
-enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2 };
+enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2, STAGE_CG1 = 3 };
 // what happens at owned nodes after the element loop
 enum : int { EPI_OP = 0, EPI_RESID = 1, EPI_CGAD = 2, EPI_RESID_MIXED = 3, EPI_DOTW = 4,
-             EPI_DOT2 = 5 };
+             EPI_DOT2 = 5, EPI_CG1 = 6 };
 // whether the kernel writes a partial sum per workgroup / runs the CG control
-enum : int { FIN_NONE = 0, FIN_PARTIAL = 1, FIN_CG = 2 };
+enum : int { FIN_NONE = 0, FIN_PARTIAL = 1, FIN_CG = 2, FIN_CG1 = 3 };
 
 template <typename T, typename TA>
 struct EbeArgs {
@@ -170,6 +219,13 @@ struct EbeArgs {
   double* partial2;           // DOT2: out = A y, partials out.y and out.out
   float* r32;                 // RESID_MIXED: inner residual -(A x - b)
   float* zero32;              // RESID_MIXED: inner solution reset
+  // single-kernel CG iteration (STAGE_CG1 / EPI_CG1 / FIN_CG1): r, dold and
+  // ad_in are R, D, AD of iteration M - 1; r_out, dnew and out receive those
+  // of iteration M; xs is updated in place
+  const T* ad_in;
+  T* r_out;
+  T* xs;
+  int first;                  // iteration 0: D = R, no update
   // reductions / state
   double* partial;            // this kernel's partials (one per workgroup)
   const double* part_in;      // FIN_CG: partials of the preceding update kernel
@@ -239,6 +295,20 @@ __device__ inline void load_phase(PhaseSlot<T, NDP, NC>& ps, int c, const int* s
   }
 }
 
+// node values read by the CG staging passes (global index, R, D, AD, X)
+template <typename T>
+struct NodeIn {
+  int g;
+  T r, d, ad, x;
+};
+// CG staging: the first node of each thread is loaded ahead of the scalar
+// reduction; the others kStageUnroll at a time (their loads issued together)
+#ifndef BHIP_STAGE_UNROLL
+#define BHIP_STAGE_UNROLL 4
+#endif
+constexpr int kStagePre = 1;
+constexpr int kStageUnroll = BHIP_STAGE_UNROLL;
+
 // PF > 0: the colour phases are software pipelined: each thread keeps the
 // element data of the next PF phases in flight in registers, so a phase
 // waits on loads issued PF phases earlier instead of paying a full HBM round
@@ -251,6 +321,8 @@ k_ebe(EbeArgs<T, TA> a) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
   __shared__ int sh_co[kMaxColors + 1];
+  __shared__ double sh_red4[FIN == FIN_CG1 ? kWaves * 4 : 1];
+  __shared__ double sh_b4[4];
   constexpr int npair = NDP * (NDP - 1) / 2;
   constexpr int ncoef = SYMQ ? 2 * npair : npair;
 
@@ -280,18 +352,74 @@ k_ebe(EbeArgs<T, TA> a) {
       load_phase<T, TA, NDP, ncoef>(pipe[q], q, sh_co, se, a.lid, a.xa, a.nslot_pad);
   }
 
-  double cg_beta = 0.0;
+  const int n0 = a.node_start[b], n1 = a.node_start[b + 1];
+  const int nown = n1 - n0;
+  const int h0 = a.halo_start[b], h1 = a.halo_start[b + 1];
+  const int nloc = nown + (h1 - h0);
+
+  // ---- early exits of the CG kernels (state written by earlier kernels) --
   if constexpr (FIN == FIN_CG) {
-    // E_m: RMRM from the update kernel's partials, stop test (gracjg.F:306-324,
-    // and the strict test of the initial residual, gracjg.F:219), BETA
     SolveState* st = a.st;
     if (a.skip_if_stalled && st->stalled) return;
     if (load_relaxed(&st->done)) return;
     if (load_relaxed(&st->final_flag)) {
       // the update kernel before this one applied the last X update
-      if (blockIdx.x == 0 && threadIdx.x == 0) store_relaxed(&st->done, ST_MAXITER);
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        store_relaxed(&st->done, ST_MAXITER);
+        store_relaxed(&st->done1, (st->final_m << 2) | ST_MAXITER);
+      }
       return;
     }
+  }
+  if constexpr (FIN == FIN_CG1) {
+    SolveState* st = a.st;
+    if (a.skip_if_stalled && st->stalled) return;
+    if (load_relaxed(&st->done)) return;
+    if (load_relaxed(&st->done1)) return;
+    if (!a.first) {
+      const int prev_stop = st->stop[1 - a.parity];
+      if (prev_stop) {
+        // the previous kernel finished the solve: publish it, no work
+        if (blockIdx.x == 0 && threadIdx.x == 0) store_relaxed(&st->done1, prev_stop);
+        return;
+      }
+    }
+  }
+
+  // ---- node values of the staging, loaded before the scalar reduction ----
+  // (the first kStagePre nodes of each thread) so that their HBM latency
+  // overlaps the reduction of the previous kernel's partial sums
+  constexpr bool kPre = STAGE == STAGE_CGDIR || STAGE == STAGE_CG1;
+  constexpr int KS = kPre ? kStagePre : 1;
+  NodeIn<T> pre[KS];
+  auto load_node = [&](int l, NodeIn<T>& v) {
+    const bool owned = l < nown;
+    v.g = owned ? n0 + l : a.halo[h0 + l - nown];
+    if constexpr (STAGE == STAGE_CGDIR) {
+      v.r = a.r[v.g];
+      v.d = a.dold[v.g];
+    } else if constexpr (STAGE == STAGE_CG1) {
+      v.r = a.r[v.g];
+      if (!a.first) {
+        v.d = a.dold[v.g];
+        v.ad = a.ad_in[v.g];
+        if (owned) v.x = a.xs[v.g];
+      }
+    }
+  };
+  if constexpr (kPre) {
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int l = threadIdx.x + k * kThreads;
+      if (l < nloc) load_node(l, pre[k]);
+    }
+  }
+
+  double cg_beta = 0.0;
+  if constexpr (FIN == FIN_CG) {
+    // E_m: RMRM from the update kernel's partials, stop test (gracjg.F:306-324,
+    // and the strict test of the initial residual, gracjg.F:219), BETA
+    SolveState* st = a.st;
     const int p = a.parity;
     const double rr = reduce_partials(a.part_in, a.n_part_in, sh_red, &sh_b);
     const int m = st->m[1 - p] + 1;
@@ -302,6 +430,7 @@ k_ebe(EbeArgs<T, TA> a) {
         st->final_m = m;
         st->final_rr = rr;
         store_relaxed(&st->done, ST_CONVERGED);
+        store_relaxed(&st->done1, (m << 2) | ST_CONVERGED);
       }
       return;
     }
@@ -313,38 +442,123 @@ k_ebe(EbeArgs<T, TA> a) {
     }
   }
 
-  const int n0 = a.node_start[b], n1 = a.node_start[b + 1];
-  const int nown = n1 - n0;
-  const int h0 = a.halo_start[b], h1 = a.halo_start[b + 1];
-  const int nloc = nown + (h1 - h0);
+  // Single-kernel CG iteration M (inner solves of the mixed path): the
+  // updates of iteration M - 1 are applied while staging,
+  //   RO = RR / D.AD, X -= RO D, R -= RO AD, D = R + BETA D,
+  // with the new ||R||^2 = RR - 2 RO R.AD + RO^2 AD.AD from the dots of
+  // iteration M - 1 (one reduction point per iteration); the kernel that
+  // finds the stop applies the last X update and records it in stop[parity].
+  T cg1_rho = (T)0, cg1_beta = (T)0;
+  if constexpr (FIN == FIN_CG1) {
+    SolveState* st = a.st;
+    const int p = a.parity;
+    const int m = st->m[1 - p] + 1;
+    if (!a.first) {
+      double s4[4];
+      reduce_partials4(a.part_in, a.n_part_in, sh_red4, sh_b4, s4);
+      const double dad = s4[0], rad = s4[1], adad = s4[2], rrp = s4[3];
+      const double rho = rrp / dad;
+      double rr = rrp - 2.0 * rho * rad + rho * rho * adad;
+      if (rr < 0.0) rr = 0.0;
+      const double thr = st->use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
+      const bool conv = rr <= thr;
+      const bool fin = m >= st->nitmax;
+      cg1_rho = (T)rho;
+      if (conv || fin) {
+        for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+          const int g = n0 + l;
+          a.xs[g] = a.xs[g] - cg1_rho * a.dold[g];
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+          st->final_m = m;
+          st->final_rr = rr;
+          st->m[p] = m;
+          st->stop[p] = (m << 2) | (conv ? ST_CONVERGED : ST_MAXITER);
+        }
+        return;
+      }
+      cg1_beta = (T)(rr / rrp);
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr[p] = rr;
+        st->m[p] = m;
+      }
+    } else {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->m[p] = m;
+    }
+  }
 
   T* ys = reinterpret_cast<T*>(smem_raw);
   T* acc = ys + ((nloc + 1) & ~1);
+  T* rs = acc + ((nown + 1) & ~1);  // EPI_CG1: R of the owned nodes
 
   // ---- stage the multiplied vector (owned + halo) into LDS ---------------
-  for (int l = threadIdx.x; l < nloc; l += blockDim.x) {
-    const bool owned = l < nown;
-    const int g = owned ? n0 + l : a.halo[h0 + l - nown];
-    T v;
-    if constexpr (STAGE == STAGE_PLAIN) {
-      v = a.y[g];
-    } else if constexpr (STAGE == STAGE_CGDIR) {
-      const T beta = (T)cg_beta;
-      v = a.r[g] + beta * a.dold[g];
-      if (owned) a.dnew[g] = v;
-    } else {
-      const double xv = a.xd[g] - (double)a.delta[g];
-      v = (T)xv;
-      if (owned) a.xnew[g] = xv;
-    }
+  auto put = [&](int l, int g, T v) {
     ys[l] = v;
     // diagonal term (mv0303.f:150-160)
-    if (owned) {
+    if (l < nown) {
       T dv;
       if constexpr (DIAG == DIAG_Q) dv = (T)a.da[g] * v;
       else if constexpr (DIAG == DIAG_ONE) dv = v;
       else dv = (T)0;
       acc[l] = dv;
+    }
+  };
+  if constexpr (kPre) {
+    auto stage_node = [&](int l, const NodeIn<T>& v) {
+      const bool owned = l < nown;
+      const int g = v.g;
+      T val;
+      if constexpr (STAGE == STAGE_CGDIR) {
+        val = v.r + (T)cg_beta * v.d;
+        if (owned) a.dnew[g] = val;
+      } else {  // STAGE_CG1
+        T rv = v.r;
+        if (a.first) {
+          val = rv;
+        } else {
+          rv = rv - cg1_rho * v.ad;
+          val = rv + cg1_beta * v.d;
+          if (owned) a.xs[g] = v.x - cg1_rho * v.d;
+        }
+        if (owned) {
+          a.r_out[g] = rv;
+          a.dnew[g] = val;
+          rs[l] = rv;
+        }
+      }
+      put(l, g, val);
+    };
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int l = threadIdx.x + k * kThreads;
+      if (l < nloc) stage_node(l, pre[k]);
+    }
+    for (int l0 = threadIdx.x + KS * kThreads; l0 < nloc; l0 += kStageUnroll * kThreads) {
+      NodeIn<T> v[kStageUnroll];
+#pragma unroll
+      for (int u = 0; u < kStageUnroll; ++u) {
+        const int l = l0 + u * kThreads;
+        if (l < nloc) load_node(l, v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kStageUnroll; ++u) {
+        const int l = l0 + u * kThreads;
+        if (l < nloc) stage_node(l, v[u]);
+      }
+    }
+  } else {
+    for (int l = threadIdx.x; l < nloc; l += blockDim.x) {
+      const bool owned = l < nown;
+      const int g = owned ? n0 + l : a.halo[h0 + l - nown];
+      T v;
+      if constexpr (STAGE == STAGE_PLAIN) {
+        v = a.y[g];
+      } else {  // STAGE_XUPD
+        const double xv = a.xd[g] - (double)a.delta[g];
+        v = (T)xv;
+        if (owned) a.xnew[g] = xv;
+      }
+      put(l, g, v);
     }
   }
   __syncthreads();
@@ -391,7 +605,7 @@ k_ebe(EbeArgs<T, TA> a) {
     }
   }
   // ---- epilogue at owned nodes -------------------------------------------
-  double part = 0.0, part2 = 0.0;
+  double part = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
   for (int l = threadIdx.x; l < nown; l += blockDim.x) {
     const int g = n0 + l;
     const T v = acc[l];
@@ -420,6 +634,14 @@ k_ebe(EbeArgs<T, TA> a) {
       a.out[g] = v;
       part += (double)v * (double)ys[l];
       part2 += (double)v * (double)v;
+    } else if constexpr (EPI == EPI_CG1) {
+      // D.AD, R.AD, AD.AD, R.R of this iteration
+      a.out[g] = v;
+      const double dv = (double)v, rv = (double)rs[l];
+      part += (double)ys[l] * dv;
+      part2 += rv * dv;
+      part3 += dv * dv;
+      part4 += rv * rv;
     } else {  // EPI_RESID_MIXED: FP64 residual, inner RHS in FP32
       const double rv = (double)v - a.b[g];
       a.r32[g] = (float)(-rv);
@@ -427,7 +649,15 @@ k_ebe(EbeArgs<T, TA> a) {
       part += rv * rv;
     }
   }
-  if constexpr (FIN != FIN_NONE) {
+  if constexpr (FIN == FIN_CG1) {
+    double v4[4] = {part, part2, part3, part4};
+    block_sum4(v4, sh_red4);
+    if (threadIdx.x == 0) {
+      double2* o = reinterpret_cast<double2*>(a.partial + (size_t)4 * b);
+      o[0] = make_double2(v4[0], v4[1]);
+      o[1] = make_double2(v4[2], v4[3]);
+    }
+  } else if constexpr (FIN != FIN_NONE) {
     double s = block_sum(part, sh_red);
     if (threadIdx.x == 0) a.partial[b] = s;
     if constexpr (EPI == EPI_DOT2) {
@@ -503,9 +733,14 @@ __global__ void k_reduce_into(const double* __restrict__ partial, int n, double*
   if (threadIdx.x == 0) *dst = s;
 }
 
+// an inner solve has applied its last update (either CG form)
+__device__ inline bool inner_finished(const SolveState* st) {
+  return st->done1 != 0 || st->stop[0] != 0 || st->stop[1] != 0;
+}
+
 // the mixed outer step must wait: solved, stalled, or its inner solve still running
 __device__ inline bool outer_blocked(const SolveState* st) {
-  return st->outer_done || st->stalled || (st->outer_count > 0 && st->done == 0);
+  return st->outer_done || st->stalled || (st->outer_count > 0 && !inner_finished(st));
 }
 
 __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int nitmax) {
@@ -519,6 +754,8 @@ __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int 
   st->final_m = 0;
   st->final_rr = 0.0;
   st->done = 0;
+  st->done1 = 0;
+  st->stop[0] = st->stop[1] = 0;
 }
 
 // GRACJG set-up (gracjg.F:189-209) from the partial sums of ||b'||^2: XL,
@@ -563,7 +800,7 @@ __global__ void k_outer_check(const double* __restrict__ partial, int n, SolveSt
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
   if (st->outer_done || st->stalled) return;
-  if (st->outer_count > 0 && st->done == 0) {
+  if (st->outer_count > 0 && !inner_finished(st)) {
     if (threadIdx.x == 0) st->stalled = 1;  // inner solve still running
     return;
   }
@@ -646,30 +883,39 @@ __global__ void k_precd1_nodes(double* __restrict__ x, double* __restrict__ b,
 }
 
 // OM 'M=DMD' on element coefficients: XM(e,c) = XM(e,c)*D(IKLE(e,j))*D(IKLE(e,i))
-// for the pair (i<j) of column c (om1111.f:583-600, om4141.f), per slot.
+// for the pair (i<j) of column c (om1111.f:583-600, om4141.f), per slot. One
+// workgroup per node block: D of its owned and halo nodes is staged in LDS,
+// so the per-slot gathers are LDS reads. xa32 (optional) receives an FP32
+// copy of the result.
 template <int NDP>
-__global__ void k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, int ncoef,
-                            int nslot_pad,
-                            const uint16_t* __restrict__ lid, const double* __restrict__ d,
-                            const int32_t* __restrict__ node_start,
-                            const int32_t* __restrict__ halo_start,
-                            const int32_t* __restrict__ halo,
-                            const int32_t* __restrict__ color_off, int nblocks) {
-  const int b = blockIdx.x;
-  if (b >= nblocks) return;
+__global__ void __launch_bounds__(kThreads)
+k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, int ncoef, int nslot_pad,
+            const uint16_t* __restrict__ lid, const double* __restrict__ d,
+            const int32_t* __restrict__ node_start, const int32_t* __restrict__ halo_start,
+            const int32_t* __restrict__ halo, const int32_t* __restrict__ color_off,
+            int nblocks) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* ds = reinterpret_cast<double*>(smem_raw);
+  const int nb = nblocks;
+  int b;
+  {
+    const int g = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = g & 7, k = g >> 3;
+    b = xcd * q + (xcd < r ? xcd : r) + k;
+  }
   const int n0 = node_start[b], nown = node_start[b + 1] - n0;
-  const int h0 = halo_start[b];
+  const int h0 = halo_start[b], nloc = nown + (halo_start[b + 1] - h0);
+  for (int l = threadIdx.x; l < nloc; l += blockDim.x)
+    ds[l] = d[l < nown ? n0 + l : halo[h0 + l - nown]];
+  __syncthreads();
   const int s0 = color_off[(size_t)b * (kMaxColors + 1)];
   const int s1 = color_off[(size_t)b * (kMaxColors + 1) + kMaxColors];
   constexpr int npair = NDP * (NDP - 1) / 2;
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
     double dv[NDP];
 #pragma unroll
-    for (int k = 0; k < NDP; ++k) {
-      int l = lid[(size_t)k * nslot_pad + s];
-      int g = l < nown ? n0 + l : halo[h0 + l - nown];
-      dv[k] = d[g];
-    }
+    for (int k = 0; k < NDP; ++k) dv[k] = ds[lid[(size_t)k * nslot_pad + s]];
     int q = 0;
 #pragma unroll
     for (int i = 0; i < NDP; ++i)
@@ -688,6 +934,17 @@ __global__ void k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, i
         ++q;
       }
   }
+}
+
+// dst[i] = src[i], i < n (16-byte vectors; both 16-byte aligned)
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, long n) {
+  const long n2 = n >> 1;
+  double2* d2 = reinterpret_cast<double2*>(dst);
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2;
+       i += (long)gridDim.x * blockDim.x)
+    d2[i] = s2[i];
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
 }
 
 // xa_slots[c][s] = xa_caller[c][slot_elem[s]]
@@ -753,9 +1010,11 @@ struct bhip_ctx {
   bhip::DevBuf<float> da32, xa32;
   // work vectors (internal order)
   bhip::DevBuf<double> vx, vb, vr, vd0, vd1, vad, vdz, vdinv, vio0, vio1;
-  bhip::DevBuf<float> fr, fd0, fd1, fad, fx0, fx1, fdz;
+  bhip::DevBuf<float> fr, fd0, fd1, fad, fx0, fx1, fdz, fr1, fad1;
   bhip::DevBuf<double> partial;  // 4 arrays of max_part: U[0], U[1], E[0], E[1]
   int max_part = 0;
+  bhip::DevBuf<double> part4;    // single-kernel CG: 2 parities x [nblocks][4]
+  bool inner_cg1 = true;         // mixed inner solver: single-kernel CG iterations
   bhip::DevBuf<bhip::SolveState> st;
   // captured CG iteration batches (key: precision, solution buffer)
   struct Graph {
@@ -765,6 +1024,13 @@ struct bhip_ctx {
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[4];
+  // the planned part of whole solves (set-up, expected iteration batches),
+  // captured once per plan and replayed as one graph
+  struct SolveGraph {
+    uint64_t key[6];
+    hipGraphExec_t exec;
+  };
+  std::vector<SolveGraph> solve_graphs;
   bool use_graphs = true;
   int prefetch_j = 0;  // register prefetch depth of the element kernels
   int iter_hint[4] = {0, 0, 0, 0};  // FP64 CG, inner CG, inner solves, CGSTAB
@@ -785,6 +1051,9 @@ static size_t lds_bytes(const Layout& L, size_t elt) {
   size_t nl = ((size_t)L.max_local + 1) & ~(size_t)1;
   return (nl + (size_t)L.max_owned) * elt;
 }
+
+// LDS of the DMD kernel: D of a block's owned and halo nodes
+static size_t dmd_lds(const Layout& L) { return ((size_t)L.max_local + 1) * sizeof(double); }
 
 static int grid_for(int n) {
   int g = (n + kThreads * 8 - 1) / (kThreads * 8);
@@ -813,7 +1082,9 @@ static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
 template <typename T, typename TA, int STAGE, int EPI, int FIN>
 static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typdia,
                              char typext) {
-  const size_t lds = lds_bytes(c->L, sizeof(T));
+  // the single-kernel CG keeps R of the owned nodes in LDS as well
+  const size_t lds = lds_bytes(c->L, sizeof(T)) +
+                     (FIN == FIN_CG1 ? ((size_t)c->L.max_owned + 2) * sizeof(T) : 0);
   dim3 grid(c->L.nblocks), block(kThreads);
   const int diag = typdia == 'Q' ? DIAG_Q : (typdia == 'I' ? DIAG_ONE : DIAG_ZERO);
   // TYPEXT '0' is stored as zero symmetric coefficients
@@ -933,7 +1204,13 @@ int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const do
                      c->stream, c->partial.p, 0.0, (long)4 * c->max_part);
   HIP_OK(hipGetLastError());
   if (const char* ev = getenv("BHIP_NO_GRAPH")) c->use_graphs = !(ev[0] == '1');
-  c->prefetch_j = 2;
+  // BHIP_INNER_CG=2: the two-kernel (GRACJG-form) inner CG of the mixed path
+  if (const char* ev = getenv("BHIP_INNER_CG")) c->inner_cg1 = !(ev[0] == '2');
+  HIP_OK(c->part4.alloc((size_t)8 * (L.nblocks + 16)));
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(8 * (L.nblocks + 16))), dim3(kThreads), 0,
+                     c->stream, c->part4.p, 0.0, (long)8 * (L.nblocks + 16));
+  HIP_OK(hipGetLastError());
+  c->prefetch_j = 1;
   if (const char* ev = getenv("BHIP_PREFETCH")) {
     int v = atoi(ev);
     if (v >= 0 && v <= 3) c->prefetch_j = v;
@@ -962,11 +1239,14 @@ int bhip_ctx_destroy(bhip_ctx* c) {
                   &c->vdinv, &c->vio0, &c->vio1, &c->partial})
     v->release();
   for (auto* v : {&c->da32, &c->xa32, &c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1,
-                  &c->fdz})
+                  &c->fdz, &c->fr1, &c->fad1})
     v->release();
+  c->part4.release();
   c->st.release();
   for (auto& gr : c->graphs)
     if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+  for (auto& sg : c->solve_graphs)
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1071,11 +1351,11 @@ int bhip_matrix_save(bhip_ctx* c) {
   const int npair = L.ndp * (L.ndp - 1) / 2;
   HIP_OK(c->da_saved.alloc(L.npoin));
   HIP_OK(c->xa_saved.alloc((size_t)2 * npair * L.nslot_pad));
-  HIP_OK(hipMemcpyAsync(c->da_saved.p, c->da.p, sizeof(double) * L.npoin,
-                        hipMemcpyDeviceToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(c->xa_saved.p, c->xa.p,
-                        sizeof(double) * (size_t)std::max(c->ncoef, npair) * L.nslot_pad,
-                        hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(kThreads), 0, c->stream, c->da_saved.p, c->da.p,
+                     (long)L.npoin);
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(kThreads), 0, c->stream, c->xa_saved.p, c->xa.p,
+                     (long)std::max(c->ncoef, npair) * L.nslot_pad);
+  HIP_OK(hipGetLastError());
   c->typdia_saved = c->typdia;
   c->typext_saved = c->typext;
   c->ncoef_saved = c->ncoef;
@@ -1088,11 +1368,11 @@ int bhip_matrix_restore(bhip_ctx* c) {
   if (!c || !c->have_saved) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int npair = L.ndp * (L.ndp - 1) / 2;
-  HIP_OK(hipMemcpyAsync(c->da.p, c->da_saved.p, sizeof(double) * L.npoin,
-                        hipMemcpyDeviceToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(c->xa.p, c->xa_saved.p,
-                        sizeof(double) * (size_t)std::max(c->ncoef_saved, npair) * L.nslot_pad,
-                        hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(kThreads), 0, c->stream, c->da.p, c->da_saved.p,
+                     (long)L.npoin);
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(kThreads), 0, c->stream, c->xa.p, c->xa_saved.p,
+                     (long)std::max(c->ncoef_saved, npair) * L.nslot_pad);
+  HIP_OK(hipGetLastError());
   c->typdia = c->typdia_saved;
   c->typext = c->typext_saved;
   c->ncoef = c->ncoef_saved;
@@ -1213,6 +1493,8 @@ static void init_cg_state(SolveState* sh, double eps2xl, int use_rr0, int nitmax
   sh->status[0] = sh->status[1] = ST_RUNNING;
   sh->rr[0] = sh->rr[1] = 1.0;
   sh->done = 0;
+  sh->done1 = 0;
+  sh->stop[0] = sh->stop[1] = 0;
   sh->final_flag = 0;
   sh->final_m = 0;
   sh->final_rr = 0.0;
@@ -1252,15 +1534,17 @@ static hipError_t enqueue_cg_batch(bhip_ctx* c, int iters, TX* x, T* r, T* d0, T
 // The graph (or plain launches) of one batch of kBatch CG iterations.
 constexpr int kBatch = 8;
 
-template <typename T, typename TA, typename TX>
-static int cg_batch_exec(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad,
-                         const TA* xa, const TA* da, hipGraphExec_t* out) {
+// A captured batch of CG iterations, cached per (solution buffer, kernel
+// family, TYPDIA/TYPEXT specialisation).
+template <typename F>
+static int cached_graph(bhip_ctx* c, const void* key, int prec, F&& enqueue,
+                        hipGraphExec_t* out) {
   *out = nullptr;
   if (!c->use_graphs) return BHIP_OK;
   int slot = -1;
   const int tag = ((int)(unsigned char)c->typdia << 8) | (int)(unsigned char)c->typext;
   for (int i = 0; i < 4; ++i)
-    if (c->graphs[i].exec && c->graphs[i].key == (const void*)x && c->graphs[i].prec == prec &&
+    if (c->graphs[i].exec && c->graphs[i].key == key && c->graphs[i].prec == prec &&
         c->graphs[i].tag == tag)
       slot = i;
   if (slot < 0) {
@@ -1273,18 +1557,67 @@ static int cg_batch_exec(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad
     }
     hipGraph_t graph;
     HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da);
+    hipError_t e = enqueue();
     hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
     HIP_OK(e);
     HIP_OK(e2);
     HIP_OK(hipGraphInstantiate(&c->graphs[slot].exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
-    c->graphs[slot].key = (const void*)x;
+    c->graphs[slot].key = key;
     c->graphs[slot].prec = prec;
     c->graphs[slot].tag = tag;
   }
   *out = c->graphs[slot].exec;
   return BHIP_OK;
+}
+
+template <typename T, typename TA, typename TX>
+static int cg_batch_exec(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad,
+                         const TA* xa, const TA* da, hipGraphExec_t* out) {
+  return cached_graph(c, (const void*)x, prec, [&]() {
+    return enqueue_cg_batch<T, TA, TX>(c, kBatch, x, r, d0, d1, ad, xa, da);
+  }, out);
+}
+
+// Runs `plan` (a function queuing kernels only) on the context stream: as a
+// graph captured on first use for this key and replayed afterwards, or
+// directly when graphs are off.
+template <typename F>
+static int run_planned(bhip_ctx* c, const uint64_t (&key)[6], F&& plan) {
+  if (!c->use_graphs) return plan(false);
+  hipGraphExec_t exec = nullptr;
+  for (auto& sg : c->solve_graphs)
+    if (std::equal(std::begin(key), std::end(key), std::begin(sg.key))) exec = sg.exec;
+  if (!exec) {
+    if (c->solve_graphs.size() >= 8) {
+      (void)hipGraphExecDestroy(c->solve_graphs.front().exec);
+      c->solve_graphs.erase(c->solve_graphs.begin());
+    }
+    hipGraph_t graph;
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = plan(true);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (rc) return rc;
+    HIP_OK(e2);
+    HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    bhip_ctx::SolveGraph sg;
+    std::copy(std::begin(key), std::end(key), std::begin(sg.key));
+    sg.exec = exec;
+    c->solve_graphs.push_back(sg);
+  }
+  HIP_OK(hipGraphLaunch(exec, c->stream));
+  return BHIP_OK;
+}
+
+static uint64_t dbits(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, sizeof(u));
+  return u;
+}
+
+static uint64_t matrix_tag(const bhip_ctx* c) {
+  return ((uint64_t)(unsigned char)c->typdia << 8) | (uint64_t)(unsigned char)c->typext;
 }
 
 template <typename T, typename TA, typename TX>
@@ -1336,9 +1669,16 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
   const int n = c->L.npoin;
   const int g = c->grid_nodes;
   SolveState* sh = c->st_host;
-  HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 0, 0.0, 0));
-  {
-    // R = A X - B (gracjg.F:213-215), partial sums of RMRM
+  hipGraphExec_t exec;
+  int rc = cg_batch_exec<double, double, double>(c, BHIP_PREC_FP64, x, c->vr.p, c->vd0.p,
+                                                 c->vd1.p, c->vad.p, c->xa.p, c->da.p, &exec);
+  if (rc) return rc;
+  int& hint = c->iter_hint[0];
+  int nb = std::max(1, (hint + 1 + kBatch - 1) / kBatch);
+  // set-up, R = A X - B (gracjg.F:213-215) with the partial sums of RMRM,
+  // E_0, and the iteration batches the previous solve needed
+  auto plan = [&](bool capturing) -> int {
+    HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 0, 0.0, 0));
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.y = x;
     a.b = b;
@@ -1346,22 +1686,22 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
     a.partial = part_u(c, 0);
     HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
-  }
-  HIP_OK((enqueue_cg_start<double, double>(c, c->vr.p, c->vd0.p, c->vad.p, c->vdz.p, c->xa.p,
-                                           c->da.p, c->L.nblocks, 0)));
-  hipGraphExec_t exec;
-  int rc = cg_batch_exec<double, double, double>(c, BHIP_PREC_FP64, x, c->vr.p, c->vd0.p,
-                                                 c->vd1.p, c->vad.p, c->xa.p, c->da.p, &exec);
-  if (rc) return rc;
-  int& hint = c->iter_hint[0];
-  int nb = std::max(1, (hint + 1 + kBatch - 1) / kBatch);
+    HIP_OK((enqueue_cg_start<double, double>(c, c->vr.p, c->vd0.p, c->vad.p, c->vdz.p, c->xa.p,
+                                             c->da.p, c->L.nblocks, 0)));
+    return enqueue_batches<double, double, double>(c, capturing ? nullptr : exec, nb, x, c->vr.p,
+                                                   c->vd0.p, c->vd1.p, c->vad.p, c->xa.p,
+                                                   c->da.p);
+  };
+  const uint64_t key[6] = {0, (uint64_t)nb, matrix_tag(c), dbits(eps), (uint64_t)nitmax,
+                           (uint64_t)(uintptr_t)x};
+  if ((rc = run_planned(c, key, plan))) return rc;
   for (;;) {
-    rc = enqueue_batches<double, double, double>(c, exec, nb, x, c->vr.p, c->vd0.p, c->vd1.p,
-                                                 c->vad.p, c->xa.p, c->da.p);
-    if (rc) return rc;
     if (read_state(c)) return BHIP_EDEVICE;
     if (sh->done) break;
     nb = std::min(nb * 2, 8);
+    rc = enqueue_batches<double, double, double>(c, exec, nb, x, c->vr.p, c->vd0.p, c->vd1.p,
+                                                 c->vad.p, c->xa.p, c->da.p);
+    if (rc) return rc;
   }
   info->relat = sh->xl_raw >= 1.0;
   if (sh->zero_rhs) {
@@ -1382,7 +1722,8 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
 static int ensure_f32(bhip_ctx* c) {
   const Layout& L = c->L;
   const int n = L.npoin;
-  for (auto* v : {&c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1, &c->fdz})
+  for (auto* v : {&c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1, &c->fdz, &c->fr1,
+                  &c->fad1})
     HIP_OK(v->alloc(n));
   HIP_OK(c->da32.alloc(n));
   const int npair = L.ndp * (L.ndp - 1) / 2;
@@ -1391,6 +1732,49 @@ static int ensure_f32(bhip_ctx* c) {
                      c->fdz.p, 0.0f, (long)n);
   HIP_OK(hipGetLastError());
   return BHIP_OK;
+}
+
+// ---- single-kernel FP32 CG (inner solves of the mixed path) --------------
+struct Cg1Bufs {
+  float* x;      // the inner solution (the correction), updated in place
+  float* r[2];   // R, D, AD by iteration parity
+  float* d[2];
+  float* ad[2];
+};
+
+static double* part4(bhip_ctx* c, int p) {
+  return c->part4.p + (size_t)p * 4 * (c->L.nblocks + 16);
+}
+
+// iteration of parity p; iteration 0 (first) takes R from r[1]
+static hipError_t launch_cg1(bhip_ctx* c, const Cg1Bufs& v, int p, int first,
+                             int skip_if_stalled) {
+  EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
+  const int q = first ? 1 : 1 - p;
+  a.r = v.r[q];
+  a.dold = v.d[q];
+  a.ad_in = v.ad[q];
+  a.r_out = v.r[p];
+  a.dnew = v.d[p];
+  a.out = v.ad[p];
+  a.xs = v.x;
+  a.partial = part4(c, p);
+  a.part_in = part4(c, 1 - p);
+  a.n_part_in = c->L.nblocks;
+  a.parity = p;
+  a.first = first;
+  a.skip_if_stalled = skip_if_stalled;
+  return launch_ebe<float, float, STAGE_CG1, EPI_CG1, FIN_CG1>(c, a, false, c->typdia,
+                                                                c->typext);
+}
+
+// iterations 1, 2, ... of a batch (parities 1, 0, 1, ...)
+static hipError_t enqueue_cg1_batch(bhip_ctx* c, int iters, const Cg1Bufs& v) {
+  for (int k = 0; k < iters; ++k) {
+    hipError_t e = launch_cg1(c, v, (k & 1) ? 0 : 1, 0, 0);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Mixed precision CG: FP64 true residual r = A'x - b' and FP64 update
@@ -1409,20 +1793,33 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   if (ensure_f32(c)) return BHIP_EDEVICE;
   const int npair = L.ndp * (L.ndp - 1) / 2;
   const long ncf = (long)(c->typext == 'Q' ? 2 * npair : npair) * L.nslot_pad;
-  if (!xa32_ready)
-    hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
-                       ncf);
-  hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
-                     (long)n);
-  HIP_OK(hipGetLastError());
   float* delta = c->fx0.p;
-  hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, delta, 0.0f, (long)n);
-  HIP_OK(hipGetLastError());
-  HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax));
+  const bool cg1 = c->inner_cg1;
+  const Cg1Bufs v1 = {delta, {c->fr.p, c->fr1.p}, {c->fd0.p, c->fd1.p}, {c->fad.p, c->fad1.p}};
   hipGraphExec_t exec;
-  int rc = cg_batch_exec<float, float, float>(c, BHIP_PREC_MIXED, delta, c->fr.p, c->fd0.p,
-                                              c->fd1.p, c->fad.p, c->xa32.p, c->da32.p, &exec);
+  int rc;
+  if (cg1)
+    rc = cached_graph(c, (const void*)delta, 2, [&]() { return enqueue_cg1_batch(c, kBatch, v1); },
+                      &exec);
+  else
+    rc = cg_batch_exec<float, float, float>(c, BHIP_PREC_MIXED, delta, c->fr.p, c->fd0.p,
+                                            c->fd1.p, c->fad.p, c->xa32.p, c->da32.p, &exec);
   if (rc) return rc;
+  // more iterations of the running inner solve (raw: plain launches, as
+  // inside a capture)
+  auto more = [&](int nbat, bool raw) -> int {
+    hipGraphExec_t ex = raw ? nullptr : exec;
+    if (!cg1)
+      return enqueue_batches<float, float, float>(c, ex, nbat, delta, c->fr.p, c->fd0.p,
+                                                  c->fd1.p, c->fad.p, c->xa32.p, c->da32.p);
+    for (int k = 0; k < nbat; ++k) {
+      if (ex)
+        HIP_OK(hipGraphLaunch(ex, c->stream));
+      else
+        HIP_OK(enqueue_cg1_batch(c, kBatch, v1));
+    }
+    return BHIP_OK;
+  };
   // outer step: x -= delta ; r = A x - b, inner RHS -r, delta = 0 ; test
   auto outer_step = [&]() -> int {
     hipLaunchKernelGGL(k_outer_xupd, dim3(g), dim3(kThreads), 0, c->stream, x, delta, n,
@@ -1431,7 +1828,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.y = x;
     a.b = b;
-    a.r32 = c->fr.p;
+    a.r32 = cg1 ? v1.r[1] : c->fr.p;
     a.zero32 = delta;
     a.partial = part_u(c, 0);
     a.skip_if_outer_done = 1;
@@ -1442,21 +1839,42 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     HIP_OK(hipGetLastError());
     return BHIP_OK;
   };
-  auto inner_solve = [&](int nbat) -> int {
-    HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
-                                           c->da32.p, L.nblocks, 1)));
-    return enqueue_batches<float, float, float>(c, exec, nbat, delta, c->fr.p, c->fd0.p,
-                                                c->fd1.p, c->fad.p, c->xa32.p, c->da32.p);
+  auto inner_solve = [&](int nbat, bool raw) -> int {
+    if (cg1)
+      HIP_OK(launch_cg1(c, v1, 0, 1, 1));
+    else
+      HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
+                                             c->da32.p, L.nblocks, 1)));
+    return more(nbat, raw);
   };
   int& hint_in = c->iter_hint[1];   // iterations of one inner solve
   int& hint_out = c->iter_hint[2];  // inner solves of one mixed solve
   const int nbat = std::max(1, (hint_in + 1 + kBatch - 1) / kBatch);
-  rc = outer_step();
-  if (rc) return rc;
-  for (int k = 0; k < std::max(hint_out, 1); ++k) {
-    if ((rc = inner_solve(nbat))) return rc;
-    if ((rc = outer_step())) return rc;
-  }
+  const int nout = std::max(hint_out, 1);
+  // the planned part: FP32 copies, set-up, then the outer and inner steps
+  // the previous solve needed (the device skips what is not needed)
+  auto plan = [&](bool capturing) -> int {
+    if (!xa32_ready)
+      hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
+                         ncf);
+    hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
+                       (long)n);
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, delta, 0.0f,
+                       (long)n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax));
+    int r = outer_step();
+    for (int k = 0; k < nout && !r; ++k) {
+      if ((r = inner_solve(nbat, capturing))) break;
+      r = outer_step();
+    }
+    return r;
+  };
+  const uint64_t key[6] = {
+      1u | (cg1 ? 2u : 0u) | (xa32_ready ? 4u : 0u) | ((uint64_t)(uint32_t)inner_nitmax << 8),
+      (uint64_t)nbat | ((uint64_t)nout << 16) | ((uint64_t)(uint32_t)nitmax << 32),
+      matrix_tag(c), dbits(eps), dbits(inner_eps), (uint64_t)(uintptr_t)x};
+  if ((rc = run_planned(c, key, plan))) return rc;
   int guard = 0;
   for (;;) {
     if (read_state(c)) return BHIP_EDEVICE;
@@ -1464,10 +1882,8 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     if (sh->inner_total >= nitmax || sh->outer_count > 100 || ++guard > 10000) break;
     if (sh->stalled) {
       // finish the running inner solve, then resume the outer loop
-      if (!sh->done) {
-        rc = enqueue_batches<float, float, float>(c, exec, 1, delta, c->fr.p, c->fd0.p,
-                                                  c->fd1.p, c->fad.p, c->xa32.p, c->da32.p);
-        if (rc) return rc;
+      if (!(sh->done1 || sh->stop[0] || sh->stop[1])) {
+        if ((rc = more(1, false))) return rc;
         continue;
       }
       sh->stalled = 0;
@@ -1476,7 +1892,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
       continue;
     }
     // the last outer test started an inner solve that was not queued
-    if ((rc = inner_solve(nbat))) return rc;
+    if ((rc = inner_solve(nbat, false))) return rc;
     if ((rc = outer_step())) return rc;
   }
   info->relat = sh->xl_raw >= 1.0;
@@ -1542,12 +1958,12 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
       // the mixed path takes its FP32 copy of D A D in the same pass
       float* x32 = mixed ? c->xa32.p : nullptr;
       if (L.ndp == 3)
-        hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), 0, c->stream,
+        hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream,
                            c->xa.p, x32, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
                            c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
                            L.nblocks);
       else
-        hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), 0, c->stream,
+        hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream,
                            c->xa.p, x32, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
                            c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
                            L.nblocks);
@@ -1608,7 +2024,13 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
     return BHIP_OK;
   };
   int rc;
-  if (precision == BHIP_PREC_MIXED) {
+  if (precision == BHIP_PREC_MIXED && c->inner_cg1) {
+    // the single-kernel CG iteration (it includes the vector updates)
+    const Cg1Bufs v1 = {c->fx0.p, {c->fr.p, c->fr1.p}, {c->fd0.p, c->fd1.p},
+                        {c->fad.p, c->fad1.p}};
+    rc = time_it([&]() { return launch_cg1(c, v1, 1, 0, 0); }, &t_dir);
+    if (rc) return rc;
+  } else if (precision == BHIP_PREC_MIXED) {
     EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
     a.r = c->fr.p;
     a.dold = c->fd0.p;
@@ -1809,6 +2231,187 @@ k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
   }
 }
 
+// One horizontal direction of the prism diffusion matrix (mt02pp.f:229-356
+// along x with F, 358-445 along y with G): d1, d2, d3 the in-plane factors of
+// the base triangle, n[6] the nodal coefficients over the column heights,
+// hh the heights, zz the node elevations, dd = XMUL / 2880 / SURFAC. Adds
+// (or stores) the 15 pair contributions into c.
+__device__ inline void mt02pp_dir(double d1, double d2, double d3, const double (&n)[6],
+                                  const double (&hh)[3], const double (&zz)[6], double dd,
+                                  double (&c)[15], bool add) {
+  const double h1 = hh[0], h2 = hh[1], h3 = hh[2];
+  const double sh1 = h1 + h2 + h3;
+  const double shh = h1 * h1 + h2 * h2 + h3 * h3;
+  const double ri = -(zz[0] * d1 + zz[1] * d2 + zz[2] * d3);
+  const double rs = -(zz[3] * d1 + zz[4] * d2 + zz[5] * d3);
+  const double r = ri + rs;
+  const double rri = r + ri + ri, rrs = r + rs + rs;
+  const double rrri = r * r + 2 * ri * ri, rrrs = r * r + 2 * rs * rs;
+  const double d12 = d1 * d2, d13 = d1 * d3, d23 = d2 * d3;
+  const double sni = n[0] + n[1] + n[2], sns = n[3] + n[4] + n[5];
+  const double snhi = n[0] * h1 + n[1] * h2 + n[2] * h3;
+  const double snhs = n[3] * h1 + n[4] * h2 + n[5] * h3;
+  const double shi = (sni * sh1 + snhi + snhi) * sh1 + sni * shh +
+                     2 * (n[0] * h1 * h1 + n[1] * h2 * h2 + n[2] * h3 * h3);
+  const double shs = (sns * sh1 + snhs + snhs) * sh1 + sns * shh +
+                     2 * (n[3] * h1 * h1 + n[4] * h2 * h2 + n[5] * h3 * h3);
+  const double snhh = shi + shs;
+  const double snhhi = snhh + shi + shi, snhhs = snhh + shs + shs;
+  const double snhi1 = (sni + n[0]) * (sh1 + h1) + snhi + n[0] * h1;
+  const double snhs1 = (sns + n[3]) * (sh1 + h1) + snhs + n[3] * h1;
+  const double snhi2 = (sni + n[1]) * (sh1 + h2) + snhi + n[1] * h2;
+  const double snhs2 = (sns + n[4]) * (sh1 + h2) + snhs + n[4] * h2;
+  const double snhi3 = (sni + n[2]) * (sh1 + h3) + snhi + n[2] * h3;
+  const double snhs3 = (sns + n[5]) * (sh1 + h3) + snhs + n[5] * h3;
+  const double hri1 = rri * snhi1 + r * snhs1, hrs1 = rrs * snhs1 + r * snhi1;
+  const double hri2 = rri * snhi2 + r * snhs2, hrs2 = rrs * snhs2 + r * snhi2;
+  const double hri3 = rri * snhi3 + r * snhs3, hrs3 = rrs * snhs3 + r * snhi3;
+  const double rr11 = 2 * (rrri * (sni + n[0] + n[0]) + rrrs * (sns + n[3] + n[3]));
+  const double rr22 = 2 * (rrri * (sni + n[1] + n[1]) + rrrs * (sns + n[4] + n[4]));
+  const double rr33 = 2 * (rrri * (sni + n[2] + n[2]) + rrrs * (sns + n[5] + n[5]));
+  const double rr12 = rrri * (sni + n[0] + n[1]) + rrrs * (sns + n[3] + n[4]);
+  const double rr13 = rrri * (sni + n[0] + n[2]) + rrrs * (sns + n[3] + n[5]);
+  const double rr23 = rrri * (sni + n[1] + n[2]) + rrrs * (sns + n[4] + n[5]);
+  const double hh12 = snhh * d12, hh13 = snhh * d13, hh23 = snhh * d23;
+  double v[15];
+  v[0] = dd * (snhhi * d12 - d1 * hri2 - d2 * hri1 + rr12);
+  v[1] = dd * (snhhi * d13 - d1 * hri3 - d3 * hri1 + rr13);
+  v[2] = dd * (-hh12 - hh13 + d1 * (hri1 - hrs1) - rr11);
+  v[3] = dd * (hh12 + d1 * hri2 - d2 * hrs1 - rr12);
+  v[4] = dd * (hh13 + d1 * hri3 - d3 * hrs1 - rr13);
+  v[5] = dd * (snhhi * d23 - d2 * hri3 - d3 * hri2 + rr23);
+  v[6] = dd * (hh12 + d2 * hri1 - d1 * hrs2 - rr12);
+  v[7] = dd * (-hh12 - hh23 + d2 * (hri2 - hrs2) - rr22);
+  v[8] = dd * (hh23 + d2 * hri3 - d3 * hrs2 - rr23);
+  v[9] = dd * (hh13 + d3 * hri1 - d1 * hrs3 - rr13);
+  v[10] = dd * (hh23 + d3 * hri2 - d2 * hrs3 - rr23);
+  v[11] = dd * (-hh13 - hh23 + d3 * (hri3 - hrs3) - rr33);
+  v[12] = dd * (snhhs * d12 + d1 * hrs2 + d2 * hrs1 + rr12);
+  v[13] = dd * (snhhs * d13 + d1 * hrs3 + d3 * hrs1 + rr13);
+  v[14] = dd * (snhhs * d23 + d2 * hrs3 + d3 * hrs2 + rr23);
+#pragma unroll
+  for (int q = 0; q < 15; ++q) c[q] = add ? c[q] + v[q] : v[q];
+}
+
+// the pairs (of the 15, row-major upper order) holding each local node
+__constant__ int kPrismRowPairs[6][5] = {{0, 1, 2, 3, 4},   {0, 5, 6, 7, 8},
+                                        {1, 5, 9, 10, 11},  {2, 6, 9, 12, 13},
+                                        {3, 7, 10, 12, 14}, {4, 8, 11, 13, 14}};
+
+// Prism element matrices per slot, diagonal assembled in LDS colour by colour.
+//   kind 200: MT01PP mass (mt01pp.f: XMUL*SURFAC/360 combinations of the
+//             column heights floored at 1e-3)
+//   kind 201: MT02PP diffusion (mt02pp.f), P1 coefficients f, g, h; h_up is
+//             H at the node NPOU0 above each node (the caller's II4 = I1 +
+//             NPOU0), h_zero cancels the vertical term, inchyd the hydrostatic
+//             inconsistency test, mon the 'MON' clipping
+__global__ void __launch_bounds__(kThreads)
+k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
+             const double* __restrict__ xel, const double* __restrict__ yel, int nelem,
+             const double* __restrict__ z, const double* __restrict__ f,
+             const double* __restrict__ gg, const double* __restrict__ h,
+             const double* __restrict__ h_up, int h_zero, int inchyd, int mon,
+             const int32_t* __restrict__ slot_elem, const uint16_t* __restrict__ lid,
+             int nslot_pad, const int32_t* __restrict__ node_start,
+             const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
+             const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
+             double* __restrict__ da, int accumulate) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* acc = reinterpret_cast<double*>(smem_raw);
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int n0 = node_start[b], nown = node_start[b + 1] - n0;
+  const int h0 = halo_start[b];
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) acc[l] = 0.0;
+  __syncthreads();
+  const int32_t* co = color_off + (size_t)b * (kMaxColors + 1);
+  const int send = co[kMaxColors];
+  for (int c = 0; c < kMaxColors; ++c) {
+    const int s0 = co[c];
+    if (s0 >= send) break;
+    const int s1 = co[c + 1];
+    for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+      const int e = slot_elem[s];
+      int l[6], g[6];
+      double zz[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        l[k] = lid[(size_t)k * nslot_pad + s];
+        g[k] = l[k] < nown ? n0 + l[k] : halo[h0 + l[k] - nown];
+        zz[k] = z[g[k]];
+      }
+      double cf[15], t[6];
+      if (kind == 200) {
+        const double coef = surfac[e] * (xmul / 360.0);
+        const double h1 = fmax(zz[3] - zz[0], 1e-3) * coef;
+        const double h2 = fmax(zz[4] - zz[1], 1e-3) * coef;
+        const double h3 = fmax(zz[5] - zz[2], 1e-3) * coef;
+        const double ht = h1 + h2 + h3;
+        cf[3] = cf[6] = h1 + h2 + ht;
+        cf[4] = cf[9] = h1 + h3 + ht;
+        cf[8] = cf[10] = h2 + h3 + ht;
+        cf[2] = 4 * h1 + ht + ht;
+        cf[7] = 4 * h2 + ht + ht;
+        cf[11] = 4 * h3 + ht + ht;
+        cf[0] = cf[12] = cf[3] + cf[3];
+        cf[1] = cf[13] = cf[4] + cf[4];
+        cf[5] = cf[14] = cf[8] + cf[8];
+        t[0] = t[3] = cf[2] + cf[2];
+        t[1] = t[4] = cf[7] + cf[7];
+        t[2] = t[5] = cf[11] + cf[11];
+      } else {
+        const double hh[3] = {zz[3] - zz[0], zz[4] - zz[1], zz[5] - zz[2]};
+        bool dry = hh[0] < 1e-4 || hh[1] < 1e-4 || hh[2] < 1e-4;
+        if (inchyd)
+          dry = dry || fmax(fmax(zz[0], zz[1]), zz[2]) > fmin(fmin(zz[3], zz[4]), zz[5]);
+        double nf[6], ng[6], nh[3];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          nf[k] = dry ? 0.0 : f[g[k]] / hh[k % 3];
+          ng[k] = dry ? 0.0 : gg[g[k]] / hh[k % 3];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nh[k] = dry ? 0.0 : (h[g[k]] + h_up[g[k]]) / hh[k];
+        const double dd = (xmul / 2880.0) / surfac[e];
+        const double x2 = xel[(size_t)nelem + e], x3 = xel[2 * (size_t)nelem + e];
+        const double y2 = yel[(size_t)nelem + e], y3 = yel[2 * (size_t)nelem + e];
+        mt02pp_dir(y2 - y3, y3, -y2, nf, hh, zz, dd, cf, false);
+        mt02pp_dir(x3 - x2, -x3, x2, ng, hh, zz, dd, cf, true);
+        const double dz = h_zero ? 0.0 : surfac[e] * (xmul / 6.0);
+        cf[2] = cf[2] - dz * nh[0];
+        cf[7] = cf[7] - dz * nh[1];
+        cf[11] = cf[11] - dz * nh[2];
+        if (mon) {
+#pragma unroll
+          for (int q = 0; q < 15; ++q) cf[q] = xmul > 0.0 ? fmin(cf[q], 0.0) : fmax(cf[q], 0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          double sum = -cf[kPrismRowPairs[k][0]];
+#pragma unroll
+          for (int j = 1; j < 5; ++j) sum = sum - cf[kPrismRowPairs[k][j]];
+          t[k] = sum;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 15; ++q) {
+        double* p = xa + (size_t)q * nslot_pad + s;
+        *p = accumulate ? *p + cf[q] : cf[q];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (l[k] < nown) acc[l[k]] += t[k];
+    }
+    __syncthreads();
+  }
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+    if (accumulate)
+      da[n0 + l] += acc[l];
+    else
+      da[n0 + l] = acc[l];
+  }
+}
+
 }  // namespace bhip
 
 extern "C" {
@@ -1849,11 +2452,11 @@ int bhip_dirich(bhip_ctx* c, double* f, double* sm, const double* fbor, const in
   // OM 'M=DMD' with W and the diagonal declared '0' (diri01.f:131-136)
   if (c->ncoef > 0) {
     if (L.ndp == 3)
-      hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->xa.p,
+      hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream, c->xa.p,
                          (float*)nullptr, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
                          c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
     else
-      hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->xa.p,
+      hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream, c->xa.p,
                          (float*)nullptr, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
                          c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
     HIP_OK(hipGetLastError());
@@ -1933,6 +2536,78 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
   c->ncoef = 3;
   c->have_matrix = true;
   return BHIP_OK;
+}
+
+// prism element matrices (host inputs): kind 200 MT01PP, 201 MT02PP
+static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
+                      const double* surfac, const double* z, const double* f, const double* g,
+                      const double* h, int h_zero, int npou0, int inchyd, int mon,
+                      int accumulate) {
+  if (!c || !surfac || !z || c->L.ndp != 6) return BHIP_EBADARG;
+  if (kind == 201 && (!xel || !yel || !f || !g || !h)) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int n = L.npoin, ne = L.nelem;
+  if (npou0 < 0 || npou0 >= n) return BHIP_EBADARG;
+  if (accumulate && (!c->have_matrix || c->typext == 'Q')) return BHIP_EBADARG;
+  if (!c->have_matrix) {
+    HIP_OK(c->da.alloc(n));
+    HIP_OK(c->xa.alloc((size_t)30 * L.nslot_pad));
+  }
+  DevBuf<double> dS, dX, dY, dZ, dF, dG, dH, dHu;
+  std::vector<double> hup;
+  auto up_nodes = [&](DevBuf<double>& dst, const double* src) -> int {
+    HIP_OK(dst.alloc(n));
+    HIP_OK(hipMemcpyAsync(c->vio0.p, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dst.p,
+                       c->vio0.p, c->perm.p, n);
+    HIP_OK(hipGetLastError());
+    // vio0 is reused by the next upload: wait for this gather
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return BHIP_OK;
+  };
+  HIP_OK(dS.alloc(ne));
+  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  int rc = up_nodes(dZ, z);
+  if (rc) return rc;
+  if (kind == 201) {
+    HIP_OK(dX.alloc((size_t)3 * ne));
+    HIP_OK(dY.alloc((size_t)3 * ne));
+    HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+    // H at node i + NPOU0 (mt02pp.f:233-235: II4 = I1 + NPOU0); the nodes of
+    // the top plane are never a prism's lower node
+    hup.assign(n, 0.0);
+    for (int i = 0; i + npou0 < n; ++i) hup[i] = h[i + npou0];
+    if ((rc = up_nodes(dF, f)) || (rc = up_nodes(dG, g)) || (rc = up_nodes(dH, h)) ||
+        (rc = up_nodes(dHu, hup.data())))
+      return rc;
+  }
+  const size_t lds = (size_t)L.max_owned * sizeof(double);
+  hipLaunchKernelGGL(k_elem_prism, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
+                     dS.p, dX.p, dY.p, ne, dZ.p, dF.p, dG.p, dH.p, dHu.p, h_zero, inchyd, mon,
+                     c->slot_elem.p, c->lid.p, L.nslot_pad, c->node_start.p, c->halo_start.p,
+                     c->halo.p, c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->typdia = 'Q';
+  c->typext = 'S';
+  c->ncoef = 15;
+  c->have_matrix = true;
+  return BHIP_OK;
+}
+
+int bhip_matrix_mt01pp(bhip_ctx* c, double xmul, const double* surfac, const double* z,
+                       int accumulate) {
+  return elem_prism(c, 200, xmul, nullptr, nullptr, surfac, z, nullptr, nullptr, nullptr, 0, 0,
+                    0, 0, accumulate);
+}
+
+int bhip_matrix_mt02pp(bhip_ctx* c, double xmul, const double* xel, const double* yel,
+                       const double* surfac, const double* z, const double* f, const double* g,
+                       const double* h, int h_zero, int npou0, int inchyd, int mon,
+                       int accumulate) {
+  return elem_prism(c, 201, xmul, xel, yel, surfac, z, f, g, h, h_zero, npou0, inchyd, mon,
+                    accumulate);
 }
 
 int bhip_matrix_mt01aa(bhip_ctx* c, double xmul, const double* surfac, int accumulate) {

@@ -1,0 +1,356 @@
+"""Time-stepped telemac2d wave equation on the GPU, with its adjoint.
+
+The forward step is the wave-equation treatment of PROPAG (sources/telemac2d/
+propag.F, TREATMENT OF THE LINEAR SYSTEM = 2: the depth increment solved
+from the system AM1 = M/dt + MATDIFUV(-1, DM1, HPROP), propag.F:664-716,
+1369-1408, 1634-1756), restated for a theta = 1 scheme without advection:
+
+  friction  CF = 2 g / (K^2 h^(1/3))            (friction_calc.f:130-134)
+            f  = CF |u| / (2 h)                  (implicit coefficient, fricti.f)
+            c  = 1 / (1/dt + f)
+  momentum  u' = c (u/dt - g grad(eta'))         (nodal, lumped projection)
+  continuity, weak form with P1 test functions, integrated by parts:
+            (M/dt + K_w) (eta' - eta) = G - K_w eta
+            K_w = int g c h grad(phi_i).grad(phi_j)   (MATDIFUV 'UV', mt02aa.f:200-240)
+            G_i = int grad(phi_i) . (h c u) / dt
+  Dirichlet values of eta' on the sea boundary enter through DIRICH
+  (dirich.f: b -= A f, rows/columns of prescribed nodes cleared).
+
+with h = eta - zb. The linear system is solved by the HIP CG behind SOLVE
+(bief.Context, diagonal preconditioning), its matrix-vector products by the
+HIP EBE mat-vec. The model has no counterpart to compare against: the
+reference differentiates telemac2d with the external dco/fortran tool
+(COMPAD, ad_solve.F), which is absent. It is checked against central finite
+differences and a tangent-linear run instead.
+
+Differentiation: the linear solve and the EBE mat-vec are torch.autograd
+Functions with hand-written adjoints (adjoint CG solve with the same
+symmetric matrix; transposed EBE mat-vec; the element outer products for the
+matrix entries) and tangent-linear rules. The time loop is a Function of its
+own that stores the state every `checkpoint` steps and, going backward,
+recomputes each segment from its checkpoint before pulling the adjoint state
+through it.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import bief
+from .mesh import Mesh
+
+GRAV = 9.81
+
+
+# ---------------------------------------------------------------------------
+# EBE operators as autograd Functions
+# ---------------------------------------------------------------------------
+def _pair_outer(ikle, a, b):
+    """Per-element symmetric pair products a_i b_j + a_j b_i for the three
+    pairs (12, 13, 23) of each triangle: the gradient of x^T A y with respect
+    to the off-diagonal coefficients of a symmetric EBE matrix."""
+    i1, i2, i3 = ikle
+    return torch.stack([a[i1] * b[i2] + a[i2] * b[i1],
+                        a[i1] * b[i3] + a[i3] * b[i1],
+                        a[i2] * b[i3] + a[i3] * b[i2]])
+
+
+class _Ops:
+    """Holds the context and connectivity the Functions need."""
+
+    def __init__(self, ctx: bief.Context, ikle: torch.Tensor, solve_kw: dict):
+        self.ctx = ctx
+        self.ikle = ikle
+        self.solve_kw = solve_kw
+        self.n_solves = 0
+        self.n_iters = 0
+
+    def set(self, da, xa):
+        self.ctx.set_matrix("Q", "S", da.contiguous(), xa.contiguous())
+
+    def solve(self, da, xa, b):
+        """A^-1 b. The right-hand side is normalised first: GRACJG's stop
+        test turns absolute when ||b'|| < 1 (XL = max(||b'||^2, 1),
+        gracjg.F:189-209), and adjoint and tangent right-hand sides are
+        small, so without it they would barely be solved."""
+        scale = float(torch.linalg.vector_norm(b))
+        if scale == 0.0:
+            return torch.zeros_like(b)
+        self.set(da, xa)
+        x = torch.zeros_like(b)
+        _, info = self.ctx.solve(x, b / scale, slv=1, precon=2, **self.solve_kw)
+        if not info["converged"]:
+            raise bief.BiefError(f"wave-equation solve did not converge: {info}")
+        self.n_solves += 1
+        self.n_iters += info["niter"]
+        return x * scale
+
+    def matvec(self, da, xa, y, trans=False):
+        self.set(da, xa)
+        return self.ctx.matvec("X=TAY" if trans else "X=AY", y.contiguous())
+
+
+def _tangent_product(ops, dda, dxa, x):
+    """(dA) x for a tangent matrix (dda, dxa), either part possibly absent."""
+    if dda is None and dxa is None:
+        return torch.zeros_like(x)
+    dda = torch.zeros_like(x) if dda is None else dda
+    dxa = torch.zeros((3, ops.ikle.shape[1]), dtype=x.dtype, device=x.device) if dxa is None \
+        else dxa
+    return ops.matvec(dda, dxa, x)
+
+
+class EbeSolve(torch.autograd.Function):
+    """x = A^-1 b for the symmetric EBE matrix (da, xa): BIEF SOLVE (CG,
+    diagonal preconditioning) on the GPU.
+    Adjoint: A lam = gx (A symmetric), db = lam, dA = -lam x^T on the
+    element pattern. Tangent: A dx = db - (dA) x."""
+
+    @staticmethod
+    def forward(ctx, ops, da, xa, b):
+        x = ops.solve(da, xa, b)
+        ctx.ops = ops
+        ctx.save_for_backward(da, xa, x)
+        ctx.save_for_forward(da, xa, x)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        ops = ctx.ops
+        da, xa, x = ctx.saved_tensors
+        lam = ops.solve(da, xa, gx.contiguous())
+        return None, -lam * x, -_pair_outer(ops.ikle, lam, x), lam
+
+    @staticmethod
+    def jvp(ctx, _ops_t, dda, dxa, db):
+        ops = ctx.ops
+        da, xa, x = ctx.saved_tensors
+        rhs = -_tangent_product(ops, dda, dxa, x)
+        if db is not None:
+            rhs = rhs + db
+        return ops.solve(da, xa, rhs.contiguous())
+
+
+class EbeMatvec(torch.autograd.Function):
+    """y = A x for the symmetric EBE matrix (da, xa) (MATVEC 'X=AY');
+    its adjoint is the transposed product 'X=TAY'."""
+
+    @staticmethod
+    def forward(ctx, ops, da, xa, x):
+        ctx.ops = ops
+        ctx.save_for_backward(da, xa, x)
+        ctx.save_for_forward(da, xa, x)
+        return ops.matvec(da, xa, x)
+
+    @staticmethod
+    def backward(ctx, gy):
+        ops = ctx.ops
+        da, xa, x = ctx.saved_tensors
+        gx = ops.matvec(da, xa, gy.contiguous(), trans=True)
+        return None, gy * x, _pair_outer(ops.ikle, gy, x), gx
+
+    @staticmethod
+    def jvp(ctx, _ops_t, dda, dxa, dx):
+        ops = ctx.ops
+        da, xa, x = ctx.saved_tensors
+        out = _tangent_product(ops, dda, dxa, x)
+        if dx is not None:
+            out = out + ops.matvec(da, xa, dx.contiguous())
+        return out
+
+
+# ---------------------------------------------------------------------------
+# the model
+# ---------------------------------------------------------------------------
+class WaveModel:
+    """Wave-equation time stepping on a triangle mesh, on the GPU.
+
+    sea: node indices with the prescribed tide eta = amp sin(2 pi t / period).
+    """
+
+    def __init__(self, mesh: Mesh, sea, dt=2.0, amp=0.2, period=600.0, hmin_speed=1e-3,
+                 precision=bief.PREC_FP64, eps=1e-10, device="cuda"):
+        if not torch.cuda.is_available():
+            raise bief.BiefError("the wave model runs on the GPU")
+        self.mesh = mesh
+        self.dev = torch.device(device)
+        self.dt, self.amp, self.period = float(dt), float(amp), float(period)
+        self.eps_u = float(hmin_speed)
+        td = lambda a, dt_=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt_,
+                                                           device=self.dev)
+        self.ikle = td(mesh.ikle, torch.long)
+        xe, ye = mesh.xel_yel()
+        self.s = td(mesh.surfac())
+        self.x2, self.x3 = td(xe[1]), td(xe[2])
+        self.y2, self.y3 = td(ye[1]), td(ye[2])
+        self.n = mesh.npoin
+        # basis-function integrals (VOLU2D)
+        vol = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        for k in range(3):
+            vol.index_add_(0, self.ikle[k], self.s / 3.0)
+        self.vol = vol
+        mask = torch.zeros(self.n, dtype=torch.bool, device=self.dev)
+        mask[torch.as_tensor(np.asarray(sea), device=self.dev)] = True
+        self.sea = mask
+        self.ctx = bief.Context(mesh)
+        self.ops = _Ops(self.ctx, self.ikle, {"eps": eps, "nitmax": 5000, "precision": precision})
+        # P1 gradient factors per element: dphi_k/dx = bx_k / (2S), dphi_k/dy = by_k / (2S)
+        self.bx = torch.stack([self.y2 - self.y3, self.y3, -self.y2])
+        self.by = torch.stack([self.x3 - self.x2, -self.x3, self.x2])
+
+    # -- assembly helpers (torch, differentiable) ---------------------------
+    def scatter(self, w):
+        """ASSVEC of element terms w (3, ne) into a node vector."""
+        out = torch.zeros(self.n, dtype=w.dtype, device=self.dev)
+        for k in range(3):
+            out = out.index_add(0, self.ikle[k], w[k])
+        return out
+
+    def mass_matvec(self, y, xmul):
+        """M y for the P1 mass matrix times xmul (MT01AA)."""
+        i1, i2, i3 = self.ikle
+        c = self.s * (xmul / 12.0)
+        y1, y2, y3 = y[i1], y[i2], y[i3]
+        return self.scatter(torch.stack([c * (2 * y1 + y2 + y3), c * (y1 + 2 * y2 + y3),
+                                         c * (y1 + y2 + 2 * y3)]))
+
+    def system(self, h, cnode):
+        """(da, xa) of AM1 = M/dt + MATDIFUV(U = g c, V = h)."""
+        i1, i2, i3 = self.ikle
+        x2, x3, y2, y3, s = self.x2, self.x3, self.y2, self.y3, self.s
+        # MT01AA (mt01aa.f:83-106)
+        m_off = s * (1.0 / (12.0 * self.dt))
+        # MT02AA 'UV' (mt02aa.f:200-240), U = g c, V = h, XMUL = 1
+        u = GRAV * cnode
+        g1, g2, g3 = h[i1], h[i2], h[i3]
+        g123 = g1 + g2 + g3
+        coef2 = u[i1] * (g1 + g123) + u[i2] * (g2 + g123) + u[i3] * (g3 + g123)
+        coef = coef2 / (48.0 * s)
+        a12 = (y3 * (y2 - y3) + x3 * (x2 - x3)) * coef
+        a13 = -(y2 * (y2 - y3) + x2 * (x2 - x3)) * coef
+        a23 = -(y2 * y3 + x2 * x3) * coef
+        xa = torch.stack([a12 + m_off, a13 + m_off, a23 + m_off])
+        t = torch.stack([-a12 - a13 + 2 * m_off, -a12 - a23 + 2 * m_off,
+                         -a13 - a23 + 2 * m_off])
+        return self.scatter(t), xa
+
+    def tide(self, t):
+        return self.amp * np.sin(2.0 * np.pi * t / self.period)
+
+    def step(self, eta, u, v, k_str, zb, t_new):
+        """One time step (eta, u, v) -> (eta', u', v') at time t_new."""
+        dt = self.dt
+        h = eta - zb
+        cf = 2.0 * GRAV / (k_str * k_str * h.pow(1.0 / 3.0))
+        speed = torch.sqrt(u * u + v * v + self.eps_u * self.eps_u)
+        f = 0.5 * cf * speed / h
+        c = 1.0 / (1.0 / dt + f)
+        da, xa = self.system(h, c)
+        # right-hand side of the increment: G - K_w eta = G - (A eta - M eta/dt)
+        qx, qy = h * c * u / dt, h * c * v / dt
+        i1, i2, i3 = self.ikle
+        qxm = (qx[i1] + qx[i2] + qx[i3]) / 3.0
+        qym = (qy[i1] + qy[i2] + qy[i3]) / 3.0
+        g_el = 0.5 * (self.bx * qxm + self.by * qym)  # S * grad(phi_k).q_mean
+        rhs = self.scatter(g_el) - EbeMatvec.apply(self.ops, da, xa, eta) + \
+            self.mass_matvec(eta, 1.0 / dt)
+        # DIRICH (diri01.f:94-139): prescribed increment on the sea boundary
+        w = (~self.sea).to(eta.dtype)
+        fb = torch.where(self.sea, self.tide(t_new) - eta, torch.zeros_like(eta))
+        rhs = rhs - EbeMatvec.apply(self.ops, da, xa, fb)
+        rhs = torch.where(self.sea, da * fb, rhs)
+        xa_d = xa * torch.stack([w[i1] * w[i2], w[i1] * w[i3], w[i2] * w[i3]])
+        deta = EbeSolve.apply(self.ops, da, xa_d, rhs)
+        eta_n = eta + deta
+        # momentum with the new free surface: lumped projection of grad(eta')
+        e1, e2, e3 = eta_n[i1], eta_n[i2], eta_n[i3]
+        gx_el = (self.bx[0] * e1 + self.bx[1] * e2 + self.bx[2] * e3) / 6.0  # S/3 * deta/dx
+        gy_el = (self.by[0] * e1 + self.by[1] * e2 + self.by[2] * e3) / 6.0
+        gxn = self.scatter(torch.stack([gx_el, gx_el, gx_el])) / self.vol
+        gyn = self.scatter(torch.stack([gy_el, gy_el, gy_el])) / self.vol
+        u_n = c * (u / dt - GRAV * gxn)
+        v_n = c * (v / dt - GRAV * gyn)
+        return eta_n, u_n, v_n
+
+    def initial_state(self, zb):
+        z = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        return z.clone(), z.clone(), z.clone()
+
+    def run(self, k_str, zb, nsteps, gauges=None, obs=None):
+        """Forward run from rest; returns the gauge series (nsteps, ngauge) or,
+        with obs, the cost sum (eta_g - obs)^2."""
+        eta, u, v = self.initial_state(zb)
+        series = []
+        cost = torch.zeros((), dtype=torch.float64, device=self.dev)
+        for n in range(nsteps):
+            eta, u, v = self.step(eta, u, v, k_str, zb, (n + 1) * self.dt)
+            if gauges is not None:
+                ge = eta[gauges]
+                series.append(ge)
+                if obs is not None:
+                    cost = cost + ((ge - obs[n]) ** 2).sum()
+        if obs is not None:
+            return cost
+        return torch.stack(series) if series else eta
+
+
+class WaveCost(torch.autograd.Function):
+    """J(K, zb) = sum over steps and gauges of (eta - obs)^2, with a
+    hand-written reverse sweep: states stored every `checkpoint` steps, each
+    segment recomputed from its checkpoint and differentiated step by step
+    (adjoint solves and transposed mat-vecs inside), last segment first."""
+
+    @staticmethod
+    def forward(ctx, model, k_str, zb, nsteps, gauges, obs, checkpoint):
+        with torch.no_grad():
+            eta, u, v = model.initial_state(zb)
+            checkpoints = {0: (eta, u, v)}
+            cost = torch.zeros((), dtype=torch.float64, device=model.dev)
+            for n in range(nsteps):
+                eta, u, v = model.step(eta, u, v, k_str, zb, (n + 1) * model.dt)
+                cost = cost + ((eta[gauges] - obs[n]) ** 2).sum()
+                if (n + 1) % checkpoint == 0 and n + 1 < nsteps:
+                    checkpoints[n + 1] = (eta, u, v)
+        ctx.model, ctx.nsteps, ctx.gauges, ctx.obs = model, nsteps, gauges, obs
+        ctx.checkpoints, ctx.checkpoint = checkpoints, checkpoint
+        ctx.save_for_backward(k_str, zb)
+        return cost
+
+    @staticmethod
+    def backward(ctx, gcost):
+        model, nsteps, gauges, obs = ctx.model, ctx.nsteps, ctx.gauges, ctx.obs
+        k_str, zb = ctx.saved_tensors
+        gk = torch.zeros_like(k_str)
+        gz = torch.zeros_like(zb)
+        # adjoint of the state at the end of the segment being processed
+        lam = [torch.zeros_like(zb) for _ in range(3)]
+        for s0 in sorted(ctx.checkpoints, reverse=True):
+            s1 = min(s0 + ctx.checkpoint, nsteps)
+            with torch.enable_grad():
+                # the segment recomputed from its checkpoint, with its graph
+                e0, u0, v0 = (t.detach().requires_grad_(True) for t in ctx.checkpoints[s0])
+                kk = k_str.detach().requires_grad_(True)
+                zz = zb.detach().requires_grad_(True)
+                e, u, v = e0, u0, v0
+                jseg = torch.zeros((), dtype=torch.float64, device=model.dev)
+                for n in range(s0, s1):
+                    e, u, v = model.step(e, u, v, kk, zz, (n + 1) * model.dt)
+                    jseg = jseg + ((e[gauges] - obs[n]) ** 2).sum()
+                outs = (e, u, v, jseg)
+                gouts = (lam[0], lam[1], lam[2], gcost)
+                g = torch.autograd.grad(outs, (e0, u0, v0, kk, zz), gouts, allow_unused=True)
+            lam = [gi if gi is not None else torch.zeros_like(zb) for gi in g[:3]]
+            if g[3] is not None:
+                gk += g[3]
+            if g[4] is not None:
+                gz += g[4]
+        return None, gk, gz, None, None, None, None
+
+
+def cost_and_gradient(model: WaveModel, k_str, zb, nsteps, gauges, obs, checkpoint=10):
+    """(J, dJ/dK, dJ/dzb) through the checkpointed adjoint."""
+    k = k_str.detach().requires_grad_(True)
+    z = zb.detach().requires_grad_(True)
+    j = WaveCost.apply(model, k, z, nsteps, gauges, obs, checkpoint)
+    gk, gz = torch.autograd.grad(j, (k, z))
+    return j.detach(), gk, gz

@@ -74,6 +74,17 @@ void orc_mt02aa(int kind, double* t, double* xm, double xmul,
 void orc_mt01pp(double* t, double* xm, double xmul, const double* surfac,
                 const double* z, const int* ikle, int nelem, int nelmax);
 
+/* prism diffusion matrix (MT02PP): diagonal tensor diffusion with P1
+ * coefficients f (x), g (y), h (z) on the prism nodes; h_zero cancels the
+ * vertical term (H of type '0'); npou0 = NPOIN2 (P1 vertical viscosity) or 0
+ * (P0 on the vertical); xel, yel: the base triangle's local coordinates
+ * (3 columns of nelmax, node 1 at the origin); z: node elevations; inchyd:
+ * hydrostatic inconsistency test; mon: 'MON' clipping. */
+void orc_mt02pp(double* t, double* xm, double xmul, const double* f, const double* g,
+                const double* h, int h_zero, int npou0, const double* xel,
+                const double* yel, const double* z, const double* surfac, const int* ikle,
+                int nelem, int nelmax, int inchyd, int mon);
+
 /* assembles element diagonal terms t (ndp columns) into da (init to zero) */
 void orc_assemble_diag(double* da, const double* t, const int* ikle,
                        int npoin, int nelem, int nelmax, int ndp);

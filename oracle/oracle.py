@@ -58,6 +58,8 @@ def lib():
         L.orc_mt01aa.argtypes = [_P, _P, _D, _P, _I, _I]
         L.orc_mt02aa.argtypes = [_I, _P, _P, _D, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I]
         L.orc_mt01pp.argtypes = [_P, _P, _D, _P, _P, _P, _I, _I]
+        L.orc_mt02pp.argtypes = [_P, _P, _D, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P,
+                                 _I, _I, _I, _I]
         L.orc_assemble_diag.argtypes = [_P, _P, _P, _I, _I, _I, _I]
         L.orc_om_dmd.argtypes = [_P, _P, _P, _C, _P, _P, _I, _I, _I, _I]
         L.orc_precd1.argtypes = [_P, _P, _P, _P, _C, _P, _P, _P, _I, _I, _I, _I, _I]
@@ -136,6 +138,20 @@ def mt01pp(surfac, z, ikle, xmul=1.0):
     t = np.empty((6, ne))
     xm = np.empty((15, ne))
     lib().orc_mt01pp(_ptr(t), _ptr(xm), xmul, _ptr(s), _ptr(_f64(z)), _ptr(_i32(ikle)), ne, ne)
+    return t, xm
+
+
+def mt02pp(f, g, h, xel, yel, z, surfac, ikle, npou0, xmul=1.0, h_zero=False, inchyd=False,
+           mon=False):
+    """Prism diffusion matrix (MT02PP); xel, yel (3, nelem) local coordinates
+    of the base triangle; returns t (6, nelem), xm (15, nelem)."""
+    s = _f64(surfac)
+    ne = s.size
+    t = np.empty((6, ne))
+    xm = np.empty((15, ne))
+    lib().orc_mt02pp(_ptr(t), _ptr(xm), xmul, _ptr(_f64(f)), _ptr(_f64(g)), _ptr(_f64(h)),
+                     int(h_zero), int(npou0), _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(_f64(z)),
+                     _ptr(s), _ptr(_i32(ikle)), ne, ne, int(inchyd), int(mon))
     return t, xm
 
 
@@ -228,6 +244,8 @@ def _bind_driver(L):
     L.ref_mt01aa.argtypes = [_I, _D, _P, _P, _P]
     L.ref_mt02aa.argtypes = [_I, _I, _I, _D, _P, _P, _P, _P, _P, _P, _I, _P, _P]
     L.ref_mt01pp.argtypes = [_I, _D, _P, _P, _P, _P, _P]
+    L.ref_mt02pp.argtypes = [_I, _I, _I, _D, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _I,
+                             _P, _P]
     return L
 
 
@@ -236,17 +254,7 @@ def ref():
     if _ref is None:
         if not have_ref():
             raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref`")
-        L = ctypes.CDLL(REF_SO)
-        L.ref_mesh_init.argtypes = [_I, _I, _I, _I, _P]
-        L.ref_matvec.argtypes = [_I, _C, _C, _P, _P, _P, _D, _P]
-        L.ref_solve.argtypes = [_I, _I, _I, _I, _D, ctypes.POINTER(ctypes.c_char), _C,
-                                _P, _P, _P, _P, ctypes.c_char_p, _I]
-        L.ref_dirich.argtypes = [_C, _C, _P, _P, _P, _P, _P, _I, _P]
-        L.ref_lump.argtypes = [_C, _C, _P, _P, _D, _P]
-        L.ref_mt01aa.argtypes = [_I, _D, _P, _P, _P]
-        L.ref_mt02aa.argtypes = [_I, _I, _I, _D, _P, _P, _P, _P, _P, _P, _I, _P, _P]
-        L.ref_mt01pp.argtypes = [_I, _D, _P, _P, _P, _P, _P]
-        _ref = L
+        _ref = _bind_driver(ctypes.CDLL(REF_SO))
     return _ref
 
 
@@ -366,3 +374,16 @@ def ref_mt01pp(surfac, z, ikle, xmul=1.0):
     xm = np.empty((30, ne))
     ref().ref_mt01pp(ne, xmul, _ptr(_f64(z)), _ptr(s), _ptr(_i32(ikle)), _ptr(t), _ptr(xm))
     return t, xm[:15]
+
+
+def ref_mt02pp(f, g, h, xel, yel, z, surfac, ikle, nplan, xmul=1.0, h_zero=False,
+               p0_vertical=False, inchyd=False, mon=False):
+    s = _f64(surfac)
+    ne = s.size
+    f = _f64(f)
+    t = np.empty((6, ne))
+    xm = np.empty((15, ne))
+    ref().ref_mt02pp(ne, f.size, nplan, xmul, _ptr(f), _ptr(_f64(g)), _ptr(_f64(h)), int(h_zero),
+                     int(p0_vertical), _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(_f64(z)), _ptr(s),
+                     _ptr(_i32(ikle)), int(inchyd), int(mon), _ptr(t), _ptr(xm))
+    return t, xm

@@ -303,4 +303,41 @@ contains
     deallocate(ik)
   end subroutine
 
+  ! MT02PP with P1 coefficients f, g, h on the prism nodes (npoin of them);
+  ! h_zero: H of type '0'; p0_vertical: H%DIMDISC = 4111
+  subroutine ref_mt02pp(nelem, npoin, nplan, xmul, f, g, h, h_zero, p0_vertical, xel, yel, z, &
+       surfac, ikle0, inchyd, mon, t, xm) bind(c, name='ref_mt02pp')
+    integer(c_int), value :: nelem, npoin, nplan, h_zero, p0_vertical, inchyd, mon
+    real(c_double), value :: xmul
+    real(c_double), intent(in) :: f(*), g(*), h(*), z(*), surfac(*)
+    real(c_double), intent(in) :: xel(nelem, 3), yel(nelem, 3)
+    integer(c_int), intent(in) :: ikle0(nelem, 6)
+    real(c_double), intent(out) :: t(nelem, 6), xm(nelem, 15)
+    type(bief_obj) :: sf, sg, sh
+    integer, allocatable :: ik(:,:)
+    double precision, allocatable :: x6(:,:), y6(:,:)
+    character(len=16) :: formul
+    allocate(ik(nelem, 6), x6(nelem, 6), y6(nelem, 6))
+    ik = ikle0 + 1
+    x6 = 0.d0
+    y6 = 0.d0
+    x6(:, 1:3) = xel
+    y6(:, 1:3) = yel
+    x6(:, 4:6) = xel
+    y6(:, 4:6) = yel
+    sf%elm = 41
+    sf%typr = 'Q'
+    sf%dimdisc = 0
+    sf%dim1 = npoin
+    sg = sf
+    sh = sf
+    if (h_zero /= 0) sh%typr = '0'
+    if (p0_vertical /= 0) sh%dimdisc = 4111
+    formul = 'MATDIF          '
+    if (mon /= 0) formul(14:16) = 'MON'
+    call mt02pp(t, xm, xmul, sf, sg, sh, f, g, h, x6, y6, z, surfac, ik, nelem, nelem, &
+         inchyd /= 0, formul, nplan)
+    deallocate(ik, x6, y6)
+  end subroutine
+
 end module ref_driver

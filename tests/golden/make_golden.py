@@ -126,12 +126,50 @@ def prism_case():
     return out
 
 
-def main():
-    np.savez_compressed(os.path.join(HERE, "ref_prism.npz"), **prism_case())
-    np.savez_compressed(os.path.join(HERE, "ref_small.npz"), **small_case())
-    np.savez_compressed(os.path.join(HERE, "ref_channel.npz"), **channel_case())
-    print("golden vectors written")
+def mt02pp_case():
+    """Prism diffusion matrix (MT02PP) over 4 planes, with nearly dry columns
+    and a crossing plane (for the hydrostatic inconsistency test): every
+    option of the routine."""
+    base = M.random_small_mesh(6, 5, seed=4)
+    rng = np.random.default_rng(40)
+    bot = -rng.uniform(1, 5, base.npoin)
+    top = rng.uniform(-0.2, 0.2, base.npoin)
+    top[:3] = bot[:3] + 5e-5
+    p = M.extrude_prisms(base, 4, bot, top)
+    # a middle plane pushed below its lower neighbour at a few nodes
+    n2 = base.npoin
+    p.z[n2 + 5:n2 + 8] = p.z[5:8] - 0.05
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    xel, yel = np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb])
+    s = base.surfac()[eb]
+    f = rng.uniform(0.5, 2, p.npoin)
+    g = rng.uniform(0.5, 2, p.npoin)
+    h = 10 ** rng.uniform(-4, -2, p.npoin)
+    out = {"z": p.z, "ikle": p.ikle, "xel": xel, "yel": yel, "surfac": s, "f": f, "g": g,
+           "h": h, "npoin2": np.array(n2), "nplan": np.array(p.nplan)}
+    for xmul in (1.0, -0.7):
+        for hz in (0, 1):
+            for p0 in (0, 1):
+                for inch in (0, 1):
+                    for mon in (0, 1):
+                        t, xm = O.ref_mt02pp(f, g, h, xel, yel, p.z, s, p.ikle, p.nplan, xmul,
+                                             bool(hz), bool(p0), bool(inch), bool(mon))
+                        key = f"mt02pp_{'p' if xmul > 0 else 'n'}{hz}{p0}{inch}{mon}"
+                        out[key + "_t"], out[key + "_xm"] = t, xm
+    return out
+
+
+CASES = {"ref_prism": prism_case, "ref_small": small_case, "ref_channel": channel_case,
+         "ref_mt02pp": mt02pp_case}
+
+
+def main(names):
+    """Writes the named fixtures (all of them by default)."""
+    for name in names or CASES:
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **CASES[name]())
+        print("golden vectors written:", name)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -1,0 +1,83 @@
+"""Adjoint of the time-stepped wave equation on the GPU: gradients of a
+gauge misfit with respect to the per-node Strickler coefficient and the
+bathymetry, checked against central finite differences, a tangent-linear
+(forward-mode) run, and across checkpoint spacings. The reference has no
+counterpart (its telemac2d gradients come from the absent dco/fortran
+tool), so these checks are the parity bar for this part."""
+import numpy as np
+import pytest
+import torch
+
+from opentelemac_amd import bief, mesh as M, wave as W
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 24
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    base = M.random_small_mesh(22, 11, seed=71, jitter=0.2)
+    m = M.Mesh(x=base.x * 100.0, y=base.y * 100.0, ikle=base.ikle, npoin2=base.npoin,
+               nelem2=base.nelem, meta={})
+    sea = np.nonzero(m.x <= 1e-9)[0]
+    model = W.WaveModel(m, sea, dt=2.0, amp=0.5, period=120.0, eps=1e-12)
+    rng = np.random.default_rng(5)
+    dev = model.dev
+    k = torch.tensor(rng.uniform(20, 60, m.npoin), dtype=torch.float64, device=dev)
+    zb = torch.tensor(-(1.0 + 2.0 * m.x / m.x.max()) - rng.uniform(0, 0.3, m.npoin),
+                      dtype=torch.float64, device=dev)
+    gauges = torch.tensor(rng.choice(m.npoin, 15, replace=False), device=dev)
+    with torch.no_grad():
+        obs = model.run(k * 0.7, zb, NSTEPS, gauges)
+    return model, k, zb, gauges, obs, rng
+
+
+def _cost(model, k, zb, gauges, obs):
+    with torch.no_grad():
+        return float(model.run(k, zb, NSTEPS, gauges, obs))
+
+
+def test_gradient_matches_central_differences(setup):
+    model, k, zb, gauges, obs, rng = setup
+    j, gk, gz = W.cost_and_gradient(model, k, zb, NSTEPS, gauges, obs, checkpoint=5)
+    assert float(j) > 0
+    assert abs(float(j) - _cost(model, k, zb, gauges, obs)) <= 1e-12 * float(j)
+    nodes = rng.choice(model.n, 6, replace=False)
+    for name, p, g, rel_step in (("K", k, gk, 1e-4), ("zb", zb, gz, 1e-4)):
+        for i in nodes:
+            h = rel_step * max(1.0, abs(float(p[i])))
+            pp, pm = p.clone(), p.clone()
+            pp[i] += h
+            pm[i] -= h
+            fd = (_cost(model, pp if name == "K" else k, pp if name == "zb" else zb, gauges,
+                        obs) -
+                  _cost(model, pm if name == "K" else k, pm if name == "zb" else zb, gauges,
+                        obs)) / (2 * h)
+            scale = float(g.abs().max())
+            assert abs(float(g[i]) - fd) <= 1e-5 * scale, (name, i, float(g[i]), fd, scale)
+
+
+def test_gradient_matches_tangent_linear(setup):
+    """<grad J, d> from the adjoint against dJ/dt along d from a forward-mode
+    (tangent-linear) run through the same solves."""
+    import torch.autograd.forward_ad as fwAD
+    model, k, zb, gauges, obs, rng = setup
+    _, gk, gz = W.cost_and_gradient(model, k, zb, NSTEPS, gauges, obs, checkpoint=4)
+    dk = torch.tensor(rng.normal(size=model.n), dtype=torch.float64, device=model.dev)
+    dz = torch.tensor(rng.normal(size=model.n) * 0.01, dtype=torch.float64, device=model.dev)
+    with fwAD.dual_level():
+        kd = fwAD.make_dual(k, dk)
+        zd = fwAD.make_dual(zb, dz)
+        jd = model.run(kd, zd, NSTEPS, gauges, obs)
+        tl = float(fwAD.unpack_dual(jd).tangent)
+    adj = float((gk * dk).sum() + (gz * dz).sum())
+    assert abs(adj - tl) <= 1e-8 * max(abs(tl), 1e-300), (adj, tl)
+
+
+def test_checkpoint_spacing_does_not_change_the_gradient(setup):
+    model, k, zb, gauges, obs, _ = setup
+    _, gk1, gz1 = W.cost_and_gradient(model, k, zb, NSTEPS, gauges, obs, checkpoint=NSTEPS)
+    _, gk2, gz2 = W.cost_and_gradient(model, k, zb, NSTEPS, gauges, obs, checkpoint=3)
+    assert torch.allclose(gk1, gk2, rtol=1e-9, atol=1e-12 * float(gk1.abs().max()))
+    assert torch.allclose(gz1, gz2, rtol=1e-9, atol=1e-12 * float(gz1.abs().max()))

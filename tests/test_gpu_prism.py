@@ -56,3 +56,102 @@ def test_prism_solve(gpu, slv):
             assert rel <= 1e-10
         else:
             assert rel <= 1e-8
+
+
+def _sloped_prisms(nx=8, ny=6, nplan=5, seed=6):
+    base = M.random_small_mesh(nx, ny, seed=seed)
+    rng = np.random.default_rng(seed + 50)
+    bot = -rng.uniform(1, 5, base.npoin)
+    top = rng.uniform(-0.2, 0.2, base.npoin)
+    top[:3] = bot[:3] + 5e-5  # nearly dry columns
+    p = M.extrude_prisms(base, nplan, bot, top)
+    n2 = base.npoin
+    p.z[n2 + 5:n2 + 8] = p.z[5:8] - 0.05  # crossing planes (inconsistency test)
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    return (base, p, np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb]),
+            base.surfac()[eb])
+
+
+def _close(got, ref, tol):
+    return np.abs(got - ref).max() <= tol * (np.abs(ref).max() + 1e-300)
+
+
+@pytest.mark.parametrize("bn", [0, 40])
+def test_device_prism_element_matrices(gpu, bn):
+    """MT01PP and MT02PP on the device, assembled by the context (diagonal
+    colour by colour in LDS), against the oracle over MT02PP's options.
+    Tolerance 1e-12 of the largest entry: the device contracts products
+    into fused multiply-adds."""
+    base, p, xel, yel, s = _sloped_prisms()
+    rng = np.random.default_rng(7)
+    f = rng.uniform(0.5, 2, p.npoin)
+    g = rng.uniform(0.5, 2, p.npoin)
+    h = 10 ** rng.uniform(-4, -2, p.npoin)
+    ctx = bief.Context(p, block_nodes=bn)
+    ctx.matrix_mt01pp(0.5, s, p.z)
+    td, da, xa = ctx.get_matrix()
+    t1, x1 = O.mt01pp(s, p.z, p.ikle, 0.5)
+    assert td == "Q" and _close(xa, x1, 1e-14)
+    assert _close(da, O.assemble_diag(t1, p.ikle, p.npoin), 1e-13)
+    for xmul in (1.0, -0.7):
+        for hz in (0, 1):
+            for p0 in (0, 1):
+                for inch in (0, 1):
+                    for mon in (0, 1):
+                        npou0 = 0 if p0 else base.npoin
+                        ctx.matrix_mt02pp(xmul, xel, yel, s, p.z, f, g, h, npou0, bool(hz),
+                                          bool(inch), bool(mon))
+                        _, da, xa = ctx.get_matrix()
+                        t, xm = O.mt02pp(f, g, h, xel, yel, p.z, s, p.ikle, npou0, xmul,
+                                         bool(hz), bool(inch), bool(mon))
+                        case = (xmul, hz, p0, inch, mon)
+                        assert _close(xa, xm, 1e-12), case
+                        assert _close(da, O.assemble_diag(t, p.ikle, p.npoin), 1e-12), case
+    # 'M=M+N': mass / dt + diffusion, as the vertical diffusion step builds it
+    ctx.matrix_mt01pp(0.5, s, p.z)
+    ctx.matrix_mt02pp(1.0, xel, yel, s, p.z, f, g, h, base.npoin, accumulate=True)
+    _, da, xa = ctx.get_matrix()
+    t2, x2 = O.mt02pp(f, g, h, xel, yel, p.z, s, p.ikle, base.npoin, 1.0)
+    assert _close(xa, x1 + x2, 1e-12)
+    assert _close(da, O.assemble_diag(t1, p.ikle, p.npoin) + O.assemble_diag(t2, p.ikle, p.npoin),
+                  1e-12)
+
+
+def test_prism_vertical_diffusion_solve(gpu):
+    """The vertical diffusion system of a sigma mesh (MT01PP / dt + MT02PP
+    with a viscosity growing log-linearly towards the surface) assembled on
+    the device and solved by CG + diagonal preconditioning, FP64 and mixed,
+    against the oracle's solve of the oracle's matrix."""
+    base = M.random_small_mesh(12, 10, seed=8)
+    rng = np.random.default_rng(9)
+    bot = -rng.uniform(2, 20, base.npoin)
+    p = M.extrude_prisms(base, 6, bot, np.zeros(base.npoin))
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    xel, yel, s = np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb]), \
+        base.surfac()[eb]
+    sig = (p.z - np.tile(bot, p.nplan)) / np.tile(-bot, p.nplan)
+    nu = 10 ** (-4 + 2 * sig)
+    dt = 10.0
+    ctx = bief.Context(p, block_nodes=64)
+    ctx.matrix_mt01pp(1.0 / dt, s, p.z)
+    ctx.matrix_mt02pp(1.0, xel, yel, s, p.z, nu, nu, nu, base.npoin, accumulate=True)
+    _, da, xa = ctx.get_matrix()
+    t1, x1 = O.mt01pp(s, p.z, p.ikle, 1.0 / dt)
+    t2, x2 = O.mt02pp(nu, nu, nu, xel, yel, p.z, s, p.ikle, base.npoin, 1.0)
+    A = O.Matrix(O.assemble_diag(t1, p.ikle, p.npoin) + O.assemble_diag(t2, p.ikle, p.npoin),
+                 x1 + x2, "Q", "S")
+    b = rng.normal(size=p.npoin)
+    x_ref, info_ref, _, _ = O.solve(A, p.ikle, b, slv=1, precon=2, eps=1e-10, nitmax=1000)
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ctx.set_matrix("Q", "S", da, xa)
+        x, info = ctx.solve(np.zeros(p.npoin), b.copy(), slv=1, precon=2, eps=1e-10,
+                            nitmax=1000, precision=prec)
+        rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+        assert info["converged"], info
+        if prec == bief.PREC_FP64:
+            assert abs(info["niter"] - info_ref.niter) <= 1, (info, info_ref.niter)
+            assert rel <= 1e-10, rel
+        else:
+            assert rel <= 1e-8, rel

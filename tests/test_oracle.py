@@ -229,3 +229,98 @@ def test_prism_mass_integrates_volume():
     t, xm = O.mt01pp(s, p.z, p.ikle, 1.0)
     h = np.mean([p.z[p.ikle[k + 3]] - p.z[p.ikle[k]] for k in range(3)], axis=0)
     assert np.allclose(t.sum(0) + 2 * xm.sum(0), s * h, rtol=1e-12)
+
+
+# ---- prism diffusion (MT02PP) ---------------------------------------------
+
+_PAIRS6 = [(i, j) for i in range(6) for j in range(i + 1, 6)]
+
+
+def _mt02pp_mesh():
+    """The mesh of the MT02PP golden vectors: sloped planes, nearly dry
+    columns and a few nodes of a middle plane pushed below the plane under
+    them."""
+    base = M.random_small_mesh(6, 5, seed=4)
+    rng = np.random.default_rng(40)
+    bot = -rng.uniform(1, 5, base.npoin)
+    top = rng.uniform(-0.2, 0.2, base.npoin)
+    top[:3] = bot[:3] + 5e-5
+    p = M.extrude_prisms(base, 4, bot, top)
+    n2 = base.npoin
+    p.z[n2 + 5:n2 + 8] = p.z[5:8] - 0.05
+    return base, p
+
+
+def _base_cols(base, p):
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    return (np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb]),
+            base.surfac()[eb])
+
+
+def test_golden_mt02pp():
+    """Bitwise against the reference's MT02PP over all its options: sign of
+    XMUL, H of type '0', P0 vertical viscosity, the hydrostatic
+    inconsistency test and 'MON'."""
+    z = np.load(os.path.join(GOLD, "ref_mt02pp.npz"))
+    base, p = _mt02pp_mesh()
+    assert np.array_equal(p.ikle, z["ikle"]) and np.array_equal(p.z, z["z"])
+    xel, yel, s = _base_cols(base, p)
+    assert np.array_equal(xel, z["xel"]) and np.array_equal(s, z["surfac"])
+    n = 0
+    for xmul in (1.0, -0.7):
+        for hz in (0, 1):
+            for p0 in (0, 1):
+                for inch in (0, 1):
+                    for mon in (0, 1):
+                        key = f"mt02pp_{'p' if xmul > 0 else 'n'}{hz}{p0}{inch}{mon}"
+                        t, xm = O.mt02pp(z["f"], z["g"], z["h"], xel, yel, p.z, s, p.ikle,
+                                         0 if p0 else base.npoin, xmul, bool(hz), bool(inch),
+                                         bool(mon))
+                        assert np.array_equal(t, z[key + "_t"]), key
+                        assert np.array_equal(xm, z[key + "_xm"]), key
+                        n += 1
+    assert n == 32
+
+
+def _energy(t, xm, ikle, u):
+    """u^T K u of the assembled matrix, from its element terms."""
+    ue = u[ikle]
+    e = (t * ue * ue).sum()
+    for q, (i, j) in enumerate(_PAIRS6):
+        e += 2.0 * (xm[q] * ue[i] * ue[j]).sum()
+    return e
+
+
+@pytest.mark.parametrize("layers", ["flat", "tilted"])
+def test_mt02pp_energy_of_linear_fields(layers):
+    """For parallel layers (flat or uniformly tilted), u = x, y or z is a P1
+    field with unit gradient along that axis, so u^T K u with a unit
+    coefficient in that direction only is the volume of the mesh. Tilted
+    layers exercise the sloping-plane terms of the horizontal parts."""
+    base = M.random_small_mesh(6, 5, seed=4)
+    bot = (0.3 * base.x + 0.1 * base.y - 3.0) if layers == "tilted" else -3.0 * np.ones(base.npoin)
+    p = M.extrude_prisms(base, 4, bot, bot + 3.0)
+    xel, yel, s = _base_cols(base, p)
+    h = np.mean([p.z[p.ikle[k + 3]] - p.z[p.ikle[k]] for k in range(3)], axis=0)
+    vol = (s * h).sum()
+    one, zero = np.ones(p.npoin), np.zeros(p.npoin)
+    for f, g, hh, u, hz in ((one, zero, zero, p.x, True), (zero, one, zero, p.y, True),
+                            (zero, zero, one, p.z, False)):
+        t, xm = O.mt02pp(f, g, hh, xel, yel, p.z, s, p.ikle, base.npoin, 1.0, hz)
+        assert abs(_energy(t, xm, p.ikle, u) / vol - 1.0) < 1e-12
+        # and constants are in the kernel (row sums vanish)
+        assert abs(_energy(t, xm, p.ikle, one)) < 1e-9 * vol
+
+
+def test_mt02pp_vertical_term_on_sloped_layers():
+    """The monotone vertical term: u = z with only H = 1 gives
+    sum_e S/3 (h1 + h2 + h3) = the volume, also on sloped columns."""
+    base, p = _mt02pp_mesh()
+    xel, yel, s = _base_cols(base, p)
+    one, zero = np.ones(p.npoin), np.zeros(p.npoin)
+    t, xm = O.mt02pp(zero, zero, one, xel, yel, p.z, s, p.ikle, base.npoin, 1.0)
+    hk = [p.z[p.ikle[k + 3]] - p.z[p.ikle[k]] for k in range(3)]
+    wet = np.min(hk, axis=0) >= 1e-4
+    vol = (s * np.mean(hk, axis=0) * wet).sum()
+    assert abs(_energy(t, xm, p.ikle, p.z) / vol - 1.0) < 1e-12
