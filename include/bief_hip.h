@@ -159,8 +159,9 @@ int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
  * the kernels of one CG iteration on the context's current (preconditioned)
  * matrix. FP64: the fused direction update + EBE mat-vec + D.AD kernel
  * (ms_dir) and the X/R update + R.R kernel (ms_upd). Mixed (after a mixed
- * solve): the FP32 inner CG's single kernel per iteration (ms_dir; ms_upd is
- * 0), or its two kernels when BHIP_INNER_CG=2 selects the two-kernel form. Scratch vectors are reused, so
+ * solve): the same two kernels of the FP32 inner CG, or, when
+ * BHIP_INNER_CG=1 selects the single-kernel inner CG, that kernel (ms_dir;
+ * ms_upd is 0). Scratch vectors are reused, so
  * call it after the solves whose results are needed. No counterpart in the
  * reference (it has no timers below the whole run, homere_telemac2d.F). */
 int bhip_time_cg_kernels(bhip_ctx* ctx, int precision, int reps,

@@ -922,12 +922,14 @@ k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, int ncoef, int ns
 #pragma unroll
       for (int j = i + 1; j < NDP; ++j) {
         double* p = xa + (size_t)q * nslot_pad + s;
-        const double v1 = (*p) * dv[j] * dv[i];
+        // elements: XM * D(j) * D(i) (om1111.f:583-600); segments (NDP 2):
+        // XM * D(GLOSEG1) * D(GLOSEG2) (omseg.f:524-536)
+        const double v1 = NDP == 2 ? (*p) * dv[i] * dv[j] : (*p) * dv[j] * dv[i];
         *p = v1;
         if (xa32) xa32[(size_t)q * nslot_pad + s] = (float)v1;
         if (ncoef > npair) {
           double* p2 = xa + (size_t)(q + npair) * nslot_pad + s;
-          const double v2 = (*p2) * dv[j] * dv[i];
+          const double v2 = NDP == 2 ? (*p2) * dv[i] * dv[j] : (*p2) * dv[j] * dv[i];
           *p2 = v2;
           if (xa32) xa32[(size_t)(q + npair) * nslot_pad + s] = (float)v2;
         }
@@ -1014,7 +1016,7 @@ struct bhip_ctx {
   bhip::DevBuf<double> partial;  // 4 arrays of max_part: U[0], U[1], E[0], E[1]
   int max_part = 0;
   bhip::DevBuf<double> part4;    // single-kernel CG: 2 parities x [nblocks][4]
-  bool inner_cg1 = true;         // mixed inner solver: single-kernel CG iterations
+  bool inner_cg1 = false;        // mixed inner solver: single-kernel CG iterations
   bhip::DevBuf<bhip::SolveState> st;
   // captured CG iteration batches (key: precision, solution buffer)
   struct Graph {
@@ -1054,6 +1056,26 @@ static size_t lds_bytes(const Layout& L, size_t elt) {
 
 // LDS of the DMD kernel: D of a block's owned and halo nodes
 static size_t dmd_lds(const Layout& L) { return ((size_t)L.max_local + 1) * sizeof(double); }
+
+static int grid_for(int n);
+
+// OM 'M=DMD' on the context's coefficients with the node vector d (and the
+// FP32 copy of the result when xa32 is given)
+static hipError_t launch_dmd(bhip_ctx* c, float* xa32, const double* d) {
+  const Layout& L = c->L;
+#define BHIP_DMD(NDP)                                                                          \
+  hipLaunchKernelGGL(k_dmd_slots<NDP>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream, \
+                     c->xa.p, xa32, c->ncoef, L.nslot_pad, c->lid.p, d, c->node_start.p,      \
+                     c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks)
+  if (L.ndp == 2)
+    BHIP_DMD(2);
+  else if (L.ndp == 3)
+    BHIP_DMD(3);
+  else
+    BHIP_DMD(6);
+#undef BHIP_DMD
+  return hipGetLastError();
+}
 
 static int grid_for(int n) {
   int g = (n + kThreads * 8 - 1) / (kThreads * 8);
@@ -1126,6 +1148,11 @@ static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typ
     if (!symq) BHIP_DIAG(3, false, false);
     else if (trans) BHIP_DIAG(3, true, true);
     else BHIP_DIAG(3, true, false);
+  } else if (c->L.ndp == 2) {
+    // segment storage: two-node "elements", one ('S') or two ('Q') terms
+    if (!symq) BHIP_DIAG(2, false, false);
+    else if (trans) BHIP_DIAG(2, true, true);
+    else BHIP_DIAG(2, true, false);
   } else {
     // prisms: 15-30 coefficients per element exceed the register budget
     if (!symq) BHIP_DIAG0(6, false, false);
@@ -1204,8 +1231,10 @@ int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const do
                      c->stream, c->partial.p, 0.0, (long)4 * c->max_part);
   HIP_OK(hipGetLastError());
   if (const char* ev = getenv("BHIP_NO_GRAPH")) c->use_graphs = !(ev[0] == '1');
-  // BHIP_INNER_CG=2: the two-kernel (GRACJG-form) inner CG of the mixed path
-  if (const char* ev = getenv("BHIP_INNER_CG")) c->inner_cg1 = !(ev[0] == '2');
+  // mixed path inner CG: two kernels per iteration (GRACJG form, measured
+  // the faster) or BHIP_INNER_CG=1: one kernel per iteration
+  c->inner_cg1 = false;
+  if (const char* ev = getenv("BHIP_INNER_CG")) c->inner_cg1 = ev[0] == '1';
   HIP_OK(c->part4.alloc((size_t)8 * (L.nblocks + 16)));
   hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(8 * (L.nblocks + 16))), dim3(kThreads), 0,
                      c->stream, c->part4.p, 0.0, (long)8 * (L.nblocks + 16));
@@ -1957,17 +1986,7 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
     if (c->ncoef > 0) {
       // the mixed path takes its FP32 copy of D A D in the same pass
       float* x32 = mixed ? c->xa32.p : nullptr;
-      if (L.ndp == 3)
-        hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream,
-                           c->xa.p, x32, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
-                           c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
-                           L.nblocks);
-      else
-        hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream,
-                           c->xa.p, x32, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p,
-                           c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
-                           L.nblocks);
-      HIP_OK(hipGetLastError());
+      HIP_OK(launch_dmd(c, x32, c->vdinv.p));
       xa32_ready = mixed;
     }
     if (precon % 2 == 0 || precon % 3 == 0) c->typdia = 'I';
@@ -2451,15 +2470,7 @@ int bhip_dirich(bhip_ctx* c, double* f, double* sm, const double* fbor, const in
   HIP_OK(hipGetLastError());
   // OM 'M=DMD' with W and the diagonal declared '0' (diri01.f:131-136)
   if (c->ncoef > 0) {
-    if (L.ndp == 3)
-      hipLaunchKernelGGL(k_dmd_slots<3>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream, c->xa.p,
-                         (float*)nullptr, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
-                         c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
-    else
-      hipLaunchKernelGGL(k_dmd_slots<6>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream, c->xa.p,
-                         (float*)nullptr, c->ncoef, L.nslot_pad, c->lid.p, c->vdinv.p, c->node_start.p,
-                         c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_dmd(c, (float*)nullptr, c->vdinv.p));
   }
   hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, hs.p, c->vd0.p,
                      c->perm.p, n);

@@ -59,7 +59,7 @@ std::string build_layout(int npoin, int nelem, int ndp, const int32_t* ikle,
                          const double* x, const double* y, int nplan,
                          int block_nodes, Layout& L) {
   if (npoin <= 0 || nelem <= 0) return "empty mesh";
-  if (ndp != 3 && ndp != 6) return "ndp must be 3 or 6";
+  if (ndp != 2 && ndp != 3 && ndp != 6) return "ndp must be 2, 3 or 6";
   if (nplan < 1) nplan = 1;
   if (nplan > 1 && npoin % nplan != 0) return "npoin is not a multiple of nplan";
   for (long i = 0; i < (long)ndp * nelem; ++i)

@@ -15,9 +15,9 @@
 ! preconditioner is used. Errors that the reference stops on (PLANTE) stop
 ! here too; non-convergence is reported in the listing, as GRACJG does.
 !
-! Supported: EBE storage (A%STO = 1), P1 triangles (11) and P1 prisms (41),
-! a single matrix or a block of one matrix, SLV 1 (CG), 6 (BiCGSTAB),
-! 7 (GMRES), PRECON 0/1, 2, 5.
+! Supported: EBE storage (A%STO = 1) on P1 triangles (11) and P1 prisms
+! (41), segment storage (A%STO = 3) on P1 triangles, a single matrix or a
+! block of one matrix, SLV 1 (CG), 6 (BiCGSTAB), 7 (GMRES), PRECON 0/1, 2, 5.
 !
 ! Compiled with -DBHIP_OWN_TYPES it uses the stand-alone type subset in
 ! bief_hip_types.f90; otherwise the real BIEF_DEF and DECLARATIONS_SPECIAL.
@@ -126,6 +126,7 @@ contains
     end select
   end function
 
+  ! off-diagonal terms per element (ndp 2: per segment, XA1 and XA2)
   integer function bhip_ncoef(ndp, typext)
     integer, intent(in) :: ndp
     character(len=1), intent(in) :: typext
@@ -198,19 +199,22 @@ module bief_hip_load
   implicit none
 contains
 
-! uploads an EBE matrix (A%D, A%X, A%TYPDIA, A%TYPEXT) to the mesh context
-subroutine bhip_load_matrix(a, mesh, h, ndp, npoin, nelem)
+! uploads a matrix (A%D, A%X, A%TYPDIA, A%TYPEXT) to the mesh context:
+! element-by-element storage (A%STO = 1) over IKLE, or segment storage
+! (A%STO = 3, P1 triangles) over GLOSEG, whose segments the context treats
+! as two-node elements. ldxa: the leading dimension of A%X.
+subroutine bhip_load_matrix(a, mesh, h, ndp, npoin, nelem, ldxa)
   type(bief_obj), intent(in), target :: a
   type(bief_mesh), intent(inout), target :: mesh
   type(c_ptr), intent(out) :: h
-  integer, intent(out) :: ndp, npoin, nelem
+  integer, intent(out) :: ndp, npoin, nelem, ldxa
   integer :: nelmax, ncoef, rc
   double precision, allocatable, target :: xa(:,:)
   double precision, target :: dummy(1)
   logical :: have_xy
   if (a%type /= 3) call bhip_fail('BHIP: A IS NOT A MATRIX', BHIP_EBADARG)
-  if (a%sto /= 1) call bhip_fail('BHIP: ELEMENT-BY-ELEMENT STORAGE (MATRIX STORAGE = 1) ONLY', &
-       BHIP_EBADARG)
+  if (a%sto /= 1 .and. a%sto /= 3) call bhip_fail( &
+       'BHIP: MATRIX STORAGE 1 (EBE) OR 3 (SEGMENTS) ONLY', BHIP_EBADARG)
   if (a%elmlin /= a%elmcol) call bhip_fail('BHIP: RECTANGULAR MATRICES NOT SUPPORTED', &
        BHIP_EBADARG)
   ndp = bhip_ndp(a%elmlin)
@@ -219,6 +223,15 @@ subroutine bhip_load_matrix(a, mesh, h, ndp, npoin, nelem)
   npoin = a%d%dim1
   nelem = mesh%nelem
   nelmax = mesh%nelmax
+  ldxa = nelmax
+  if (a%sto == 3) then
+    if (ndp /= 3) call bhip_fail('BHIP: SEGMENT STORAGE ON P1 TRIANGLES ONLY', BHIP_EBADARG)
+    ndp = 2
+    nelem = mesh%nseg
+    nelmax = mesh%gloseg%maxdim1
+    ! XA2 follows XA1 directly: XA(NSEG+1) (matvct.f:685-692)
+    ldxa = nelem
+  end if
   ! prisms: the nodes of one vertical column share (x, y), so the
   ! coordinate order keeps columns contiguous without the plane count
   have_xy = associated(mesh%x) .and. associated(mesh%y)
@@ -226,14 +239,20 @@ subroutine bhip_load_matrix(a, mesh, h, ndp, npoin, nelem)
   if (have_xy) then
     have_xy = size(mesh%x%r) >= npoin .and. size(mesh%y%r) >= npoin
   end if
-  if (have_xy) then
+  if (a%sto == 3) then
+    if (have_xy) then
+      h = bhip_mesh_ctx(npoin, nelem, nelmax, ndp, mesh%gloseg%i, mesh%x%r, mesh%y%r, .true., 1)
+    else
+      h = bhip_mesh_ctx(npoin, nelem, nelmax, ndp, mesh%gloseg%i, dummy, dummy, .false., 1)
+    end if
+  else if (have_xy) then
     h = bhip_mesh_ctx(npoin, nelem, nelmax, ndp, mesh%ikle%i, mesh%x%r, mesh%y%r, .true., 1)
   else
     h = bhip_mesh_ctx(npoin, nelem, nelmax, ndp, mesh%ikle%i, dummy, dummy, .false., 1)
   end if
   ncoef = bhip_ncoef(ndp, a%typext)
   allocate(xa(nelem, max(ncoef, 1)))
-  if (ncoef > 0) call bhip_pack_xa(a%x%r, nelmax, nelem, ncoef, xa)
+  if (ncoef > 0) call bhip_pack_xa(a%x%r, ldxa, nelem, ncoef, xa)
   rc = bhip_matrix_set(h, a%typdia, a%typext, c_loc(a%d%r(1)), c_loc(xa(1, 1)))
   deallocate(xa)
   if (rc /= BHIP_OK) call bhip_fail('BHIP: MATRIX UPLOAD FAILED', rc)
@@ -262,7 +281,7 @@ SUBROUTINE SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX)
   TYPE(BHIP_SOLVER_CFG) :: SC
   TYPE(BHIP_SOLVE_INFO) :: INFO
   TYPE(C_PTR) :: H
-  INTEGER :: NDP, NPOIN, NELEM, NCOEF, RC, UNIT
+  INTEGER :: NDP, NPOIN, NELEM, NCOEF, RC, UNIT, LDXA
   CHARACTER(KIND=C_CHAR) :: TD
   CHARACTER(LEN=6) :: SNAME
   DOUBLE PRECISION, ALLOCATABLE, TARGET :: XA(:,:)
@@ -287,7 +306,7 @@ SUBROUTINE SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX)
   IF (CFG%SLV /= 1 .AND. CFG%SLV /= 6 .AND. CFG%SLV /= 7) THEN
     CALL BHIP_FAIL('BHIP SOLVE: SOLVER NOT AVAILABLE (1 CG, 6 CGSTAB, 7 GMRES)', BHIP_EBADARG)
   END IF
-  CALL BHIP_LOAD_MATRIX(PA, MESH, H, NDP, NPOIN, NELEM)
+  CALL BHIP_LOAD_MATRIX(PA, MESH, H, NDP, NPOIN, NELEM, LDXA)
   SC%SLV = CFG%SLV
   SC%PRECON = CFG%PRECON
   SC%KRYLOV = CFG%KRYLOV
@@ -306,7 +325,7 @@ SUBROUTINE SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX)
   ALLOCATE(XA(NELEM, MAX(NCOEF, 1)))
   IF (NCOEF > 0) THEN
     RC = BHIP_MATRIX_GET(H, TD, C_LOC(PA%D%R(1)), C_LOC(XA(1, 1)))
-    CALL BHIP_UNPACK_XA(XA, NELEM, MESH%NELMAX, NCOEF, PA%X%R)
+    CALL BHIP_UNPACK_XA(XA, NELEM, LDXA, NCOEF, PA%X%R)
   ELSE
     RC = BHIP_MATRIX_GET(H, TD, C_LOC(PA%D%R(1)), C_LOC(DUMMY(1)))
   END IF
@@ -361,14 +380,14 @@ SUBROUTINE MATVEC(OP, X, A, Y, C, MESH, LEGO)
   TYPE(BIEF_MESH), INTENT(INOUT), TARGET :: MESH
   LOGICAL, INTENT(IN), OPTIONAL          :: LEGO
   TYPE(C_PTR) :: H
-  INTEGER :: NDP, NPOIN, NELEM, CODE, RC
+  INTEGER :: NDP, NPOIN, NELEM, CODE, RC, LDXA
   IF (PRESENT(LEGO)) THEN
     ! the non-assembled W hand-over between calls (matvec.f:279-292)
     IF (.NOT. LEGO) CALL BHIP_FAIL('BHIP MATVEC: LEGO=.FALSE. NOT SUPPORTED', BHIP_EBADARG)
   END IF
   CODE = BHIP_OPCODE(OP)
   IF (CODE < 0) CALL BHIP_FAIL('BHIP MATVEC: UNKNOWN OPERATION '//OP, BHIP_EBADARG)
-  CALL BHIP_LOAD_MATRIX(A, MESH, H, NDP, NPOIN, NELEM)
+  CALL BHIP_LOAD_MATRIX(A, MESH, H, NDP, NPOIN, NELEM, LDXA)
   IF (Y%DIM1 < NPOIN .OR. SIZE(X%R) < NPOIN) THEN
     CALL BHIP_FAIL('BHIP MATVEC: VECTOR SIZES', BHIP_EBADARG)
   END IF

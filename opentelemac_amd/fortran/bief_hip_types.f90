@@ -15,10 +15,10 @@ module bief_hip_types
     integer :: type = 0            ! 2 vector, 3 matrix, 4 block
     character(len=6) :: name = ' '
     integer :: elm = 0             ! 11 P1 triangle, 41 P1 prism
-    integer :: dim1 = 0, maxdim1 = 0, dim2 = 1
+    integer :: dim1 = 0, maxdim1 = 0, dim2 = 1, maxdim2 = 1
     double precision, pointer, dimension(:) :: r => null()
     integer, pointer, dimension(:) :: i => null()
-    integer :: sto = 1             ! 1: element by element
+    integer :: sto = 1             ! 1: element by element, 3: segments
     integer :: elmlin = 0, elmcol = 0
     character(len=1) :: typdia = '0', typext = '0'
     type(bief_obj), pointer :: d => null(), x => null()
@@ -29,8 +29,8 @@ module bief_hip_types
   type bief_mesh
     character(len=6) :: name = 'MESH  '
     integer, pointer :: nelem => null(), nelmax => null(), npoin => null()
-    integer, pointer :: dim1 => null(), typelm => null()
-    type(bief_obj), pointer :: ikle => null(), x => null(), y => null()
+    integer, pointer :: dim1 => null(), typelm => null(), nseg => null()
+    type(bief_obj), pointer :: ikle => null(), x => null(), y => null(), gloseg => null()
   end type
 
   type slvcfg

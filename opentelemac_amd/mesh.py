@@ -67,6 +67,40 @@ class Mesh:
         return 0.5 * (xe[1] * ye[2] - xe[2] * ye[1])
 
 
+def segments(ikle):
+    """Edges of a triangle mesh for segment storage (MATRIX STORAGE = 3):
+    gloseg (2, nseg), 0-based, first node < second node, in order of first
+    appearance over the elements and their edges 12, 23, 31; eltseg (3, nelem)
+    the segment of each element edge (12, 23, 31)."""
+    ik = np.asarray(ikle, dtype=np.int64)
+    ne = ik.shape[1]
+    a = np.concatenate([ik[0], ik[1], ik[2]])
+    b = np.concatenate([ik[1], ik[2], ik[0]])
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    key = lo * (int(ik.max()) + 1) + hi
+    # element-major order of the edges (e, then local edge)
+    order = np.arange(3 * ne).reshape(3, ne).T.ravel()
+    uniq, first, inv = np.unique(key[order], return_index=True, return_inverse=True)
+    rank = np.empty(uniq.size, dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(uniq.size)
+    seg_of = np.empty(3 * ne, dtype=np.int64)
+    seg_of[order] = rank[inv]
+    nseg = uniq.size
+    gloseg = np.empty((2, nseg), dtype=np.int64)
+    gloseg[0, seg_of] = lo
+    gloseg[1, seg_of] = hi
+    return gloseg, seg_of.reshape(3, ne)
+
+
+def segment_mesh(mesh: "Mesh"):
+    """The segments of a triangle mesh as two-node elements (the layout of a
+    matrix in segment storage) and the element-to-segment map."""
+    gl, es = segments(mesh.ikle)
+    seg = Mesh(x=mesh.x, y=mesh.y, ikle=gl, npoin2=mesh.npoin, nelem2=gl.shape[1],
+               meta={"storage": "segments"})
+    return seg, es
+
+
 def triangle_areas(x, y, ikle):
     xe = x[ikle]
     ye = y[ikle]

@@ -88,6 +88,31 @@ def wave_matrix(mesh: Mesh, depth, dt=1.0, theta_u=1.0, theta_h=1.0, friction=No
     return da, np.ascontiguousarray(xa), "Q", "S"
 
 
+def ebe_to_segments(xa, eltseg, ikle, gloseg, typext="S"):
+    """Assembles element off-diagonal terms into segment storage (the
+    operation of ASSEX3, assex3.f): XA(s) = sum over the elements holding
+    segment s of their term for that edge. 'S': xa (3, ne) pairs 12, 13, 23
+    -> (1, nseg). 'Q': xa (6, ne) with the transposed pairs 21, 31, 32 ->
+    (2, nseg), XA1 = A(g1, g2), XA2 = A(g2, g1)."""
+    nseg = gloseg.shape[1]
+    # element edge k (12, 23, 31) -> pair column and its local nodes
+    cols = ((0, 0, 1), (2, 1, 2), (1, 2, 0))  # (column of pair (i<j), node a, node b)
+    out = np.zeros((2 if typext == "Q" else 1, nseg))
+    for k, (c, na, nb) in enumerate(cols):
+        s = eltseg[k]
+        if typext == "S":
+            out[0] += np.bincount(s, xa[c], minlength=nseg)
+        else:
+            # xa[c] = A(i, j) for local i < j, xa[c + 3] = A(j, i)
+            i_loc, j_loc = min(na, nb), max(na, nb)
+            gi = ikle[i_loc]
+            forward = gi == gloseg[0, s]  # A(i, j) is A(g1, g2)
+            a_ij, a_ji = xa[c], xa[c + 3]
+            out[0] += np.bincount(s, np.where(forward, a_ij, a_ji), minlength=nseg)
+            out[1] += np.bincount(s, np.where(forward, a_ji, a_ij), minlength=nseg)
+    return out
+
+
 def boundary_nodes(mesh: Mesh):
     """Nodes on edges that belong to a single triangle."""
     ik = mesh.ikle

@@ -113,10 +113,79 @@ void orc_matvct(int transposed, double* x, const double* da, char typdia,
   orc_assvec(x, ikle, nelem, nelmax, ndp, w);
 }
 
+/* MVSEG (mvseg.f:110-728) for a square matrix in segment storage
+ * (MATRIX STORAGE = 3): gloseg holds the two nodes of each segment
+ * (GLOSEG(NSEG,2)), xa the off-diagonal terms XA1 = A(g1,g2) in column 1
+ * and, for TYPEXT 'Q', XA2 = A(g2,g1) in column 2. The diagonal part of
+ * each operation first (the OV calls), then the segments in order, each
+ * adding to its two nodes. */
+static void orc_mvseg(int op, double* x, const double* da, char typdia, const double* xa,
+                      char typext, const double* y, double c, const int* gloseg, int npoin,
+                      int nseg, int nsegmax) {
+  const int trans = op >= ORC_TAY;
+  const double* xa1 = xa;
+  const double* xa2 = typext == 'Q' ? xa + nsegmax : xa;
+  if (trans) { /* X=TAY family: XA2 towards GLOSEG1, XA1 towards GLOSEG2 */
+    const double* tmp = xa1;
+    xa1 = xa2;
+    xa2 = tmp;
+  }
+  for (int i = 0; i < npoin; ++i) {
+    const double d = typdia == 'Q' ? da[i] : 1.0;
+    const int diag = typdia == 'Q' || typdia == 'I';
+    switch (op) {
+      case ORC_AY: case ORC_TAY:
+        x[i] = !diag ? 0.0 : (typdia == 'Q' ? y[i] * d : y[i]);
+        break;
+      case ORC_CAY:
+        x[i] = !diag ? 0.0 : (typdia == 'Q' ? c * y[i] * d : c * y[i]);
+        break;
+      case ORC_NAY: case ORC_NTAY:
+        x[i] = !diag ? 0.0 : (typdia == 'Q' ? -(y[i] * d) : -y[i]);
+        break;
+      case ORC_XPAY: case ORC_XPTAY:
+        if (diag) x[i] = typdia == 'Q' ? x[i] + y[i] * d : x[i] + y[i];
+        break;
+      case ORC_XMAY: case ORC_XMTAY:
+        if (diag) x[i] = typdia == 'Q' ? x[i] - y[i] * d : x[i] - y[i];
+        break;
+      default: /* X=X+CAY, X=X+CTAY */
+        if (diag) x[i] = typdia == 'Q' ? x[i] + c * y[i] * d : x[i] + c * y[i];
+        break;
+    }
+  }
+  if (typext != 'Q' && typext != 'S') return;
+  const int* g1 = gloseg;
+  const int* g2 = gloseg + nsegmax;
+  for (int s = 0; s < nseg; ++s) {
+    const int a = g1[s], b = g2[s];
+    switch (op) {
+      case ORC_AY: case ORC_TAY: case ORC_XPAY: case ORC_XPTAY:
+        x[a] = x[a] + xa1[s] * y[b];
+        x[b] = x[b] + xa2[s] * y[a];
+        break;
+      case ORC_NAY: case ORC_NTAY: case ORC_XMAY: case ORC_XMTAY:
+        x[a] = x[a] - xa1[s] * y[b];
+        x[b] = x[b] - xa2[s] * y[a];
+        break;
+      default: /* X=CAY, X=X+CAY, X=X+CTAY */
+        x[a] = x[a] + c * xa1[s] * y[b];
+        x[b] = x[b] + c * xa2[s] * y[a];
+        break;
+    }
+  }
+}
+
 void orc_matvec(int op, double* x, const double* da, char typdia,
                 const double* xa, char typext, const double* y, double c,
                 const int* ikle, int npoin, int nelem, int nelmax, int ndp,
                 double* w, double* t) {
+  if (ndp == 2) {
+    /* segment storage: MATVCT calls MVSEG with the operation itself when X
+     * and Y are different vectors (matvec.f:289-306) */
+    orc_mvseg(op, x, da, typdia, xa, typext, y, c, ikle, npoin, nelem, nelmax);
+    return;
+  }
   /* matvec.f:349-380: T = A Y or A^T Y, then X combined with T by OS */
   int trans = (op >= ORC_TAY);
   orc_matvct(trans, t, da, typdia, xa, typext, y, ikle, npoin, nelem, nelmax,
@@ -291,11 +360,15 @@ void orc_om_dmd(double* da, char* typdia, double* xa, char typext,
           }
           ++idx;
         }
-      /* XM(e,c) = XM(e,c) * D(IKLE(e,j)) * D(IKLE(e,i)) */
+      /* XM(e,c) = XM(e,c) * D(IKLE(e,j)) * D(IKLE(e,i)); segments:
+       * XM(s) = XM(s) * D(GLOSEG(s,1)) * D(GLOSEG(s,2)) (omseg.f:524-536) */
       double* col = xa + (long)c * nelmax;
       const int* ii = ikle + (long)i * nelmax;
       const int* jj = ikle + (long)j * nelmax;
-      for (int e = 0; e < nelem; ++e) col[e] = col[e] * d[jj[e]] * d[ii[e]];
+      if (ndp == 2)
+        for (int e = 0; e < nelem; ++e) col[e] = col[e] * d[ii[e]] * d[jj[e]];
+      else
+        for (int e = 0; e < nelem; ++e) col[e] = col[e] * d[jj[e]] * d[ii[e]];
     }
   }
 }

@@ -246,6 +246,7 @@ def _bind_driver(L):
     L.ref_mt01pp.argtypes = [_I, _D, _P, _P, _P, _P, _P]
     L.ref_mt02pp.argtypes = [_I, _I, _I, _D, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _I,
                              _P, _P]
+    L.ref_mesh_segments.argtypes = [_I, _P]
     return L
 
 
@@ -264,11 +265,14 @@ class RefMesh:
     replaced by the HIP host layer)."""
     _current = {}
 
-    def __init__(self, ikle, npoin2, nelem2, nplan=1, loader=None):
+    def __init__(self, ikle, npoin2, nelem2, nplan=1, loader=None, gloseg=None):
+        """gloseg (2, nseg), 0-based: the matrix of the driver is in segment
+        storage (MATRIX STORAGE = 3) over these segments of the triangles."""
         self.ikle = _i32(ikle)
         self.ndp = self.ikle.shape[0]
         self.args = (npoin2, nelem2, nplan)
         self.loader = loader or ref
+        self.gloseg = None if gloseg is None else _i32(gloseg)
 
     @property
     def lib(self):
@@ -278,6 +282,8 @@ class RefMesh:
         key = self.loader.__name__
         if RefMesh._current.get(key) is not self:
             self.lib.ref_mesh_init(self.ndp, *self.args, _ptr(self.ikle))
+            if self.gloseg is not None:
+                self.lib.ref_mesh_segments(self.gloseg.shape[1], _ptr(self.gloseg))
             RefMesh._current[key] = self
 
 

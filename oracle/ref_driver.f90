@@ -130,6 +130,38 @@ contains
     ready = .true.
   end subroutine
 
+  ! Segment storage (MATRIX STORAGE = 3) on the current triangle mesh:
+  ! GLOSEG (nseg, 2) arrives 0-based, structure of arrays; the driver's
+  ! matrix is re-allocated with storage 3 and the load/store helpers then
+  ! treat the segments as two-node elements.
+  subroutine ref_mesh_segments(nseg, gloseg0) bind(c, name='ref_mesh_segments')
+    integer(c_int), value :: nseg
+    integer(c_int), intent(in) :: gloseg0(*)
+    integer :: cfgm(2)
+    mesh%nseg = nseg
+    mesh%nds(11, 2) = nseg
+    deallocate(mesh%gloseg%i)
+    call set_int_obj_alloc(mesh%gloseg, nseg, 2)
+    mesh%gloseg%i(1:2 * nseg) = gloseg0(1:2 * nseg) + 1
+    nullify(ma%d, ma%x)
+    cfgm(1) = 3
+    cfgm(2) = 1
+    call bief_allmat(ma, 'A     ', ielm_s, ielm_s, cfgm, 'Q', 'Q', mesh)
+    ndp_s = 2
+    nelem_s = nseg
+  end subroutine
+
+  subroutine set_int_obj_alloc(obj, n1, n2)
+    type(bief_obj), intent(inout) :: obj
+    integer, intent(in) :: n1, n2
+    obj%dim1 = n1
+    obj%maxdim1 = n1
+    obj%dim2 = n2
+    obj%maxdim2 = n2
+    allocate(obj%i(max(n1 * n2, 1)))
+    obj%i = 0
+  end subroutine
+
   subroutine ref_mesh_free_all()
     ! objects of the previous mesh are dropped (not freed: this is a test
     ! driver and meshes are few) so BIEF allocates them afresh

@@ -160,8 +160,52 @@ def mt02pp_case():
     return out
 
 
+def seg_case():
+    """Segment storage (MATRIX STORAGE = 3, MVSEG / OMSEG): all MATVEC
+    operations, SOLVE (CG, BiCGSTAB, GMRES x PRECON 0/2/5), DIRICH, LUMP on
+    the segments of a small triangle mesh."""
+    m = M.random_small_mesh(9, 7, seed=3)
+    gl, es = M.segments(m.ikle)
+    rm = O.RefMesh(m.ikle, m.npoin, m.nelem, gloseg=gl)
+    rng = np.random.default_rng(50)
+    out = {"gloseg": gl, "eltseg": es}
+    da = rng.uniform(2, 3, m.npoin)
+    out["da"] = da
+    for typext, nc in (("S", 1), ("Q", 2)):
+        xa = rng.uniform(-1, 1, (nc, gl.shape[1]))
+        out[f"xa_{typext}"] = xa
+        for typdia in "QI0":
+            A = O.Matrix(da, xa, typdia, typext)
+            for op, code in O.OPS.items():
+                y = rng.normal(size=m.npoin)
+                x0 = rng.normal(size=m.npoin)
+                key = f"mv_{typdia}{typext}_{code}"
+                out[key + "_y"], out[key + "_x0"] = y, x0
+                out[key] = O.ref_matvec(rm, op, A, y, x0, c=0.37)
+    sysd = S.wave_system(m, 1.0 + m.x / 4.0, dt=2.0)
+    xs = S.ebe_to_segments(sysd["xa"], es, m.ikle, gl, "S")
+    out["spd_da"], out["spd_xa"], out["spd_b"], out["spd_x0"] = sysd["da"], xs, sysd["b"], \
+        sysd["x0"]
+    A = O.Matrix(sysd["da"], xs, "Q", "S")
+    for slv in (1, 6, 7):
+        for precon in (0, 2, 5):
+            x, info, A2, b2 = O.ref_solve(rm, A, sysd["b"], sysd["x0"], slv=slv, precon=precon,
+                                          krylov=5, eps=1e-10, nitmax=500)
+            key = f"solve_{slv}_{precon}"
+            out[key + "_x"], out[key + "_niter"] = x, np.array(info["niter"])
+            out[key + "_xa"], out[key + "_b"] = A2.xa, b2
+    limdir = np.where(rng.uniform(size=m.npoin) < 0.2, 5, 2).astype(np.int32)
+    fbor = np.where(limdir == 5, rng.normal(size=m.npoin), 0.0)
+    sm = rng.normal(size=m.npoin)
+    out["dir_limdir"], out["dir_fbor"], out["dir_sm"] = limdir, fbor, sm
+    f, A3, sm2 = O.ref_dirich(rm, A, sm, fbor, limdir)
+    out["dir_f"], out["dir_xa"], out["dir_sm_out"] = f, A3.xa, sm2
+    out["lump"] = O.ref_lump(rm, A, 0.3)
+    return out
+
+
 CASES = {"ref_prism": prism_case, "ref_small": small_case, "ref_channel": channel_case,
-         "ref_mt02pp": mt02pp_case}
+         "ref_mt02pp": mt02pp_case, "ref_seg": seg_case}
 
 
 def main(names):

@@ -72,3 +72,46 @@ def test_dropin_dirich_lump(gpu, small):
     assert np.allclose(sm2, smo, rtol=1e-13, atol=1e-13 * np.abs(smo).max())
     lo = O.lump(A, m.ikle, 0.5)
     assert np.allclose(O.ref_lump(rm, A, 0.5), lo, rtol=1e-13, atol=1e-13 * np.abs(lo).max())
+
+
+@pytest.fixture(scope="module")
+def small_seg(small):
+    m, _ = small
+    gl, es = M.segments(m.ikle)
+    return m, gl, es, O.RefMesh(m.ikle, m.npoin, m.nelem, loader=O.dropin, gloseg=gl)
+
+
+@pytest.mark.parametrize("typext", ["S", "Q"])
+def test_dropin_segment_matvec(gpu, small_seg, typext):
+    """MATVEC on a matrix in segment storage (A%STO = 3) through the host
+    layer: the context is built over MESH%GLOSEG."""
+    m, gl, _, rm = small_seg
+    rng = np.random.default_rng(11)
+    da = rng.uniform(1, 2, m.npoin)
+    xa = rng.normal(size=(2 if typext == "Q" else 1, gl.shape[1]))
+    for typdia in "QI":
+        A = O.Matrix(da, xa, typdia, typext)
+        for op in O.OPS:
+            y = rng.normal(size=m.npoin)
+            x0 = rng.normal(size=m.npoin)
+            got = O.ref_matvec(rm, op, A, y, x0, c=0.3)
+            ref = O.matvec(op, A, gl, y, x0, c=0.3)
+            assert np.abs(got - ref).max() <= 1e-13 * (np.abs(ref).max() + 1), (typdia, op)
+
+
+@pytest.mark.parametrize("slv", [1, 6, 7])
+def test_dropin_segment_solve(gpu, small_seg, slv):
+    m, gl, es, rm = small_seg
+    sysd = S.wave_system(m, 1.0 + m.x / 3.0, dt=0.5 if slv == 7 else 2.0)
+    xs = S.ebe_to_segments(sysd["xa"], es, m.ikle, gl, "S")
+    A = O.Matrix(sysd["da"], xs, "Q", "S")
+    x_o, info_o, A_o, b_o = O.solve(A, gl, sysd["b"], sysd["x0"], slv=slv, precon=2,
+                                    krylov=5, eps=1e-10, nitmax=600)
+    x, info, A2, b2 = O.ref_solve(rm, A, sysd["b"], sysd["x0"], slv=slv, precon=2,
+                                  krylov=5, eps=1e-10, nitmax=600)
+    tol_it = 1 if slv != 6 else max(4, info_o.niter // 10)
+    assert abs(info["niter"] - info_o.niter) <= tol_it
+    assert np.linalg.norm(x - x_o) <= (1e-10 if slv != 6 else 1e-8) * np.linalg.norm(x_o)
+    assert A2.typdia == A_o.typdia
+    assert np.array_equal(A2.xa, A_o.xa)
+    assert np.allclose(b2, b_o, rtol=1e-14, atol=0)
