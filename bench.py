@@ -109,10 +109,13 @@ def main():
     ap.add_argument("--cache", default=None, help="npz file to reuse the synthetic workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (profiler counter scale)")
-    ap.add_argument("--workload", default="estuary", choices=["estuary", "prisms", "adjoint"],
+    ap.add_argument("--workload", default="estuary",
+                    choices=["estuary", "prisms", "adjoint", "ensemble"],
                     help="estuary: the 2M-node wave-equation CG (the headline line); prisms: "
                          "the 5M-node sigma-mesh diffusion and Poisson systems (parity "
                          "configuration, measured the same way)")
+    ap.add_argument("--adj-fd", type=int, default=20,
+                    help="adjoint workload: nodes per parameter checked by central differences")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -137,8 +140,11 @@ def main():
         return
     if args.workload == "prisms":
         return bench_prisms(args, dev)
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(args)), flush=True)
+        return
     if args.workload == "adjoint":
-        print(json.dumps(bench_adjoint(args)), flush=True)
+        print(json.dumps(bench_adjoint(args, n_fd=args.adj_fd)), flush=True)
         return
     m, sysd = build_workload(args.npoin, args.seed, args.cache)
     t = time.time()
@@ -433,6 +439,97 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
             "fd_max_rel_error": fd_err, "fd_nodes_per_parameter": n_fd,
             "tangent_linear_rel_error": abs(adj - tl) / max(abs(tl), 1e-300),
             "solver": "mixed (FP32 EBE CG, FP64 refinement), eps 1e-10"}
+
+
+def bench_ensemble(args, nsys=256):
+    """256 channel meshes (2000 m x 200 m, the 20,451-node channel's shape)
+    of 5k to 200k nodes, log-uniform (seeded), each with its own wave-equation
+    step (depth 1 -> 5 m with a bump, 0.2 m tide at x = 0, dt = 1 s); all
+    solved as one ensemble (one context, one launch per kernel covering every
+    mesh, FP64 CG + PRECON 2 to 1e-8 with per-mesh scalars), its node-block
+    size chosen by the autotuner on the first 32 meshes. A step restores the
+    assembled matrices and solves every mesh; p50/p99 of the step time over
+    the timed steps. Beside it: the same meshes one context each, solved one
+    after the other."""
+    import torch
+    from opentelemac_amd import bief, mesh as M, system as S, tune
+    rng = np.random.default_rng(args.seed + 3)
+    sizes = np.exp(rng.uniform(np.log(5000), np.log(200000), nsys)).astype(int)
+    t = time.time()
+    meshes, systems = [], []
+    for k, n in enumerate(sizes):
+        m = M.channel_mesh_n(int(n), seed=args.seed * 1000 + k)
+        meshes.append(m)
+        systems.append(S.wave_system(m, M.channel_depth(m), dt=1.0))
+    total = sum(m.npoin for m in meshes)
+    log(f"ensemble of {nsys} meshes, {total} nodes, built in {time.time() - t:.1f}s")
+    cat = lambda key, sel=None: np.ascontiguousarray(np.concatenate(
+        [s[key] for s in (systems if sel is None else systems[:sel])], axis=-1))
+
+    def make(bn, sel=32):
+        return bief.Ensemble(meshes[:sel], block_nodes=bn)
+
+    def load(ctx, sel=32):
+        ctx.set_matrix("Q", "S", cat("da", sel), cat("xa", sel))
+        ctx.solve(cat("x0", sel), cat("b", sel).copy(), slv=1, precon=2, eps=1e-8, nitmax=5000)
+        ctx.set_matrix("Q", "S", cat("da", sel), cat("xa", sel))
+
+    best, table = tune.block_nodes(make, load, candidates=(256, 512, 1024, 2048))
+    log(f"block size {best}: {table}")
+    t = time.time()
+    ens = bief.Ensemble(meshes, block_nodes=best)
+    log(f"ensemble context in {time.time() - t:.1f}s")
+    dev = torch.device("cuda:0")
+    ens.set_matrix("Q", "S", torch.from_numpy(cat("da")).to(dev),
+                   torch.from_numpy(cat("xa")).to(dev))
+    ens.save_matrix()
+    x0 = torch.from_numpy(cat("x0")).to(dev)
+    b0 = torch.from_numpy(cat("b")).to(dev)
+    x, b = torch.empty_like(x0), torch.empty_like(b0)
+
+    def step():
+        ens.restore_matrix()
+        x.copy_(x0)
+        b.copy_(b0)
+        return ens.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000)[1]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        info = step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    infos = ens.infos()
+    lat_ms = np.array(lat) * 1e3
+    # the same systems, one context each, one after the other
+    ctxs = [bief.Context(m, block_nodes=best) for m in meshes]
+    dsys = [(torch.from_numpy(sd["da"]).to(dev), torch.from_numpy(np.ascontiguousarray(sd["xa"])).to(dev),
+             torch.from_numpy(sd["x0"]).to(dev), torch.from_numpy(sd["b"]).to(dev)) for sd in systems]
+    seq = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for ctx, (da, xa, xx, bb) in zip(ctxs, dsys):
+            ctx.set_matrix("Q", "S", da, xa)
+            ctx.solve(xx.clone(), bb.clone(), slv=1, precon=2, eps=1e-8, nitmax=5000)
+        torch.cuda.synchronize()
+        seq.append(time.perf_counter() - t0)
+    return {"workload": f"ensemble of {nsys} channel meshes, 5k-200k nodes (log-uniform)",
+            "systems": nsys, "total_nodes": int(total),
+            "total_elements": int(sum(m.nelem for m in meshes)),
+            "block_nodes": best, "autotune_ms_per_iteration": table,
+            "ms_per_ensemble_step_p50": float(np.percentile(lat_ms, 50)),
+            "ms_per_ensemble_step_p99": float(np.percentile(lat_ms, 99)),
+            "ensemble_steps_per_s": float(1e3 / lat_ms.mean()),
+            "systems_per_s": float(nsys * 1e3 / lat_ms.mean()),
+            "iterations_max": int(info["niter"]),
+            "iterations_median": float(np.median([i["niter"] for i in infos])),
+            "all_converged": bool(all(i["converged"] for i in infos)),
+            "sequential_ms_per_round": float(min(seq) * 1e3),
+            "speedup_vs_sequential": float(min(seq) * 1e3 / lat_ms.mean())}
 
 
 def _cpu_model():

@@ -115,11 +115,29 @@ int bhip_layout_check(int npoin, int nelem, int ndp, const int32_t* ikle,
                       const double* x, const double* y, int nplan,
                       int block_nodes, int64_t* stats);
 
+/* Ensemble of nsys independent meshes in one context (no counterpart in the
+ * reference, which solves one mesh per run): every entry point then works on
+ * the union of the meshes -- vectors are the meshes' vectors one after the
+ * other, xa the meshes' element terms one after the other (nelem = the sum)
+ * -- and one launch covers all meshes. bhip_solve_dev runs FP64 CG (SLV 1,
+ * PRECON 0/2/5) with the scalars, stop test and iteration count of each
+ * mesh its own; bhip_batch_info returns each mesh's outcome. ikle holds the
+ * meshes' connectivities one after the other (mesh s: ndp x nelem[s], SoA,
+ * its own 0-based node ids); x, y (may be NULL) the coordinates. ndp 3
+ * (triangles) or 2 (segment storage). */
+int bhip_batch_create(int nsys, const int* npoin, const int* nelem, int ndp,
+                      const int32_t* ikle, const double* x, const double* y,
+                      int block_nodes, bhip_ctx** out);
+int bhip_batch_info(const bhip_ctx* ctx, int nsys, bhip_solve_info* infos);
+
 /* ---- matrix ------------------------------------------------------------ */
 
 /* Loads an EBE matrix (replaces the A%D / A%X / A%TYPDIA / A%TYPEXT fields
  * BIEF_OBJ hands to MATVEC/SOLVE, bief_def.f:74-196). typdia 'Q','I','0';
- * typext 'S','Q','0'. The host variant copies from host memory. */
+ * typext 'S','Q','0'. The host variant copies from host memory and returns
+ * when the copy is done; the _dev variant queues the copy on the context
+ * stream, so the caller keeps da_dev / xa_dev valid until that stream has
+ * run it (any later synchronising call on the context guarantees it). */
 int bhip_matrix_set(bhip_ctx* ctx, char typdia, char typext, const double* da,
                     const double* xa);
 int bhip_matrix_set_dev(bhip_ctx* ctx, char typdia, char typext,

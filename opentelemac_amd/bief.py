@@ -97,6 +97,8 @@ def lib():
         L.bhip_lump.argtypes = [_P, _D, _P]
         L.bhip_matrix_mt01aa.argtypes = [_P, _D, _P, _I]
         L.bhip_matrix_mt02aa.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _I]
+        L.bhip_batch_create.argtypes = [_I, _P, _P, _I, _P, _P, _P, _I, ctypes.POINTER(_P)]
+        L.bhip_batch_info.argtypes = [_P, _I, ctypes.POINTER(SolveInfo)]
         L.bhip_matrix_mt01pp.argtypes = [_P, _D, _P, _P, _I]
         L.bhip_matrix_mt02pp.argtypes = [_P, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I]
         _lib = L
@@ -201,6 +203,12 @@ class Context:
             self._sync_torch()
             rc = lib().bhip_matrix_set_dev(self._h, typdia.encode(), typext.encode(),
                                            da.data_ptr(), xa.data_ptr() if xa is not None else None)
+            # the copy runs later on the context stream: torch's stream waits
+            # for it, so memory torch hands out again is only written after
+            # the copy has read it. (Not record_stream: the context's stream
+            # may be destroyed before the tensor is freed, and torch would
+            # then record an event on a stream that no longer exists.)
+            self._join_torch()
         else:
             da = _host_f64(da)
             xa = _host_f64(xa) if xa is not None else None
@@ -362,3 +370,48 @@ class Context:
         d = info.as_dict()
         d["status"] = STATUS[rc]
         return x, d
+
+
+class Ensemble(Context):
+    """Several independent meshes in one context (bhip_batch_create): the
+    vectors and element terms of all meshes one after the other; one launch
+    covers all meshes, and FP64 CG solves every system with its own scalars
+    and stop test."""
+
+    def __init__(self, meshes, block_nodes: int = 0):
+        ndp = meshes[0].ndp
+        if any(m.ndp != ndp for m in meshes):
+            raise BiefError("an ensemble holds meshes of one element type")
+        self.meshes = list(meshes)
+        self.nsys = len(meshes)
+        npo = np.array([m.npoin for m in meshes], dtype=np.int32)
+        nel = np.array([m.nelem for m in meshes], dtype=np.int32)
+        self.node_off = np.concatenate([[0], np.cumsum(npo)]).astype(np.int64)
+        self.elem_off = np.concatenate([[0], np.cumsum(nel)]).astype(np.int64)
+        self.npoin = int(self.node_off[-1])
+        self.nelem = int(self.elem_off[-1])
+        self.ndp = ndp
+        self.mesh = None
+        self._ikle = np.ascontiguousarray(
+            np.concatenate([np.ascontiguousarray(m.ikle, dtype=np.int32).ravel() for m in meshes]))
+        x = np.ascontiguousarray(np.concatenate([m.x for m in meshes]), dtype=np.float64)
+        y = np.ascontiguousarray(np.concatenate([m.y for m in meshes]), dtype=np.float64)
+        self._npo, self._nel = npo, nel
+        h = _P()
+        rc = lib().bhip_batch_create(self.nsys, npo.ctypes.data, nel.ctypes.data, ndp,
+                                     self._ikle.ctypes.data, x.ctypes.data, y.ctypes.data,
+                                     int(block_nodes), ctypes.byref(h))
+        _check(rc, "bhip_batch_create")
+        self._h = h
+        self.typdia = "0"
+        self.typext = "0"
+
+    def split(self, v):
+        """The per-mesh pieces of an ensemble node vector."""
+        return [v[self.node_off[i]:self.node_off[i + 1]] for i in range(self.nsys)]
+
+    def infos(self):
+        """Outcome of the last solve for each mesh."""
+        arr = (SolveInfo * self.nsys)()
+        _check(lib().bhip_batch_info(self._h, self.nsys, arr), "bhip_batch_info")
+        return [a.as_dict() for a in arr]

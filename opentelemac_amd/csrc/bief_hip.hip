@@ -169,6 +169,15 @@ __device__ inline double reduce_partials(const double* __restrict__ partial, int
   return s;
 }
 
+// XCD-aware block order: workgroup g is dispatched to XCD g % 8; each XCD
+// gets a contiguous range of node blocks so that blocks sharing halo nodes
+// share an L2. Bijective for any block count.
+__device__ inline int xcd_block(int g, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = g & 7, k = g >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
 __device__ inline int load_relaxed(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -235,6 +244,10 @@ struct EbeArgs {
   int skip_if_stalled;        // first direction kernel of a scheduled inner solve
   int skip_if_done;           // kernels of a running solve after it finished
   int skip_if_outer_done;     // mixed outer residual: skip once the solve is done
+  // ensembles: system of each block and block range of each system (NULL
+  // for one system: all blocks, partials [0, n_part_in))
+  const int32_t* block_sys;
+  const int32_t* sys_blocks;
 };
 
 // column of the coefficient array holding A(i, j), i != j
@@ -329,14 +342,12 @@ k_ebe(EbeArgs<T, TA> a) {
   // XCD-aware mapping: workgroup g is dispatched to XCD g % 8; give each XCD
   // a contiguous range of node blocks so blocks sharing halo nodes share an
   // L2. Bijective for any block count.
-  const int nb = a.nblocks;
-  int b;
-  {
-    const int g = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = g & 7, k = g >> 3;
-    b = xcd * q + (xcd < r ? xcd : r) + k;
-  }
+  const int b = xcd_block(blockIdx.x, a.nblocks);
+  // the system (mesh of an ensemble) of this block, its state and the range
+  // of the partial sums (one per block) that belong to it
+  const int sys = a.block_sys ? a.block_sys[b] : 0;
+  const int pb0 = a.sys_blocks ? a.sys_blocks[sys] : 0;
+  const int pb1 = a.sys_blocks ? a.sys_blocks[sys + 1] : a.n_part_in;
   if (a.skip_if_outer_done && outer_blocked(a.st)) return;
   if (a.skip_if_done && load_relaxed(&a.st->done)) return;
   const int32_t* co = a.color_off + (size_t)b * (kMaxColors + 1);
@@ -359,12 +370,12 @@ k_ebe(EbeArgs<T, TA> a) {
 
   // ---- early exits of the CG kernels (state written by earlier kernels) --
   if constexpr (FIN == FIN_CG) {
-    SolveState* st = a.st;
+    SolveState* st = a.st + sys;
     if (a.skip_if_stalled && st->stalled) return;
     if (load_relaxed(&st->done)) return;
     if (load_relaxed(&st->final_flag)) {
       // the update kernel before this one applied the last X update
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (b == pb0 && threadIdx.x == 0) {
         store_relaxed(&st->done, ST_MAXITER);
         store_relaxed(&st->done1, (st->final_m << 2) | ST_MAXITER);
       }
@@ -419,14 +430,14 @@ k_ebe(EbeArgs<T, TA> a) {
   if constexpr (FIN == FIN_CG) {
     // E_m: RMRM from the update kernel's partials, stop test (gracjg.F:306-324,
     // and the strict test of the initial residual, gracjg.F:219), BETA
-    SolveState* st = a.st;
+    SolveState* st = a.st + sys;
     const int p = a.parity;
-    const double rr = reduce_partials(a.part_in, a.n_part_in, sh_red, &sh_b);
+    const double rr = reduce_partials(a.part_in + pb0, pb1 - pb0, sh_red, &sh_b);
     const int m = st->m[1 - p] + 1;
     const double thr = st->use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
     const bool conv = (m == 0) ? (rr < thr) : (rr <= thr);
     if (conv) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (b == pb0 && threadIdx.x == 0) {
         st->final_m = m;
         st->final_rr = rr;
         store_relaxed(&st->done, ST_CONVERGED);
@@ -435,7 +446,7 @@ k_ebe(EbeArgs<T, TA> a) {
       return;
     }
     cg_beta = (m == 0) ? 0.0 : rr / st->rr[1 - p];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (b == pb0 && threadIdx.x == 0) {
       st->rr[p] = rr;
       st->m[p] = m;
       st->status[p] = (m >= st->nitmax) ? ST_FINAL : ST_RUNNING;
@@ -674,25 +685,33 @@ k_ebe(EbeArgs<T, TA> a) {
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kThreads)
 k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
-            const T* __restrict__ ad, int n, const double* __restrict__ part_e, int n_part_e,
-            double* __restrict__ part_u, SolveState* st, int parity) {
+            const T* __restrict__ ad, const int32_t* __restrict__ node_start, int nblocks,
+            const int32_t* __restrict__ block_sys, const int32_t* __restrict__ sys_blocks,
+            const double* __restrict__ part_e, double* __restrict__ part_u, SolveState* st0,
+            int parity) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
+  // one workgroup per node block, in the direction kernel's block order (its
+  // AD and D of these nodes were written through the same XCD's L2)
+  const int b = xcd_block(blockIdx.x, nblocks);
+  const int sys = block_sys ? block_sys[b] : 0;
+  const int pb0 = sys_blocks ? sys_blocks[sys] : 0;
+  const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
+  SolveState* st = st0 + sys;
   if (load_relaxed(&st->done)) return;
   const int q = 1 - parity;  // slot of iteration m - 1
   const int status = st->status[q];
   // RO = RMRM / D.AD of iteration m - 1 (gracjg.F:380-383)
-  const double dad = reduce_partials(part_e, n_part_e, sh_red, &sh_b);
+  const double dad = reduce_partials(part_e + pb0, pb1 - pb0, sh_red, &sh_b);
   const double rho = st->rr[q] / dad;
   const T rhot = (T)rho;
-  const int stride = gridDim.x * blockDim.x;
+  const int n0 = node_start[b], n1 = node_start[b + 1];
   if (status == ST_FINAL) {
     // NITMAX reached: the last X = X - RO D, then stop (gracjg.F:387-409)
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-      x[i] = x[i] - (TX)rhot * (TX)d[i];
-    // every workgroup runs this branch (status[q] was written by an earlier
-    // kernel); the next direction kernel turns final_flag into done
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int i = n0 + threadIdx.x; i < n1; i += blockDim.x) x[i] = x[i] - (TX)rhot * (TX)d[i];
+    // every workgroup of the system runs this branch (status[q] was written
+    // by an earlier kernel); the next direction kernel turns final_flag into done
+    if (b == pb0 && threadIdx.x == 0) {
       st->final_m = st->m[q];
       st->final_rr = st->rr[q];
       st->rho[q] = rho;
@@ -701,15 +720,15 @@ k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
     return;
   }
   double part = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  for (int i = n0 + threadIdx.x; i < n1; i += blockDim.x) {
     x[i] = x[i] - (TX)rhot * (TX)d[i];
     const T rv = r[i] - rhot * ad[i];
     r[i] = rv;
     part += (double)rv * (double)rv;
   }
-  double s = block_sum(part, sh_red);
-  if (threadIdx.x == 0) part_u[blockIdx.x] = s;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->rho[q] = rho;
+  double sm = block_sum(part, sh_red);
+  if (threadIdx.x == 0) part_u[b] = sm;
+  if (b == pb0 && threadIdx.x == 0) st->rho[q] = rho;
 }
 
 // ---------------------------------------------------------------------------
@@ -723,6 +742,18 @@ __global__ void k_dot_partials(const T* x, const T* y, int n, double* partial) {
     part += (double)x[i] * (double)y[i];
   double s = block_sum(part, sh_red);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// partial[b] = x.y over the nodes of block b (one workgroup per block)
+__global__ void k_dot_blocks(const double* __restrict__ x, const double* __restrict__ y,
+                             const int32_t* __restrict__ node_start, double* partial) {
+  __shared__ double sh_red[kWaves];
+  const int b = blockIdx.x;
+  double part = 0.0;
+  for (int i = node_start[b] + threadIdx.x; i < node_start[b + 1]; i += blockDim.x)
+    part += x[i] * y[i];
+  double s = block_sum(part, sh_red);
+  if (threadIdx.x == 0) partial[b] = s;
 }
 
 // one workgroup: *dst = fixed-order sum of n partials
@@ -761,11 +792,17 @@ __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int 
 // GRACJG set-up (gracjg.F:189-209) from the partial sums of ||b'||^2: XL,
 // the zero right-hand-side exit, the relative/absolute switch; then the
 // state of a fresh CG solve (FP64) or of the mixed outer loop.
-__global__ void k_cg_setup(const double* __restrict__ partial, int n, double eps, int nitmax,
-                           int mixed, double inner_eps, int inner_nitmax, SolveState* st) {
+// One workgroup per system; partial holds one sum per node block.
+__global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
+                           const int32_t* __restrict__ sys_blocks, double eps, int nitmax,
+                           int mixed, double inner_eps, int inner_nitmax, SolveState* st0) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
-  const double xl_raw = reduce_partials(partial, n, sh_red, &sh_b);
+  const int sys = blockIdx.x;
+  SolveState* st = st0 + sys;
+  const int pb0 = sys_blocks ? sys_blocks[sys] : 0;
+  const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
+  const double xl_raw = reduce_partials(partial + pb0, pb1 - pb0, sh_red, &sh_b);
   if (threadIdx.x != 0) return;
   const double xl = xl_raw < 1.0 ? 1.0 : xl_raw;
   st->xl_raw = xl_raw;
@@ -1017,7 +1054,14 @@ struct bhip_ctx {
   int max_part = 0;
   bhip::DevBuf<double> part4;    // single-kernel CG: 2 parities x [nblocks][4]
   bool inner_cg1 = false;        // mixed inner solver: single-kernel CG iterations
-  bhip::DevBuf<bhip::SolveState> st;
+  bhip::DevBuf<bhip::SolveState> st;  // one per system
+  // ensembles of independent meshes (nsys > 1): the system of each node
+  // block, the block range and node/element offsets of each system
+  int nsys = 1;
+  bhip::DevBuf<int32_t> block_sys, sys_blocks;
+  std::vector<int32_t> sys_blocks_h;
+  std::vector<int64_t> sys_node_off, sys_elem_off;
+  std::vector<bhip_solve_info> sys_info;  // per system, last solve
   // captured CG iteration batches (key: precision, solution buffer)
   struct Graph {
     const void* key = nullptr;
@@ -1048,6 +1092,9 @@ struct bhip_ctx {
 };
 
 namespace bhip {
+
+// the per-system arrays of an ensemble; NULL for one system
+static const int32_t* sys_ptr(const DevBuf<int32_t>& v) { return v.n ? v.p : nullptr; }
 
 static size_t lds_bytes(const Layout& L, size_t elt) {
   size_t nl = ((size_t)L.max_local + 1) & ~(size_t)1;
@@ -1097,6 +1144,8 @@ static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
   a.da = da;
   a.partial = c->partial.p;
   a.st = c->st.p;
+  a.block_sys = sys_ptr(c->block_sys);
+  a.sys_blocks = sys_ptr(c->sys_blocks);
   return a;
 }
 
@@ -1173,18 +1222,8 @@ using namespace bhip;
 // ===========================================================================
 // C ABI
 // ===========================================================================
-extern "C" {
-
-int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const double* x,
-                    const double* y, int nplan, int block_nodes, bhip_ctx** out) {
-  if (!out || !ikle) return BHIP_EBADARG;
-  *out = nullptr;
-  std::unique_ptr<bhip_ctx> c(new bhip_ctx());
-  std::string err = build_layout(npoin, nelem, ndp, ikle, x, y, nplan, block_nodes, c->L);
-  if (!err.empty()) {
-    fprintf(stderr, "libbiefhip: bhip_ctx_create: %s\n", err.c_str());
-    return BHIP_EBADARG;
-  }
+// Device side of a context whose layout (and, for ensembles, systems) is set.
+static int ctx_setup(std::unique_ptr<bhip_ctx>& c, bhip_ctx** out) {
   const Layout& L = c->L;
   c->lds_f64 = lds_bytes(L, sizeof(double));
   c->lds_f32 = lds_bytes(L, sizeof(float));
@@ -1244,10 +1283,20 @@ int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const do
     int v = atoi(ev);
     if (v >= 0 && v <= 3) c->prefetch_j = v;
   }
-  HIP_OK(c->st.alloc(1));
-  HIP_OK(hipHostMalloc((void**)&c->st_host, sizeof(SolveState), hipHostMallocDefault));
-  std::memset(c->st_host, 0, sizeof(SolveState));
-  HIP_OK(up(c->st.p, c->st_host, sizeof(SolveState)));
+  HIP_OK(c->st.alloc(c->nsys));
+  HIP_OK(hipHostMalloc((void**)&c->st_host, sizeof(SolveState) * c->nsys, hipHostMallocDefault));
+  std::memset(c->st_host, 0, sizeof(SolveState) * c->nsys);
+  HIP_OK(up(c->st.p, c->st_host, sizeof(SolveState) * c->nsys));
+  c->sys_info.assign(c->nsys, bhip_solve_info());
+  if (c->nsys > 1) {
+    std::vector<int32_t> bs(L.nblocks);
+    for (int sy = 0; sy < c->nsys; ++sy)
+      for (int b = c->sys_blocks_h[sy]; b < c->sys_blocks_h[sy + 1]; ++b) bs[b] = sy;
+    HIP_OK(c->block_sys.alloc(L.nblocks));
+    HIP_OK(c->sys_blocks.alloc(c->nsys + 1));
+    HIP_OK(up(c->block_sys.p, bs.data(), bs.size() * 4));
+    HIP_OK(up(c->sys_blocks.p, c->sys_blocks_h.data(), c->sys_blocks_h.size() * 4));
+  }
   hipLaunchKernelGGL(k_fill<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
                      c->vdz.p, 0.0, (long)n);
   HIP_OK(hipGetLastError());
@@ -1255,6 +1304,51 @@ int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const do
   *out = c.release();
   return BHIP_OK;
 }
+
+extern "C" {
+
+int bhip_ctx_create(int npoin, int nelem, int ndp, const int32_t* ikle, const double* x,
+                    const double* y, int nplan, int block_nodes, bhip_ctx** out) {
+  if (!out || !ikle) return BHIP_EBADARG;
+  *out = nullptr;
+  std::unique_ptr<bhip_ctx> c(new bhip_ctx());
+  std::string err = build_layout(npoin, nelem, ndp, ikle, x, y, nplan, block_nodes, c->L);
+  if (!err.empty()) {
+    fprintf(stderr, "libbiefhip: bhip_ctx_create: %s\n", err.c_str());
+    return BHIP_EBADARG;
+  }
+  return ctx_setup(c, out);
+}
+
+int bhip_batch_create(int nsys, const int* npoin, const int* nelem, int ndp,
+                      const int32_t* ikle, const double* x, const double* y, int block_nodes,
+                      bhip_ctx** out) {
+  if (!out || !ikle || !npoin || !nelem || nsys < 1 || (ndp != 2 && ndp != 3)) return BHIP_EBADARG;
+  *out = nullptr;
+  std::unique_ptr<bhip_ctx> c(new bhip_ctx());
+  std::vector<int32_t> ikle_union;
+  std::string err = build_layout_multi(nsys, npoin, nelem, ndp, ikle, x, y, block_nodes, c->L,
+                                       c->sys_blocks_h, ikle_union);
+  if (!err.empty()) {
+    fprintf(stderr, "libbiefhip: bhip_batch_create: %s\n", err.c_str());
+    return BHIP_EBADARG;
+  }
+  c->nsys = nsys;
+  c->sys_node_off.assign(nsys + 1, 0);
+  c->sys_elem_off.assign(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    c->sys_node_off[s + 1] = c->sys_node_off[s] + npoin[s];
+    c->sys_elem_off[s + 1] = c->sys_elem_off[s] + nelem[s];
+  }
+  return ctx_setup(c, out);
+}
+
+int bhip_batch_info(const bhip_ctx* c, int nsys, bhip_solve_info* infos) {
+  if (!c || !infos || nsys != c->nsys) return BHIP_EBADARG;
+  for (int s = 0; s < nsys; ++s) infos[s] = c->sys_info[s];
+  return BHIP_OK;
+}
+
 
 int bhip_ctx_destroy(bhip_ctx* c) {
   if (!c) return BHIP_OK;
@@ -1351,8 +1445,10 @@ static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double
                        c->stream, c->xa.p, src, c->slot_elem.p, L.nslot, L.nslot_pad, L.nelem,
                        ncoef);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(c->stream));
-    staging.release();
+    if (kind == hipMemcpyHostToDevice) {
+      HIP_OK(hipStreamSynchronize(c->stream));
+      staging.release();
+    }
   } else {
     hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(L.nslot_pad)), dim3(kThreads), 0,
                        c->stream, c->xa.p, 0.0, (long)npair * L.nslot_pad);
@@ -1362,7 +1458,9 @@ static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double
   c->typext = typext;
   c->ncoef = ncoef;
   c->have_matrix = true;
-  HIP_OK(hipStreamSynchronize(c->stream));
+  // host arrays are read before returning; device arrays asynchronously,
+  // in stream order on the context stream
+  if (kind == hipMemcpyHostToDevice) HIP_OK(hipStreamSynchronize(c->stream));
   return BHIP_OK;
 }
 
@@ -1496,14 +1594,14 @@ int bhip_matvec(bhip_ctx* c, int op, double cst, const double* y, double* x) {
 namespace bhip {
 
 static int read_state(bhip_ctx* c) {
-  HIP_OK(hipMemcpyAsync(c->st_host, c->st.p, sizeof(SolveState), hipMemcpyDeviceToHost,
+  HIP_OK(hipMemcpyAsync(c->st_host, c->st.p, sizeof(SolveState) * c->nsys, hipMemcpyDeviceToHost,
                         c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return BHIP_OK;
 }
 
 static int write_state(bhip_ctx* c) {
-  HIP_OK(hipMemcpyAsync(c->st.p, c->st_host, sizeof(SolveState), hipMemcpyHostToDevice,
+  HIP_OK(hipMemcpyAsync(c->st.p, c->st_host, sizeof(SolveState) * c->nsys, hipMemcpyHostToDevice,
                         c->stream));
   return BHIP_OK;
 }
@@ -1532,18 +1630,23 @@ static void init_cg_state(SolveState* sh, double eps2xl, int use_rr0, int nitmax
 // One batch of `iters` CG iterations (each U then E; an even count keeps the
 // direction ping-pong aligned). All kernels exit at once when the solve is
 // done, so a batch can run past convergence at no cost but launches.
+// the X/R update kernel of iteration parity p (one workgroup per node block)
+template <typename T, typename TX>
+static hipError_t launch_update(bhip_ctx* c, TX* x, T* r, const T* d, const T* ad, int p) {
+  hipLaunchKernelGGL((k_cg_update<T, TX>), dim3(c->L.nblocks), dim3(kThreads), 0, c->stream, x,
+                     r, d, ad, c->node_start.p, c->L.nblocks, sys_ptr(c->block_sys),
+                     sys_ptr(c->sys_blocks), part_e(c, 1 - p), part_u(c, p), c->st.p, p);
+  return hipGetLastError();
+}
+
 template <typename T, typename TA, typename TX>
 static hipError_t enqueue_cg_batch(bhip_ctx* c, int iters, TX* x, T* r, T* d0, T* d1, T* ad,
                                    const TA* xa, const TA* da) {
-  const int n = c->L.npoin;
-  const int g = c->grid_nodes;
   for (int k = 0; k < iters; ++k) {
     const int p = (k & 1) ? 0 : 1;  // iterations 1, 2, 3, ... of the batch
     T* dprev = (p == 1) ? d0 : d1;
     T* dnext = (p == 1) ? d1 : d0;
-    hipLaunchKernelGGL((k_cg_update<T, TX>), dim3(g), dim3(kThreads), 0, c->stream, x, r,
-                       dprev, ad, n, part_e(c, 1 - p), c->L.nblocks, part_u(c, p), c->st.p, p);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_update<T, TX>(c, x, r, dprev, ad, p);
     if (e != hipSuccess) return e;
     EbeArgs<T, TA> a = base_args<T, TA>(c, xa, da);
     a.r = r;
@@ -1552,7 +1655,7 @@ static hipError_t enqueue_cg_batch(bhip_ctx* c, int iters, TX* x, T* r, T* d0, T
     a.out = ad;
     a.partial = part_e(c, p);
     a.part_in = part_u(c, p);
-    a.n_part_in = g;
+    a.n_part_in = c->L.nblocks;
     a.parity = p;
     e = launch_ebe<T, TA, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia, c->typext);
     if (e != hipSuccess) return e;
@@ -1681,11 +1784,11 @@ static hipError_t enqueue_cg_start(bhip_ctx* c, T* r, T* d0, T* ad, const T* dz,
 
 static hipError_t enqueue_bb_setup(bhip_ctx* c, const double* b, double eps, int nitmax,
                                    int mixed, double inner_eps, int inner_nitmax) {
-  const int g = c->grid_nodes;
-  hipLaunchKernelGGL(k_dot_partials<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b,
-                     c->L.npoin, part_u(c, 1));
-  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 1), g, eps,
-                     nitmax, mixed, inner_eps, inner_nitmax, c->st.p);
+  hipLaunchKernelGGL(k_dot_blocks, dim3(c->L.nblocks), dim3(kThreads), 0, c->stream, b, b,
+                     c->node_start.p, part_u(c, 1));
+  hipLaunchKernelGGL(k_cg_setup, dim3(c->nsys), dim3(kThreads), 0, c->stream, part_u(c, 1),
+                     c->L.nblocks, sys_ptr(c->sys_blocks), eps, nitmax, mixed, inner_eps,
+                     inner_nitmax, c->st.p);
   return hipGetLastError();
 }
 
@@ -1724,28 +1827,61 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
   const uint64_t key[6] = {0, (uint64_t)nb, matrix_tag(c), dbits(eps), (uint64_t)nitmax,
                            (uint64_t)(uintptr_t)x};
   if ((rc = run_planned(c, key, plan))) return rc;
+  // every system of an ensemble runs until its own stop; batches go on
+  // while one of them has not stopped
   for (;;) {
     if (read_state(c)) return BHIP_EDEVICE;
-    if (sh->done) break;
+    bool all = true;
+    for (int sy = 0; sy < c->nsys; ++sy) all = all && sh[sy].done != 0;
+    if (all) break;
     nb = std::min(nb * 2, 8);
     rc = enqueue_batches<double, double, double>(c, exec, nb, x, c->vr.p, c->vd0.p, c->vd1.p,
                                                  c->vad.p, c->xa.p, c->da.p);
     if (rc) return rc;
   }
-  info->relat = sh->xl_raw >= 1.0;
-  if (sh->zero_rhs) {
-    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
-    HIP_OK(hipGetLastError());
-    info->niter = 0;
-    info->relres = 0.0;
-    info->converged = 1;
+  int maxm = 0, nconv = 0, nzero = 0;
+  double maxrel = 0.0;
+  for (int sy = 0; sy < c->nsys; ++sy) {
+    const SolveState& q = sh[sy];
+    bhip_solve_info& si = c->sys_info[sy];
+    std::memset(&si, 0, sizeof(si));
+    si.relat = q.xl_raw >= 1.0;
+    if (q.zero_rhs) {
+      // X = 0 (gracjg.F:194-202) on this system's nodes
+      const int b0 = c->nsys > 1 ? c->sys_blocks_h[sy] : 0;
+      const int b1 = c->nsys > 1 ? c->sys_blocks_h[sy + 1] : c->L.nblocks;
+      const int n0 = c->L.node_start[b0], n1 = c->L.node_start[b1];
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x + n0, 0.0,
+                         (long)(n1 - n0));
+      HIP_OK(hipGetLastError());
+      si.converged = 1;
+      ++nzero;
+      ++nconv;
+      continue;
+    }
+    si.niter = q.final_m;
+    si.relres = std::sqrt(q.final_rr / q.xl);
+    si.converged = q.done == ST_CONVERGED;
+    nconv += si.converged;
+    maxm = std::max(maxm, q.final_m);
+    maxrel = std::max(maxrel, si.relres);
+  }
+  (void)n;
+  if (nzero == c->nsys) {
+    *info = c->sys_info[0];
     return BHIP_ZERO_RHS;
   }
-  hint = sh->final_m;
-  info->niter = sh->final_m;
-  info->relres = std::sqrt(sh->final_rr / sh->xl);
-  info->converged = sh->done == ST_CONVERGED;
-  return sh->done == ST_CONVERGED ? BHIP_OK : BHIP_MAXITER;
+  hint = maxm;
+  if (c->nsys == 1) {
+    *info = c->sys_info[0];
+  } else {
+    // the ensemble: largest count and precision, converged if all did
+    info->niter = maxm;
+    info->relres = maxrel;
+    info->relat = c->sys_info[0].relat;
+    info->converged = nconv == c->nsys;
+  }
+  return info->converged ? BHIP_OK : BHIP_MAXITER;
 }
 
 static int ensure_f32(bhip_ctx* c) {
@@ -1964,6 +2100,8 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
   if (cfg->slv != 1 && cfg->slv != 6 && cfg->slv != 7) return BHIP_EBADARG;
   if (cfg->slv == 7 && (cfg->krylov < 1 || cfg->krylov > 20)) return BHIP_EBADARG;
   if (cfg->slv != 1 && cfg->precision == BHIP_PREC_MIXED) return BHIP_EBADARG;
+  // ensembles: FP64 CG (per-system scalars in its kernels)
+  if (c->nsys > 1 && (cfg->slv != 1 || cfg->precision != BHIP_PREC_FP64)) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
@@ -2024,11 +2162,13 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
   if (precision == BHIP_PREC_MIXED && !c->fr.p) return BHIP_EBADARG;
   const int n = c->L.npoin;
   const int g = c->grid_nodes;
-  SolveState* sh = c->st_host;
-  std::memset(sh, 0, sizeof(SolveState));
   // never converges, never reaches NITMAX: every launch does its full work
-  init_cg_state(sh, -1.0, 0, 0x7fffffff);
-  sh->m[0] = sh->m[1] = 1;
+  for (int sy = 0; sy < c->nsys; ++sy) {
+    SolveState* sh = c->st_host + sy;
+    std::memset(sh, 0, sizeof(SolveState));
+    init_cg_state(sh, -1.0, 0, 0x7fffffff);
+    sh->m[0] = sh->m[1] = 1;
+  }
   if (write_state(c)) return BHIP_EDEVICE;
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
@@ -2057,7 +2197,7 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
     a.out = c->fad.p;
     a.partial = part_e(c, 0);
     a.part_in = part_u(c, 0);
-    a.n_part_in = g;
+    a.n_part_in = c->L.nblocks;
     a.parity = 0;
     rc = time_it([&]() {
       return launch_ebe<float, float, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia,
@@ -2065,10 +2205,7 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
     }, &t_dir);
     if (rc) return rc;
     rc = time_it([&]() {
-      hipLaunchKernelGGL((k_cg_update<float, float>), dim3(g), dim3(kThreads), 0, c->stream,
-                         c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, n, part_e(c, 0), c->L.nblocks,
-                         part_u(c, 1), c->st.p, 1);
-      return hipGetLastError();
+      return launch_update<float, float>(c, c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, 1);
     }, &t_upd);
     if (rc) return rc;
   } else {
@@ -2079,7 +2216,7 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
     a.out = c->vad.p;
     a.partial = part_e(c, 0);
     a.part_in = part_u(c, 0);
-    a.n_part_in = g;
+    a.n_part_in = c->L.nblocks;
     a.parity = 0;
     rc = time_it([&]() {
       return launch_ebe<double, double, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia,
@@ -2087,10 +2224,7 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
     }, &t_dir);
     if (rc) return rc;
     rc = time_it([&]() {
-      hipLaunchKernelGGL((k_cg_update<double, double>), dim3(g), dim3(kThreads), 0, c->stream,
-                         c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, n, part_e(c, 0), c->L.nblocks,
-                         part_u(c, 1), c->st.p, 1);
-      return hipGetLastError();
+      return launch_update<double, double>(c, c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, 1);
     }, &t_upd);
     if (rc) return rc;
   }

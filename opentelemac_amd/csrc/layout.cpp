@@ -230,6 +230,11 @@ std::string validate_layout(const Layout& L, const int32_t* ikle) {
     if (L.iperm[L.perm[i]] != i) return "perm not a bijection";
   }
   if (L.node_start[0] != 0 || L.node_start[L.nblocks] != npoin) return "node ranges";
+  std::vector<int32_t> block_of(npoin, -1);
+  for (int b = 0; b < L.nblocks; ++b) {
+    if (L.node_start[b + 1] < L.node_start[b]) return "node ranges not monotone";
+    for (int g = L.node_start[b]; g < L.node_start[b + 1]; ++g) block_of[g] = b;
+  }
   // every element appears in every block owning one of its nodes, once
   std::vector<int32_t> seen_count(nelem, 0);
   std::vector<int32_t> need(nelem, 0);
@@ -238,7 +243,7 @@ std::string validate_layout(const Layout& L, const int32_t* ikle) {
     int nbk = 0;
     for (int k = 0; k < ndp; ++k) {
       int g = L.iperm[ikle[(long)k * nelem + e]];
-      int bb = g / L.block_nodes;
+      int bb = block_of[g];
       bool dup = false;
       for (int q = 0; q < nbk; ++q) dup |= (blocks[q] == bb);
       if (!dup) blocks[nbk++] = bb;
@@ -283,4 +288,81 @@ std::string validate_layout(const Layout& L, const int32_t* ikle) {
   return "";
 }
 
+// Layout of an ensemble of independent meshes: each mesh is laid out on its
+// own (its blocks hold only its nodes) and the layouts are concatenated, so
+// one launch covers every mesh. Node m of mesh s is node noff[s] + m of the
+// union, element e is element eoff[s] + e.
+std::string build_layout_multi(int nsys, const int* npoin, const int* nelem, int ndp,
+                               const int32_t* ikle, const double* x, const double* y,
+                               int block_nodes, Layout& L, std::vector<int32_t>& sys_blocks,
+                               std::vector<int32_t>& ikle_union) {
+  if (nsys < 1) return "no mesh";
+  std::vector<Layout> parts(nsys);
+  std::vector<int64_t> noff(nsys + 1, 0), eoff(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    noff[s + 1] = noff[s] + npoin[s];
+    eoff[s + 1] = eoff[s] + nelem[s];
+  }
+  if (noff[nsys] > INT32_MAX || eoff[nsys] * ndp > INT32_MAX) return "ensemble too large";
+  const int NP = (int)noff[nsys], NE = (int)eoff[nsys];
+  ikle_union.assign((size_t)ndp * NE, 0);
+  for (int s = 0; s < nsys; ++s) {
+    const int32_t* ik = ikle + (size_t)ndp * eoff[s];
+    std::string err = build_layout(npoin[s], nelem[s], ndp, ik, x ? x + noff[s] : nullptr,
+                                   y ? y + noff[s] : nullptr, 1, block_nodes, parts[s]);
+    if (!err.empty()) return "mesh " + std::to_string(s) + ": " + err;
+    for (int k = 0; k < ndp; ++k)
+      for (int e = 0; e < nelem[s]; ++e)
+        ikle_union[(size_t)k * NE + eoff[s] + e] = (int32_t)(ik[(size_t)k * nelem[s] + e] + noff[s]);
+  }
+  L = Layout();
+  L.npoin = NP;
+  L.nelem = NE;
+  L.ndp = ndp;
+  L.nplan = 1;
+  L.block_nodes = parts[0].block_nodes;
+  L.perm.resize(NP);
+  L.iperm.resize(NP);
+  sys_blocks.assign(nsys + 1, 0);
+  int nslot = 0;
+  for (int s = 0; s < nsys; ++s) nslot += parts[s].nslot;
+  L.nslot = nslot;
+  L.nslot_pad = (nslot + 255) / 256 * 256;
+  L.lid.assign((size_t)ndp * L.nslot_pad, 0);
+  L.slot_elem.resize(nslot);
+  L.node_start.push_back(0);
+  L.halo_start.push_back(0);
+  int so = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const Layout& P = parts[s];
+    const int no = (int)noff[s], eo = (int)eoff[s], ho = (int)L.halo.size();
+    for (int i = 0; i < P.npoin; ++i) {
+      L.perm[no + i] = no + P.perm[i];
+      L.iperm[no + i] = no + P.iperm[i];
+    }
+    for (int b = 0; b < P.nblocks; ++b) {
+      L.node_start.push_back(no + P.node_start[b + 1]);
+      L.halo_start.push_back(ho + P.halo_start[b + 1]);
+      for (int c = 0; c <= kMaxColors; ++c)
+        L.color_off.push_back(so + P.color_off[(size_t)b * (kMaxColors + 1) + c]);
+      L.ncolor.push_back(P.ncolor[b]);
+    }
+    for (int32_t h : P.halo) L.halo.push_back(no + h);
+    for (int k = 0; k < ndp; ++k)
+      std::copy(P.lid.begin() + (size_t)k * P.nslot_pad,
+                P.lid.begin() + (size_t)k * P.nslot_pad + P.nslot,
+                L.lid.begin() + (size_t)k * L.nslot_pad + so);
+    for (int q = 0; q < P.nslot; ++q) L.slot_elem[so + q] = eo + P.slot_elem[q];
+    sys_blocks[s + 1] = sys_blocks[s] + P.nblocks;
+    L.max_colors = std::max(L.max_colors, P.max_colors);
+    L.max_local = std::max(L.max_local, P.max_local);
+    L.max_owned = std::max(L.max_owned, P.max_owned);
+    so += P.nslot;
+  }
+  L.nblocks = sys_blocks[nsys];
+  L.halo_total = (int64_t)L.halo.size();
+  return validate_layout(L, ikle_union.data());
+}
+
 }  // namespace bhip
+

@@ -47,6 +47,17 @@ std::string build_layout(int npoin, int nelem, int ndp, const int32_t* ikle,
                          const double* x, const double* y, int nplan,
                          int block_nodes, Layout& L);
 
+// Layout of an ensemble of nsys independent meshes (each laid out on its
+// own, then concatenated; no block holds nodes of two meshes). ikle holds
+// the meshes' connectivities one after the other (mesh s: ndp x nelem[s],
+// structure of arrays, its own 0-based node ids); x, y (may be NULL) the
+// coordinates concatenated. sys_blocks receives the block range of each
+// mesh, ikle_union the connectivity of the union in global node ids.
+std::string build_layout_multi(int nsys, const int* npoin, const int* nelem, int ndp,
+                               const int32_t* ikle, const double* x, const double* y,
+                               int block_nodes, Layout& L, std::vector<int32_t>& sys_blocks,
+                               std::vector<int32_t>& ikle_union);
+
 // Independent check of every invariant the kernels rely on.
 std::string validate_layout(const Layout& L, const int32_t* ikle);
 

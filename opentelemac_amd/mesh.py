@@ -67,6 +67,33 @@ class Mesh:
         return 0.5 * (xe[1] * ye[2] - xe[2] * ye[1])
 
 
+def channel_mesh_n(npoin_target: int, seed: int = 0, length=2000.0, width=200.0,
+                   jitter: float = 0.2) -> Mesh:
+    """A length x width channel (the 20,451-node channel's shape) with about
+    npoin_target nodes: a perturbed structured grid, each cell split along a
+    random diagonal. Built without Python loops (ensembles of many meshes)."""
+    rng = np.random.default_rng(seed)
+    ny = max(2, int(round(np.sqrt(npoin_target * width / length))) - 1)
+    nx = max(2, int(round(npoin_target / (ny + 1))) - 1)
+    dx, dy = length / nx, width / ny
+    X, Y = np.meshgrid(np.arange(nx + 1) * dx, np.arange(ny + 1) * dy, indexing="ij")
+    inner_x = (X > 0) & (X < length - 1e-9)
+    inner_y = (Y > 0) & (Y < width - 1e-9)
+    X = X + jitter * dx * rng.uniform(-1, 1, X.shape) * inner_x
+    Y = Y + jitter * dy * rng.uniform(-1, 1, Y.shape) * inner_y
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    a = (i * (ny + 1) + j).ravel()
+    b = ((i + 1) * (ny + 1) + j).ravel()
+    c, d = b + 1, a + 1
+    flip = rng.uniform(size=a.size) < 0.5
+    t1 = np.where(flip, np.stack([a, b, c]), np.stack([a, b, d]))
+    t2 = np.where(flip, np.stack([a, c, d]), np.stack([b, c, d]))
+    ikle = np.concatenate([t1, t2], axis=1).astype(np.int32)
+    return Mesh(x=np.ascontiguousarray(X.ravel()), y=np.ascontiguousarray(Y.ravel()),
+                ikle=np.ascontiguousarray(ikle), npoin2=X.size, nelem2=ikle.shape[1],
+                meta={"kind": "channel", "seed": seed, "nx": nx, "ny": ny})
+
+
 def segments(ikle):
     """Edges of a triangle mesh for segment storage (MATRIX STORAGE = 3):
     gloseg (2, nseg), 0-based, first node < second node, in order of first

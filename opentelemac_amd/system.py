@@ -157,6 +157,32 @@ def dirich_host(da, xa, ikle, sm, fbor, limdir, kdir=5):
     return f, da.copy(), xa, sm
 
 
+def drop_depth(mesh: Mesh):
+    """Initial depth of the reference's gouttedo example, a Gaussian drop of
+    2.4 m on 2.4 m of water centred in the 20.1 m basin
+    (examples/telemac2d/gouttedo/user_fortran/condin.f:64-68)."""
+    eikon = ((mesh.x - 10.05) ** 2 + (mesh.y - 10.05) ** 2) / 4.0
+    return 2.4 * (1.0 + np.exp(-eikon))
+
+
+def drop_system(mesh: Mesh, dt=0.04, theta=0.6):
+    """First wave-equation step of the gouttedo example: closed basin (walls
+    only, no Dirichlet node), time step 0.04 s and implicitation 0.6 for depth
+    and velocity (examples/telemac2d/gouttedo/t2d_gouttedo.cas), linearised
+    about the initial depth; the right-hand side is the mass term (M/dt) H0
+    of the fluid at rest. Returns dict(da, xa, typdia, typext, b, x0)."""
+    h0 = drop_depth(mesh)
+    da, xa, td, te = wave_matrix(mesh, h0, dt, theta, theta)
+    t1, x1 = mt01aa(mesh.surfac(), 1.0 / dt)
+    i1, i2, i3 = mesh.ikle
+    a12, a13, a23 = x1
+    w = np.stack([t1[0] * h0[i1] + a12 * h0[i2] + a13 * h0[i3],
+                  a12 * h0[i1] + t1[1] * h0[i2] + a23 * h0[i3],
+                  a13 * h0[i1] + a23 * h0[i2] + t1[2] * h0[i3]])
+    b = assemble(w, mesh.ikle, mesh.npoin)
+    return {"da": da, "xa": xa, "typdia": td, "typext": te, "b": b, "x0": h0.copy()}
+
+
 def wave_system(mesh: Mesh, depth, dt=1.0, theta=1.0, inflow=None, amplitude=0.2,
                 period=600.0, t=150.0):
     """Matrix and right-hand side of one wave-equation step from rest with a

@@ -64,26 +64,37 @@ class _Ops:
         self.solve_kw = solve_kw
         self.n_solves = 0
         self.n_iters = 0
+        # the last solution of each kind of solve (forward, adjoint,
+        # tangent): the initial guess of the next one of that kind, since
+        # successive time steps give close solutions
+        self.guess = {}
+
+    def reset_guesses(self):
+        self.guess = {}
 
     def set(self, da, xa):
         self.ctx.set_matrix("Q", "S", da.contiguous(), xa.contiguous())
 
-    def solve(self, da, xa, b):
+    def solve(self, da, xa, b, kind="forward"):
         """A^-1 b. The right-hand side is normalised first: GRACJG's stop
         test turns absolute when ||b'|| < 1 (XL = max(||b'||^2, 1),
         gracjg.F:189-209), and adjoint and tangent right-hand sides are
-        small, so without it they would barely be solved."""
-        scale = float(torch.linalg.vector_norm(b))
-        if scale == 0.0:
-            return torch.zeros_like(b)
+        small, so without it they would barely be solved. The previous
+        solution of the same kind is the initial guess."""
+        # on the device (no host round trip); a zero b stays zero and takes
+        # the solver's zero right-hand-side exit
+        scale = torch.clamp(torch.linalg.vector_norm(b), min=1e-300)
         self.set(da, xa)
-        x = torch.zeros_like(b)
+        g = self.guess.get(kind)
+        x = g / scale if g is not None and g.shape == b.shape else torch.zeros_like(b)
         _, info = self.ctx.solve(x, b / scale, slv=1, precon=2, **self.solve_kw)
         if not info["converged"]:
             raise bief.BiefError(f"wave-equation solve did not converge: {info}")
         self.n_solves += 1
         self.n_iters += info["niter"]
-        return x * scale
+        x = x * scale
+        self.guess[kind] = x.detach()
+        return x
 
     def matvec(self, da, xa, y, trans=False):
         self.set(da, xa)
@@ -118,7 +129,7 @@ class EbeSolve(torch.autograd.Function):
     def backward(ctx, gx):
         ops = ctx.ops
         da, xa, x = ctx.saved_tensors
-        lam = ops.solve(da, xa, gx.contiguous())
+        lam = ops.solve(da, xa, gx.contiguous(), kind="adjoint")
         return None, -lam * x, -_pair_outer(ops.ikle, lam, x), lam
 
     @staticmethod
@@ -128,7 +139,7 @@ class EbeSolve(torch.autograd.Function):
         rhs = -_tangent_product(ops, dda, dxa, x)
         if db is not None:
             rhs = rhs + db
-        return ops.solve(da, xa, rhs.contiguous())
+        return ops.solve(da, xa, rhs.contiguous(), kind="tangent")
 
 
 class EbeMatvec(torch.autograd.Function):
@@ -252,12 +263,12 @@ class WaveModel:
         qxm = (qx[i1] + qx[i2] + qx[i3]) / 3.0
         qym = (qy[i1] + qy[i2] + qy[i3]) / 3.0
         g_el = 0.5 * (self.bx * qxm + self.by * qym)  # S * grad(phi_k).q_mean
-        rhs = self.scatter(g_el) - EbeMatvec.apply(self.ops, da, xa, eta) + \
-            self.mass_matvec(eta, 1.0 / dt)
-        # DIRICH (diri01.f:94-139): prescribed increment on the sea boundary
+        # DIRICH (diri01.f:94-139): prescribed increment fb on the sea
+        # boundary, b -= A fb; one product A (eta + fb) serves both terms
         w = (~self.sea).to(eta.dtype)
         fb = torch.where(self.sea, self.tide(t_new) - eta, torch.zeros_like(eta))
-        rhs = rhs - EbeMatvec.apply(self.ops, da, xa, fb)
+        rhs = self.scatter(g_el) - EbeMatvec.apply(self.ops, da, xa, eta + fb) + \
+            self.mass_matvec(eta, 1.0 / dt)
         rhs = torch.where(self.sea, da * fb, rhs)
         xa_d = xa * torch.stack([w[i1] * w[i2], w[i1] * w[i3], w[i2] * w[i3]])
         deta = EbeSolve.apply(self.ops, da, xa_d, rhs)
@@ -279,6 +290,7 @@ class WaveModel:
     def run(self, k_str, zb, nsteps, gauges=None, obs=None):
         """Forward run from rest; returns the gauge series (nsteps, ngauge) or,
         with obs, the cost sum (eta_g - obs)^2."""
+        self.ops.reset_guesses()
         eta, u, v = self.initial_state(zb)
         series = []
         cost = torch.zeros((), dtype=torch.float64, device=self.dev)
@@ -302,17 +314,25 @@ class WaveCost(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, model, k_str, zb, nsteps, gauges, obs, checkpoint):
+        model.ops.reset_guesses()
+        last = ((nsteps - 1) // checkpoint) * checkpoint  # start of the last segment
         with torch.no_grad():
             eta, u, v = model.initial_state(zb)
             checkpoints = {0: (eta, u, v)}
             cost = torch.zeros((), dtype=torch.float64, device=model.dev)
-            for n in range(nsteps):
+            for n in range(last):
                 eta, u, v = model.step(eta, u, v, k_str, zb, (n + 1) * model.dt)
                 cost = cost + ((eta[gauges] - obs[n]) ** 2).sum()
-                if (n + 1) % checkpoint == 0 and n + 1 < nsteps:
+                if (n + 1) % checkpoint == 0:
                     checkpoints[n + 1] = (eta, u, v)
+        # the last segment keeps its graph: the reverse sweep starts there
+        # without recomputing it
+        with torch.enable_grad():
+            seg = _Segment(model, checkpoints[last], k_str, zb)
+            jlast = seg.run(last, nsteps, gauges, obs)
+        cost = cost + jlast.detach()
         ctx.model, ctx.nsteps, ctx.gauges, ctx.obs = model, nsteps, gauges, obs
-        ctx.checkpoints, ctx.checkpoint = checkpoints, checkpoint
+        ctx.checkpoints, ctx.checkpoint, ctx.last_segment = checkpoints, checkpoint, seg
         ctx.save_for_backward(k_str, zb)
         return cost
 
@@ -324,27 +344,49 @@ class WaveCost(torch.autograd.Function):
         gz = torch.zeros_like(zb)
         # adjoint of the state at the end of the segment being processed
         lam = [torch.zeros_like(zb) for _ in range(3)]
+        model.ops.guess.pop("adjoint", None)
         for s0 in sorted(ctx.checkpoints, reverse=True):
             s1 = min(s0 + ctx.checkpoint, nsteps)
             with torch.enable_grad():
-                # the segment recomputed from its checkpoint, with its graph
-                e0, u0, v0 = (t.detach().requires_grad_(True) for t in ctx.checkpoints[s0])
-                kk = k_str.detach().requires_grad_(True)
-                zz = zb.detach().requires_grad_(True)
-                e, u, v = e0, u0, v0
-                jseg = torch.zeros((), dtype=torch.float64, device=model.dev)
-                for n in range(s0, s1):
-                    e, u, v = model.step(e, u, v, kk, zz, (n + 1) * model.dt)
-                    jseg = jseg + ((e[gauges] - obs[n]) ** 2).sum()
-                outs = (e, u, v, jseg)
-                gouts = (lam[0], lam[1], lam[2], gcost)
-                g = torch.autograd.grad(outs, (e0, u0, v0, kk, zz), gouts, allow_unused=True)
+                if ctx.last_segment is not None:
+                    seg, ctx.last_segment = ctx.last_segment, None
+                else:
+                    # the segment recomputed from its checkpoint, with its graph
+                    seg = _Segment(model, ctx.checkpoints[s0], k_str, zb)
+                    seg.run(s0, s1, gauges, obs)
+                g = seg.grad(lam, gcost)
             lam = [gi if gi is not None else torch.zeros_like(zb) for gi in g[:3]]
             if g[3] is not None:
                 gk += g[3]
             if g[4] is not None:
                 gz += g[4]
         return None, gk, gz, None, None, None, None
+
+
+class _Segment:
+    """Steps s0..s1-1 from a stored state, with their autograd graph."""
+
+    def __init__(self, model, state, k_str, zb):
+        self.model = model
+        self.e0, self.u0, self.v0 = (t.detach().requires_grad_(True) for t in state)
+        self.kk = k_str.detach().requires_grad_(True)
+        self.zz = zb.detach().requires_grad_(True)
+
+    def run(self, s0, s1, gauges, obs):
+        m = self.model
+        e, u, v = self.e0, self.u0, self.v0
+        j = torch.zeros((), dtype=torch.float64, device=m.dev)
+        for n in range(s0, s1):
+            e, u, v = m.step(e, u, v, self.kk, self.zz, (n + 1) * m.dt)
+            j = j + ((e[gauges] - obs[n]) ** 2).sum()
+        self.outs = (e, u, v, j)
+        return j
+
+    def grad(self, lam, gcost):
+        """Adjoints of the segment's start state and of K, zb, given those
+        of its end state (lam) and of the cost."""
+        return torch.autograd.grad(self.outs, (self.e0, self.u0, self.v0, self.kk, self.zz),
+                                   (lam[0], lam[1], lam[2], gcost), allow_unused=True)
 
 
 def cost_and_gradient(model: WaveModel, k_str, zb, nsteps, gauges, obs, checkpoint=10):

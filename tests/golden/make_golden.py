@@ -204,8 +204,28 @@ def seg_case():
     return out
 
 
+def gouttedo_case():
+    """The reference's gouttedo mesh (geo_gouttedo.slf, 4624 nodes) and the
+    first step of its drop: CG, BiCGSTAB and GMRES(3) with PRECON 2 at
+    eps 1e-10 from a zero initial guess, in element and in segment storage."""
+    from opentelemac_amd import serafin as SF
+    m = SF.read(os.path.join(HERE, "geo_gouttedo.slf")).mesh()
+    gl, es = M.segments(m.ikle)
+    sysd = S.drop_system(m)
+    out = {}
+    for sto, ikle, xa in (("ebe", m.ikle, sysd["xa"]),
+                          ("seg", gl, S.ebe_to_segments(sysd["xa"], es, m.ikle, gl, "S"))):
+        rm = O.RefMesh(m.ikle, m.npoin, m.nelem, gloseg=gl if sto == "seg" else None)
+        A = O.Matrix(sysd["da"], xa, "Q", "S")
+        for slv in (1, 6, 7):
+            x, info, _, _ = O.ref_solve(rm, A, sysd["b"], np.zeros(m.npoin), slv=slv,
+                                        precon=2, krylov=3, eps=1e-10, nitmax=500)
+            out[f"{sto}_{slv}_x"], out[f"{sto}_{slv}_niter"] = x, np.array(info["niter"])
+    return out
+
+
 CASES = {"ref_prism": prism_case, "ref_small": small_case, "ref_channel": channel_case,
-         "ref_mt02pp": mt02pp_case, "ref_seg": seg_case}
+         "ref_mt02pp": mt02pp_case, "ref_seg": seg_case, "ref_gouttedo": gouttedo_case}
 
 
 def main(names):

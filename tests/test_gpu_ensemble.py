@@ -1,0 +1,87 @@
+"""Ensembles of independent meshes in one context (bhip_batch_create): one
+launch covers every mesh, and FP64 CG runs each system with its own scalars,
+stop test and iteration count. Each system is checked against the oracle's
+solve of that system alone."""
+import numpy as np
+import pytest
+
+from opentelemac_amd import bief, mesh as M, system as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(sizes, seed=0):
+    meshes, systems = [], []
+    for k, n in enumerate(sizes):
+        m = M.channel_mesh_n(n, seed=seed + k)
+        meshes.append(m)
+        systems.append(S.wave_system(m, M.channel_depth(m), dt=1.0 + 0.5 * k))
+    return meshes, systems
+
+
+def _cat(systems, key):
+    return np.concatenate([s[key] for s in systems], axis=-1)
+
+
+@pytest.mark.parametrize("bn", [0, 300])
+def test_ensemble_cg_matches_each_system(gpu, bn):
+    sizes = [600, 2500, 150, 4000, 1200]
+    meshes, systems = _members(sizes)
+    ens = bief.Ensemble(meshes, block_nodes=bn)
+    ens.set_matrix("Q", "S", _cat(systems, "da"), _cat(systems, "xa"))
+    x, info = ens.solve(_cat(systems, "x0"), _cat(systems, "b"), slv=1, precon=2, eps=1e-10,
+                        nitmax=2000)
+    infos = ens.infos()
+    assert info["converged"] and len(infos) == len(meshes)
+    for m, sysd, xs, si in zip(meshes, systems, ens.split(x), infos):
+        A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+        xr, ir, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2, eps=1e-10,
+                               nitmax=2000)
+        assert si["converged"]
+        assert abs(si["niter"] - ir.niter) <= 1, (si, ir.niter)
+        assert np.linalg.norm(xs - xr) <= 1e-10 * np.linalg.norm(xr)
+    assert info["niter"] == max(si["niter"] for si in infos)
+
+
+def test_ensemble_matvec_and_zero_rhs(gpu):
+    """The ensemble's product is the meshes' products side by side; a
+    system with a zero right-hand side takes GRACJG's X = 0 exit while the
+    others are solved."""
+    meshes, systems = _members([800, 300, 1500], seed=10)
+    ens = bief.Ensemble(meshes)
+    da, xa = _cat(systems, "da"), _cat(systems, "xa")
+    ens.set_matrix("Q", "S", da, xa)
+    rng = np.random.default_rng(2)
+    y = rng.normal(size=ens.npoin)
+    got = ens.matvec("X=AY", y)
+    for m, sysd, ys, gs in zip(meshes, systems, ens.split(y), ens.split(got)):
+        ref = O.matvec("X=AY", O.Matrix(sysd["da"], sysd["xa"], "Q", "S"), m.ikle, ys)
+        assert np.abs(gs - ref).max() <= 1e-13 * (np.abs(ref).max() + 1)
+    b = _cat(systems, "b").copy()
+    b[ens.node_off[1]:ens.node_off[2]] = 0.0
+    x0 = rng.normal(size=ens.npoin)
+    x, info = ens.solve(x0.copy(), b, slv=1, precon=2, eps=1e-10, nitmax=2000)
+    infos = ens.infos()
+    assert all(si["converged"] for si in infos)
+    assert np.all(ens.split(x)[1] == 0.0) and infos[1]["niter"] == 0
+    m, sysd = meshes[2], systems[2]
+    xr, ir, _, _ = O.solve(O.Matrix(sysd["da"], sysd["xa"], "Q", "S"), m.ikle, sysd["b"],
+                           ens.split(x0)[2], slv=1, precon=2, eps=1e-10, nitmax=2000)
+    assert abs(infos[2]["niter"] - ir.niter) <= 1
+    assert np.linalg.norm(ens.split(x)[2] - xr) <= 1e-10 * np.linalg.norm(xr)
+
+
+def test_ensemble_nitmax_per_system(gpu):
+    meshes, systems = _members([1000, 3000], seed=20)
+    ens = bief.Ensemble(meshes)
+    ens.set_matrix("Q", "S", _cat(systems, "da"), _cat(systems, "xa"))
+    x, info = ens.solve(_cat(systems, "x0"), _cat(systems, "b"), slv=1, precon=2, eps=1e-14,
+                        nitmax=7)
+    infos = ens.infos()
+    assert not info["converged"]
+    for m, sysd, xs, si in zip(meshes, systems, ens.split(x), infos):
+        xr, ir, _, _ = O.solve(O.Matrix(sysd["da"], sysd["xa"], "Q", "S"), m.ikle, sysd["b"],
+                               sysd["x0"], slv=1, precon=2, eps=1e-14, nitmax=7)
+        assert si["niter"] == ir.niter == 7
+        assert np.linalg.norm(xs - xr) <= 1e-10 * np.linalg.norm(xr)

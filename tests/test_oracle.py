@@ -378,3 +378,23 @@ def test_segment_storage_equals_element_storage(typext):
     for op, ref in (("X=AY", D @ y), ("X=TAY", D.T @ y)):
         got = O.matvec(op, O.Matrix(da, xs, "Q", typext), gl, y)
         assert np.allclose(got, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max()), op
+
+
+def test_golden_gouttedo_real_mesh():
+    """The reference's gouttedo mesh read from its SERAFIN file, first step
+    of the drop: the oracle reproduces the reference's solves bit for bit in
+    element and in segment storage."""
+    from opentelemac_amd import serafin as SF
+    from opentelemac_amd import system as S
+    z = np.load(os.path.join(GOLD, "ref_gouttedo.npz"))
+    m = SF.read(os.path.join(GOLD, "geo_gouttedo.slf")).mesh()
+    gl, es = M.segments(m.ikle)
+    sysd = S.drop_system(m)
+    for sto, ikle, xa in (("ebe", m.ikle, sysd["xa"]),
+                          ("seg", gl, S.ebe_to_segments(sysd["xa"], es, m.ikle, gl, "S"))):
+        A = O.Matrix(sysd["da"], xa, "Q", "S")
+        for slv in (1, 6, 7):
+            x, info, _, _ = O.solve(A, ikle, sysd["b"], np.zeros(m.npoin), slv=slv, precon=2,
+                                    krylov=3, eps=1e-10, nitmax=500)
+            assert info.niter == int(z[f"{sto}_{slv}_niter"]), (sto, slv)
+            assert np.array_equal(x, z[f"{sto}_{slv}_x"]), (sto, slv)
