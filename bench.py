@@ -366,8 +366,15 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     (seeded), misfit at 200 random gauges against a run with the friction
     perturbed by a uniform factor in [0.8, 1.2], checkpoints every 10 steps,
     mixed-precision solves (FP32 EBE CG inside FP64 refinement) to 1e-10.
-    Reports forward and gradient times, their ratio, central differences at
-    20 random nodes (each parameter) and a tangent-linear check."""
+
+    Timed on the device model (HIP step kernels and their hand-written
+    adjoint, DeviceWaveModel): forward run, and forward + reverse sweeps for
+    the gradient, with and without checkpoint recomputation. Beside it the
+    torch model of the same equations (WaveModel: torch element-wise work
+    around the HIP mat-vec and solve, autograd for the rest). Checks: central
+    differences at 20 random nodes per parameter on the device model, and
+    the device gradient against a tangent-linear (forward-mode) run of the
+    torch model."""
     import torch
     import torch.autograd.forward_ad as fwAD
     from opentelemac_amd import bief, mesh as M, system as S, wave as W
@@ -377,37 +384,52 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     bnd = S.boundary_nodes(m)
     sea = bnd[m.x[bnd] > m.x.max() - 1.0]
     log(f"adjoint mesh {m.npoin} nodes, built in {time.time() - t:.1f}s")
-    model = W.WaveModel(m, sea, dt=2.0, amp=0.2, period=600.0, precision=bief.PREC_MIXED,
-                        eps=1e-10)
+    kw = dict(dt=2.0, amp=0.2, period=600.0, precision=bief.PREC_MIXED, eps=1e-10)
+    model = W.DeviceWaveModel(m, sea, **kw)
+    tmodel = W.WaveModel(m, sea, **kw)
     dev = model.dev
     rng = np.random.default_rng(args.seed + 2)
     k = torch.tensor(rng.uniform(20, 60, m.npoin), dtype=torch.float64, device=dev)
     zb = torch.tensor(-depth, dtype=torch.float64, device=dev)
     gauges = torch.tensor(rng.choice(m.npoin, 200, replace=False), device=dev)
-    with torch.no_grad():
-        obs = model.run(k * torch.tensor(rng.uniform(0.8, 1.2, m.npoin), device=dev), zb, nsteps,
-                        gauges)
+    obs = model.run(k * torch.tensor(rng.uniform(0.8, 1.2, m.npoin), device=dev), zb, nsteps,
+                    gauges)
+
+    def timed(fn):
+        fn()  # warm-up (graphs, allocator)
+        torch.cuda.synchronize()
+        model.n_solves = model.n_iters = 0
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
 
     def forward():
-        with torch.no_grad():
-            return float(model.run(k, zb, nsteps, gauges, obs))
+        return float(model.run(k, zb, nsteps, gauges, obs))
 
-    forward()  # warm-up (graphs, allocator)
+    t_fwd, j0 = timed(forward)
+    fwd_solves, fwd_iters = model.n_solves, model.n_iters
+    t_grad, (j, gk, gz) = timed(
+        lambda: model.cost_and_gradient(k, zb, nsteps, gauges, obs, checkpoint))
+    grad_solves = model.n_solves
+    t_grad_all, _ = timed(lambda: model.cost_and_gradient(k, zb, nsteps, gauges, obs, nsteps))
+    log(f"device model: forward {t_fwd:.3f}s, gradient {t_grad:.3f}s (checkpoints every "
+        f"{checkpoint}), {t_grad_all:.3f}s (all states kept)")
+    # the torch model of the same equations
+    with torch.no_grad():
+        tmodel.run(k, zb, nsteps, gauges, obs)
     torch.cuda.synchronize()
-    model.ops.n_solves = model.ops.n_iters = 0
     t0 = time.perf_counter()
-    j0 = forward()
+    with torch.no_grad():
+        tmodel.run(k, zb, nsteps, gauges, obs)
     torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    fwd_solves, fwd_iters = model.ops.n_solves, model.ops.n_iters
-    W.cost_and_gradient(model, k, zb, nsteps, gauges, obs, checkpoint)  # warm-up
+    tt_fwd = time.perf_counter() - t0
+    W.cost_and_gradient(tmodel, k, zb, nsteps, gauges, obs, checkpoint)
     torch.cuda.synchronize()
-    model.ops.n_solves = model.ops.n_iters = 0
     t0 = time.perf_counter()
-    j, gk, gz = W.cost_and_gradient(model, k, zb, nsteps, gauges, obs, checkpoint)
+    W.cost_and_gradient(tmodel, k, zb, nsteps, gauges, obs, checkpoint)
     torch.cuda.synchronize()
-    t_grad = time.perf_counter() - t0
-    grad_solves = model.ops.n_solves
+    tt_grad = time.perf_counter() - t0
     # central differences at n_fd random nodes for each parameter
     fd_err = {}
     for name, p, g in (("strickler", k, gk), ("bathymetry", zb, gz)):
@@ -427,15 +449,19 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     dk = torch.tensor(rng.normal(size=m.npoin), dtype=torch.float64, device=dev)
     dz = torch.tensor(rng.normal(size=m.npoin) * 0.01, dtype=torch.float64, device=dev)
     with fwAD.dual_level():
-        jd = model.run(fwAD.make_dual(k, dk), fwAD.make_dual(zb, dz), nsteps, gauges, obs)
+        jd = tmodel.run(fwAD.make_dual(k, dk), fwAD.make_dual(zb, dz), nsteps, gauges, obs)
         tl = float(fwAD.unpack_dual(jd).tangent)
     adj = float((gk * dk).sum() + (gz * dz).sum())
     return {"workload": "wave-equation adjoint, 500k nodes, 50 steps, checkpoints every 10",
             "npoin": m.npoin, "nelem": m.nelem, "steps": nsteps, "cost": j0,
             "forward_s": t_fwd, "gradient_s": t_grad,
             "adjoint_over_forward": round(t_grad / t_fwd, 3),
+            "gradient_s_all_states_kept": t_grad_all,
+            "adjoint_over_forward_all_states_kept": round(t_grad_all / t_fwd, 3),
             "forward_solves": fwd_solves, "forward_cg_iterations": fwd_iters,
             "gradient_solves": grad_solves,
+            "torch_model": {"forward_s": tt_fwd, "gradient_s": tt_grad,
+                            "adjoint_over_forward": round(tt_grad / tt_fwd, 3)},
             "fd_max_rel_error": fd_err, "fd_nodes_per_parameter": n_fd,
             "tangent_linear_rel_error": abs(adj - tl) / max(abs(tl), 1e-300),
             "solver": "mixed (FP32 EBE CG, FP64 refinement), eps 1e-10"}

@@ -172,6 +172,14 @@ int bhip_solve(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x,
                double* b, bhip_solve_info* info);
 int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
                    double* b_dev, bhip_solve_info* info);
+/* The same SOLVE with device vectors in the context's internal node order
+ * (bhip_ctx_perm), for callers that keep their fields in that order: no
+ * permutation at the boundary. */
+int bhip_solve_dev_internal(bhip_ctx* ctx, const bhip_solver_cfg* cfg,
+                            double* x_dev, double* b_dev, bhip_solve_info* info);
+/* The context's node order: perm[i] is the caller's node of internal node
+ * i (npoin entries, host memory). */
+int bhip_ctx_perm(const bhip_ctx* ctx, int32_t* perm);
 
 /* Measurement: average duration (ms, HIP events on the context stream) of
  * the kernels of one CG iteration on the context's current (preconditioned)
@@ -231,6 +239,59 @@ int bhip_matrix_mt02pp(bhip_ctx* ctx, double xmul, const double* xel,
                        const double* f, const double* g, const double* h,
                        int h_zero, int npou0, int inchyd, int mon,
                        int accumulate);
+
+/* ---- the time-stepped wave equation and its adjoint ---------------------- *
+ * One time step of telemac2d's wave-equation treatment (PROPAG,
+ * propag.F:664-716, 1369-1408, 1634-1756; theta = 1, no advection) with
+ * Strickler friction (friction_calc.f:130-134) and a prescribed free surface
+ * on sea nodes, as the device harness runs it, and the transposed kernels of
+ * its hand-written adjoint. No single routine of the reference corresponds:
+ * PROPAG builds the same system from MATMAS (MT01AA), MATDIFUV (MT02AA 'UV'),
+ * VECTOR and DIRICH calls. All node vectors are device pointers in the
+ * context's internal order (bhip_ctx_perm); triangles only. */
+typedef struct {
+  double dt;     /* time step (s) */
+  double grav;   /* gravity */
+  double eps_u;  /* speed floor in |u| = sqrt(u^2 + v^2 + eps_u^2) */
+  double tide;   /* free surface prescribed on sea nodes at the new time */
+} bhip_wave_params;
+
+/* Slot geometry (xel, yel [3*nelem] local coordinates, surfac [nelem], caller
+ * element order), basis integrals, and the sea nodes (sea [npoin], caller
+ * order, nonzero = prescribed). */
+int bhip_wave_setup(bhip_ctx* ctx, const double* xel, const double* yel,
+                    const double* surfac, const int32_t* sea);
+/* The system of the step from the state (eta, u, v), Strickler kstr and
+ * bottom zb: A = M/dt + MATDIFUV(g c, h) into the context's matrix (TYPDIA
+ * 'Q', TYPEXT 'S', rows and columns of sea nodes cleared off the diagonal)
+ * and its right-hand side rhs (D fb on sea nodes). */
+int bhip_wave_assemble(bhip_ctx* ctx, const bhip_wave_params* p,
+                       const double* eta, const double* u, const double* v,
+                       const double* kstr, const double* zb, double* rhs);
+/* The new state from the increment deta = A^-1 rhs: eta' = eta + deta and
+ * the momentum update with the lumped gradient of eta'. Uses the friction
+ * factor of the last bhip_wave_assemble. */
+int bhip_wave_momentum(bhip_ctx* ctx, const bhip_wave_params* p,
+                       const double* eta, const double* deta, const double* u,
+                       const double* v, double* eta_n, double* u_n,
+                       double* v_n);
+/* Adjoint of the momentum update with respect to eta' (after
+ * bhip_wave_assemble of the same step): etot = ebar_n + grad^T(...), the
+ * right-hand side of the adjoint solve. */
+int bhip_wave_momentum_adj(bhip_ctx* ctx, const bhip_wave_params* p,
+                           const double* ubar_n, const double* vbar_n,
+                           const double* ebar_n, double* etot);
+/* Adjoint of the system, its right-hand side and the node quantities, given
+ * lam = A^-1 etot: writes the adjoint state of the step's start (ebar, ubar,
+ * vbar) and adds to the adjoints of kstr and zb (kbar, zbbar). */
+int bhip_wave_assemble_adj(bhip_ctx* ctx, const bhip_wave_params* p,
+                           const double* lam, const double* deta,
+                           const double* eta, const double* u, const double* v,
+                           const double* u_n, const double* v_n,
+                           const double* ubar_n, const double* vbar_n,
+                           const double* etot, const double* kstr, double* ebar,
+                           double* ubar, double* vbar, double* kbar,
+                           double* zbbar);
 
 #ifdef __cplusplus
 }

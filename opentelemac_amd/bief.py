@@ -50,6 +50,12 @@ class SolverCfg(ctypes.Structure):
                 ("inner_eps", ctypes.c_double), ("inner_nitmax", ctypes.c_int)]
 
 
+class WaveParams(ctypes.Structure):
+    """bhip_wave_params: time step, gravity, speed floor, tide on sea nodes."""
+    _fields_ = [("dt", ctypes.c_double), ("grav", ctypes.c_double), ("eps_u", ctypes.c_double),
+                ("tide", ctypes.c_double)]
+
+
 class SolveInfo(ctypes.Structure):
     _fields_ = [("niter", ctypes.c_int), ("inner_iters", ctypes.c_int),
                 ("outer_iters", ctypes.c_int), ("relat", ctypes.c_int),
@@ -101,6 +107,15 @@ def lib():
         L.bhip_batch_info.argtypes = [_P, _I, ctypes.POINTER(SolveInfo)]
         L.bhip_matrix_mt01pp.argtypes = [_P, _D, _P, _P, _I]
         L.bhip_matrix_mt02pp.argtypes = [_P, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I]
+        L.bhip_solve_dev_internal.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
+                                              ctypes.POINTER(SolveInfo)]
+        L.bhip_ctx_perm.argtypes = [_P, _P]
+        _W = ctypes.POINTER(WaveParams)
+        L.bhip_wave_setup.argtypes = [_P, _P, _P, _P, _P]
+        L.bhip_wave_assemble.argtypes = [_P, _W] + [_P] * 6
+        L.bhip_wave_momentum.argtypes = [_P, _W] + [_P] * 7
+        L.bhip_wave_momentum_adj.argtypes = [_P, _W] + [_P] * 4
+        L.bhip_wave_assemble_adj.argtypes = [_P, _W] + [_P] * 16
         _lib = L
     return _lib
 
@@ -331,6 +346,30 @@ class Context:
                "bhip_matvec")
         return x
 
+    def perm(self):
+        """The context's node order: internal node i is the caller's perm[i]."""
+        p = np.empty(self.npoin, dtype=np.int32)
+        _check(lib().bhip_ctx_perm(self._h, p.ctypes.data), "bhip_ctx_perm")
+        return p
+
+    def solve_internal(self, x, b, slv=1, precon=2, krylov=5, nitmax=1000, eps=1e-8,
+                       precision=PREC_FP64, inner_eps=1e-4, inner_nitmax=0):
+        """SOLVE on torch CUDA vectors in the context's internal order
+        (bhip_solve_dev_internal); otherwise as solve()."""
+        if not _is_torch(x) or not _is_torch(b):
+            raise BiefError("solve_internal takes device tensors")
+        cfg = SolverCfg(slv, precon, krylov, nitmax, eps, precision, inner_eps, inner_nitmax)
+        info = SolveInfo()
+        self._sync_torch()
+        rc = lib().bhip_solve_dev_internal(self._h, ctypes.byref(cfg), x.data_ptr(),
+                                           b.data_ptr(), ctypes.byref(info))
+        _check(rc, "bhip_solve_dev_internal")
+        self._join_torch()
+        self.typdia = "I" if precon in (2,) else self.typdia
+        d = info.as_dict()
+        d["status"] = STATUS[rc]
+        return x, d
+
     def time_cg_kernels(self, precision=PREC_FP64, reps=50):
         """Average ms per launch of the CG direction/mat-vec kernel and the
         X/R update kernel (HIP events on the context stream)."""
@@ -370,6 +409,53 @@ class Context:
         d = info.as_dict()
         d["status"] = STATUS[rc]
         return x, d
+
+
+class WaveStep:
+    """The device wave-equation step and its adjoint kernels on a context
+    (bhip_wave_*): node vectors are torch CUDA float64 tensors in the
+    context's internal order (Context.perm)."""
+
+    def __init__(self, ctx: Context, xel, yel, surfac, sea_mask):
+        self.ctx = ctx
+        xel = _host_f64(xel)
+        yel = _host_f64(yel)
+        surfac = _host_f64(surfac)
+        sea = np.ascontiguousarray(sea_mask, dtype=np.int32)
+        if xel.size != 3 * ctx.nelem or sea.size != ctx.npoin:
+            raise BiefError("wave setup: array sizes do not match the mesh")
+        _check(lib().bhip_wave_setup(ctx._h, xel.ctypes.data, yel.ctypes.data, surfac.ctypes.data,
+                                     sea.ctypes.data), "bhip_wave_setup")
+
+    @staticmethod
+    def params(dt, grav, eps_u, tide):
+        return WaveParams(float(dt), float(grav), float(eps_u), float(tide))
+
+    def _run(self, name, p, *tensors):
+        for t in tensors:
+            if not (_is_torch(t) and t.is_cuda and t.dtype.is_floating_point and
+                    t.element_size() == 8 and t.is_contiguous() and t.numel() == self.ctx.npoin):
+                raise BiefError(f"{name}: node vectors must be contiguous float64 CUDA tensors "
+                                f"of {self.ctx.npoin} entries")
+        self.ctx._sync_torch()
+        _check(getattr(lib(), name)(self.ctx._h, ctypes.byref(p),
+                                    *(t.data_ptr() for t in tensors)), name)
+        self.ctx._join_torch()
+
+    def assemble(self, p, eta, u, v, kstr, zb, rhs):
+        self._run("bhip_wave_assemble", p, eta, u, v, kstr, zb, rhs)
+        self.ctx.typdia, self.ctx.typext = "Q", "S"
+
+    def momentum(self, p, eta, deta, u, v, eta_n, u_n, v_n):
+        self._run("bhip_wave_momentum", p, eta, deta, u, v, eta_n, u_n, v_n)
+
+    def momentum_adj(self, p, ubar_n, vbar_n, ebar_n, etot):
+        self._run("bhip_wave_momentum_adj", p, ubar_n, vbar_n, ebar_n, etot)
+
+    def assemble_adj(self, p, lam, deta, eta, u, v, u_n, v_n, ubar_n, vbar_n, etot, kstr,
+                     ebar, ubar, vbar, kbar, zbbar):
+        self._run("bhip_wave_assemble_adj", p, lam, deta, eta, u, v, u_n, v_n, ubar_n, vbar_n,
+                  etot, kstr, ebar, ubar, vbar, kbar, zbbar)
 
 
 class Ensemble(Context):

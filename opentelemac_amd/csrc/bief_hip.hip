@@ -1089,6 +1089,12 @@ struct bhip_ctx {
   bool have_saved = false;
   bhip::SolveState* st_host = nullptr;  // pinned
   size_t lds_f64 = 0, lds_f32 = 0;
+  // the time-stepped wave equation on the device (wave.inl): slot geometry
+  // [5][nslot_pad] (x2, x3, y2, y3, S of the slot's element) and node fields
+  // in internal order (1 - sea mask, basis integrals, h, c, qx, qy, y, and
+  // the assembled diagonal before preconditioning)
+  bhip::DevBuf<double> wv_geo, wv_w, wv_vol, wv_h, wv_c, wv_qx, wv_qy, wv_y, wv_da;
+  bool wv_ready = false;
 };
 
 namespace bhip {
@@ -1366,6 +1372,9 @@ int bhip_ctx_destroy(bhip_ctx* c) {
     v->release();
   c->part4.release();
   c->st.release();
+  for (auto* v : {&c->wv_geo, &c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_c, &c->wv_qx, &c->wv_qy,
+                  &c->wv_y, &c->wv_da})
+    v->release();
   for (auto& gr : c->graphs)
     if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
   for (auto& sg : c->solve_graphs)
@@ -2087,8 +2096,21 @@ extern "C" {
 
 extern "C" {
 
-int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, double* b_dev,
-                   bhip_solve_info* info) {
+}  // extern "C"
+
+namespace bhip {
+// out[i] = in[i] (* s[i]); no alignment assumed beyond that of a double
+__global__ void k_copy_scaled(double* __restrict__ out, const double* __restrict__ in,
+                              const double* __restrict__ s, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = s ? in[i] * s[i] : in[i];
+}
+}  // namespace bhip
+
+// SOLVE on device vectors, in the caller's node order or (internal) in the
+// context's own order, which skips the permutations.
+static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, double* b_dev,
+                     bhip_solve_info* info, bool internal) {
   if (!c || !cfg || !x_dev || !b_dev || !c->have_matrix) return BHIP_EBADARG;
   bhip_solve_info dummy;
   if (!info) info = &dummy;
@@ -2106,10 +2128,17 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
   const int n = L.npoin;
   const int g = c->grid_nodes;
   // caller order -> internal order
-  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, x_dev,
-                     c->perm.p, n);
-  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vb.p, b_dev,
-                     c->perm.p, n);
+  if (internal) {
+    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, x_dev,
+                       nullptr, n);
+    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, c->vb.p, b_dev,
+                       nullptr, n);
+  } else {
+    hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, x_dev,
+                       c->perm.p, n);
+    hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vb.p, b_dev,
+                       c->perm.p, n);
+  }
   HIP_OK(hipGetLastError());
   const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
   const bool mixed = cfg->precision == BHIP_PREC_MIXED;
@@ -2144,16 +2173,41 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
   }
   if (rc != BHIP_OK && rc != BHIP_MAXITER && rc != BHIP_ZERO_RHS) return rc;
   // X <- X*D (solve.f:637-641), back to caller order; B <- D*B as the reference leaves it
-  if (diag)
-    hipLaunchKernelGGL(k_scatter_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
-                       c->vdinv.p, c->perm.p, n);
-  else
-    hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
-                       c->perm.p, n);
-  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, b_dev, c->vb.p,
-                     c->perm.p, n);
+  if (internal) {
+    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
+                       diag ? c->vdinv.p : nullptr, n);
+    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, b_dev, c->vb.p,
+                       nullptr, n);
+  } else {
+    if (diag)
+      hipLaunchKernelGGL(k_scatter_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev,
+                         c->vx.p, c->vdinv.p, c->perm.p, n);
+    else
+      hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, x_dev,
+                         c->vx.p, c->perm.p, n);
+    hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, b_dev,
+                       c->vb.p, c->perm.p, n);
+  }
   HIP_OK(hipGetLastError());
   return rc;
+}
+
+extern "C" {
+
+int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, double* b_dev,
+                   bhip_solve_info* info) {
+  return solve_dev(c, cfg, x_dev, b_dev, info, false);
+}
+
+int bhip_solve_dev_internal(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev,
+                            double* b_dev, bhip_solve_info* info) {
+  return solve_dev(c, cfg, x_dev, b_dev, info, true);
+}
+
+int bhip_ctx_perm(const bhip_ctx* c, int32_t* perm) {
+  if (!c || !perm) return BHIP_EBADARG;
+  std::memcpy(perm, c->L.perm.data(), sizeof(int32_t) * c->L.perm.size());
+  return BHIP_OK;
 }
 
 int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
@@ -2766,3 +2820,5 @@ int bhip_matrix_mt02aa(bhip_ctx* c, int kind, double xmul, const double* xel, co
 }
 
 }  // extern "C"
+
+#include "wave.inl"

@@ -396,3 +396,198 @@ def cost_and_gradient(model: WaveModel, k_str, zb, nsteps, gauges, obs, checkpoi
     j = WaveCost.apply(model, k, z, nsteps, gauges, obs, checkpoint)
     gk, gz = torch.autograd.grad(j, (k, z))
     return j.detach(), gk, gz
+
+
+# ---------------------------------------------------------------------------
+# the same model on the HIP step kernels, with the hand-written adjoint sweep
+# ---------------------------------------------------------------------------
+class DeviceWaveModel:
+    """The wave-equation model of WaveModel with each step in three HIP
+    launches (bhip_wave_assemble: node quantities + system + right-hand side
+    written straight into the context; SOLVE on internal-order vectors;
+    bhip_wave_momentum) and its adjoint in the transposed kernels
+    (bhip_wave_momentum_adj, adjoint SOLVE, bhip_wave_assemble_adj). State
+    vectors stay in the context's internal node order for the whole run;
+    inputs and gradients are converted at the ends.
+
+    cost_and_gradient keeps the state every `checkpoint` steps; going
+    backward it recomputes each segment from its checkpoint, keeping that
+    segment's states (the last segment's come from the forward run), and
+    pulls the adjoint state through it step by step."""
+
+    def __init__(self, mesh: Mesh, sea, dt=2.0, amp=0.2, period=600.0, hmin_speed=1e-3,
+                 precision=bief.PREC_FP64, eps=1e-10, device="cuda"):
+        if not torch.cuda.is_available():
+            raise bief.BiefError("the wave model runs on the GPU")
+        self.mesh = mesh
+        self.dev = torch.device(device)
+        self.dt, self.amp, self.period = float(dt), float(amp), float(period)
+        self.eps_u = float(hmin_speed)
+        self.n = mesh.npoin
+        self.ctx = bief.Context(mesh)
+        xe, ye = mesh.xel_yel()
+        sea_mask = np.zeros(mesh.npoin, dtype=np.int32)
+        sea_mask[np.asarray(sea)] = 1
+        self.kern = bief.WaveStep(self.ctx, xe, ye, mesh.surfac(), sea_mask)
+        perm = self.ctx.perm()
+        iperm = np.empty_like(perm)
+        iperm[perm] = np.arange(perm.size, dtype=np.int32)
+        self.perm = torch.as_tensor(perm.astype(np.int64), device=self.dev)
+        self.iperm = torch.as_tensor(iperm.astype(np.int64), device=self.dev)
+        self.solve_kw = {"eps": eps, "nitmax": 5000, "precision": precision}
+        self.guess = {}
+        self.n_solves = 0
+        self.n_iters = 0
+
+    # -- node order ----------------------------------------------------------
+    def to_internal(self, t):
+        return t[self.perm].contiguous()
+
+    def to_caller(self, t):
+        out = torch.empty_like(t)
+        out[self.perm] = t
+        return out
+
+    def _zeros(self):
+        return torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+
+    def tide(self, t):
+        return self.amp * np.sin(2.0 * np.pi * t / self.period)
+
+    def _params(self, t_new):
+        return bief.WaveStep.params(self.dt, GRAV, self.eps_u, self.tide(t_new))
+
+    def _solve(self, rhs, kind):
+        """A^-1 rhs on the context's current matrix; right-hand side
+        normalised (GRACJG's stop test is absolute below ||b'|| = 1,
+        gracjg.F:189-209), previous solution of the same kind as initial
+        guess."""
+        scale = torch.clamp(torch.linalg.vector_norm(rhs), min=1e-300)
+        g = self.guess.get(kind)
+        x = g / scale if g is not None else self._zeros()
+        b = rhs / scale
+        _, info = self.ctx.solve_internal(x, b, slv=1, precon=2, **self.solve_kw)
+        if not info["converged"]:
+            raise bief.BiefError(f"wave-equation solve did not converge: {info}")
+        self.n_solves += 1
+        self.n_iters += info["niter"]
+        x = x * scale
+        self.guess[kind] = x
+        return x
+
+    # -- forward -------------------------------------------------------------
+    def step(self, eta, u, v, k_str, zb, t_new):
+        """(eta, u, v) -> (eta', u', v', deta), internal order."""
+        p = self._params(t_new)
+        rhs = self._zeros()
+        self.kern.assemble(p, eta, u, v, k_str, zb, rhs)
+        deta = self._solve(rhs, "forward")
+        eta_n, u_n, v_n = self._zeros(), self._zeros(), self._zeros()
+        self.kern.momentum(p, eta, deta, u, v, eta_n, u_n, v_n)
+        return eta_n, u_n, v_n, deta
+
+    def run(self, k_str, zb, nsteps, gauges=None, obs=None):
+        """As WaveModel.run (caller-order inputs, gauges as caller node ids)."""
+        self.guess = {}
+        k, z = self.to_internal(k_str), self.to_internal(zb)
+        g = self.iperm[gauges] if gauges is not None else None
+        eta, u, v = self._zeros(), self._zeros(), self._zeros()
+        series = []
+        cost = torch.zeros((), dtype=torch.float64, device=self.dev)
+        for n in range(nsteps):
+            eta, u, v, _ = self.step(eta, u, v, k, z, (n + 1) * self.dt)
+            if g is not None:
+                ge = eta[g]
+                series.append(ge)
+                if obs is not None:
+                    cost = cost + ((ge - obs[n]) ** 2).sum()
+        if obs is not None:
+            return cost
+        return torch.stack(series) if series else self.to_caller(eta)
+
+    # -- adjoint -------------------------------------------------------------
+    def forward_sweep(self, k_str, zb, nsteps, gauges, obs, checkpoint=10):
+        """J = sum over steps and gauges of (eta - obs)^2, and the tape of the
+        reverse sweep: the state every `checkpoint` steps and the states and
+        increments of the last segment."""
+        self.guess = {}
+        k, z = self.to_internal(k_str), self.to_internal(zb)
+        g = self.iperm[gauges]
+        last = ((nsteps - 1) // checkpoint) * checkpoint
+        eta, u, v = self._zeros(), self._zeros(), self._zeros()
+        checkpoints = {0: (eta, u, v)}
+        seg_last = [(eta, u, v)] if last == 0 else None
+        deta_last = []
+        cost = torch.zeros((), dtype=torch.float64, device=self.dev)
+        for n in range(nsteps):
+            eta, u, v, deta = self.step(eta, u, v, k, z, (n + 1) * self.dt)
+            cost = cost + ((eta[g] - obs[n]) ** 2).sum()
+            if n + 1 == last:
+                seg_last = [(eta, u, v)]
+            elif n + 1 > last:
+                seg_last.append((eta, u, v))
+            if n >= last:
+                deta_last.append(deta)
+            if (n + 1) % checkpoint == 0 and n + 1 <= last:
+                checkpoints[n + 1] = (eta, u, v)
+        tape = {"k": k, "z": z, "g": g, "obs": obs, "nsteps": nsteps, "checkpoint": checkpoint,
+                "checkpoints": checkpoints, "last": last, "seg_last": (seg_last, deta_last)}
+        return cost, tape
+
+    def reverse_sweep(self, tape, gcost=1.0):
+        """(dJ/dK, dJ/dzb) times gcost, caller order, from forward_sweep's tape."""
+        k, z, g, obs = tape["k"], tape["z"], tape["g"], tape["obs"]
+        nsteps, checkpoint = tape["nsteps"], tape["checkpoint"]
+        kbar, zbar = self._zeros(), self._zeros()
+        lam_e, lam_u, lam_v = self._zeros(), self._zeros(), self._zeros()
+        self.guess.pop("adjoint", None)
+        for s0 in sorted(tape["checkpoints"], reverse=True):
+            s1 = min(s0 + checkpoint, nsteps)
+            if s0 == tape["last"]:
+                states, detas = tape["seg_last"]
+            else:
+                # the segment recomputed from its checkpoint
+                states, detas = [tape["checkpoints"][s0]], []
+                for n in range(s0, s1):
+                    e, uu, vv, d = self.step(*states[-1], k, z, (n + 1) * self.dt)
+                    states.append((e, uu, vv))
+                    detas.append(d)
+            for n in range(s1 - 1, s0 - 1, -1):
+                (e0, u0, v0), (e1, u1, v1) = states[n - s0], states[n - s0 + 1]
+                p = self._params((n + 1) * self.dt)
+                ebar_n = lam_e.clone()
+                ebar_n.index_add_(0, g, (2.0 * gcost) * (e1[g] - obs[n]))
+                # the step's system again, then the transposed step
+                self.kern.assemble(p, e0, u0, v0, k, z, self._zeros())
+                etot = self._zeros()
+                self.kern.momentum_adj(p, lam_u, lam_v, ebar_n, etot)
+                lam = self._solve(etot, "adjoint")
+                eb, ub, vb = self._zeros(), self._zeros(), self._zeros()
+                self.kern.assemble_adj(p, lam, detas[n - s0], e0, u0, v0, u1, v1, lam_u, lam_v,
+                                       etot, k, eb, ub, vb, kbar, zbar)
+                lam_e, lam_u, lam_v = eb, ub, vb
+        return self.to_caller(kbar), self.to_caller(zbar)
+
+    def cost_and_gradient(self, k_str, zb, nsteps, gauges, obs, checkpoint=10):
+        """(J, dJ/dK, dJ/dzb) through the checkpointed adjoint sweep."""
+        cost, tape = self.forward_sweep(k_str, zb, nsteps, gauges, obs, checkpoint)
+        gk, gz = self.reverse_sweep(tape)
+        return cost, gk, gz
+
+
+class DeviceWaveCost(torch.autograd.Function):
+    """J(K, zb) of DeviceWaveModel behind autograd: forward is the
+    checkpointed forward sweep, backward the hand-written reverse sweep."""
+
+    @staticmethod
+    def forward(ctx, model, k_str, zb, nsteps, gauges, obs, checkpoint):
+        cost, tape = model.forward_sweep(k_str.detach(), zb.detach(), nsteps, gauges, obs,
+                                         checkpoint)
+        ctx.model, ctx.tape = model, tape
+        return cost
+
+    @staticmethod
+    def backward(ctx, gcost):
+        gk, gz = ctx.model.reverse_sweep(ctx.tape, gcost)
+        ctx.tape = None
+        return None, gk, gz, None, None, None, None
