@@ -110,7 +110,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (profiler counter scale)")
     ap.add_argument("--workload", default="estuary",
-                    choices=["estuary", "prisms", "adjoint", "ensemble"],
+                    choices=["estuary", "prisms", "adjoint", "ensemble", "supg"],
                     help="estuary: the 2M-node wave-equation CG (the headline line); prisms: "
                          "the 5M-node sigma-mesh diffusion and Poisson systems (parity "
                          "configuration, measured the same way)")
@@ -145,6 +145,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps(bench_adjoint(args, n_fd=args.adj_fd)), flush=True)
+        return
+    if args.workload == "supg":
+        print(json.dumps(bench_supg(args)), flush=True)
         return
     m, sysd = build_workload(args.npoin, args.seed, args.cache)
     t = time.time()
@@ -465,6 +468,77 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
             "fd_max_rel_error": fd_err, "fd_nodes_per_parameter": n_fd,
             "tangent_linear_rel_error": abs(adj - tl) / max(abs(tl), 1e-300),
             "solver": "mixed (FP32 EBE CG, FP64 refinement), eps 1e-10"}
+
+
+def bench_supg(args, dt=10.0, nu=0.1):
+    """The non-symmetric system of the 2M-node estuary: CVDFTR's SUPG tracer
+    system (classical SUPG, implicit P1 diffusion nu = 0.1 m^2/s, dt = 10 s)
+    with a seeded divergence-free velocity field of at most 2 m/s (a
+    streamfunction of 16 Gaussians), a zero Dirichlet value on the sea
+    boundary and a tracer of Gaussian patches, assembled on the device
+    (MT01AA, MT05AA, MT03AA with KSUPG in place, MT02AA, OM) and solved by
+    GMRES(5) with PRECON 2 to eps 1e-8 (FP64). A step restores the
+    assembled matrix and solves."""
+    import torch
+    from opentelemac_amd import bief, mesh as M, system as S, tracer
+    t = time.time()
+    m = M.estuary_mesh(args.npoin, seed=args.seed)
+    rng = np.random.default_rng(args.seed + 7)
+    x, y = m.x, m.y
+    lx, ly = x.max() - x.min(), y.max() - y.min()
+    # streamfunction psi = sum a_k exp(-r_k^2 / (2 s_k^2)); u = dpsi/dy, v = -dpsi/dx
+    u = np.zeros(m.npoin)
+    v = np.zeros(m.npoin)
+    for _ in range(16):
+        cx, cy = x.min() + rng.uniform() * lx, y.min() + rng.uniform() * ly
+        sg = rng.uniform(0.05, 0.2) * min(lx, ly)
+        a = rng.normal()
+        e = a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * sg * sg))
+        u += -e * (y - cy) / (sg * sg)
+        v += e * (x - cx) / (sg * sg)
+    sc = 2.0 / np.hypot(u, v).max()
+    u, v = u * sc, v * sc
+    fn = np.zeros(m.npoin)
+    for _ in range(8):
+        cx, cy = x.min() + rng.uniform() * lx, y.min() + rng.uniform() * ly
+        fn += np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * (0.03 * lx) ** 2))
+    bnd = S.boundary_nodes(m)
+    sea = bnd[m.x[bnd] > m.x.max() - 1.0]
+    log(f"supg mesh {m.npoin} nodes, built in {time.time() - t:.1f}s")
+    ctx = bief.Context(m)
+    t = time.time()
+    sm = tracer.supg_system(ctx, m, u, v, np.full(m.npoin, nu), dt, fn)
+    limdir = np.full(m.npoin, 2, dtype=np.int32)
+    limdir[sea] = 5
+    f, sm = ctx.dirich(fn, sm, np.zeros(m.npoin), limdir)
+    log(f"system assembled on the device in {time.time() - t:.1f}s")
+    ctx.save_matrix()
+    dev = torch.device("cuda:0")
+    x0 = torch.from_numpy(f).to(dev)
+    b0 = torch.from_numpy(sm).to(dev)
+    xx, bb = torch.empty_like(x0), torch.empty_like(b0)
+
+    def step():
+        ctx.restore_matrix()
+        xx.copy_(x0)
+        bb.copy_(b0)
+        return ctx.solve(xx, bb, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000)[1]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        info = step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    ms = float(np.median(lat)) * 1e3
+    return {"workload": "SUPG tracer system (CVDFTR), 2M-node estuary, GMRES(5) + PRECON 2",
+            "npoin": m.npoin, "nelem": m.nelem, "dt_s": dt, "nu": nu, "eps": 1e-8,
+            "umax": 2.0, "ms_per_solve": ms, "solves_per_s": 1e3 / ms,
+            "gmres_restarts": info["niter"], "converged": bool(info["converged"]),
+            "relres": info["relres"], "dtype": "f64"}
 
 
 def bench_ensemble(args, nsys=256):

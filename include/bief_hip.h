@@ -216,6 +216,8 @@ int bhip_lump(bhip_ctx* ctx, double xmul, double* diag);
  *  - MT02AA P1 isotropic (mt02aa.f:247-290): u [npoin], v ignored. */
 int bhip_matrix_mt01aa(bhip_ctx* ctx, double xmul, const double* surfac,
                        int accumulate);
+/* (accumulate = 1 into a non-symmetric 'Q' matrix adds the symmetric
+ * element terms to both halves, as OM 'M=M+N' does) */
 int bhip_matrix_mt02aa(bhip_ctx* ctx, int kind, double xmul,
                        const double* xel, const double* yel,
                        const double* surfac, const double* u, const double* v,
@@ -239,6 +241,29 @@ int bhip_matrix_mt02pp(bhip_ctx* ctx, double xmul, const double* xel,
                        const double* f, const double* g, const double* h,
                        int h_zero, int npou0, int inchyd, int mon,
                        int accumulate);
+
+/* SUPG element matrices of CVDFTR (cvdftr.F:398-560) on the device, P1
+ * triangles, assembled into the context's matrix as MATRIX does (host
+ * inputs in the caller's numbering; xel, yel [3*nelem], surfac [nelem]):
+ *  kind 3: MASUPG, MT03AA (mt03aa.f), f, g per element, u, v per node; 'Q'
+ *  kind 4: MAUGUG, MT04AA (mt04aa.f), u, v per node; symmetric
+ *  kind 5: MATVGR, MT05AA (mt05aa.f), u, v per node, or per element with
+ *          BHIP_SUPG_U_P0; BHIP_SUPG_UPWIND selects FORMUL(16:16) = 'N'; 'Q'
+ * BHIP_SUPG_KSUPG: the element vectors (f, g of kind 3, the per-element
+ * u, v of kind 5) are KSUPG(1, u, v) of the nodal u, v given (ksupg.f ->
+ * kspg11.f), computed on the device. op: 0 'M=N', 1 'M=M+N', 2 'M=M+TN'
+ * (OM, om1111.f); a symmetric target becomes 'Q' first when needed. */
+#define BHIP_SUPG_U_P0 1
+#define BHIP_SUPG_UPWIND 2
+#define BHIP_SUPG_KSUPG 4
+int bhip_matrix_supg(bhip_ctx* ctx, int kind, double xmul, const double* xel,
+                     const double* yel, const double* surfac, const double* f,
+                     const double* g, const double* u, const double* v, int flags,
+                     int op);
+/* OM on the context's matrix M, with N its saved copy (bhip_matrix_save)
+ * (sources/utils/bief/om.f -> om1111.f): "M=X(M)" (the non-symmetric form
+ * of M), "M=CN" (M = c M), "M=M+CN", "M=M+N", "M=M+TN" (N transposed). */
+int bhip_matrix_om(bhip_ctx* ctx, const char* op, double c);
 
 /* ---- the time-stepped wave equation and its adjoint ---------------------- *
  * One time step of telemac2d's wave-equation treatment (PROPAG,

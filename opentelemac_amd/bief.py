@@ -110,6 +110,8 @@ def lib():
         L.bhip_solve_dev_internal.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
                                               ctypes.POINTER(SolveInfo)]
         L.bhip_ctx_perm.argtypes = [_P, _P]
+        L.bhip_matrix_supg.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I]
+        L.bhip_matrix_om.argtypes = [_P, ctypes.c_char_p, _D]
         _W = ctypes.POINTER(WaveParams)
         L.bhip_wave_setup.argtypes = [_P, _P, _P, _P, _P]
         L.bhip_wave_assemble.argtypes = [_P, _W] + [_P] * 6
@@ -235,6 +237,7 @@ class Context:
     def save_matrix(self):
         """OM 'M=N' into the context's saved copy (device to device)."""
         _check(lib().bhip_matrix_save(self._h), "bhip_matrix_save")
+        self._saved_typext = self.typext
 
     def restore_matrix(self):
         """OM 'M=N' back from the saved copy (queued on the context stream)."""
@@ -275,7 +278,7 @@ class Context:
         s = _host_f64(surfac)
         _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), s.ctypes.data, int(accumulate)),
                "bhip_matrix_mt01aa")
-        self.typdia, self.typext = "Q", "S"
+        self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     def matrix_mt02aa(self, kind, xmul, xel, yel, surfac, u, v=None, accumulate=False):
         """MATRIX(... 'MATDIF' / 'MATDIFUV' ...): kind 1 P1 isotropic
@@ -289,7 +292,33 @@ class Context:
                                         yel.ctypes.data, s.ctypes.data, u.ctypes.data,
                                         v.ctypes.data if v is not None else None,
                                         int(accumulate)), "bhip_matrix_mt02aa")
-        self.typdia, self.typext = "Q", "S"
+        self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
+
+    SUPG_U_P0, SUPG_UPWIND, SUPG_KSUPG = 1, 2, 4
+    _OM_OPS = {"M=N": 0, "M=M+N": 1, "M=M+TN": 2}
+
+    def matrix_supg(self, kind, xmul, xel, yel, surfac, u, v, f=None, g=None, flags=0, op="M=N"):
+        """MATRIX(... 'MASUPG' (kind 3, MT03AA) / 'MAUGUG' (4, MT04AA) /
+        'MATVGR' (5, MT05AA) ...) with OM op 'M=N', 'M=M+N' or 'M=M+TN'.
+        flags: SUPG_U_P0 (u, v per element), SUPG_UPWIND (MATVGR's 'N'
+        variant), SUPG_KSUPG (the element vectors from KSUPG(1, u, v))."""
+        arrs = [_host_f64(a) for a in (xel, yel, surfac, u, v)]
+        fg = [_host_f64(a) if a is not None else None for a in (f, g)]
+        _check(lib().bhip_matrix_supg(self._h, int(kind), float(xmul),
+                                      *(a.ctypes.data for a in arrs[:3]),
+                                      *(a.ctypes.data if a is not None else None for a in fg),
+                                      arrs[3].ctypes.data, arrs[4].ctypes.data, int(flags),
+                                      self._OM_OPS[op]), "bhip_matrix_supg")
+        sym = kind == 4 and (op == "M=N" or (op == "M=M+N" and self.typext == "S"))
+        self.typdia, self.typext = "Q", ("S" if sym else "Q")
+
+    def matrix_om(self, op, c=0.0):
+        """OM on the context's matrix with N = its saved copy (save_matrix):
+        'M=X(M)', 'M=CN' (N = M), 'M=M+CN', 'M=M+N', 'M=M+TN' (om1111.f)."""
+        _check(lib().bhip_matrix_om(self._h, op.ljust(8).encode(), float(c)), "bhip_matrix_om")
+        if op.startswith("M=X(M)") or (op.startswith("M=M+") and
+                                        getattr(self, "_saved_typext", "S") == "Q"):
+            self.typext = "Q"
 
     def matrix_mt01pp(self, xmul, surfac, z, accumulate=False):
         """MATRIX(... 'MATMAS' ...) on prisms (mt01pp.f): surfac is the base

@@ -2357,7 +2357,7 @@ k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
            int nslot_pad, const int32_t* __restrict__ node_start,
            const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
            const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
-           double* __restrict__ da, int accumulate) {
+           double* __restrict__ da, int accumulate, int target_q) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* acc = reinterpret_cast<double*>(smem_raw);
   const int b = blockIdx.x;
@@ -2411,17 +2411,12 @@ k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
         t2 = -a12 - a23;
         t3 = -a13 - a23;
       }
-      double* p12 = xa + s;
-      double* p13 = xa + (size_t)nslot_pad + s;
-      double* p23 = xa + 2 * (size_t)nslot_pad + s;
-      if (accumulate) {
-        *p12 += a12;
-        *p13 += a13;
-        *p23 += a23;
-      } else {
-        *p12 = a12;
-        *p13 = a13;
-        *p23 = a23;
+      // a symmetric element matrix; into a non-symmetric ('Q') target it
+      // goes to both halves (OM 'M=M+N' of an 'S' N into a 'Q' M)
+      const double av[3] = {a12, a13, a23};
+      for (int cc = 0; cc < (target_q ? 6 : 3); ++cc) {
+        double* p = xa + (size_t)cc * nslot_pad + s;
+        *p = accumulate ? *p + av[cc % 3] : av[cc % 3];
       }
       const double tk[3] = {t1, t2, t3};
 #pragma unroll
@@ -2696,7 +2691,9 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
   if (kind == 2 && !v) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin, ne = L.nelem;
-  if (accumulate && (!c->have_matrix || c->typext == 'Q')) return BHIP_EBADARG;
+  if (accumulate && (!c->have_matrix || c->typdia != 'Q' || c->typext == '0'))
+    return BHIP_EBADARG;
+  const int target_q = accumulate && c->typext == 'Q';
   if (!c->have_matrix) {
     HIP_OK(c->da.alloc(n));
     HIP_OK(c->xa.alloc((size_t)6 * L.nslot_pad));
@@ -2725,14 +2722,14 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
   hipLaunchKernelGGL(k_elem_tri, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
                      dS.p, dX.p, dY.p, ne, dU.p, v ? dV.p : dU.p, c->slot_elem.p, c->lid.p,
                      L.nslot_pad, c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
-                     L.nblocks, c->xa.p, c->da.p, accumulate);
+                     L.nblocks, c->xa.p, c->da.p, accumulate, target_q);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->stream));
   // MATRIY sets TYPDIA 'Q', TYPEXT 'S' (matriy.f:375-376); 'M=M+N' of two
-  // symmetric matrices stays symmetric
+  // symmetric matrices stays symmetric, into a 'Q' one stays 'Q'
   c->typdia = 'Q';
-  c->typext = 'S';
-  c->ncoef = 3;
+  c->typext = target_q ? 'Q' : 'S';
+  c->ncoef = target_q ? 6 : 3;
   c->have_matrix = true;
   return BHIP_OK;
 }
@@ -2747,7 +2744,9 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
   const Layout& L = c->L;
   const int n = L.npoin, ne = L.nelem;
   if (npou0 < 0 || npou0 >= n) return BHIP_EBADARG;
-  if (accumulate && (!c->have_matrix || c->typext == 'Q')) return BHIP_EBADARG;
+  if (accumulate && (!c->have_matrix || c->typdia != 'Q' || c->typext == '0'))
+    return BHIP_EBADARG;
+  const int target_q = accumulate && c->typext == 'Q';
   if (!c->have_matrix) {
     HIP_OK(c->da.alloc(n));
     HIP_OK(c->xa.alloc((size_t)30 * L.nslot_pad));
@@ -2822,3 +2821,4 @@ int bhip_matrix_mt02aa(bhip_ctx* c, int kind, double xmul, const double* xel, co
 }  // extern "C"
 
 #include "wave.inl"
+#include "supg.inl"

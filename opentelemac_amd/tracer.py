@@ -1,0 +1,50 @@
+"""The SUPG tracer system of CVDFTR on the device.
+
+sources/utils/bief/cvdftr.F:398-560 builds, for an advected and diffused
+tracer with classical SUPG (OPTADV = 1):
+
+  TE  = KSUPG(1, U)                                  SUPG element vectors
+  AM2 = MATVGR(1, U) + MASUPG(1, TE, U)               advection + SUPG
+  AM1 = X(MATMAS(1/dt)) + T(MATVGR(1/dt, TE as P0))   mass + SUPG test term
+  SM  = AM1 (dt SM0 + FN) + (TETAT - 1) AM2 FN
+  AM1 = AM1 + TETAT AM2
+
+then adds the diffusion (MATDIF, cvdftr.F:694-733; here implicit with an
+isotropic P1 coefficient), applies DIRICH and calls SOLVE with a
+non-symmetric solver (GMRES). Every matrix here is built on the device:
+the element kernels of MT01AA, MT02AA, MT03AA, MT05AA (with KSUPG computed
+in place) and the OM operations work on the context's slot matrix and its
+saved copy, in the reference's order.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import bief
+
+
+def supg_system(ctx: bief.Context, mesh, u, v, nu, dt, fn, tetat=1.0, sm0=None):
+    """Assembles AM1 into the context (non-symmetric, 'Q') and returns the
+    right-hand side SM (host array). u, v, nu, fn: node arrays (caller
+    order); sm0: the source term SM before the step (zero by default)."""
+    xe, ye = mesh.xel_yel()
+    s = mesh.surfac()
+    fn = np.ascontiguousarray(fn, dtype=np.float64)
+    K = bief.Context.SUPG_KSUPG
+    # AM2: centred advection plus the SUPG term (cvdftr.F:398-409)
+    ctx.matrix_supg(5, 1.0, xe, ye, s, u, v, op="M=N")
+    ctx.matrix_supg(3, 1.0, xe, ye, s, u, v, flags=K, op="M=M+N")
+    t2 = ctx.matvec("X=AY", fn)
+    ctx.save_matrix()
+    # AM1: mass over dt made non-symmetric, plus the transposed SUPG test
+    # term with the element vectors as a P0 velocity (cvdftr.F:448, 513-523)
+    ctx.matrix_mt01aa(1.0 / dt, s)
+    ctx.matrix_om("M=X(M)")
+    ctx.matrix_supg(5, 1.0 / dt, xe, ye, s, u, v, flags=K, op="M=M+TN")
+    rhs = fn if sm0 is None else dt * np.asarray(sm0, dtype=np.float64) + fn
+    sm = ctx.matvec("X=AY", rhs)
+    sm = sm + (tetat - 1.0) * t2
+    ctx.matrix_om("M=M+CN", tetat)
+    # implicit diffusion (MATDIF, P1 isotropic)
+    ctx.matrix_mt02aa(1, 1.0, xe, ye, s, nu, accumulate=True)
+    return sm

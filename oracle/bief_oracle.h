@@ -85,6 +85,26 @@ void orc_mt02pp(double* t, double* xm, double xmul, const double* f, const doubl
                 const double* yel, const double* z, const double* surfac, const int* ikle,
                 int nelem, int nelmax, int inchyd, int mon);
 
+/* SUPG element routines (bief_oracle_supg.c), P1 triangles; xm in the 'Q'
+ * order (12,13,23,21,31,32) except MT04AA ('S', 12,13,23).
+ *  KSPG11 (ksupg.f -> kspg11.f): SUPG element vectors kx, ky (per element)
+ *  MT03AA (MASUPG): f, g per element, u, v per node
+ *  MT04AA (MAUGUG): u, v per node, symmetric
+ *  MT05AA (MATVGR): u, v per element (u_p0) or per node; upwind: the 'N'
+ *  variant of FORMUL(16:16) */
+void orc_kspg11(double* kx, double* ky, const double* xel, const double* yel,
+                const double* u, const double* v, const int* ikle, int nelem, int nelmax,
+                double xmul);
+void orc_mt03aa(double* t, double* xm, double xmul, const double* f, const double* g,
+                const double* u, const double* v, const double* xel, const double* yel,
+                const double* surfac, const int* ikle, int nelem, int nelmax);
+void orc_mt04aa(double* t, double* xm, double xmul, const double* u, const double* v,
+                const double* xel, const double* yel, const double* surfac, const int* ikle,
+                int nelem, int nelmax);
+void orc_mt05aa(double* t, double* xm, double xmul, const double* u, const double* v,
+                int u_p0, int upwind, const double* xel, const double* yel, const int* ikle,
+                int nelem, int nelmax);
+
 /* assembles element diagonal terms t (ndp columns) into da (init to zero) */
 void orc_assemble_diag(double* da, const double* t, const int* ikle,
                        int npoin, int nelem, int nelmax, int ndp);

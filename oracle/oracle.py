@@ -61,6 +61,10 @@ def lib():
         L.orc_mt02pp.argtypes = [_P, _P, _D, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P,
                                  _I, _I, _I, _I]
         L.orc_assemble_diag.argtypes = [_P, _P, _P, _I, _I, _I, _I]
+        L.orc_kspg11.argtypes = [_P, _P, _P, _P, _P, _P, _P, _I, _I, _D]
+        L.orc_mt03aa.argtypes = [_P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I]
+        L.orc_mt04aa.argtypes = [_P, _P, _D, _P, _P, _P, _P, _P, _P, _I, _I]
+        L.orc_mt05aa.argtypes = [_P, _P, _D, _P, _P, _I, _I, _P, _P, _P, _I, _I]
         L.orc_om_dmd.argtypes = [_P, _P, _P, _C, _P, _P, _I, _I, _I, _I]
         L.orc_precd1.argtypes = [_P, _P, _P, _P, _C, _P, _P, _P, _I, _I, _I, _I, _I]
         L.orc_solve.restype = _I
@@ -129,6 +133,47 @@ def mt02aa(kind, xel, yel, surfac, ikle, u, v=None, xmul=1.0, mon=False, ldu=0):
     lib().orc_mt02aa(kind, _ptr(t), _ptr(xm), xmul, _ptr(u), _ptr(v), ldu,
                      _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(s), _ptr(_i32(ikle)), ne, ne,
                      1 if mon else 0)
+    return t, xm
+
+
+def kspg11(xel, yel, ikle, u, v, xmul=1.0):
+    """KSUPG (ksupg.f -> kspg11.f): the SUPG element vectors (kx, ky)."""
+    ik = _i32(ikle)
+    ne = ik.shape[1]
+    kx, ky = np.empty(ne), np.empty(ne)
+    lib().orc_kspg11(_ptr(kx), _ptr(ky), _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(_f64(u)),
+                     _ptr(_f64(v)), _ptr(ik), ne, ne, xmul)
+    return kx, ky
+
+
+def mt03aa(f, g, u, v, xel, yel, surfac, ikle, xmul=1.0):
+    """MASUPG (mt03aa.f): t (3, ne), xm (6, ne) in the 'Q' order."""
+    s = _f64(surfac)
+    ne = s.size
+    t, xm = np.empty((3, ne)), np.empty((6, ne))
+    lib().orc_mt03aa(_ptr(t), _ptr(xm), xmul, _ptr(_f64(f)), _ptr(_f64(g)), _ptr(_f64(u)),
+                     _ptr(_f64(v)), _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(s), _ptr(_i32(ikle)),
+                     ne, ne)
+    return t, xm
+
+
+def mt04aa(u, v, xel, yel, surfac, ikle, xmul=1.0):
+    """MAUGUG (mt04aa.f): t (3, ne), xm (3, ne) symmetric."""
+    s = _f64(surfac)
+    ne = s.size
+    t, xm = np.empty((3, ne)), np.empty((3, ne))
+    lib().orc_mt04aa(_ptr(t), _ptr(xm), xmul, _ptr(_f64(u)), _ptr(_f64(v)), _ptr(_f64(xel)),
+                     _ptr(_f64(yel)), _ptr(s), _ptr(_i32(ikle)), ne, ne)
+    return t, xm
+
+
+def mt05aa(u, v, xel, yel, ikle, xmul=1.0, u_p0=False, upwind=False):
+    """MATVGR (mt05aa.f): t (3, ne), xm (6, ne) in the 'Q' order."""
+    ik = _i32(ikle)
+    ne = ik.shape[1]
+    t, xm = np.empty((3, ne)), np.empty((6, ne))
+    lib().orc_mt05aa(_ptr(t), _ptr(xm), xmul, _ptr(_f64(u)), _ptr(_f64(v)), int(u_p0),
+                     int(upwind), _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(ik), ne, ne)
     return t, xm
 
 
@@ -247,6 +292,10 @@ def _bind_driver(L):
     L.ref_mt02pp.argtypes = [_I, _I, _I, _D, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _I,
                              _P, _P]
     L.ref_mesh_segments.argtypes = [_I, _P]
+    L.ref_kspg11.argtypes = [_I, _D, _P, _P, _P, _P, _P, _P, _P]
+    L.ref_mt03aa.argtypes = [_I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.ref_mt04aa.argtypes = [_I, _D, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.ref_mt05aa.argtypes = [_I, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P]
     return L
 
 
@@ -373,6 +422,43 @@ def ref_mt02aa(kind, xel, yel, surfac, ikle, u, v=None, xmul=1.0, mon=False, npo
     return t, xm
 
 
+def ref_kspg11(xel, yel, ikle, u, v, xmul=1.0):
+    ik = _i32(ikle)
+    ne = ik.shape[1]
+    kx, ky = np.empty(ne), np.empty(ne)
+    ref().ref_kspg11(ne, xmul, _ptr(_f64(u)), _ptr(_f64(v)), _ptr(_f64(xel)), _ptr(_f64(yel)),
+                     _ptr(ik), _ptr(kx), _ptr(ky))
+    return kx, ky
+
+
+def ref_mt03aa(f, g, u, v, xel, yel, surfac, ikle, xmul=1.0):
+    s = _f64(surfac)
+    ne = s.size
+    t, xm = np.empty((3, ne)), np.empty((6, ne))
+    ref().ref_mt03aa(ne, xmul, _ptr(_f64(f)), _ptr(_f64(g)), _ptr(_f64(u)), _ptr(_f64(v)),
+                     _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(s), _ptr(_i32(ikle)), _ptr(t),
+                     _ptr(xm))
+    return t, xm
+
+
+def ref_mt04aa(u, v, xel, yel, surfac, ikle, xmul=1.0):
+    s = _f64(surfac)
+    ne = s.size
+    t, xm = np.empty((3, ne)), np.empty((3, ne))
+    ref().ref_mt04aa(ne, xmul, _ptr(_f64(u)), _ptr(_f64(v)), _ptr(_f64(xel)), _ptr(_f64(yel)),
+                     _ptr(s), _ptr(_i32(ikle)), _ptr(t), _ptr(xm))
+    return t, xm
+
+
+def ref_mt05aa(u, v, xel, yel, ikle, xmul=1.0, u_p0=False, upwind=False):
+    ik = _i32(ikle)
+    ne = ik.shape[1]
+    t, xm = np.empty((3, ne)), np.empty((6, ne))
+    ref().ref_mt05aa(ne, xmul, 10 if u_p0 else 11, int(upwind), _ptr(_f64(u)), _ptr(_f64(v)),
+                     _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(ik), _ptr(t), _ptr(xm))
+    return t, xm
+
+
 def ref_mt01pp(surfac, z, ikle, xmul=1.0):
     s = _f64(surfac)
     ne = s.size
@@ -393,3 +479,56 @@ def ref_mt02pp(f, g, h, xel, yel, z, surfac, ikle, nplan, xmul=1.0, h_zero=False
                      int(p0_vertical), _ptr(_f64(xel)), _ptr(_f64(yel)), _ptr(_f64(z)), _ptr(s),
                      _ptr(_i32(ikle)), int(inchyd), int(mon), _ptr(t), _ptr(xm))
     return t, xm
+
+
+# ---------------------------------------------------------------------------
+# CVDFTR's SUPG system (cvdftr.F:398-560), assembled from element matrices
+# ---------------------------------------------------------------------------
+def _om_xm(xa):
+    """OM 'M=X(M)': the non-symmetric form of a symmetric triangle matrix."""
+    return np.concatenate([xa, xa]) if xa.shape[0] == 3 else xa.copy()
+
+
+def supg_system(mesh, u, v, nu, dt, fn, tetat=1.0, reference=False):
+    """The tracer system of CVDFTR with classical SUPG (OPTADV = 1), an
+    implicit P1 isotropic diffusion nu (MATDIF, TETATD = 1) and a zero
+    source, built from the oracle's element routines (or, with
+    reference=True, from the reference's own through the driver):
+
+      TE = KSUPG(1, U)                                    (cvdftr.F:406)
+      AM2 = MATVGR(1, U) + MASUPG(1, TE, U)               (:398-409)
+      AM1 = X(MATMAS(1/dt)) + T(MATVGR(1/dt, TE as P0))   (:448, :513-523)
+      SM = AM1 FN + (TETAT - 1) AM2 FN                    (:541-546)
+      AM1 = AM1 + TETAT AM2 + MATDIF(nu)                  (:550, :694-733)
+
+    Returns (Matrix 'Q', sm). The OM operations are element-wise and
+    restated here with numpy; the diagonal assembly and the products are
+    the oracle's (ASSVEC order)."""
+    xe, ye = mesh.xel_yel()
+    s = mesh.surfac()
+    ik = mesh.ikle
+    n = mesh.npoin
+    P = globals()
+    k = (lambda name: P["ref_" + name]) if reference else (lambda name: P[name])
+    te1, te2 = k("kspg11")(xe, ye, ik, u, v, 1.0)
+    t5, x5 = k("mt05aa")(u, v, xe, ye, ik, 1.0)
+    t3, x3 = k("mt03aa")(te1, te2, u, v, xe, ye, s, ik, 1.0)
+    am2_d = assemble_diag(t5, ik, n) + assemble_diag(t3, ik, n)
+    am2_x = x5 + x3
+    am2 = Matrix(am2_d, am2_x, "Q", "Q")
+    t2 = matvec("X=AY", am2, ik, fn)
+    t1, x1 = k("mt01aa")(s, 1.0 / dt)
+    am1_d = assemble_diag(t1, ik, n)
+    am1_x = _om_xm(x1)
+    tp, xp = k("mt05aa")(te1, te2, xe, ye, ik, 1.0 / dt, u_p0=True)
+    am1_d = am1_d + assemble_diag(tp, ik, n)
+    am1_x = am1_x + np.concatenate([xp[3:], xp[:3]])
+    am1 = Matrix(am1_d, am1_x, "Q", "Q")
+    sm = matvec("X=AY", am1, ik, fn)
+    sm = sm + (tetat - 1.0) * t2
+    am1_d = am1_d + tetat * am2_d
+    am1_x = am1_x + tetat * am2_x
+    td, xd = k("mt02aa")(1, xe, ye, s, ik, nu, None, 1.0)
+    am1_d = am1_d + assemble_diag(td, ik, n)
+    am1_x = am1_x + _om_xm(xd)
+    return Matrix(am1_d, am1_x, "Q", "Q"), sm

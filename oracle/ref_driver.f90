@@ -2,7 +2,7 @@
 ! oracle/_ref/libbief_ref.a). It is written for this project: it fills a
 ! BIEF_MESH with the fields the linear-system path reads, then calls the
 ! reference routines SOLVE, MATVEC, DIRICH, LUMP, MT01AA, MT02AA, MT01PP,
-! MT02PP unchanged.
+! MT02PP, and the SUPG routines KSPG11, MT03AA, MT04AA, MT05AA unchanged.
 !
 ! TEST INFRASTRUCTURE ONLY (checker + "reference" CPU baseline). Entry points
 ! are bind(C) so tests/ and bench.py's cpu_baseline leg can load the shared
@@ -370,6 +370,86 @@ contains
     call mt02pp(t, xm, xmul, sf, sg, sh, f, g, h, x6, y6, z, surfac, ik, nelem, nelem, &
          inchyd /= 0, formul, nplan)
     deallocate(ik, x6, y6)
+  end subroutine
+
+  ! KSPG11: SUPG element vectors (P0) from P1 velocities
+  subroutine ref_kspg11(nelem, xmul, u, v, xel, yel, ikle0, kx, ky) bind(c, name='ref_kspg11')
+    integer(c_int), value :: nelem
+    real(c_double), value :: xmul
+    real(c_double), intent(in) :: u(*), v(*), xel(nelem, 3), yel(nelem, 3)
+    integer(c_int), intent(in) :: ikle0(nelem, 3)
+    real(c_double), intent(out) :: kx(nelem), ky(nelem)
+    integer, allocatable :: ik(:,:)
+    allocate(ik(nelem, 3))
+    ik = ikle0 + 1
+    call kspg11(kx, ky, xel, yel, u, v, ik, nelem, nelem, xmul)
+    deallocate(ik)
+  end subroutine
+
+  ! MT03AA (MASUPG) with P0 F, G and P1 U, V; xm in the 'Q' order 12,13,23,21,31,32
+  subroutine ref_mt03aa(nelem, xmul, f, g, u, v, xel, yel, surfac, ikle0, t, xm) &
+       bind(c, name='ref_mt03aa')
+    integer(c_int), value :: nelem
+    real(c_double), value :: xmul
+    real(c_double), intent(in) :: f(*), g(*), u(*), v(*), xel(nelem, 3), yel(nelem, 3)
+    real(c_double), intent(in) :: surfac(*)
+    integer(c_int), intent(in) :: ikle0(nelem, 3)
+    real(c_double), intent(out) :: t(nelem, 3), xm(nelem, 6)
+    type(bief_obj) :: sf, sg, su, sv
+    integer, allocatable :: ik(:,:)
+    allocate(ik(nelem, 4))
+    ik(:, 1:3) = ikle0 + 1
+    ik(:, 4) = ik(:, 1)
+    sf%elm = 10
+    sg%elm = 10
+    su%elm = 11
+    sv%elm = 11
+    call mt03aa(t(1,1), xm(1,1), xm(1,2), xm(1,4), t(1,2), xm(1,3), xm(1,5), xm(1,6), &
+         t(1,3), xmul, sf, sg, su, sv, f, g, u, v, xel, yel, surfac, &
+         ik(1,1), ik(1,2), ik(1,3), ik(1,4), nelem, nelem)
+    deallocate(ik)
+  end subroutine
+
+  ! MT04AA (MAUGUG) with P1 U, V; symmetric xm 12,13,23
+  subroutine ref_mt04aa(nelem, xmul, u, v, xel, yel, surfac, ikle0, t, xm) &
+       bind(c, name='ref_mt04aa')
+    integer(c_int), value :: nelem
+    real(c_double), value :: xmul
+    real(c_double), intent(in) :: u(*), v(*), xel(nelem, 3), yel(nelem, 3), surfac(*)
+    integer(c_int), intent(in) :: ikle0(nelem, 3)
+    real(c_double), intent(out) :: t(nelem, 3), xm(nelem, 3)
+    type(bief_obj) :: su, sv
+    integer, allocatable :: ik(:,:)
+    allocate(ik(nelem, 3))
+    ik = ikle0 + 1
+    su%elm = 11
+    sv%elm = 11
+    call mt04aa(t(1,1), xm(1,1), xm(1,2), t(1,2), xm(1,3), t(1,3), xmul, su, sv, u, v, &
+         xel, yel, surfac, ik, nelem, nelem)
+    deallocate(ik)
+  end subroutine
+
+  ! MT05AA (MATVGR): u_elm 11 (P1 U, V) or 10 (P0 U, V); upwind /= 0 selects
+  ! the FORMUL(16:16) = 'N' variant; xm in the 'Q' order
+  subroutine ref_mt05aa(nelem, xmul, u_elm, upwind, u, v, xel, yel, ikle0, t, xm) &
+       bind(c, name='ref_mt05aa')
+    integer(c_int), value :: nelem, u_elm, upwind
+    real(c_double), value :: xmul
+    real(c_double), intent(in) :: u(*), v(*), xel(nelem, 3), yel(nelem, 3)
+    integer(c_int), intent(in) :: ikle0(nelem, 3)
+    real(c_double), intent(out) :: t(nelem, 3), xm(nelem, 6)
+    type(bief_obj) :: su, sv
+    integer, allocatable :: ik(:,:)
+    character(len=16) :: formul
+    allocate(ik(nelem, 3))
+    ik = ikle0 + 1
+    su%elm = u_elm
+    sv%elm = u_elm
+    formul = 'MATVGR          '
+    if (upwind /= 0) formul(16:16) = 'N'
+    call mt05aa(t(1,1), xm(1,1), xm(1,2), xm(1,4), t(1,2), xm(1,3), xm(1,5), xm(1,6), &
+         t(1,3), xmul, su, sv, u, v, xel, yel, ik, nelem, nelem, formul)
+    deallocate(ik)
   end subroutine
 
 end module ref_driver

@@ -224,8 +224,48 @@ def gouttedo_case():
     return out
 
 
+def supg_flow(m):
+    """A rigid rotation about the gouttedo basin's centre, |u| <= 2 m/s."""
+    w = 2.0 / (10.05 * np.sqrt(2.0))
+    return -w * (m.y - 10.05), w * (m.x - 10.05)
+
+
+def supg_case():
+    """SUPG element routines (KSPG11, MT03AA, MT04AA, MT05AA in its P1, P0
+    and upwind forms) on the small mesh, and CVDFTR's SUPG tracer system on
+    the gouttedo mesh (rotating flow, nu = 0.1, dt = 0.5 s, a Gaussian
+    patch) solved by GMRES(5) and BiCGSTAB with PRECON 2 at eps 1e-10."""
+    from opentelemac_amd import serafin as SF
+    out = {}
+    m = M.random_small_mesh(9, 7, seed=3)
+    xe, ye = m.xel_yel()
+    sf = m.surfac()
+    rng = np.random.default_rng(60)
+    u, v = rng.uniform(-2, 2, m.npoin), rng.uniform(-2, 2, m.npoin)
+    out["small_u"], out["small_v"] = u, v
+    kx, ky = O.ref_kspg11(xe, ye, m.ikle, u, v, 1.0)
+    out["kspg11_kx"], out["kspg11_ky"] = kx, ky
+    out["mt03aa_t"], out["mt03aa_xm"] = O.ref_mt03aa(kx, ky, u, v, xe, ye, sf, m.ikle, 0.7)
+    out["mt04aa_t"], out["mt04aa_xm"] = O.ref_mt04aa(u, v, xe, ye, sf, m.ikle, 0.3)
+    out["mt05aa_t"], out["mt05aa_xm"] = O.ref_mt05aa(u, v, xe, ye, m.ikle, 1.3)
+    out["mt05p0_t"], out["mt05p0_xm"] = O.ref_mt05aa(kx, ky, xe, ye, m.ikle, 2.0, u_p0=True)
+    out["mt05n_t"], out["mt05n_xm"] = O.ref_mt05aa(u, v, xe, ye, m.ikle, 1.0, upwind=True)
+    g = SF.read(os.path.join(HERE, "geo_gouttedo.slf")).mesh()
+    gu, gv = supg_flow(g)
+    fn = np.exp(-((g.x - 6.0) ** 2 + (g.y - 10.0) ** 2) / 4.0)
+    A, sm = O.supg_system(g, gu, gv, np.full(g.npoin, 0.1), 0.5, fn, reference=True)
+    out["sys_da"], out["sys_xa"], out["sys_sm"] = A.da, A.xa, sm
+    rm = O.RefMesh(g.ikle, g.npoin, g.nelem)
+    for slv in (7, 6):
+        x, info, _, _ = O.ref_solve(rm, A, sm, fn, slv=slv, precon=2, krylov=5, eps=1e-10,
+                                    nitmax=500)
+        out[f"solve_{slv}_x"], out[f"solve_{slv}_niter"] = x, np.array(info["niter"])
+    return out
+
+
 CASES = {"ref_prism": prism_case, "ref_small": small_case, "ref_channel": channel_case,
-         "ref_mt02pp": mt02pp_case, "ref_seg": seg_case, "ref_gouttedo": gouttedo_case}
+         "ref_mt02pp": mt02pp_case, "ref_seg": seg_case, "ref_gouttedo": gouttedo_case,
+         "ref_supg": supg_case}
 
 
 def main(names):

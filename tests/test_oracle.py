@@ -398,3 +398,128 @@ def test_golden_gouttedo_real_mesh():
                                     krylov=3, eps=1e-10, nitmax=500)
             assert info.niter == int(z[f"{sto}_{slv}_niter"]), (sto, slv)
             assert np.array_equal(x, z[f"{sto}_{slv}_x"]), (sto, slv)
+
+
+# ---- SUPG element routines and CVDFTR's tracer system --------------------
+
+@pytest.fixture(scope="module")
+def gs():
+    return np.load(os.path.join(GOLD, "ref_supg.npz"))
+
+
+def test_golden_supg_elements(gs):
+    m = M.random_small_mesh(9, 7, seed=3)
+    xe, ye = m.xel_yel()
+    sf = m.surfac()
+    u, v = gs["small_u"], gs["small_v"]
+    kx, ky = O.kspg11(xe, ye, m.ikle, u, v, 1.0)
+    assert np.array_equal(kx, gs["kspg11_kx"]) and np.array_equal(ky, gs["kspg11_ky"])
+    for key, (t, xm) in (("mt03aa", O.mt03aa(kx, ky, u, v, xe, ye, sf, m.ikle, 0.7)),
+                         ("mt04aa", O.mt04aa(u, v, xe, ye, sf, m.ikle, 0.3)),
+                         ("mt05aa", O.mt05aa(u, v, xe, ye, m.ikle, 1.3)),
+                         ("mt05p0", O.mt05aa(kx, ky, xe, ye, m.ikle, 2.0, u_p0=True)),
+                         ("mt05n", O.mt05aa(u, v, xe, ye, m.ikle, 1.0, upwind=True))):
+        assert np.array_equal(t, gs[key + "_t"]), key
+        assert np.array_equal(xm, gs[key + "_xm"]), key
+
+
+def test_golden_supg_system_and_solves(gs):
+    """CVDFTR's SUPG system on the gouttedo mesh, GMRES(5) and BiCGSTAB:
+    the oracle reproduces the reference bit for bit, counts included."""
+    from opentelemac_amd import serafin as SF
+    g = SF.read(os.path.join(GOLD, "geo_gouttedo.slf")).mesh()
+    w = 2.0 / (10.05 * np.sqrt(2.0))
+    gu, gv = -w * (g.y - 10.05), w * (g.x - 10.05)
+    fn = np.exp(-((g.x - 6.0) ** 2 + (g.y - 10.0) ** 2) / 4.0)
+    A, sm = O.supg_system(g, gu, gv, np.full(g.npoin, 0.1), 0.5, fn)
+    assert np.array_equal(A.da, gs["sys_da"]) and np.array_equal(A.xa, gs["sys_xa"])
+    assert np.array_equal(sm, gs["sys_sm"])
+    for slv in (7, 6):
+        x, info, _, _ = O.solve(A, g.ikle, sm, fn, slv=slv, precon=2, krylov=5, eps=1e-10,
+                                nitmax=500)
+        assert info.niter == int(gs[f"solve_{slv}_niter"])
+        assert np.array_equal(x, gs[f"solve_{slv}_x"])
+
+
+def _grad_factors(m):
+    """b_k, c_k with grad(phi_k) = (b_k, c_k) / (2S), node 1 at the origin."""
+    xe, ye = m.xel_yel()
+    b = np.stack([ye[1] - ye[2], ye[2], -ye[1]])
+    c = np.stack([xe[2] - xe[1], -xe[2], xe[1]])
+    return b, c
+
+
+def _full(t, xm):
+    """(3, 3, ne) element matrices from t and xm ('Q' order or 'S')."""
+    ne = t.shape[1]
+    a = np.zeros((3, 3, ne))
+    for k in range(3):
+        a[k, k] = t[k]
+    pairs = ((0, 1), (0, 2), (1, 2))
+    for c, (i, j) in enumerate(pairs):
+        a[i, j] = xm[c]
+        a[j, i] = xm[c + 3] if xm.shape[0] == 6 else xm[c]
+    return a
+
+
+def test_supg_closed_forms_constant_velocity():
+    """With a constant velocity U and a constant vector K on each element:
+    MATVGR = S/3 U.grad(phi_j), MASUPG = S (K.grad(phi_i))(U.grad(phi_j)),
+    MAUGUG = S (U.grad(phi_i))(U.grad(phi_j)); row i of MATVGR does not
+    depend on i (a transposed operand would fail)."""
+    m = M.random_small_mesh(8, 6, seed=11)
+    xe, ye = m.xel_yel()
+    sf = m.surfac()
+    b, c = _grad_factors(m)
+    U, V = 1.3, -0.4
+    u, v = np.full(m.npoin, U), np.full(m.npoin, V)
+    rng = np.random.default_rng(3)
+    F, G = rng.normal(size=m.nelem), rng.normal(size=m.nelem)
+    ug = (U * b + V * c) / (2 * sf)            # U.grad(phi_j), (3, ne)
+    kg = (F * b + G * c) / (2 * sf)            # K.grad(phi_i)
+    a5 = _full(*O.mt05aa(u, v, xe, ye, m.ikle, 0.9))
+    a3 = _full(*O.mt03aa(F, G, u, v, xe, ye, sf, m.ikle, 0.9))
+    a4 = _full(*O.mt04aa(u, v, xe, ye, sf, m.ikle, 0.9))
+    for i in range(3):
+        for j in range(3):
+            assert np.allclose(a5[i, j], 0.9 * sf / 3 * ug[j], rtol=1e-12, atol=1e-14)
+            assert np.allclose(a3[i, j], 0.9 * sf * kg[i] * ug[j], rtol=1e-12, atol=1e-14)
+            assert np.allclose(a4[i, j], 0.9 * sf * ug[i] * ug[j], rtol=1e-12, atol=1e-14)
+    # the P0 form of MATVGR with the same constant velocity is the same matrix
+    a5p0 = _full(*O.mt05aa(np.full(m.nelem, U), np.full(m.nelem, V), xe, ye, m.ikle, 0.9,
+                           u_p0=True))
+    assert np.allclose(a5p0, a5, rtol=1e-12, atol=1e-14)
+
+
+def test_supg_vectors_follow_the_velocity():
+    """KSUPG is h/3 times the unit mean velocity, h at most the largest
+    element height, and linear in XMUL."""
+    m = M.random_small_mesh(8, 6, seed=12)
+    xe, ye = m.xel_yel()
+    rng = np.random.default_rng(4)
+    u, v = rng.uniform(-2, 2, m.npoin), rng.uniform(-2, 2, m.npoin)
+    kx, ky = O.kspg11(xe, ye, m.ikle, u, v, 1.0)
+    um, vm = u[m.ikle].sum(0), v[m.ikle].sum(0)
+    assert np.allclose(kx * vm - ky * um, 0.0, atol=1e-13)
+    assert np.all(kx * um + ky * vm >= 0)
+    s = m.surfac()
+    sides = [np.hypot(xe[1], ye[1]), np.hypot(xe[2] - xe[1], ye[2] - ye[1]),
+             np.hypot(xe[2], ye[2])]
+    hmax = np.max([2 * s / l for l in sides], axis=0)
+    assert np.all(np.hypot(kx, ky) <= 0.33333333 * hmax * (1 + 1e-14))
+    kx2, ky2 = O.kspg11(xe, ye, m.ikle, u, v, 2.5)
+    assert np.allclose(kx2, 2.5 * kx, rtol=1e-15) and np.allclose(ky2, 2.5 * ky, rtol=1e-15)
+
+
+def test_supg_planted_transpose_is_caught():
+    """A transposed MATVGR (an easy mistake in the 'Q' column order) breaks
+    the constant-velocity closed form: checked on the oracle's own output."""
+    m = M.random_small_mesh(6, 5, seed=13)
+    xe, ye = m.xel_yel()
+    b, c = _grad_factors(m)
+    sf = m.surfac()
+    u, v = np.full(m.npoin, 1.0), np.full(m.npoin, 0.5)
+    t, xm = O.mt05aa(u, v, xe, ye, m.ikle, 1.0)
+    wrong = _full(t, np.concatenate([xm[3:], xm[:3]]))
+    ug = (1.0 * b + 0.5 * c) / (2 * sf)
+    assert not np.allclose(wrong[0, 1], sf / 3 * ug[1], rtol=1e-6)
