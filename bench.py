@@ -302,7 +302,21 @@ def bench_prisms(args, dev):
     nu = 10.0 ** (-4.0 + 2.0 * sig)
     log(f"prisms {p.npoin} nodes {p.nelem} elements, built in {time.time() - t:.1f}s")
     t = time.time()
-    ctx = bief.Context(p, block_nodes=args.block_nodes)
+    bn, table = args.block_nodes, None
+    if bn == 0:
+        # node-block size by the autotuner, on the diffusion system
+        from opentelemac_amd import tune
+
+        def load(c):
+            c.matrix_mt01pp(1.0 / 2.0, s, p.z)
+            c.matrix_mt02pp(1.0, xel, yel, s, p.z, nu, nu, nu, base.npoin, accumulate=True)
+            c.solve(np.zeros(p.npoin), np.ones(p.npoin), slv=1, precon=2, eps=1e-8,
+                    nitmax=20000)
+
+        bn, table = tune.block_nodes(lambda b: bief.Context(p, block_nodes=b), load,
+                                     candidates=(1024, 2048), reps=10)
+        log(f"autotuned block size {bn}: {table}")
+    ctx = bief.Context(p, block_nodes=bn)
     st = ctx.stats()
     log(f"context in {time.time() - t:.1f}s: {st}")
     rng = np.random.default_rng(args.seed)
@@ -311,7 +325,7 @@ def bench_prisms(args, dev):
            "higher_is_better": True, "data": "synthetic",
            "config": {"workload": "sigma-mesh prisms, 10 planes x 501,491 nodes",
                       "npoin": p.npoin, "nelem": p.nelem, "dt_s": 2.0, "eps": 1e-8},
-           "layout": st, "systems": {}}
+           "layout": st, "block_nodes": bn, "autotune_ms_per_iteration": table, "systems": {}}
     top = np.zeros(p.npoin, dtype=np.int32)
     top[(nplan - 1) * base.npoin:] = 5
     for name in ("diffusion", "poisson"):
