@@ -314,7 +314,7 @@ def bench_prisms(args, dev):
                     nitmax=20000)
 
         bn, table = tune.block_nodes(lambda b: bief.Context(p, block_nodes=b), load,
-                                     candidates=(1024, 2048), reps=10)
+                                     candidates=(1024, 2048, 2560), reps=10)
         log(f"autotuned block size {bn}: {table}")
     ctx = bief.Context(p, block_nodes=bn)
     st = ctx.stats()
