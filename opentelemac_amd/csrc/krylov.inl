@@ -252,6 +252,18 @@ __global__ void k_scale_into(double* __restrict__ x, const double* __restrict__ 
     x[i] = cst * y[i];
 }
 
+// V <- V / sqrt(nrm2[0]) unless the norm is below the Krylov-reduction
+// threshold of gmres.f:274 (the host then sees the breakdown in the
+// Hessenberg column and ignores what follows)
+__global__ void k_scale_inv_norm(double* __restrict__ v, const double* __restrict__ nrm2,
+                                 int n) {
+  const double r = sqrt(nrm2[0]);
+  if (!(r >= 1e-20)) return;
+  const double s = 1.0 / r;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v[i] = s * v[i];
+}
+
 // One modified Gram-Schmidt step: H(i,j) = reduced partials of w.V_i;
 // w = src - H(i,j) V_i (gmres.f:262-265); partial sums of w.next (next = the
 // next basis vector, or w itself for the norm).
@@ -333,7 +345,12 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
   const int g = c->grid_nodes;
   const int K0 = krylov;
   if (c->gm.alloc((size_t)2 * K0 * n)) return BHIP_EDEVICE;
-  if (c->gm_scal.alloc(64)) return BHIP_EDEVICE;
+  // device scalars: ||B||^2, ||R0||^2, then one Hessenberg column per
+  // Arnoldi step (H(0..j, j) and ||w||^2), then the last column
+  const int colw = K0 + 1;
+  auto col = [&](int j) { return 2 + j * colw; };
+  const int lastoff = col(K0 - 1);
+  if (c->gm_scal.alloc((size_t)lastoff + K0 + 2)) return BHIP_EDEVICE;
   if (c->bicg_part.alloc((size_t)(K0 + 2) * c->max_part)) return BHIP_EDEVICE;
   double* Vb = c->gm.p;
   double* AVb = c->gm.p + (size_t)K0 * n;
@@ -348,7 +365,8 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
     HIP_OK(hipStreamSynchronize(c->stream));
     return BHIP_OK;
   };
-  double hs[64];
+  std::vector<double> hsv((size_t)lastoff + K0 + 2);
+  double* hs = hsv.data();
   // NB = ||B|| clamped to 1 (gmres.f:188-194)
   hipLaunchKernelGGL(k_dot_partials<double>, dim3(g), dim3(kThreads), 0, c->stream, b, b, n,
                      part_u(c, 0));
@@ -399,8 +417,10 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
             c, a, false, c->typdia, c->typext)));
         return BHIP_OK;
       };
-      bool lastcol = false;
-      for (int j = 0; j < k - 1; ++j) {
+      // the Arnoldi steps are queued back to back: each normalises its new
+      // basis vector on the device, and the host reads all the Hessenberg
+      // columns once per cycle
+      for (int j = 0; j < K0 - 1; ++j) {
         if (matvec_w(j, Vp(0))) return BHIP_EDEVICE;
         // V_{j+1} from AV_j by modified Gram-Schmidt over V_0..V_j
         int npin = L.nblocks;
@@ -410,46 +430,48 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
           double* pin = mp + (size_t)(i & 1) * stride;
           double* pout = mp + (size_t)((i + 1) & 1) * stride;
           hipLaunchKernelGGL(k_mgs, dim3(g), dim3(kThreads), 0, c->stream, Vp(j + 1), src,
-                             Vp(i), next, n, pin, npin, pout, ds + 2 + i);
+                             Vp(i), next, n, pin, npin, pout, ds + col(j) + i);
           npin = g;
         }
-        HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream,
-                           mp + (size_t)((j + 1) & 1) * stride, g, ds + 2 + j + 1);
-        HIP_OK(hipGetLastError());
-        if (fetch(hs, 2 + j + 2)) return BHIP_EDEVICE;
-        for (int i = 0; i <= j; ++i) H[i][j] = hs[2 + i];
-        H[j + 1][j] = std::sqrt(hs[2 + j + 1]);
-        if (H[j + 1][j] < 1e-20) {
-          // reduction of the Krylov space (gmres.f:274-283)
-          k = j + 1;
-          lastcol = true;
-          break;
-        }
-        hipLaunchKernelGGL(k_scale_into, dim3(g), dim3(kThreads), 0, c->stream, Vp(j + 1),
-                           Vp(j + 1), 1.0 / H[j + 1][j], n);
+                           mp + (size_t)((j + 1) & 1) * stride, g, ds + col(j) + j + 1);
+        hipLaunchKernelGGL(k_scale_inv_norm, dim3(g), dim3(kThreads), 0, c->stream, Vp(j + 1),
+                           ds + col(j) + j + 1, n);
         HIP_OK(hipGetLastError());
       }
-      (void)lastcol;
       // K-th column (gmres.f:285-296): AV_K, H(I,K) = AV_K.V_I,
       // H(K+1,K) = sqrt(max(||AV_K||^2 - sum H(I,K)^2, 0))
-      {
+      auto last_column = [&](int kk) -> int {
         EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
-        a.y = Vp(k - 1);
-        a.out = AVp(k - 1);
+        a.y = Vp(kk - 1);
+        a.out = AVp(kk - 1);
         a.op = BHIP_OP_AY;
         HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_OP, FIN_NONE>(
             c, a, false, c->typdia, c->typext)));
+        hipLaunchKernelGGL((k_multidot<20>), dim3(g), dim3(kThreads), 0, c->stream,
+                           AVp(kk - 1), Vb, kk, n, mp, stride);
+        hipLaunchKernelGGL(k_reduce_many, dim3(1), dim3(kThreads), 0, c->stream, mp, g, stride,
+                           kk + 1, ds + lastoff);
+        HIP_OK(hipGetLastError());
+        return BHIP_OK;
+      };
+      if (last_column(K0)) return BHIP_EDEVICE;
+      if (fetch(hs, lastoff + K0 + 1)) return BHIP_EDEVICE;
+      for (int j = 0; j < K0 - 1; ++j) {
+        for (int i = 0; i <= j; ++i) H[i][j] = hs[col(j) + i];
+        H[j + 1][j] = std::sqrt(hs[col(j) + j + 1]);
+        if (H[j + 1][j] < 1e-20) {
+          // reduction of the Krylov space (gmres.f:274-283): the last
+          // column is that of V_j, computed again
+          k = j + 1;
+          if (last_column(k)) return BHIP_EDEVICE;
+          if (fetch(hs, lastoff + k + 1)) return BHIP_EDEVICE;
+          break;
+        }
       }
-      hipLaunchKernelGGL((k_multidot<20>), dim3(g), dim3(kThreads), 0, c->stream, AVp(k - 1),
-                         Vb, k, n, mp, stride);
-      hipLaunchKernelGGL(k_reduce_many, dim3(1), dim3(kThreads), 0, c->stream, mp, g, stride,
-                         k + 1, ds + 2);
-      HIP_OK(hipGetLastError());
-      if (fetch(hs, 2 + k + 1)) return BHIP_EDEVICE;
-      double hk = hs[2];
+      double hk = hs[lastoff];
       for (int i = 0; i < k; ++i) {
-        H[i][k - 1] = hs[3 + i];
+        H[i][k - 1] = hs[lastoff + 1 + i];
         hk = hk - H[i][k - 1] * H[i][k - 1];
       }
       H[k][k - 1] = std::sqrt(std::max(hk, 0.0));
