@@ -169,6 +169,32 @@ __device__ inline double reduce_partials(const double* __restrict__ partial, int
   return s;
 }
 
+// reduce_partials in two steps, for kernels that want other loads in
+// flight meanwhile: load_partials issues this thread's share of the partial
+// sums first (indices clamped, no branches around the loads); sum_partials
+// adds them in reduce_partials' order, so the result is the same to the bit.
+template <int P>
+__device__ inline void load_partials(double (&pv)[P], const double* __restrict__ partial,
+                                     int n) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) pv[k] = partial[min((int)threadIdx.x + k * kThreads, n - 1)];
+}
+template <int P>
+__device__ inline double sum_partials(const double (&pv)[P], const double* __restrict__ partial,
+                                      int n, double* sh, double* bcast) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < P; ++k)
+    if ((int)threadIdx.x + k * kThreads < n) s += pv[k];
+  for (int i = threadIdx.x + P * kThreads; i < n; i += blockDim.x) s += partial[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) *bcast = s;
+  __syncthreads();
+  s = *bcast;
+  __syncthreads();
+  return s;
+}
+
 // XCD-aware block order: workgroup g is dispatched to XCD g % 8; each XCD
 // gets a contiguous range of node blocks so that blocks sharing halo nodes
 // share an L2. Bijective for any block count.
@@ -320,6 +346,7 @@ struct NodeIn {
 #define BHIP_STAGE_UNROLL 4
 #endif
 constexpr int kStagePre = 1;
+constexpr int kEbePart = 8;  // partial sums per thread loaded ahead in the CG kernels
 constexpr int kStageUnroll = BHIP_STAGE_UNROLL;
 
 // PF > 0: the colour phases are software pipelined: each thread keeps the
@@ -354,6 +381,12 @@ k_ebe(EbeArgs<T, TA> a) {
   if (threadIdx.x <= kMaxColors) sh_co[threadIdx.x] = co[threadIdx.x];
   __syncthreads();
   const int se = sh_co[kMaxColors];
+
+  // the partial sums of the previous kernel first (FIN_CG): loads issued
+  // before the element and node loads below, so that adding them up waits
+  // for them alone while the rest stays in flight
+  double pvr[FIN == FIN_CG ? kEbePart : 1];
+  if constexpr (FIN == FIN_CG) load_partials(pvr, a.part_in + pb0, pb1 - pb0);
 
   // first PF phases in flight before the vector staging
   PhaseSlot<T, NDP, ncoef> pipe[PF > 0 ? PF : 1];
@@ -432,7 +465,7 @@ k_ebe(EbeArgs<T, TA> a) {
     // and the strict test of the initial residual, gracjg.F:219), BETA
     SolveState* st = a.st + sys;
     const int p = a.parity;
-    const double rr = reduce_partials(a.part_in + pb0, pb1 - pb0, sh_red, &sh_b);
+    const double rr = sum_partials(pvr, a.part_in + pb0, pb1 - pb0, sh_red, &sh_b);
     const int m = st->m[1 - p] + 1;
     const double thr = st->use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
     const bool conv = (m == 0) ? (rr < thr) : (rr <= thr);
@@ -682,6 +715,9 @@ k_ebe(EbeArgs<T, TA> a) {
 // CG vector update: X -= RO D ; R -= RO AD ; RMRM = R.R (gracjg.F:284-296,
 // 387-388), with the state transitions of the loop test (gracjg.F:306-324)
 // ---------------------------------------------------------------------------
+constexpr int kUpdPre = 4;   // nodes per thread loaded ahead in k_cg_update
+constexpr int kUpdPart = 8;  // partial sums per thread loaded ahead
+
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kThreads)
 k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
@@ -699,13 +735,34 @@ k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
   const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
   SolveState* st = st0 + sys;
   if (load_relaxed(&st->done)) return;
+  const int n0 = node_start[b], n1 = node_start[b + 1];
   const int q = 1 - parity;  // slot of iteration m - 1
   const int status = st->status[q];
+  const double rrq = st->rr[q];
+  // loads first, arithmetic after: this thread's share of the partial sums
+  // of D.AD, then its first kUpdPre nodes, so that the HBM latency of the
+  // node data overlaps the reduction (the sum keeps reduce_partials' order).
+  // Indices are clamped rather than branched on, so no load waits on another.
+  const double* pe = part_e + pb0;
+  const int np = pb1 - pb0;
+  double pv[kUpdPart];
+  load_partials(pv, pe, np);
+  // (FP64 keeps fewer nodes in flight: its registers would cost occupancy)
+  constexpr int kPre = sizeof(T) == 4 ? kUpdPre : kUpdPre / 2;
+  T dv[kPre], adv[kPre], rv0[kPre];
+  TX xv[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int i = min(n0 + (int)threadIdx.x + k * kThreads, n1 - 1);
+    dv[k] = d[i];
+    adv[k] = ad[i];
+    rv0[k] = r[i];
+    xv[k] = x[i];
+  }
   // RO = RMRM / D.AD of iteration m - 1 (gracjg.F:380-383)
-  const double dad = reduce_partials(part_e + pb0, pb1 - pb0, sh_red, &sh_b);
-  const double rho = st->rr[q] / dad;
+  const double dad = sum_partials(pv, pe, np, sh_red, &sh_b);
+  const double rho = rrq / dad;
   const T rhot = (T)rho;
-  const int n0 = node_start[b], n1 = node_start[b + 1];
   if (status == ST_FINAL) {
     // NITMAX reached: the last X = X - RO D, then stop (gracjg.F:387-409)
     for (int i = n0 + threadIdx.x; i < n1; i += blockDim.x) x[i] = x[i] - (TX)rhot * (TX)d[i];
@@ -720,7 +777,17 @@ k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
     return;
   }
   double part = 0.0;
-  for (int i = n0 + threadIdx.x; i < n1; i += blockDim.x) {
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int i = n0 + threadIdx.x + k * kThreads;
+    if (i < n1) {
+      x[i] = xv[k] - (TX)rhot * (TX)dv[k];
+      const T rv = rv0[k] - rhot * adv[k];
+      r[i] = rv;
+      part += (double)rv * (double)rv;
+    }
+  }
+  for (int i = n0 + threadIdx.x + kPre * kThreads; i < n1; i += blockDim.x) {
     x[i] = x[i] - (TX)rhot * (TX)d[i];
     const T rv = r[i] - rhot * ad[i];
     r[i] = rv;
