@@ -346,6 +346,10 @@ struct NodeIn {
 #define BHIP_STAGE_UNROLL 4
 #endif
 constexpr int kStagePre = 1;
+#ifndef BHIP_STAGE_PRE_F32
+#define BHIP_STAGE_PRE_F32 1
+#endif
+constexpr int kStagePreF32 = BHIP_STAGE_PRE_F32;
 constexpr int kEbePart = 8;  // partial sums per thread loaded ahead in the CG kernels
 constexpr int kStageUnroll = BHIP_STAGE_UNROLL;
 
@@ -434,7 +438,7 @@ k_ebe(EbeArgs<T, TA> a) {
   // (the first kStagePre nodes of each thread) so that their HBM latency
   // overlaps the reduction of the previous kernel's partial sums
   constexpr bool kPre = STAGE == STAGE_CGDIR || STAGE == STAGE_CG1;
-  constexpr int KS = kPre ? kStagePre : 1;
+  constexpr int KS = kPre ? (sizeof(T) == 4 ? kStagePreF32 : kStagePre) : 1;
   NodeIn<T> pre[KS];
   auto load_node = [&](int l, NodeIn<T>& v) {
     const bool owned = l < nown;
