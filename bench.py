@@ -390,8 +390,8 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     torch model of the same equations (WaveModel: torch element-wise work
     around the HIP mat-vec and solve, autograd for the rest). Checks: central
     differences at 20 random nodes per parameter on the device model, and
-    the device gradient against a tangent-linear (forward-mode) run of the
-    torch model."""
+    the device gradient against an FP64 tangent-linear (forward-mode) run of
+    the torch model."""
     import torch
     import torch.autograd.forward_ad as fwAD
     from opentelemac_amd import bief, mesh as M, system as S, wave as W
@@ -463,10 +463,14 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
             fd = (jp - jm) / (2 * h)
             worst = max(worst, abs(float(g[i]) - fd) / scale)
         fd_err[name] = worst
+    # the tangent-linear reference: forward mode through the torch model in
+    # FP64 (FP64 CG to 1e-12)
+    tl_model = W.WaveModel(m, sea, dt=2.0, amp=0.2, period=600.0, precision=bief.PREC_FP64,
+                           eps=1e-12)
     dk = torch.tensor(rng.normal(size=m.npoin), dtype=torch.float64, device=dev)
     dz = torch.tensor(rng.normal(size=m.npoin) * 0.01, dtype=torch.float64, device=dev)
     with fwAD.dual_level():
-        jd = tmodel.run(fwAD.make_dual(k, dk), fwAD.make_dual(zb, dz), nsteps, gauges, obs)
+        jd = tl_model.run(fwAD.make_dual(k, dk), fwAD.make_dual(zb, dz), nsteps, gauges, obs)
         tl = float(fwAD.unpack_dual(jd).tangent)
     adj = float((gk * dk).sum() + (gz * dz).sum())
     return {"workload": "wave-equation adjoint, 500k nodes, 50 steps, checkpoints every 10",
