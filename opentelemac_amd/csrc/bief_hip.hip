@@ -85,7 +85,9 @@ struct SolveState {
                        // after the one that applied the last update
   int stop[2];         // single-kernel inner CG: set by the kernel of iteration M
                        // that applied the last update, (M << 2) | status, by parity
-  int pad4;
+  int dpar;            // reliable updates: parity of the direction the next inner
+                       // start continues from, -1 for a fresh start
+  int reliable;        // mixed path: inner CG continued across outer steps
 };
 
 // ---------------------------------------------------------------------------
@@ -866,7 +868,8 @@ __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int 
 // One workgroup per system; partial holds one sum per node block.
 __global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
                            const int32_t* __restrict__ sys_blocks, double eps, int nitmax,
-                           int mixed, double inner_eps, int inner_nitmax, SolveState* st0) {
+                           int mixed, double inner_eps, int inner_nitmax, int reliable,
+                           SolveState* st0) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
   const int sys = blockIdx.x;
@@ -884,6 +887,8 @@ __global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
   st->outer_count = 0;
   st->inner_total = 0;
   st->rr_outer = 0.0;
+  st->dpar = -1;
+  st->reliable = mixed && reliable;
   if (!mixed) {
     reset_cg(st, eps * eps * xl, 0, nitmax);
   } else {
@@ -914,16 +919,45 @@ __global__ void k_outer_check(const double* __restrict__ partial, int n, SolveSt
   }
   const double rr = reduce_partials(partial, n, sh_red, &sh_b);
   if (threadIdx.x != 0) return;
-  if (st->outer_count > 0) st->inner_total += st->final_m;
+  // a continued inner solve starts at M = 1 and counts one product fewer
+  if (st->outer_count > 0) st->inner_total += st->final_m - (st->dpar >= 0 ? 1 : 0);
   st->rr_outer = rr;
   if (rr <= st->eps2_outer) {
     st->outer_done = 1;
     st->done = ST_CONVERGED;
     return;
   }
+  // reliable updates: the next inner solve keeps the last direction D of
+  // this one, and its first BETA is the new residual's ||R||^2 over the
+  // ||R||^2 of the iteration that built that direction
+  // Iteration M of a solve that started at M0 (0 fresh, 1 continued) keeps
+  // its ||R||^2 and direction in slot (M - M0) & 1.
+  const int mf = st->final_m;
+  const int m0 = st->dpar >= 0 ? 1 : 0;
+  const int slot = (mf - 1 - m0) & 1;
+  const bool cont = st->reliable && st->outer_count > 0 && mf - 1 >= m0 &&
+                    st->done == ST_CONVERGED;
+  const double rprev = cont ? st->rr[slot] : 1.0;
   st->outer_count += 1;
   reset_cg(st, st->inner_eps2, 1, st->inner_nitmax);
   st->rr0 = rr;
+  st->dpar = -1;
+  if (cont) {
+    st->m[1] = 0;
+    st->rr[1] = rprev;
+    st->dpar = slot;
+  }
+}
+
+// the direction a continued inner solve starts from, copied to dst (the
+// start kernel reads it while writing d0, so it cannot stay in d0)
+__global__ void k_inner_dir(float* __restrict__ dst, const float* __restrict__ d0,
+                            const float* __restrict__ d1, int n, const SolveState* st) {
+  const int par = st->dpar;
+  if (par < 0 || st->outer_done) return;
+  const float* src = par ? d1 : d0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // mixed outer update x <- x - delta (owned nodes, in place), skipped once done
@@ -1125,6 +1159,7 @@ struct bhip_ctx {
   int max_part = 0;
   bhip::DevBuf<double> part4;    // single-kernel CG: 2 parities x [nblocks][4]
   bool inner_cg1 = false;        // mixed inner solver: single-kernel CG iterations
+  bool reliable = true;          // mixed inner CG continued across outer steps
   bhip::DevBuf<bhip::SolveState> st;  // one per system
   // ensembles of independent meshes (nsys > 1): the system of each node
   // block, the block range and node/element offsets of each system
@@ -1351,6 +1386,10 @@ static int ctx_setup(std::unique_ptr<bhip_ctx>& c, bhip_ctx** out) {
   // the faster) or BHIP_INNER_CG=1: one kernel per iteration
   c->inner_cg1 = false;
   if (const char* ev = getenv("BHIP_INNER_CG")) c->inner_cg1 = ev[0] == '1';
+  // reliable updates in the two-kernel inner CG; BHIP_RELIABLE=0 restarts
+  // every inner solve from D = R instead
+  c->reliable = true;
+  if (const char* ev = getenv("BHIP_RELIABLE")) c->reliable = ev[0] != '0';
   HIP_OK(c->part4.alloc((size_t)8 * (L.nblocks + 16)));
   hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(8 * (L.nblocks + 16))), dim3(kThreads), 0,
                      c->stream, c->part4.p, 0.0, (long)8 * (L.nblocks + 16));
@@ -1863,12 +1902,13 @@ static hipError_t enqueue_cg_start(bhip_ctx* c, T* r, T* d0, T* ad, const T* dz,
 }
 
 static hipError_t enqueue_bb_setup(bhip_ctx* c, const double* b, double eps, int nitmax,
-                                   int mixed, double inner_eps, int inner_nitmax) {
+                                   int mixed, double inner_eps, int inner_nitmax,
+                                   int reliable = 0) {
   hipLaunchKernelGGL(k_dot_blocks, dim3(c->L.nblocks), dim3(kThreads), 0, c->stream, b, b,
                      c->node_start.p, part_u(c, 1));
   hipLaunchKernelGGL(k_cg_setup, dim3(c->nsys), dim3(kThreads), 0, c->stream, part_u(c, 1),
                      c->L.nblocks, sys_ptr(c->sys_blocks), eps, nitmax, mixed, inner_eps,
-                     inner_nitmax, c->st.p);
+                     inner_nitmax, reliable, c->st.p);
   return hipGetLastError();
 }
 
@@ -2084,12 +2124,19 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     HIP_OK(hipGetLastError());
     return BHIP_OK;
   };
+  const bool reliable = !cg1 && c->reliable;
   auto inner_solve = [&](int nbat, bool raw) -> int {
-    if (cg1)
+    if (cg1) {
       HIP_OK(launch_cg1(c, v1, 0, 1, 1));
-    else
+    } else {
+      // a continued solve reads its previous direction from fdz (zero for a
+      // fresh start, where BETA = 0)
+      if (reliable)
+        hipLaunchKernelGGL(k_inner_dir, dim3(g), dim3(kThreads), 0, c->stream, c->fdz.p,
+                           c->fd0.p, c->fd1.p, n, c->st.p);
       HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
                                              c->da32.p, L.nblocks, 1)));
+    }
     return more(nbat, raw);
   };
   int& hint_in = c->iter_hint[1];   // iterations of one inner solve
@@ -2107,7 +2154,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, delta, 0.0f,
                        (long)n);
     HIP_OK(hipGetLastError());
-    HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax));
+    HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax, reliable ? 1 : 0));
     int r = outer_step();
     for (int k = 0; k < nout && !r; ++k) {
       if ((r = inner_solve(nbat, capturing))) break;
@@ -2116,7 +2163,8 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     return r;
   };
   const uint64_t key[6] = {
-      1u | (cg1 ? 2u : 0u) | (xa32_ready ? 4u : 0u) | ((uint64_t)(uint32_t)inner_nitmax << 8),
+      1u | (cg1 ? 2u : 0u) | (xa32_ready ? 4u : 0u) | (reliable ? 8u : 0u) |
+          ((uint64_t)(uint32_t)inner_nitmax << 8),
       (uint64_t)nbat | ((uint64_t)nout << 16) | ((uint64_t)(uint32_t)nitmax << 32),
       matrix_tag(c), dbits(eps), dbits(inner_eps), (uint64_t)(uintptr_t)x};
   if ((rc = run_planned(c, key, plan))) return rc;
