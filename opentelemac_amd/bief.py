@@ -341,10 +341,17 @@ class Context:
         self.typdia, self.typext = "Q", "S"
 
     # ---- MATVEC ---------------------------------------------------------
+    def _on_own_stream(self):
+        import torch
+        return torch.cuda.current_stream().cuda_stream == self.stream_ptr()
+
     def _sync_torch(self):
         """Orders the context stream after the work queued so far on torch's
-        current stream (an event wait on the device, no host round trip)."""
+        current stream (an event wait on the device, no host round trip);
+        nothing to do when torch already runs on the context's stream."""
         import torch
+        if self._on_own_stream():
+            return
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream())
         self.torch_stream().wait_event(ev)
@@ -352,6 +359,8 @@ class Context:
     def _join_torch(self):
         """Orders torch's current stream after the context stream's work."""
         import torch
+        if self._on_own_stream():
+            return
         ev = torch.cuda.Event()
         ev.record(self.torch_stream())
         torch.cuda.current_stream().wait_event(ev)

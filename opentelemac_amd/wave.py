@@ -438,6 +438,10 @@ class DeviceWaveModel:
         self.guess = {}
         self.n_solves = 0
         self.n_iters = 0
+        # torch's element-wise work runs on the context's own stream, so the
+        # library calls need no event handshakes (inputs and results are
+        # ordered with the caller's stream at the public methods)
+        self.stream = self.ctx.torch_stream()
 
     # -- node order ----------------------------------------------------------
     def to_internal(self, t):
@@ -450,6 +454,10 @@ class DeviceWaveModel:
 
     def _zeros(self):
         return torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+
+    def _out(self):
+        """A node vector that a kernel writes in full (no fill needed)."""
+        return torch.empty(self.n, dtype=torch.float64, device=self.dev)
 
     def tide(self, t):
         return self.amp * np.sin(2.0 * np.pi * t / self.period)
@@ -479,15 +487,38 @@ class DeviceWaveModel:
     def step(self, eta, u, v, k_str, zb, t_new):
         """(eta, u, v) -> (eta', u', v', deta), internal order."""
         p = self._params(t_new)
-        rhs = self._zeros()
+        rhs = self._out()
         self.kern.assemble(p, eta, u, v, k_str, zb, rhs)
         deta = self._solve(rhs, "forward")
-        eta_n, u_n, v_n = self._zeros(), self._zeros(), self._zeros()
+        eta_n, u_n, v_n = self._out(), self._out(), self._out()
         self.kern.momentum(p, eta, deta, u, v, eta_n, u_n, v_n)
         return eta_n, u_n, v_n, deta
 
+    def _on_stream(self, fn, *args):
+        """fn(*args) on the context's stream, ordered after the caller's
+        stream's work and before what the caller queues next; returned
+        tensors are marked as used by the caller's stream."""
+        caller = torch.cuda.current_stream()
+        self.stream.wait_stream(caller)
+        with torch.cuda.stream(self.stream):
+            out = fn(*args)
+        caller.wait_stream(self.stream)
+        stack = [out]
+        while stack:
+            o = stack.pop()
+            if torch.is_tensor(o):
+                o.record_stream(caller)
+            elif isinstance(o, (tuple, list)):
+                stack.extend(o)
+            elif isinstance(o, dict):
+                stack.extend(o.values())
+        return out
+
     def run(self, k_str, zb, nsteps, gauges=None, obs=None):
         """As WaveModel.run (caller-order inputs, gauges as caller node ids)."""
+        return self._on_stream(self._run, k_str, zb, nsteps, gauges, obs)
+
+    def _run(self, k_str, zb, nsteps, gauges, obs):
         self.guess = {}
         k, z = self.to_internal(k_str), self.to_internal(zb)
         g = self.iperm[gauges] if gauges is not None else None
@@ -510,6 +541,9 @@ class DeviceWaveModel:
         """J = sum over steps and gauges of (eta - obs)^2, and the tape of the
         reverse sweep: the state every `checkpoint` steps and the states and
         increments of the last segment."""
+        return self._on_stream(self._forward_sweep, k_str, zb, nsteps, gauges, obs, checkpoint)
+
+    def _forward_sweep(self, k_str, zb, nsteps, gauges, obs, checkpoint):
         self.guess = {}
         k, z = self.to_internal(k_str), self.to_internal(zb)
         g = self.iperm[gauges]
@@ -536,6 +570,9 @@ class DeviceWaveModel:
 
     def reverse_sweep(self, tape, gcost=1.0):
         """(dJ/dK, dJ/dzb) times gcost, caller order, from forward_sweep's tape."""
+        return self._on_stream(self._reverse_sweep, tape, gcost)
+
+    def _reverse_sweep(self, tape, gcost):
         k, z, g, obs = tape["k"], tape["z"], tape["g"], tape["obs"]
         nsteps, checkpoint = tape["nsteps"], tape["checkpoint"]
         kbar, zbar = self._zeros(), self._zeros()
@@ -558,11 +595,11 @@ class DeviceWaveModel:
                 ebar_n = lam_e.clone()
                 ebar_n.index_add_(0, g, (2.0 * gcost) * (e1[g] - obs[n]))
                 # the step's system again, then the transposed step
-                self.kern.assemble(p, e0, u0, v0, k, z, self._zeros())
-                etot = self._zeros()
+                self.kern.assemble(p, e0, u0, v0, k, z, self._out())
+                etot = self._out()
                 self.kern.momentum_adj(p, lam_u, lam_v, ebar_n, etot)
                 lam = self._solve(etot, "adjoint")
-                eb, ub, vb = self._zeros(), self._zeros(), self._zeros()
+                eb, ub, vb = self._out(), self._out(), self._out()
                 self.kern.assemble_adj(p, lam, detas[n - s0], e0, u0, v0, u1, v1, lam_u, lam_v,
                                        etot, k, eb, ub, vb, kbar, zbar)
                 lam_e, lam_u, lam_v = eb, ub, vb
