@@ -195,7 +195,43 @@ def main():
         results[name]["ms_upd_kernel"] = ms_upd
         its = info["niter"]
         results[name]["cg_iters_per_s"] = its * args.steps / el
+        # per-solve latency: each solve on its own, synchronised
+        lat = []
+        for _ in range(10):
+            t1 = time.perf_counter()
+            step(prec)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t1)
+        results[name]["ms_per_solve_median"] = 1e3 * float(np.median(lat))
+        results[name]["ms_per_solve_max"] = 1e3 * float(np.max(lat))
         log(name, json.dumps(results[name]))
+
+    # per-iteration algorithmic bytes of CG (matrix and connectivity once, ten
+    # vector touches per node) and the rate they imply
+    ne, nprime = m.nelem, m.npoin
+    b_it = {"fp64": ne * (3 * 8 + 12) + nprime * 10 * 8,
+            "mixed": ne * (3 * 4 + 12) + nprime * 10 * 4}
+    for name in ("fp64", "mixed"):
+        results[name]["algorithmic_bytes_per_cg_iteration"] = b_it[name]
+        results[name]["algorithmic_GBps_cg"] = results[name]["cg_iters_per_s"] * b_it[name] / 1e9
+
+    # the host-pointer path (what the Fortran SOLVE does each call): upload
+    # A, X, B from pageable host memory, solve in FP64, copy X and the
+    # scaled B back
+    da_h, xa_h = sysd["da"], np.ascontiguousarray(sysd["xa"])
+    hl = []
+    for k in range(4):
+        t1 = time.perf_counter()
+        ctx.set_matrix("Q", "S", da_h, xa_h)
+        xh, hinfo = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-8,
+                              nitmax=5000, precision=bief.PREC_FP64)
+        if k:
+            hl.append(time.perf_counter() - t1)
+    host_path = {"ms_per_solve_median": 1e3 * float(np.median(hl)), "iterations": hinfo["niter"],
+                 "bytes_host_to_device": int(da_h.nbytes + xa_h.nbytes + 2 * sysd["b"].nbytes),
+                 "bytes_device_to_host": int(2 * sysd["b"].nbytes),
+                 "note": "bhip_matrix_set + bhip_solve from pageable numpy arrays, FP64"}
+    log("host path", json.dumps(host_path))
 
     # true residual ||b - A x|| / ||b|| of both solutions (FP64 mat-vec)
     ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
@@ -206,7 +242,6 @@ def main():
     rel_mixed_vs_fp64 = float(np.linalg.norm(sols["mixed"] - sols["fp64"])
                               / np.linalg.norm(sols["fp64"]))
 
-    ne, nprime = m.nelem, m.npoin
     # algorithmic bytes per launch of the dominant kernels (int32
     # connectivity, 3 coefficients per element, node vectors):
     #  * FP64 CG direction + EBE mat-vec + D.AD: R, D_old read; D, AD written
@@ -233,7 +268,8 @@ def main():
                 "avg_launch_ms": results["mixed"]["ms_dir_kernel"],
                 "fp64_kernel": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                 "algorithmic_bytes_per_launch": bytes_f64,
-                                "avg_launch_ms": results["fp64"]["ms_dir_kernel"]}}
+                                "avg_launch_ms": results["fp64"]["ms_dir_kernel"]},
+                "copy_same_bytes": _copy_rate(dev, bytes_f32)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -243,6 +279,8 @@ def main():
                                                   / np.linalg.norm(xcpu))
             cpu["gpu_mixed_rel_l2_vs_cpu"] = float(np.linalg.norm(sols["mixed"] - xcpu)
                                                    / np.linalg.norm(xcpu))
+            cpu["cg_iters_per_s"] = cpu["value"] * cpu["iterations"]
+            cpu["algorithmic_GBps_cg"] = cpu["cg_iters_per_s"] * b_it["fp64"] / 1e9
             cpu["host_cpu"] = _cpu_model()
             cpu["nproc"] = os.cpu_count()
         except Exception as e:  # reported, never silently substituted
@@ -270,11 +308,32 @@ def main():
         "mixed": mixed,
         "mixed_speedup_vs_fp64": round(results["fp64"]["ms_per_solve"] / mixed["ms_per_solve"], 3),
         "mixed_vs_fp64_rel_l2": rel_mixed_vs_fp64,
+        "host_pointer_path": host_path,
         "layout": st,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def _copy_rate(dev, nbytes, reps=50):
+    """A plain device copy moving nbytes in all (nbytes / 2 read, nbytes / 2
+    written), timed with events over reps launches: the practical rate of
+    that much traffic on this GPU, beside the kernel's."""
+    import torch
+    src = torch.ones(max(1, nbytes // 16), dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * src.numel() * 8
+    return {"bytes": moved, "avg_launch_ms": ms, "GBps": round(moved / (ms * 1e-3) / 1e9, 1)}
 
 
 def bench_prisms(args, dev):

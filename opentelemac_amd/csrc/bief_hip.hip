@@ -35,6 +35,7 @@
 namespace bhip {
 
 constexpr int kThreads = 256;
+static_assert(kThreads == kPhaseSlots, "colour phases are balanced for this workgroup size");
 constexpr int kWaves = kThreads / 64;
 
 #define HIP_OK(expr)                                              \

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -122,6 +123,8 @@ std::string build_layout(int npoin, int nelem, int ndp, const int32_t* ikle,
   slot_elem.reserve((size_t)(nelem * 1.3) + 16);
   for (auto& c : lid_cols) c.reserve((size_t)(nelem * 1.3) + 16);
 
+  const char* bal_env = getenv("BHIP_COLOUR_BALANCE");
+  const bool balance = !(bal_env && bal_env[0] == '0');
   for (int b = 0; b < L.nblocks; ++b) {
     const int n0 = L.node_start[b], n1 = L.node_start[b + 1];
     const int nown = n1 - n0;
@@ -179,6 +182,42 @@ std::string build_layout(int npoin, int nelem, int ndp, const int32_t* ikle,
       for (int k = 0; k < ndp; ++k) {
         int l = local_of(L.iperm[ikle[(long)k * nelem + e]]);
         if (l < nown) owned_mask[l] |= (1ull << c);
+      }
+    }
+    // balance: first-fit fills the first colours beyond one slot per thread
+    // of the workgroup, and a phase's slots past that count are loaded
+    // outside the kernel's prefetch. Move elements out of such colours into
+    // a colour with room that none of their owned nodes uses (each owned
+    // node has at most one element per colour, so its mask bit is that
+    // element's alone). BHIP_COLOUR_BALANCE=0 keeps first-fit as it is.
+    if (balance) {
+      std::vector<int> ccount(ncol, 0);
+      for (size_t s = 0; s < belems.size(); ++s) ccount[bcolor[s]]++;
+      for (int pass = 0; pass < 4; ++pass) {
+        bool over = false;
+        for (size_t s = 0; s < belems.size(); ++s) {
+          const int o = bcolor[s];
+          if (ccount[o] <= kPhaseSlots) continue;
+          const int e = belems[s];
+          int lo[8];
+          uint64_t used = 0;
+          for (int k = 0; k < ndp; ++k) {
+            lo[k] = local_of(L.iperm[ikle[(long)k * nelem + e]]);
+            if (lo[k] < nown) used |= owned_mask[lo[k]];
+          }
+          int c = 0;
+          while (c < ncol && (c == o || ccount[c] >= kPhaseSlots || (used >> c & 1))) ++c;
+          if (c == ncol) {
+            over = true;
+            continue;
+          }
+          for (int k = 0; k < ndp; ++k)
+            if (lo[k] < nown) owned_mask[lo[k]] = (owned_mask[lo[k]] & ~(1ull << o)) | (1ull << c);
+          bcolor[s] = c;
+          ccount[o]--;
+          ccount[c]++;
+        }
+        if (!over) break;
       }
     }
     // slots sorted by colour (stable: keeps the locality order inside)

@@ -18,6 +18,9 @@
 namespace bhip {
 
 constexpr int kMaxColors = 48;
+// threads of an element-kernel workgroup: one colour phase holds at most
+// this many element instances where the colouring allows it
+constexpr int kPhaseSlots = 256;
 
 struct Layout {
   int npoin = 0, nelem = 0, ndp = 3, nplan = 1;
