@@ -205,9 +205,21 @@ int bhip_dirich(bhip_ctx* ctx, double* f, double* sm, const double* fbor,
 /* LUMP: DIAG = A * (XMUL everywhere) (sources/utils/bief/lump.f) */
 int bhip_lump(bhip_ctx* ctx, double xmul, double* diag);
 
+/* The mesh's element geometry, kept on the device in the context's element
+ * order: XEL, YEL [3*nelem] (node 1 at the origin) and SURFAC [nelem], as
+ * INBIEF computes them once per mesh (inbief.f:384-397, for MESH%XEL,
+ * MESH%YEL, MESH%SURFAC). Afterwards the element-matrix entries below take
+ * xel = yel = surfac = NULL to use it: no per-call upload, and slot-ordered
+ * reads in their kernels. Triangles and prisms (the base triangle's
+ * values per prism). A later call replaces it. */
+int bhip_ctx_set_geometry(bhip_ctx* ctx, const double* xel, const double* yel,
+                          const double* surfac);
+
 /* Element matrices on the device, assembled into the context's matrix as
  * BIEF's MATRIX does (matrix.f: element matrix, diagonal by ASSVEC, then
- * OM 'M=N' or 'M=M+N'). accumulate = 0: 'M=N'; 1: 'M=M+N'.
+ * OM 'M=N' or 'M=M+N'). accumulate = 0: 'M=N'; 1: 'M=M+N'. xel, yel and
+ * surfac all NULL: the geometry of bhip_ctx_set_geometry (here and in the
+ * prism and SUPG entries).
  *  - MATMAS, MT01AA (mt01aa.f:83-106): mass, xmul*S/12 (off-diag) and
  *    xmul*S/6 (diagonal); surfac[nelem]
  *  - MATDIFUV, MT02AA 'UV' (mt02aa.f:200-240): diffusion with the P1

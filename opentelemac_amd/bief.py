@@ -112,6 +112,7 @@ def lib():
         L.bhip_ctx_perm.argtypes = [_P, _P]
         L.bhip_matrix_supg.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I]
         L.bhip_matrix_om.argtypes = [_P, ctypes.c_char_p, _D]
+        L.bhip_ctx_set_geometry.argtypes = [_P, _P, _P, _P]
         _W = ctypes.POINTER(WaveParams)
         L.bhip_wave_setup.argtypes = [_P, _P, _P, _P, _P]
         L.bhip_wave_assemble.argtypes = [_P, _W] + [_P] * 6
@@ -134,6 +135,14 @@ def _is_torch(a):
 
 def _host_f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _host_or_none(a):
+    return None if a is None else _host_f64(a)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
 
 
 def layout_check(mesh, block_nodes=0):
@@ -273,25 +282,31 @@ class Context:
         _check(lib().bhip_lump(self._h, float(xmul), d.ctypes.data), "bhip_lump")
         return d
 
+    def set_geometry(self, xel, yel, surfac):
+        """Keeps the mesh's XEL, YEL, SURFAC on the device (what INBIEF
+        computes once, inbief.f:384-397); the matrix_* methods then take
+        xel = yel = surfac = None."""
+        arrs = [_host_f64(a) for a in (xel, yel, surfac)]
+        _check(lib().bhip_ctx_set_geometry(self._h, *(a.ctypes.data for a in arrs)),
+               "bhip_ctx_set_geometry")
+        self.has_geometry = True
+
     def matrix_mt01aa(self, xmul, surfac, accumulate=False):
-        """MATRIX(... 'MATMAS' ...) with OM 'M=N' / 'M=M+N' (mt01aa.f)."""
-        s = _host_f64(surfac)
-        _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), s.ctypes.data, int(accumulate)),
+        """MATRIX(... 'MATMAS' ...) with OM 'M=N' / 'M=M+N' (mt01aa.f).
+        surfac None: the geometry of set_geometry."""
+        s = _host_or_none(surfac)
+        _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), _ptr(s), int(accumulate)),
                "bhip_matrix_mt01aa")
         self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     def matrix_mt02aa(self, kind, xmul, xel, yel, surfac, u, v=None, accumulate=False):
         """MATRIX(... 'MATDIF' / 'MATDIFUV' ...): kind 1 P1 isotropic
         viscosity U, kind 2 coefficient U*V (mt02aa.f)."""
-        xel = _host_f64(xel)
-        yel = _host_f64(yel)
-        s = _host_f64(surfac)
+        xel, yel, s, v = (_host_or_none(a) for a in (xel, yel, surfac, v))
         u = _host_f64(u)
-        v = _host_f64(v) if v is not None else None
-        _check(lib().bhip_matrix_mt02aa(self._h, int(kind), float(xmul), xel.ctypes.data,
-                                        yel.ctypes.data, s.ctypes.data, u.ctypes.data,
-                                        v.ctypes.data if v is not None else None,
-                                        int(accumulate)), "bhip_matrix_mt02aa")
+        _check(lib().bhip_matrix_mt02aa(self._h, int(kind), float(xmul), _ptr(xel), _ptr(yel),
+                                        _ptr(s), u.ctypes.data, _ptr(v), int(accumulate)),
+               "bhip_matrix_mt02aa")
         self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     SUPG_U_P0, SUPG_UPWIND, SUPG_KSUPG = 1, 2, 4
@@ -302,13 +317,12 @@ class Context:
         'MATVGR' (5, MT05AA) ...) with OM op 'M=N', 'M=M+N' or 'M=M+TN'.
         flags: SUPG_U_P0 (u, v per element), SUPG_UPWIND (MATVGR's 'N'
         variant), SUPG_KSUPG (the element vectors from KSUPG(1, u, v))."""
-        arrs = [_host_f64(a) for a in (xel, yel, surfac, u, v)]
-        fg = [_host_f64(a) if a is not None else None for a in (f, g)]
-        _check(lib().bhip_matrix_supg(self._h, int(kind), float(xmul),
-                                      *(a.ctypes.data for a in arrs[:3]),
-                                      *(a.ctypes.data if a is not None else None for a in fg),
-                                      arrs[3].ctypes.data, arrs[4].ctypes.data, int(flags),
-                                      self._OM_OPS[op]), "bhip_matrix_supg")
+        geo = [_host_or_none(a) for a in (xel, yel, surfac)]
+        fg = [_host_or_none(a) for a in (f, g)]
+        u, v = _host_f64(u), _host_f64(v)
+        _check(lib().bhip_matrix_supg(self._h, int(kind), float(xmul), *(_ptr(a) for a in geo),
+                                      *(_ptr(a) for a in fg), u.ctypes.data, v.ctypes.data,
+                                      int(flags), self._OM_OPS[op]), "bhip_matrix_supg")
         sym = kind == 4 and (op == "M=N" or (op == "M=M+N" and self.typext == "S"))
         self.typdia, self.typext = "Q", ("S" if sym else "Q")
 
@@ -323,9 +337,9 @@ class Context:
     def matrix_mt01pp(self, xmul, surfac, z, accumulate=False):
         """MATRIX(... 'MATMAS' ...) on prisms (mt01pp.f): surfac is the base
         triangle area of each prism, z the node elevations."""
-        s = _host_f64(surfac)
+        s = _host_or_none(surfac)
         z = _host_f64(z)
-        _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), s.ctypes.data, z.ctypes.data,
+        _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), _ptr(s), z.ctypes.data,
                                         int(accumulate)), "bhip_matrix_mt01pp")
         self.typdia, self.typext = "Q", "S"
 
@@ -334,8 +348,9 @@ class Context:
         """MATRIX(... 'MATDIF' ...) on prisms (mt02pp.f): diagonal diffusion
         tensor (f, g, h) per node; npou0 = NPOIN2 (P1 vertical viscosity) or
         0 (P0 on the vertical)."""
-        arrs = [_host_f64(a) for a in (xel, yel, surfac, z, f, g, h)]
-        _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[a.ctypes.data for a in arrs],
+        arrs = [_host_or_none(a) for a in (xel, yel, surfac)] + \
+            [_host_f64(a) for a in (z, f, g, h)]
+        _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[_ptr(a) for a in arrs],
                                         int(h_zero), int(npou0), int(inchyd), int(mon),
                                         int(accumulate)), "bhip_matrix_mt02pp")
         self.typdia, self.typext = "Q", "S"

@@ -1117,10 +1117,16 @@ __global__ void k_matrix_from_slots(double* __restrict__ dst, const double* __re
 // ---------------------------------------------------------------------------
 // host side: context
 // ---------------------------------------------------------------------------
+// Device buffer owned by its holder: freed by release() or when it goes out
+// of scope (temporaries of an entry point, whose work has been waited for).
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   hipError_t alloc(size_t count) {
     if (p && n >= count) return hipSuccess;
     if (p) (void)hipFree(p);
@@ -1159,6 +1165,8 @@ struct bhip_ctx {
   bhip::DevBuf<double> partial;  // 4 arrays of max_part: U[0], U[1], E[0], E[1]
   int max_part = 0;
   bhip::DevBuf<double> part4;    // single-kernel CG: 2 parities x [nblocks][4]
+  bhip::DevBuf<double> geo;      // element geometry per slot (bhip_ctx_set_geometry)
+  bool have_geo = false;
   bool inner_cg1 = false;        // mixed inner solver: single-kernel CG iterations
   bool reliable = true;          // mixed inner CG continued across outer steps
   bhip::DevBuf<bhip::SolveState> st;  // one per system
@@ -1482,6 +1490,7 @@ int bhip_ctx_destroy(bhip_ctx* c) {
                   &c->fdz, &c->fr1, &c->fad1})
     v->release();
   c->part4.release();
+  c->geo.release();
   c->st.release();
   for (auto* v : {&c->wv_geo, &c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_c, &c->wv_qx, &c->wv_qy,
                   &c->wv_y, &c->wv_da})
@@ -2462,6 +2471,55 @@ __global__ void k_gather_any(T* __restrict__ out, const T* __restrict__ in,
     out[i] = in[idx[i]];
 }
 
+// Element geometry of slot s: XEL, YEL of nodes 2 and 3 (node 1 at the
+// origin, inbief.f:384-397) and SURFAC. From the context's slot-ordered copy
+// (geo: [5][nslot_pad], one coalesced read per value) when it is given, else
+// from the caller's arrays by element e (a gather per value).
+struct SlotGeo {
+  double x2, x3, y2, y3, s;
+};
+__device__ inline SlotGeo load_geo(const double* __restrict__ geo, size_t np, int s,
+                                   const double* __restrict__ xel,
+                                   const double* __restrict__ yel,
+                                   const double* __restrict__ surfac, size_t ne, int e,
+                                   bool xy) {
+  SlotGeo g = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (geo) {
+    g.s = geo[4 * np + s];
+    if (xy) {
+      g.x2 = geo[s];
+      g.x3 = geo[np + s];
+      g.y2 = geo[2 * np + s];
+      g.y3 = geo[3 * np + s];
+    }
+  } else {
+    g.s = surfac[e];
+    if (xy) {
+      g.x2 = xel[ne + e];
+      g.x3 = xel[2 * ne + e];
+      g.y2 = yel[ne + e];
+      g.y3 = yel[2 * ne + e];
+    }
+  }
+  return g;
+}
+
+// geo[c][s] for c = XEL2, XEL3, YEL2, YEL3, SURFAC of slot s's element
+__global__ void k_geo_slots(double* __restrict__ geo, const double* __restrict__ xel,
+                            const double* __restrict__ yel, const double* __restrict__ surfac,
+                            const int32_t* __restrict__ slot_elem, int nslot, int nslot_pad,
+                            int nelem) {
+  const size_t np = nslot_pad, ne = nelem;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslot; s += gridDim.x * blockDim.x) {
+    const int e = slot_elem[s];
+    geo[s] = xel[ne + e];
+    geo[np + s] = xel[2 * ne + e];
+    geo[2 * np + s] = yel[ne + e];
+    geo[3 * np + s] = yel[2 * ne + e];
+    geo[4 * np + s] = surfac[e];
+  }
+}
+
 // Element matrices per slot (each element instance computes the same values)
 // and the assembly of their diagonal terms onto owned nodes in LDS, colour
 // by colour (same deterministic scheme as the mat-vec).
@@ -2477,7 +2535,8 @@ k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
            int nslot_pad, const int32_t* __restrict__ node_start,
            const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
            const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
-           double* __restrict__ da, int accumulate, int target_q) {
+           double* __restrict__ da, int accumulate, int target_q,
+           const double* __restrict__ geo) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* acc = reinterpret_cast<double*>(smem_raw);
   const int b = blockIdx.x;
@@ -2500,17 +2559,18 @@ k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
         l[k] = lid[(size_t)k * nslot_pad + s];
         g[k] = l[k] < nown ? n0 + l[k] : halo[h0 + l[k] - nown];
       }
+      const SlotGeo G = load_geo(geo, nslot_pad, s, xel, yel, surfac, nelem, e, kind != 100);
       double a12, a13, a23, t1, t2, t3;
       if (kind == 100) {
-        const double det = surfac[e] * (xmul / 12.0);
+        const double det = G.s * (xmul / 12.0);
         a12 = a13 = a23 = det;
         t1 = t2 = t3 = det + det;
       } else {
-        const double x2 = xel[(size_t)nelem + e], x3 = xel[2 * (size_t)nelem + e];
-        const double y2 = yel[(size_t)nelem + e], y3 = yel[2 * (size_t)nelem + e];
+        const double x2 = G.x2, x3 = G.x3;
+        const double y2 = G.y2, y3 = G.y3;
         if (kind == 2) {
           const double g1 = v_int[g[0]], g2 = v_int[g[1]], g3 = v_int[g[2]];
-          const double coef1 = (xmul / 48.0) / surfac[e];
+          const double coef1 = (xmul / 48.0) / G.s;
           const double g123 = g1 + g2 + g3;
           const double coef2 = u_int[g[0]] * (g1 + g123) + u_int[g[1]] * (g2 + g123) +
                                u_int[g[2]] * (g3 + g123);
@@ -2519,7 +2579,7 @@ k_elem_tri(int kind, double xmul, const double* __restrict__ surfac,
           a23 = -(y2 * y3 + x2 * x3) * coef2 * coef1;
         } else {
           const double somvx = (u_int[g[0]] + u_int[g[1]] + u_int[g[2]]) * (xmul / 12.0) /
-                               surfac[e];
+                               G.s;
           const double x2x3 = x2 * x3 + y2 * y3;
           const double x2aux = x2 * (-x2 + x3) + y2 * (-y2 + y3);
           const double x3aux = x3 * (-x2 + x3) + y3 * (-y2 + y3);
@@ -2637,7 +2697,7 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
              int nslot_pad, const int32_t* __restrict__ node_start,
              const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
              const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
-             double* __restrict__ da, int accumulate) {
+             double* __restrict__ da, int accumulate, const double* __restrict__ geo) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* acc = reinterpret_cast<double*>(smem_raw);
   const int b = blockIdx.x;
@@ -2662,9 +2722,10 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
         g[k] = l[k] < nown ? n0 + l[k] : halo[h0 + l[k] - nown];
         zz[k] = z[g[k]];
       }
+      const SlotGeo G = load_geo(geo, nslot_pad, s, xel, yel, surfac, nelem, e, kind != 200);
       double cf[15], t[6];
       if (kind == 200) {
-        const double coef = surfac[e] * (xmul / 360.0);
+        const double coef = G.s * (xmul / 360.0);
         const double h1 = fmax(zz[3] - zz[0], 1e-3) * coef;
         const double h2 = fmax(zz[4] - zz[1], 1e-3) * coef;
         const double h3 = fmax(zz[5] - zz[2], 1e-3) * coef;
@@ -2694,12 +2755,12 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) nh[k] = dry ? 0.0 : (h[g[k]] + h_up[g[k]]) / hh[k];
-        const double dd = (xmul / 2880.0) / surfac[e];
-        const double x2 = xel[(size_t)nelem + e], x3 = xel[2 * (size_t)nelem + e];
-        const double y2 = yel[(size_t)nelem + e], y3 = yel[2 * (size_t)nelem + e];
+        const double dd = (xmul / 2880.0) / G.s;
+        const double x2 = G.x2, x3 = G.x3;
+        const double y2 = G.y2, y3 = G.y3;
         mt02pp_dir(y2 - y3, y3, -y2, nf, hh, zz, dd, cf, false);
         mt02pp_dir(x3 - x2, -x3, x2, ng, hh, zz, dd, cf, true);
-        const double dz = h_zero ? 0.0 : surfac[e] * (xmul / 6.0);
+        const double dz = h_zero ? 0.0 : G.s * (xmul / 6.0);
         cf[2] = cf[2] - dz * nh[0];
         cf[7] = cf[7] - dz * nh[1];
         cf[11] = cf[11] - dz * nh[2];
@@ -2806,8 +2867,12 @@ int bhip_lump(bhip_ctx* c, double xmul, double* diag) {
 static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
                        const double* surfac, const double* u, const double* v,
                        int accumulate) {
-  if (!c || !surfac || c->L.ndp != 3) return BHIP_EBADARG;
-  if (kind != 100 && (!xel || !yel || !u)) return BHIP_EBADARG;
+  if (!c || c->L.ndp != 3) return BHIP_EBADARG;
+  // XEL, YEL, SURFAC all NULL: the context's geometry (bhip_ctx_set_geometry)
+  const bool use_geo = !surfac && !xel && !yel;
+  if (use_geo ? !c->have_geo : (!surfac || (kind != 100 && (!xel || !yel))))
+    return BHIP_EBADARG;
+  if (kind != 100 && !u) return BHIP_EBADARG;
   if (kind == 2 && !v) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin, ne = L.nelem;
@@ -2819,15 +2884,21 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
     HIP_OK(c->xa.alloc((size_t)6 * L.nslot_pad));
   }
   DevBuf<double> dS, dX, dY, dU, dV;
-  HIP_OK(dS.alloc(ne));
-  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  if (!use_geo) {
+    HIP_OK(dS.alloc(ne));
+    HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  }
   if (kind != 100) {
-    HIP_OK(dX.alloc((size_t)3 * ne));
-    HIP_OK(dY.alloc((size_t)3 * ne));
+    if (!use_geo) {
+      HIP_OK(dX.alloc((size_t)3 * ne));
+      HIP_OK(dY.alloc((size_t)3 * ne));
+      HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
+                            c->stream));
+      HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
+                            c->stream));
+    }
     HIP_OK(dU.alloc(n));
     HIP_OK(dV.alloc(n));
-    HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(c->vio0.p, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dU.p,
                        c->vio0.p, c->perm.p, n);
@@ -2842,7 +2913,8 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
   hipLaunchKernelGGL(k_elem_tri, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
                      dS.p, dX.p, dY.p, ne, dU.p, v ? dV.p : dU.p, c->slot_elem.p, c->lid.p,
                      L.nslot_pad, c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
-                     L.nblocks, c->xa.p, c->da.p, accumulate, target_q);
+                     L.nblocks, c->xa.p, c->da.p, accumulate, target_q,
+                     use_geo ? c->geo.p : nullptr);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->stream));
   // MATRIY sets TYPDIA 'Q', TYPEXT 'S' (matriy.f:375-376); 'M=M+N' of two
@@ -2859,8 +2931,12 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
                       const double* surfac, const double* z, const double* f, const double* g,
                       const double* h, int h_zero, int npou0, int inchyd, int mon,
                       int accumulate) {
-  if (!c || !surfac || !z || c->L.ndp != 6) return BHIP_EBADARG;
-  if (kind == 201 && (!xel || !yel || !f || !g || !h)) return BHIP_EBADARG;
+  if (!c || !z || c->L.ndp != 6) return BHIP_EBADARG;
+  // XEL, YEL, SURFAC all NULL: the context's geometry (bhip_ctx_set_geometry)
+  const bool use_geo = !surfac && !xel && !yel;
+  if (use_geo ? !c->have_geo : (!surfac || (kind == 201 && (!xel || !yel))))
+    return BHIP_EBADARG;
+  if (kind == 201 && (!f || !g || !h)) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin, ne = L.nelem;
   if (npou0 < 0 || npou0 >= n) return BHIP_EBADARG;
@@ -2883,15 +2959,21 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
     HIP_OK(hipStreamSynchronize(c->stream));
     return BHIP_OK;
   };
-  HIP_OK(dS.alloc(ne));
-  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  if (!use_geo) {
+    HIP_OK(dS.alloc(ne));
+    HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  }
   int rc = up_nodes(dZ, z);
   if (rc) return rc;
   if (kind == 201) {
-    HIP_OK(dX.alloc((size_t)3 * ne));
-    HIP_OK(dY.alloc((size_t)3 * ne));
-    HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+    if (!use_geo) {
+      HIP_OK(dX.alloc((size_t)3 * ne));
+      HIP_OK(dY.alloc((size_t)3 * ne));
+      HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
+                            c->stream));
+      HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
+                            c->stream));
+    }
     // H at node i + NPOU0 (mt02pp.f:233-235: II4 = I1 + NPOU0); the nodes of
     // the top plane are never a prism's lower node
     hup.assign(n, 0.0);
@@ -2904,13 +2986,35 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
   hipLaunchKernelGGL(k_elem_prism, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
                      dS.p, dX.p, dY.p, ne, dZ.p, dF.p, dG.p, dH.p, dHu.p, h_zero, inchyd, mon,
                      c->slot_elem.p, c->lid.p, L.nslot_pad, c->node_start.p, c->halo_start.p,
-                     c->halo.p, c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate);
+                     c->halo.p, c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate,
+                     use_geo ? c->geo.p : nullptr);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->stream));
   c->typdia = 'Q';
   c->typext = 'S';
   c->ncoef = 15;
   c->have_matrix = true;
+  return BHIP_OK;
+}
+
+int bhip_ctx_set_geometry(bhip_ctx* c, const double* xel, const double* yel,
+                          const double* surfac) {
+  if (!c || !xel || !yel || !surfac || (c->L.ndp != 3 && c->L.ndp != 6)) return BHIP_EBADARG;
+  const Layout& L = c->L;
+  const int ne = L.nelem;
+  DevBuf<double> dS, dX, dY;
+  HIP_OK(dS.alloc(ne));
+  HIP_OK(dX.alloc((size_t)3 * ne));
+  HIP_OK(dY.alloc((size_t)3 * ne));
+  HIP_OK(c->geo.alloc((size_t)5 * L.nslot_pad));
+  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_geo_slots, dim3(grid_for(L.nslot)), dim3(kThreads), 0, c->stream,
+                     c->geo.p, dX.p, dY.p, dS.p, c->slot_elem.p, L.nslot, L.nslot_pad, ne);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->have_geo = true;
   return BHIP_OK;
 }
 

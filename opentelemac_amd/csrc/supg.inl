@@ -59,6 +59,7 @@ struct SupgArgs {
   const double* surfac;  // per element (caller order)
   const double* xel;     // [3][nelem]
   const double* yel;
+  const double* geo;     // or the context's slot-ordered geometry (then the three above unused)
   int nelem;
   const double* f;  // per element, MT03AA (unless SUPG_KSUPG)
   const double* g;
@@ -101,8 +102,9 @@ __global__ void __launch_bounds__(kThreads) k_elem_supg(SupgArgs a) {
         l[k] = a.lid[(size_t)k * np + s];
         g[k] = l[k] < nown ? n0 + l[k] : a.halo[hb + l[k] - nown];
       }
-      const double x2e = a.xel[ne + e], x3e = a.xel[2 * ne + e];
-      const double y2e = a.yel[ne + e], y3e = a.yel[2 * ne + e];
+      const SlotGeo G = load_geo(a.geo, np, s, a.xel, a.yel, a.surfac, ne, e, true);
+      const double x2e = G.x2, x3e = G.x3;
+      const double y2e = G.y2, y3e = G.y3;
       const bool p0 = (a.flags & SUPG_U_P0) && !(a.flags & SUPG_KSUPG);
       double u1 = 0, u2 = 0, u3 = 0, v1 = 0, v2 = 0, v3 = 0;
       if (!p0) {
@@ -123,7 +125,7 @@ __global__ void __launch_bounds__(kThreads) k_elem_supg(SupgArgs a) {
           ff = a.f[e];
           gg = a.g[e];
         }
-        const double den = (a.xmul / 12.0) / a.surfac[e];
+        const double den = (a.xmul / 12.0) / G.s;
         const double kkx = ff * den, kky = gg * den;
         const double u123 = u1 + u2 + u3, v123 = v1 + v2 + v3;
         const double p1 = x2e * v123 - y2e * u123;
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(kThreads) k_elem_supg(SupgArgs a) {
         m[2][1] = -m[1][1] - m[0][1];
         m[2][2] = -m[0][2] - m[1][2];
       } else if (a.kind == 4) {
-        const double c = (a.xmul / 48.0) / a.surfac[e];
+        const double c = (a.xmul / 48.0) / G.s;
         const double u123 = u1 + u2 + u3, v123 = v1 + v2 + v3;
         const double uu = u1 * u123 + u2 * u2 + u2 * u3 + u3 * u3;
         const double uv = u1 * (v123 + v1) + u2 * (v123 + v2) + u3 * (v123 + v3);
@@ -279,7 +281,10 @@ extern "C" {
 int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
                      const double* surfac, const double* f, const double* g, const double* u,
                      const double* v, int flags, int op) {
-  if (!c || c->L.ndp != 3 || !xel || !yel || !surfac || !u || !v) return BHIP_EBADARG;
+  if (!c || c->L.ndp != 3 || !u || !v) return BHIP_EBADARG;
+  // XEL, YEL, SURFAC all NULL: the context's geometry (bhip_ctx_set_geometry)
+  const bool use_geo = !surfac && !xel && !yel;
+  if (use_geo ? !c->have_geo : (!surfac || !xel || !yel)) return BHIP_EBADARG;
   if (kind != 3 && kind != 4 && kind != 5) return BHIP_EBADARG;
   if (op < 0 || op > 2 || (flags & ~(SUPG_U_P0 | SUPG_UPWIND | SUPG_KSUPG))) return BHIP_EBADARG;
   if (kind == 3 && !(flags & SUPG_KSUPG) && (!f || !g)) return BHIP_EBADARG;
@@ -309,12 +314,14 @@ int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, cons
   }
   const bool u_p0 = (flags & SUPG_U_P0) && !(flags & SUPG_KSUPG);
   DevBuf<double> dS, dX, dY, dF, dG, dU, dV;
-  HIP_OK(dS.alloc(ne));
-  HIP_OK(dX.alloc((size_t)3 * ne));
-  HIP_OK(dY.alloc((size_t)3 * ne));
-  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+  if (!use_geo) {
+    HIP_OK(dS.alloc(ne));
+    HIP_OK(dX.alloc((size_t)3 * ne));
+    HIP_OK(dY.alloc((size_t)3 * ne));
+    HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+  }
   if (kind == 3 && !(flags & SUPG_KSUPG)) {
     HIP_OK(dF.alloc(ne));
     HIP_OK(dG.alloc(ne));
@@ -344,6 +351,7 @@ int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, cons
   a.surfac = dS.p;
   a.xel = dX.p;
   a.yel = dY.p;
+  a.geo = use_geo ? c->geo.p : nullptr;
   a.nelem = ne;
   a.f = dF.p;
   a.g = dG.p;

@@ -14,7 +14,8 @@ isotropic P1 coefficient), applies DIRICH and calls SOLVE with a
 non-symmetric solver (GMRES). Every matrix here is built on the device:
 the element kernels of MT01AA, MT02AA, MT03AA, MT05AA (with KSUPG computed
 in place) and the OM operations work on the context's slot matrix and its
-saved copy, in the reference's order.
+saved copy, in the reference's order. The mesh geometry (XEL, YEL, SURFAC)
+is put on the device once per context and read from there.
 """
 from __future__ import annotations
 
@@ -27,24 +28,26 @@ def supg_system(ctx: bief.Context, mesh, u, v, nu, dt, fn, tetat=1.0, sm0=None):
     """Assembles AM1 into the context (non-symmetric, 'Q') and returns the
     right-hand side SM (host array). u, v, nu, fn: node arrays (caller
     order); sm0: the source term SM before the step (zero by default)."""
-    xe, ye = mesh.xel_yel()
-    s = mesh.surfac()
+    if not getattr(ctx, "has_geometry", False):
+        xe, ye = mesh.xel_yel()
+        ctx.set_geometry(xe, ye, mesh.surfac())
+    G = (None, None, None)  # the geometry kept on the device
     fn = np.ascontiguousarray(fn, dtype=np.float64)
     K = bief.Context.SUPG_KSUPG
     # AM2: centred advection plus the SUPG term (cvdftr.F:398-409)
-    ctx.matrix_supg(5, 1.0, xe, ye, s, u, v, op="M=N")
-    ctx.matrix_supg(3, 1.0, xe, ye, s, u, v, flags=K, op="M=M+N")
+    ctx.matrix_supg(5, 1.0, *G, u, v, op="M=N")
+    ctx.matrix_supg(3, 1.0, *G, u, v, flags=K, op="M=M+N")
     t2 = ctx.matvec("X=AY", fn)
     ctx.save_matrix()
     # AM1: mass over dt made non-symmetric, plus the transposed SUPG test
     # term with the element vectors as a P0 velocity (cvdftr.F:448, 513-523)
-    ctx.matrix_mt01aa(1.0 / dt, s)
+    ctx.matrix_mt01aa(1.0 / dt, None)
     ctx.matrix_om("M=X(M)")
-    ctx.matrix_supg(5, 1.0 / dt, xe, ye, s, u, v, flags=K, op="M=M+TN")
+    ctx.matrix_supg(5, 1.0 / dt, *G, u, v, flags=K, op="M=M+TN")
     rhs = fn if sm0 is None else dt * np.asarray(sm0, dtype=np.float64) + fn
     sm = ctx.matvec("X=AY", rhs)
     sm = sm + (tetat - 1.0) * t2
     ctx.matrix_om("M=M+CN", tetat)
     # implicit diffusion (MATDIF, P1 isotropic)
-    ctx.matrix_mt02aa(1, 1.0, xe, ye, s, nu, accumulate=True)
+    ctx.matrix_mt02aa(1, 1.0, *G, nu, accumulate=True)
     return sm

@@ -155,3 +155,23 @@ def test_prism_vertical_diffusion_solve(gpu):
             assert rel <= 1e-10, rel
         else:
             assert rel <= 1e-8, rel
+
+
+def test_prism_device_geometry_matches_caller_arrays(gpu):
+    """MT01PP / MT02PP from the geometry kept on the device (set_geometry,
+    then xel = yel = surfac = None) give the same bits as from the caller's
+    arrays (same layout, same kernels, slot-ordered reads)."""
+    base, p, xel, yel, s = _sloped_prisms()
+    rng = np.random.default_rng(8)
+    f, g = rng.uniform(0.5, 2, p.npoin), rng.uniform(0.5, 2, p.npoin)
+    h = 10 ** rng.uniform(-4, -2, p.npoin)
+    a, b = bief.Context(p, block_nodes=40), bief.Context(p, block_nodes=40)
+    b.set_geometry(xel, yel, s)
+    for ctx, geo in ((a, (xel, yel, s)), (b, (None, None, None))):
+        ctx.matrix_mt01pp(0.5, geo[2], p.z)
+        ctx.matrix_mt02pp(1.0, *geo, p.z, f, g, h, base.npoin, accumulate=True)
+    _, da_a, xa_a = a.get_matrix()
+    _, da_b, xa_b = b.get_matrix()
+    assert np.array_equal(da_a, da_b) and np.array_equal(xa_a, xa_b)
+    with pytest.raises(bief.BiefError):
+        a.matrix_mt01pp(0.5, None, p.z)  # no geometry kept on this context

@@ -141,6 +141,37 @@ def test_supg_solves_match_reference(gpu, drop, slv):
     assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
 
 
+def test_device_geometry_matches_caller_arrays(gpu, small):
+    """Element matrices from the geometry kept on the device (set_geometry,
+    then xel = yel = surfac = None) are the same bits as from the caller's
+    arrays, for every element kind (same layout, same kernels)."""
+    m, (xe, ye), sf, u, v = small
+    a, b = bief.Context(m, block_nodes=40), bief.Context(m, block_nodes=40)
+    b.set_geometry(xe, ye, sf)
+    kx, ky = O.kspg11(xe, ye, m.ikle, u, v, 1.0)
+    C = bief.Context
+    cases = [
+        lambda c, g: c.matrix_mt01aa(0.5, g[2]),
+        lambda c, g: c.matrix_mt02aa(1, 1.0, *g, u),
+        lambda c, g: c.matrix_mt02aa(2, 0.7, *g, u, v),
+        lambda c, g: c.matrix_supg(3, 0.7, *g, u, v, f=kx, g=ky),
+        lambda c, g: c.matrix_supg(3, 0.7, *g, u, v, flags=K),
+        lambda c, g: c.matrix_supg(4, 0.3, *g, u, v),
+        lambda c, g: c.matrix_supg(5, 1.3, *g, u, v),
+        lambda c, g: c.matrix_supg(5, 1.0, *g, u, v, flags=C.SUPG_UPWIND),
+        lambda c, g: c.matrix_supg(5, 2.0, *g, kx, ky, flags=C.SUPG_U_P0),
+        lambda c, g: c.matrix_supg(5, 1.0, *g, u, v, flags=K, op="M=M+TN"),
+    ]
+    for i, build in enumerate(cases):
+        build(a, (xe, ye, sf))
+        build(b, (None, None, None))
+        _, da_a, xa_a = a.get_matrix()
+        _, da_b, xa_b = b.get_matrix()
+        assert np.array_equal(da_a, da_b) and np.array_equal(xa_a, xa_b), i
+    with pytest.raises(bief.BiefError):
+        a.matrix_mt01aa(0.5, None)  # no geometry kept on this context
+
+
 def test_supg_rejects_bad_arguments(gpu, small):
     m, (xe, ye), sf, u, v = small
     ctx = bief.Context(m)
