@@ -58,3 +58,26 @@ def test_device_element_matrices(gpu, bn):
     assert np.allclose(xa, x3, rtol=1e-14, atol=1e-14 * np.abs(x3).max())
     d3 = O.assemble_diag(t3, m.ikle, m.npoin)
     assert np.allclose(da, d3, rtol=1e-12, atol=1e-12 * np.abs(d3).max())
+
+
+def test_element_entries_free_their_device_buffers(gpu):
+    """Repeated element-matrix calls from the caller's arrays (each uploads
+    XEL, YEL, SURFAC and the node arrays into temporary device buffers) leave
+    the device's free memory where it was: the temporaries are freed when
+    the call returns. 30 calls on a 100k-node channel would hold about 400 MB
+    if they were not."""
+    import torch
+
+    m = M.channel_mesh(400, 250)
+    xe, ye = m.xel_yel()
+    s = m.surfac()
+    u = np.full(m.npoin, 0.1)
+    ctx = bief.Context(m)
+    ctx.matrix_mt02aa(1, 1.0, xe, ye, s, u)  # first call: the context's own buffers
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(30):
+        ctx.matrix_mt02aa(1, 1.0, xe, ye, s, u)
+        ctx.matrix_mt01aa(0.5, s, accumulate=True)
+    torch.cuda.synchronize()
+    assert free0 - torch.cuda.mem_get_info()[0] < 64 * 2**20
