@@ -141,6 +141,31 @@ def _host_or_none(a):
     return None if a is None else _host_f64(a)
 
 
+def _host_n(a, n, what):
+    """float64 contiguous host array of exactly n values: the C entry points
+    read n values from the pointer, so a shorter array is refused here."""
+    h = _host_f64(a)
+    if h.size != n:
+        raise BiefError(f"{what}: {h.size} values, expected {n}")
+    return h
+
+
+def _host_n_or_none(a, n, what):
+    return None if a is None else _host_n(a, n, what)
+
+
+def _dev_n(t, n, what):
+    """A torch tensor handed to a device entry point: contiguous float64 on
+    a GPU with exactly n values (the kernels read n values from its
+    address; a host tensor's address is no device address)."""
+    import torch
+    if not (_is_torch(t) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise BiefError(f"{what}: expected a contiguous float64 CUDA tensor")
+    if t.numel() != n:
+        raise BiefError(f"{what}: {t.numel()} values, expected {n}")
+    return t
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data
 
@@ -226,6 +251,11 @@ class Context:
     def set_matrix(self, typdia: str, typext: str, da, xa):
         """A%D, A%X (ncoef x nelem), A%TYPDIA, A%TYPEXT."""
         if _is_torch(da):
+            npair = self.ndp * (self.ndp - 1) // 2
+            ncoef = {"Q": 2 * npair, "S": npair}.get(typext, 0)
+            _dev_n(da, self.npoin, "set_matrix: da")
+            if ncoef:
+                _dev_n(xa, ncoef * self.nelem, "set_matrix: xa")
             self._sync_torch()
             rc = lib().bhip_matrix_set_dev(self._h, typdia.encode(), typext.encode(),
                                            da.data_ptr(), xa.data_ptr() if xa is not None else None)
@@ -236,10 +266,14 @@ class Context:
             # then record an event on a stream that no longer exists.)
             self._join_torch()
         else:
-            da = _host_f64(da)
-            xa = _host_f64(xa) if xa is not None else None
-            rc = lib().bhip_matrix_set(self._h, typdia.encode(), typext.encode(), da.ctypes.data,
-                                       xa.ctypes.data if xa is not None else None)
+            da = _host_n_or_none(da, self.npoin, "set_matrix: da")
+            npair = self.ndp * (self.ndp - 1) // 2
+            ncoef = {"Q": 2 * npair, "S": npair}.get(typext, 0)
+            # (TYPEXT '0' reads no off-diagonal terms)
+            xa = _host_n(xa, ncoef * self.nelem, "set_matrix: xa") \
+                if xa is not None and ncoef else None
+            rc = lib().bhip_matrix_set(self._h, typdia.encode(), typext.encode(), _ptr(da),
+                                       _ptr(xa))
         _check(rc, "bhip_matrix_set")
         self.typdia, self.typext = typdia, typext
 
@@ -268,10 +302,13 @@ class Context:
         """DIRICH with domain-sized FBOR/LIMDIR (dirich.f -> diri01.f ->
         diraux.f). Returns (f, sm); the context's matrix gets the rows and
         columns of prescribed nodes zeroed."""
-        f = _host_f64(f).copy()
-        sm = _host_f64(sm).copy()
-        fb = _host_f64(fbor)
+        n = self.npoin
+        f = _host_n(f, n, "dirich: f").copy()
+        sm = _host_n(sm, n, "dirich: sm").copy()
+        fb = _host_n(fbor, n, "dirich: fbor")
         ld = np.ascontiguousarray(limdir, dtype=np.int32)
+        if ld.size != n:
+            raise BiefError(f"dirich: limdir: {ld.size} values, expected {n}")
         _check(lib().bhip_dirich(self._h, f.ctypes.data, sm.ctypes.data, fb.ctypes.data,
                                  ld.ctypes.data, int(kdir)), "bhip_dirich")
         return f, sm
@@ -286,7 +323,9 @@ class Context:
         """Keeps the mesh's XEL, YEL, SURFAC on the device (what INBIEF
         computes once, inbief.f:384-397); the matrix_* methods then take
         xel = yel = surfac = None."""
-        arrs = [_host_f64(a) for a in (xel, yel, surfac)]
+        ne = self.nelem
+        arrs = [_host_n(xel, 3 * ne, "xel"), _host_n(yel, 3 * ne, "yel"),
+                _host_n(surfac, ne, "surfac")]
         _check(lib().bhip_ctx_set_geometry(self._h, *(a.ctypes.data for a in arrs)),
                "bhip_ctx_set_geometry")
         self.has_geometry = True
@@ -294,7 +333,7 @@ class Context:
     def matrix_mt01aa(self, xmul, surfac, accumulate=False):
         """MATRIX(... 'MATMAS' ...) with OM 'M=N' / 'M=M+N' (mt01aa.f).
         surfac None: the geometry of set_geometry."""
-        s = _host_or_none(surfac)
+        s = _host_n_or_none(surfac, self.nelem, "surfac")
         _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), _ptr(s), int(accumulate)),
                "bhip_matrix_mt01aa")
         self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
@@ -302,8 +341,10 @@ class Context:
     def matrix_mt02aa(self, kind, xmul, xel, yel, surfac, u, v=None, accumulate=False):
         """MATRIX(... 'MATDIF' / 'MATDIFUV' ...): kind 1 P1 isotropic
         viscosity U, kind 2 coefficient U*V (mt02aa.f)."""
-        xel, yel, s, v = (_host_or_none(a) for a in (xel, yel, surfac, v))
-        u = _host_f64(u)
+        ne, n = self.nelem, self.npoin
+        xel, yel = _host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel")
+        s = _host_n_or_none(surfac, ne, "surfac")
+        u, v = _host_n(u, n, "u"), _host_n_or_none(v, n, "v")
         _check(lib().bhip_matrix_mt02aa(self._h, int(kind), float(xmul), _ptr(xel), _ptr(yel),
                                         _ptr(s), u.ctypes.data, _ptr(v), int(accumulate)),
                "bhip_matrix_mt02aa")
@@ -317,9 +358,12 @@ class Context:
         'MATVGR' (5, MT05AA) ...) with OM op 'M=N', 'M=M+N' or 'M=M+TN'.
         flags: SUPG_U_P0 (u, v per element), SUPG_UPWIND (MATVGR's 'N'
         variant), SUPG_KSUPG (the element vectors from KSUPG(1, u, v))."""
-        geo = [_host_or_none(a) for a in (xel, yel, surfac)]
-        fg = [_host_or_none(a) for a in (f, g)]
-        u, v = _host_f64(u), _host_f64(v)
+        ne, n = self.nelem, self.npoin
+        geo = [_host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel"),
+               _host_n_or_none(surfac, ne, "surfac")]
+        fg = [_host_n_or_none(a, ne, "f, g") for a in (f, g)]
+        nu = ne if (flags & self.SUPG_U_P0) and not (flags & self.SUPG_KSUPG) else n
+        u, v = _host_n(u, nu, "u"), _host_n(v, nu, "v")
         _check(lib().bhip_matrix_supg(self._h, int(kind), float(xmul), *(_ptr(a) for a in geo),
                                       *(_ptr(a) for a in fg), u.ctypes.data, v.ctypes.data,
                                       int(flags), self._OM_OPS[op]), "bhip_matrix_supg")
@@ -337,8 +381,8 @@ class Context:
     def matrix_mt01pp(self, xmul, surfac, z, accumulate=False):
         """MATRIX(... 'MATMAS' ...) on prisms (mt01pp.f): surfac is the base
         triangle area of each prism, z the node elevations."""
-        s = _host_or_none(surfac)
-        z = _host_f64(z)
+        s = _host_n_or_none(surfac, self.nelem, "surfac")
+        z = _host_n(z, self.npoin, "z")
         _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), _ptr(s), z.ctypes.data,
                                         int(accumulate)), "bhip_matrix_mt01pp")
         self.typdia, self.typext = "Q", "S"
@@ -348,8 +392,10 @@ class Context:
         """MATRIX(... 'MATDIF' ...) on prisms (mt02pp.f): diagonal diffusion
         tensor (f, g, h) per node; npou0 = NPOIN2 (P1 vertical viscosity) or
         0 (P0 on the vertical)."""
-        arrs = [_host_or_none(a) for a in (xel, yel, surfac)] + \
-            [_host_f64(a) for a in (z, f, g, h)]
+        ne, n = self.nelem, self.npoin
+        arrs = [_host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel"),
+                _host_n_or_none(surfac, ne, "surfac")] + \
+            [_host_n(a, n, "z, f, g, h") for a in (z, f, g, h)]
         _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[_ptr(a) for a in arrs],
                                         int(h_zero), int(npou0), int(inchyd), int(mon),
                                         int(accumulate)), "bhip_matrix_mt02pp")
@@ -388,13 +434,15 @@ class Context:
             import torch
             if x is None:
                 x = torch.zeros_like(y)
+            _dev_n(y, self.npoin, "matvec: y")
+            _dev_n(x, self.npoin, "matvec: x")
             self._sync_torch()
             _check(lib().bhip_matvec_dev(self._h, code, float(c), y.data_ptr(), x.data_ptr()),
                    "bhip_matvec_dev")
             self._join_torch()
             return x
-        y = _host_f64(y)
-        x = np.zeros(self.npoin) if x is None else _host_f64(x).copy()
+        y = _host_n(y, self.npoin, "matvec: y")
+        x = np.zeros(self.npoin) if x is None else _host_n(x, self.npoin, "matvec: x").copy()
         _check(lib().bhip_matvec(self._h, code, float(c), y.ctypes.data, x.ctypes.data),
                "bhip_matvec")
         return x
@@ -411,6 +459,8 @@ class Context:
         (bhip_solve_dev_internal); otherwise as solve()."""
         if not _is_torch(x) or not _is_torch(b):
             raise BiefError("solve_internal takes device tensors")
+        _dev_n(x, self.npoin, "solve: x")
+        _dev_n(b, self.npoin, "solve: b")
         cfg = SolverCfg(slv, precon, krylov, nitmax, eps, precision, inner_eps, inner_nitmax)
         info = SolveInfo()
         self._sync_torch()
@@ -440,14 +490,16 @@ class Context:
         cfg = SolverCfg(slv, precon, krylov, nitmax, eps, precision, inner_eps, inner_nitmax)
         info = SolveInfo()
         if _is_torch(x):
+            _dev_n(x, self.npoin, "solve: x")
+            _dev_n(b, self.npoin, "solve: b")
             self._sync_torch()
             rc = lib().bhip_solve_dev(self._h, ctypes.byref(cfg), x.data_ptr(), b.data_ptr(),
                                       ctypes.byref(info))
             _check(rc, "bhip_solve_dev")
             self._join_torch()
         else:
-            xh = _host_f64(x)
-            bh = _host_f64(b)
+            xh = _host_n(x, self.npoin, "solve: x")
+            bh = _host_n(b, self.npoin, "solve: b")
             if not xh.flags.writeable or xh is x:
                 xh = xh.copy()
             rc = lib().bhip_solve(self._h, ctypes.byref(cfg), xh.ctypes.data, bh.ctypes.data,

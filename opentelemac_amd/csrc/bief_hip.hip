@@ -1545,6 +1545,7 @@ static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double
   const Layout& L = c->L;
   const int npair = L.ndp * (L.ndp - 1) / 2;
   const int ncoef = typext == 'Q' ? 2 * npair : (typext == 'S' ? npair : 0);
+  if (ncoef > 0 && !xa) return BHIP_EBADARG;
   HIP_OK(c->da.alloc(L.npoin));
   // coefficient storage always holds the non-symmetric width, so a 'S'
   // matrix can be turned 'Q' later without reallocation

@@ -81,3 +81,49 @@ def test_element_entries_free_their_device_buffers(gpu):
         ctx.matrix_mt01aa(0.5, s, accumulate=True)
     torch.cuda.synchronize()
     assert free0 - torch.cuda.mem_get_info()[0] < 64 * 2**20
+
+
+def test_host_entries_refuse_wrong_sizes(gpu):
+    """The host-array entry points read a fixed number of values from each
+    pointer; arrays of another length are refused before any call."""
+    m = M.random_small_mesh(8, 7, seed=23)
+    ctx = bief.Context(m)
+    n, ne = m.npoin, m.nelem
+    da, xa = np.ones(n), np.zeros((3, ne))
+    with pytest.raises(bief.BiefError):
+        ctx.set_matrix("Q", "S", da[:-1], xa)
+    with pytest.raises(bief.BiefError):
+        ctx.set_matrix("Q", "Q", da, xa)            # 'Q' needs 6 columns
+    with pytest.raises(bief.BiefError):
+        ctx.set_matrix("Q", "S", da, None)          # 'S' needs its coefficients
+    ctx.set_matrix("Q", "S", da, xa)
+    with pytest.raises(bief.BiefError):
+        ctx.matvec("X=AY", np.ones(n + 1))
+    with pytest.raises(bief.BiefError):
+        ctx.solve(np.zeros(n), np.ones(n - 1))
+    xe, ye = m.xel_yel()
+    with pytest.raises(bief.BiefError):
+        ctx.set_geometry(xe, ye, m.surfac()[:-1])
+    with pytest.raises(bief.BiefError):
+        ctx.matrix_mt02aa(1, 1.0, xe, ye, m.surfac(), np.ones(n - 2))
+
+
+def test_device_entries_refuse_host_or_short_tensors(gpu):
+    """A CPU tensor's address is no device address, and a short tensor would
+    be read past its end: the device entry points refuse both."""
+    import torch
+
+    m = M.random_small_mesh(8, 7, seed=24)
+    ctx = bief.Context(m)
+    n = m.npoin
+    ctx.set_matrix("Q", "S", np.ones(n), np.zeros((3, m.nelem)))
+    with pytest.raises(bief.BiefError):
+        ctx.matvec("X=AY", torch.ones(n, dtype=torch.float64))          # host tensor
+    with pytest.raises(bief.BiefError):
+        ctx.matvec("X=AY", torch.ones(n - 1, dtype=torch.float64, device=gpu))
+    with pytest.raises(bief.BiefError):
+        ctx.solve(torch.zeros(n, dtype=torch.float64, device=gpu),
+                  torch.ones(n, dtype=torch.float32, device=gpu))     # FP32 right-hand side
+    with pytest.raises(bief.BiefError):
+        ctx.set_matrix("Q", "S", torch.ones(n, dtype=torch.float64, device=gpu),
+                       torch.zeros((2, m.nelem), dtype=torch.float64, device=gpu))
