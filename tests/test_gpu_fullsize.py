@@ -70,3 +70,60 @@ def test_fullsize_fp64_iterations_match_reference(gpu, estuary):
     _, info = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-8,
                         nitmax=5000)
     assert info["niter"] == 41, info
+
+
+@pytest.fixture(scope="module")
+def prisms():
+    """The 5,014,910-node sigma mesh: the 500k estuary base extruded to 10
+    planes over its seeded depth (8,996,184 prisms), with its geometry."""
+    base = M.estuary_mesh(500_000, seed=0)
+    depth = M.smooth_random_depth(base, seed=0)
+    p = M.extrude_prisms(base, 10, -depth, np.zeros(base.npoin))
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    return base, p, np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb]), \
+        base.surfac()[eb]
+
+
+def test_fullsize_prisms_assembly_and_solve(gpu, prisms):
+    """MT01PP integrates the mesh volume (all entries of the mass matrix sum
+    to sum S (h1 + h2 + h3) / 3); MT02PP with unit coefficients annihilates
+    constants; the diffusion system (MT01PP / dt + MT02PP) solves to the
+    1e-8 stop rule (on the diagonally scaled system) in FP64, in 54
+    iterations +-1 as the reference's SOLVE does on this system (flang build
+    of sources/utils/bief: 54 iterations, relative precision 0.9619190E-08),
+    and in mixed precision, with true residuals ||b - A x|| / ||b|| below
+    1e-7."""
+    base, p, xel, yel, s = prisms
+    assert p.npoin == 5_014_910 and p.nelem == 8_996_184
+    ctx = bief.Context(p, block_nodes=2560)
+    ctx.set_geometry(xel, yel, s)
+    ctx.matrix_mt01pp(1.0, None, p.z)
+    _, da, xa = ctx.get_matrix()
+    ik = p.ikle
+    h = (p.z[ik[3]] - p.z[ik[0]]) + (p.z[ik[4]] - p.z[ik[1]]) + (p.z[ik[5]] - p.z[ik[2]])
+    vol = float(np.sum(s * h / 3.0))
+    assert abs(da.sum() + 2.0 * xa.sum() - vol) <= 1e-12 * vol
+    one = np.ones(p.npoin)
+    ctx.matrix_mt02pp(1.0, None, None, None, p.z, one, one, one, base.npoin)
+    _, da2, _ = ctx.get_matrix()
+    assert np.abs(ctx.matvec("X=AY", one)).max() <= 1e-12 * np.abs(da2).max()
+    sig = np.repeat(np.linspace(0.0, 1.0, 10), base.npoin)
+    nu = 10.0 ** (-4.0 + 2.0 * sig)
+    b = np.random.default_rng(12).normal(size=p.npoin)
+    sols = {}
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ctx.matrix_mt01pp(0.5, None, p.z)
+        ctx.matrix_mt02pp(1.0, None, None, None, p.z, nu, nu, nu, base.npoin, accumulate=True)
+        if prec == bief.PREC_FP64:
+            ctx.save_matrix()
+        x, info = ctx.solve(np.zeros(p.npoin), b.copy(), slv=1, precon=2, eps=1e-8, nitmax=5000,
+                            precision=prec)
+        assert info["converged"], (prec, info)
+        if prec == bief.PREC_FP64:
+            assert abs(info["niter"] - 54) <= 1, info
+        sols[prec] = x
+    ctx.restore_matrix()
+    for prec, x in sols.items():
+        true = np.linalg.norm(b - ctx.matvec("X=AY", x)) / np.linalg.norm(b)
+        assert true <= 1e-7, (prec, true)
