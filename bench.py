@@ -572,27 +572,7 @@ def bench_supg(args, dt=10.0, nu=0.1):
     from opentelemac_amd import bief, mesh as M, system as S, tracer
     t = time.time()
     m = M.estuary_mesh(args.npoin, seed=args.seed)
-    rng = np.random.default_rng(args.seed + 7)
-    x, y = m.x, m.y
-    lx, ly = x.max() - x.min(), y.max() - y.min()
-    # streamfunction psi = sum a_k exp(-r_k^2 / (2 s_k^2)); u = dpsi/dy, v = -dpsi/dx
-    u = np.zeros(m.npoin)
-    v = np.zeros(m.npoin)
-    for _ in range(16):
-        cx, cy = x.min() + rng.uniform() * lx, y.min() + rng.uniform() * ly
-        sg = rng.uniform(0.05, 0.2) * min(lx, ly)
-        a = rng.normal()
-        e = a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * sg * sg))
-        u += -e * (y - cy) / (sg * sg)
-        v += e * (x - cx) / (sg * sg)
-    sc = 2.0 / np.hypot(u, v).max()
-    u, v = u * sc, v * sc
-    fn = np.zeros(m.npoin)
-    for _ in range(8):
-        cx, cy = x.min() + rng.uniform() * lx, y.min() + rng.uniform() * ly
-        fn += np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * (0.03 * lx) ** 2))
-    bnd = S.boundary_nodes(m)
-    sea = bnd[m.x[bnd] > m.x.max() - 1.0]
+    u, v, fn, sea = tracer.estuary_flow(m, seed=args.seed)
     log(f"supg mesh {m.npoin} nodes, built in {time.time() - t:.1f}s")
     ctx = bief.Context(m)
     t = time.time()

@@ -51,3 +51,33 @@ def supg_system(ctx: bief.Context, mesh, u, v, nu, dt, fn, tetat=1.0, sm0=None):
     # implicit diffusion (MATDIF, P1 isotropic)
     ctx.matrix_mt02aa(1, 1.0, *G, nu, accumulate=True)
     return sm
+
+
+def estuary_flow(mesh, seed=0, umax=2.0):
+    """The synthetic tracer case on an estuary mesh: a divergence-free
+    velocity (u, v) = (dpsi/dy, -dpsi/dx) of a streamfunction of 16 seeded
+    Gaussians, scaled to a largest speed of umax; a tracer of 8 Gaussian
+    patches; and the sea-boundary nodes (x within 1 m of the largest x),
+    where the tracer is prescribed. Returns (u, v, fn, sea)."""
+    from . import system as S
+    rng = np.random.default_rng(seed + 7)
+    x, y = mesh.x, mesh.y
+    lx, ly = x.max() - x.min(), y.max() - y.min()
+    u = np.zeros(mesh.npoin)
+    v = np.zeros(mesh.npoin)
+    for _ in range(16):
+        cx, cy = x.min() + rng.uniform() * lx, y.min() + rng.uniform() * ly
+        sg = rng.uniform(0.05, 0.2) * min(lx, ly)
+        a = rng.normal()
+        e = a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * sg * sg))
+        u += -e * (y - cy) / (sg * sg)
+        v += e * (x - cx) / (sg * sg)
+    sc = umax / np.hypot(u, v).max()
+    u, v = u * sc, v * sc
+    fn = np.zeros(mesh.npoin)
+    for _ in range(8):
+        cx, cy = x.min() + rng.uniform() * lx, y.min() + rng.uniform() * ly
+        fn += np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * (0.03 * lx) ** 2))
+    bnd = S.boundary_nodes(mesh)
+    sea = bnd[mesh.x[bnd] > mesh.x.max() - 1.0]
+    return u, v, fn, sea

@@ -8,12 +8,17 @@ where the oracle would take minutes.
     reference's own SOLVE (flang build of sources/utils/bief) on this
     system, and its true residual ||b - A x|| / ||b|| is below 1e-8;
   * the mixed path (FP32 inner CG in FP64 refinement) reaches the same
-    stop rule and agrees with the FP64 solution to 1e-7.
+    stop rule and agrees with the FP64 solution to 1e-7;
+  * CVDFTR's SUPG tracer system assembled on the device equals the
+    oracle's (the C restatement, here at full size) to 1e-13, and GMRES(5)
+    solves it in the reference's 3 restarts;
+  * on the 5M-node prism mesh, see the prism test.
 Building the mesh and the system takes about a minute on the host."""
 import numpy as np
 import pytest
 
-from opentelemac_amd import bief, mesh as M, system as S
+from opentelemac_amd import bief, mesh as M, system as S, tracer
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -127,3 +132,27 @@ def test_fullsize_prisms_assembly_and_solve(gpu, prisms):
     for prec, x in sols.items():
         true = np.linalg.norm(b - ctx.matvec("X=AY", x)) / np.linalg.norm(b)
         assert true <= 1e-7, (prec, true)
+
+
+def test_fullsize_supg_system_and_gmres(gpu, estuary):
+    """The SUPG tracer system of the 2M-node estuary (the synthetic flow of
+    tracer.estuary_flow, nu = 0.1, dt = 10 s): the device assembly against
+    the oracle's at full size (which equals the reference's own routines bit
+    for bit on this system), DIRICH on the sea boundary, GMRES(5) + PRECON 2
+    to 1e-8 in the reference's count (its SOLVE: 3 restarts, relative
+    precision 0.2229581E-08)."""
+    m, _ = estuary
+    u, v, fn, sea = tracer.estuary_flow(m, seed=0)
+    nu = np.full(m.npoin, 0.1)
+    ctx = bief.Context(m)
+    sm = tracer.supg_system(ctx, m, u, v, nu, 10.0, fn)
+    _, da, xa = ctx.get_matrix()
+    A, smo = O.supg_system(m, u, v, nu, 10.0, fn)
+    for got, ref in ((da, A.da), (xa, A.xa), (sm, smo)):
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    limdir = np.full(m.npoin, 2, dtype=np.int32)
+    limdir[sea] = 5
+    f, sm2 = ctx.dirich(fn, sm, np.zeros(m.npoin), limdir)
+    x, info = ctx.solve(f, sm2, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000)
+    assert info["converged"] and abs(info["niter"] - 3) <= 1, info
+    assert abs(info["relres"] - 2.229581e-09) <= 1e-3 * 2.229581e-09, info
