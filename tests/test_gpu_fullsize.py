@@ -156,3 +156,46 @@ def test_fullsize_supg_system_and_gmres(gpu, estuary):
     x, info = ctx.solve(f, sm2, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000)
     assert info["converged"] and abs(info["niter"] - 3) <= 1, info
     assert abs(info["relres"] - 2.229581e-09) <= 1e-3 * 2.229581e-09, info
+
+
+def test_fullsize_adjoint_gradient(gpu):
+    """The gradient of the gauge misfit through the device wave model at
+    the adjoint workload's size (the 500k-node estuary, 50 steps of 2 s,
+    200 gauges, checkpoints every 10 steps, mixed solves to 1e-10):
+      * with checkpoints it agrees with the gradient that keeps every state
+        to 1e-6 (measured: 3e-7; the two sweeps' solves each stop at their
+        own 1e-10 rule, so they agree to that accuracy, not to the bit);
+      * along a random direction it matches central differences of the
+        cost to 1e-5 of sum |g_i d_i| (Strickler and bathymetry; steps of
+        1e-5 and 1e-6 of the parameter's largest value)."""
+    import torch
+    from opentelemac_amd import wave as W
+
+    m = M.estuary_mesh(500_000, seed=0)
+    depth = M.smooth_random_depth(m, seed=0)
+    bnd = S.boundary_nodes(m)
+    sea = bnd[m.x[bnd] > m.x.max() - 1.0]
+    model = W.DeviceWaveModel(m, sea, dt=2.0, amp=0.2, period=600.0,
+                              precision=bief.PREC_MIXED, eps=1e-10)
+    dev = model.dev
+    rng = np.random.default_rng(2)
+    k = torch.tensor(rng.uniform(20, 60, m.npoin), dtype=torch.float64, device=dev)
+    zb = torch.tensor(-depth, dtype=torch.float64, device=dev)
+    gauges = torch.tensor(rng.choice(m.npoin, 200, replace=False), device=dev)
+    obs = model.run(k * torch.tensor(rng.uniform(0.8, 1.2, m.npoin), device=dev), zb, 50, gauges)
+    _, gk, gz = model.cost_and_gradient(k, zb, 50, gauges, obs, 10)
+    _, gk_all, gz_all = model.cost_and_gradient(k, zb, 50, gauges, obs, 50)
+    for g, g_all in ((gk, gk_all), (gz, gz_all)):
+        rel = float(torch.linalg.norm(g - g_all) / torch.linalg.norm(g_all))
+        assert rel <= 1e-6, rel
+    for p, g, scale in ((k, gk, 1e-5), (zb, gz, 1e-6)):
+        d = torch.tensor(rng.normal(size=m.npoin), dtype=torch.float64, device=dev)
+        h = scale * float(p.abs().max())
+        p0 = p.clone()
+        p.copy_(p0 + h * d)
+        jp = float(model.run(k, zb, 50, gauges, obs))
+        p.copy_(p0 - h * d)
+        jm = float(model.run(k, zb, 50, gauges, obs))
+        p.copy_(p0)
+        fd = (jp - jm) / (2 * h)
+        assert abs(fd - float(g @ d)) <= 1e-5 * float((g * d).abs().sum()), (fd, float(g @ d))
