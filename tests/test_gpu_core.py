@@ -146,6 +146,29 @@ def test_solve_repeatable_bitwise(gpu):
     assert np.array_equal(outs[0][0], outs[1][0])
 
 
+@pytest.mark.parametrize("prec", [bief.PREC_FP64, bief.PREC_MIXED])
+def test_solve_independent_of_history(gpu, prec):
+    """A solve's result does not depend on the solves the context ran
+    before it (another right-hand side, another tolerance): the queued
+    iteration counts, cached graphs and scratch vectors those leave behind
+    change nothing, to the bit."""
+    m = M.channel_mesh(100, 20, 500.0, 100.0)
+    sysd = S.wave_system(m, 1.0 + m.x / 100.0, dt=3.0)
+    ctx = bief.Context(m, block_nodes=128)
+    other = np.random.default_rng(5).normal(size=m.npoin) * np.abs(sysd["b"]).max()
+
+    def solve(b, eps=1e-10):
+        ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+        return ctx.solve(sysd["x0"].copy(), b.copy(), slv=1, precon=2, eps=eps, precision=prec)
+
+    x1, i1 = solve(sysd["b"])
+    solve(other)
+    solve(sysd["b"], eps=1e-5)
+    x2, i2 = solve(sysd["b"])
+    assert i1["niter"] == i2["niter"] and i1["outer_iters"] == i2["outer_iters"]
+    assert np.array_equal(x1, x2)
+
+
 def test_context_reuse_across_preconditioners(gpu):
     """One context, successive solves with PRECON 0, 2, 5 (different diagonal
     kinds of the preconditioned matrix), each against the oracle."""
