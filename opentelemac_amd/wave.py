@@ -550,6 +550,10 @@ class DeviceWaveModel:
         last = ((nsteps - 1) // checkpoint) * checkpoint
         eta, u, v = self._zeros(), self._zeros(), self._zeros()
         checkpoints = {0: (eta, u, v)}
+        # the forward solve's initial guess at each checkpoint (the previous
+        # step's solution), so that a recomputed segment repeats the forward
+        # sweep's solves exactly
+        guesses = {0: None}
         seg_last = [(eta, u, v)] if last == 0 else None
         deta_last = []
         cost = torch.zeros((), dtype=torch.float64, device=self.dev)
@@ -564,8 +568,10 @@ class DeviceWaveModel:
                 deta_last.append(deta)
             if (n + 1) % checkpoint == 0 and n + 1 <= last:
                 checkpoints[n + 1] = (eta, u, v)
+                guesses[n + 1] = self.guess.get("forward")
         tape = {"k": k, "z": z, "g": g, "obs": obs, "nsteps": nsteps, "checkpoint": checkpoint,
-                "checkpoints": checkpoints, "last": last, "seg_last": (seg_last, deta_last)}
+                "checkpoints": checkpoints, "guesses": guesses, "last": last,
+                "seg_last": (seg_last, deta_last)}
         return cost, tape
 
     def reverse_sweep(self, tape, gcost=1.0):
@@ -583,8 +589,13 @@ class DeviceWaveModel:
             if s0 == tape["last"]:
                 states, detas = tape["seg_last"]
             else:
-                # the segment recomputed from its checkpoint
+                # the segment recomputed from its checkpoint, its first solve
+                # from the forward sweep's initial guess there
                 states, detas = [tape["checkpoints"][s0]], []
+                if tape["guesses"][s0] is None:
+                    self.guess.pop("forward", None)
+                else:
+                    self.guess["forward"] = tape["guesses"][s0]
                 for n in range(s0, s1):
                     e, uu, vv, d = self.step(*states[-1], k, z, (n + 1) * self.dt)
                     states.append((e, uu, vv))
