@@ -319,12 +319,16 @@ class WaveCost(torch.autograd.Function):
         with torch.no_grad():
             eta, u, v = model.initial_state(zb)
             checkpoints = {0: (eta, u, v)}
+            # the forward solve's initial guess at each checkpoint, so that a
+            # recomputed segment repeats the forward sweep's solves exactly
+            guesses = {0: None}
             cost = torch.zeros((), dtype=torch.float64, device=model.dev)
             for n in range(last):
                 eta, u, v = model.step(eta, u, v, k_str, zb, (n + 1) * model.dt)
                 cost = cost + ((eta[gauges] - obs[n]) ** 2).sum()
                 if (n + 1) % checkpoint == 0:
                     checkpoints[n + 1] = (eta, u, v)
+                    guesses[n + 1] = model.ops.guess.get("forward")
         # the last segment keeps its graph: the reverse sweep starts there
         # without recomputing it
         with torch.enable_grad():
@@ -333,6 +337,7 @@ class WaveCost(torch.autograd.Function):
         cost = cost + jlast.detach()
         ctx.model, ctx.nsteps, ctx.gauges, ctx.obs = model, nsteps, gauges, obs
         ctx.checkpoints, ctx.checkpoint, ctx.last_segment = checkpoints, checkpoint, seg
+        ctx.guesses = guesses
         ctx.save_for_backward(k_str, zb)
         return cost
 
@@ -351,7 +356,12 @@ class WaveCost(torch.autograd.Function):
                 if ctx.last_segment is not None:
                     seg, ctx.last_segment = ctx.last_segment, None
                 else:
-                    # the segment recomputed from its checkpoint, with its graph
+                    # the segment recomputed from its checkpoint, with its
+                    # graph, its first solve from the forward initial guess
+                    if ctx.guesses[s0] is None:
+                        model.ops.guess.pop("forward", None)
+                    else:
+                        model.ops.guess["forward"] = ctx.guesses[s0]
                     seg = _Segment(model, ctx.checkpoints[s0], k_str, zb)
                     seg.run(s0, s1, gauges, obs)
                 g = seg.grad(lam, gcost)
