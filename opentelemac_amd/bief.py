@@ -502,6 +502,10 @@ class Context:
             bh = _host_n(b, self.npoin, "solve: b")
             if not xh.flags.writeable or xh is x:
                 xh = xh.copy()
+            # SOLVE overwrites B with D*B: never write into the caller's
+            # array through the C ABI (a read-only view would be clobbered)
+            if not bh.flags.writeable or bh is b:
+                bh = bh.copy()
             rc = lib().bhip_solve(self._h, ctypes.byref(cfg), xh.ctypes.data, bh.ctypes.data,
                                   ctypes.byref(info))
             _check(rc, "bhip_solve")

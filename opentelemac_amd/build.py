@@ -34,9 +34,7 @@ def _run(cmd, cwd=None):
 
 def build_hip(verbose=False):
     out = os.path.join(HERE, "libbiefhip.so")
-    srcs = [os.path.join(CSRC, f) for f in ("bief_hip.hip", "bief_krylov.hip", "bief_elem.hip",
-                                            "layout.cpp")]
-    srcs = [s for s in srcs if os.path.exists(s)]
+    srcs = [os.path.join(CSRC, f) for f in ("bief_hip.hip", "layout.cpp")]
     tmp = out + ".tmp"
     log = _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp])
     os.replace(tmp, out)

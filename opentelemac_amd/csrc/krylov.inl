@@ -266,8 +266,10 @@ __global__ void k_scale_inv_norm(double* __restrict__ v, const double* __restric
 
 // One modified Gram-Schmidt step: H(i,j) = reduced partials of w.V_i;
 // w = src - H(i,j) V_i (gmres.f:262-265); partial sums of w.next (next = the
-// next basis vector, or w itself for the norm).
-__global__ void k_mgs(double* __restrict__ w, const double* __restrict__ src,
+// next basis vector, or w itself for the norm). w and src may be the same
+// buffer (in-place steps after the first): each element is read before it is
+// written at the same index, so neither is declared __restrict__.
+__global__ void k_mgs(double* w, const double* src,
                       const double* __restrict__ vi, const double* __restrict__ next, int n,
                       const double* __restrict__ part_in, int npart_in,
                       double* __restrict__ part_out, double* __restrict__ hij) {

@@ -325,9 +325,11 @@ SUBROUTINE SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX)
   ALLOCATE(XA(NELEM, MAX(NCOEF, 1)))
   IF (NCOEF > 0) THEN
     RC = BHIP_MATRIX_GET(H, TD, C_LOC(PA%D%R(1)), C_LOC(XA(1, 1)))
+    IF (RC /= BHIP_OK) CALL BHIP_FAIL('BHIP SOLVE: MATRIX READ-BACK FAILED', RC)
     CALL BHIP_UNPACK_XA(XA, NELEM, LDXA, NCOEF, PA%X%R)
   ELSE
     RC = BHIP_MATRIX_GET(H, TD, C_LOC(PA%D%R(1)), C_LOC(DUMMY(1)))
+    IF (RC /= BHIP_OK) CALL BHIP_FAIL('BHIP SOLVE: MATRIX READ-BACK FAILED', RC)
   END IF
   DEALLOCATE(XA)
   PA%TYPDIA = TD

@@ -155,3 +155,109 @@ def write(path, s: Serafin):
             out.append(rec(np.asarray(v, dtype=e + rtype).tobytes()))
     with open(path, "wb") as fh:
         fh.write(b"".join(out))
+
+
+# ---------------------------------------------------------------------------
+# Boundary-conditions file (.cli), read as LECLIM does
+# ---------------------------------------------------------------------------
+# Codes of the boundary-conditions file (sources/utils/bief/declarations_telemac.f:30-45)
+KENT, KENTU, KSORT, KADH, KLOG, KINC = 5, 6, 4, 0, 2, 1
+# Technical codes the solvers work with (declarations_telemac.f:58-68)
+KNEU, KDIR, KDDL = 1, 2, 3
+
+
+@dataclass
+class Boundary:
+    """The columns of a boundary-conditions file, one entry per boundary
+    point in file order (the boundary numbering K). nbor is 0-based."""
+    lihbor: np.ndarray
+    liubor: np.ndarray
+    livbor: np.ndarray
+    hbor: np.ndarray
+    ubor: np.ndarray
+    vbor: np.ndarray
+    chbord: np.ndarray
+    litbor: np.ndarray
+    tbor: np.ndarray
+    atbor: np.ndarray
+    btbor: np.ndarray
+    nbor: np.ndarray
+    colour: np.ndarray
+
+    @property
+    def nptfr(self):
+        return self.nbor.size
+
+
+def read_cli(path, npoin=None) -> Boundary:
+    """Reads a boundary-conditions file as GET_BND_VALUE_SRF
+    (sources/utils/hermes/utils_serafin.F:1608-1630) and GET_BND_NUMBERING
+    do: one list-directed record of 13 values per boundary point, LIHBOR,
+    LIUBOR, LIVBOR, HBOR, UBOR, VBOR, AUBOR (CHBORD), LITBOR, TBOR, ATBOR,
+    BTBOR, N (the global node number, 1-based) and K (the colour); whatever
+    follows the 13th value on a line is ignored. Then LECLIM's corrections
+    (sources/utils/bief/leclim.f:158-185): a no-slip code on the depth
+    becomes a wall, an incident-wave code on the depth a free exit, and
+    UBOR/VBOR are zeroed where the velocity is not prescribed.
+
+    Raises ValueError on a short record, a non-numeric field or (npoin
+    given) a node number outside 1..npoin, where the reference stops."""
+    rows = []
+    with open(path, "r", encoding="latin-1") as fh:
+        for ln, line in enumerate(fh, 1):
+            tok = line.split()
+            if not tok:
+                continue
+            if len(tok) < 13:
+                raise ValueError(f"{path}:{ln}: {len(tok)} values, 13 expected")
+            try:
+                rows.append([float(t) for t in tok[:13]])
+            except ValueError as e:
+                raise ValueError(f"{path}:{ln}: {e}") from None
+    a = np.array(rows, dtype=np.float64).reshape(-1, 13)
+    ints = lambda j: a[:, j].astype(np.int64)  # noqa: E731
+    b = Boundary(lihbor=ints(0), liubor=ints(1), livbor=ints(2), hbor=a[:, 3].copy(),
+                 ubor=a[:, 4].copy(), vbor=a[:, 5].copy(), chbord=a[:, 6].copy(),
+                 litbor=ints(7), tbor=a[:, 8].copy(), atbor=a[:, 9].copy(),
+                 btbor=a[:, 10].copy(), nbor=ints(11) - 1, colour=ints(12))
+    if b.nbor.size and (b.nbor.min() < 0 or (npoin is not None and b.nbor.max() >= npoin)):
+        raise ValueError(f"{path}: boundary node number outside 1..{npoin}")
+    # LECLIM (leclim.f:158-185)
+    b.lihbor[b.lihbor == KADH] = KLOG
+    b.lihbor[b.lihbor == KINC] = KSORT
+    b.ubor[(b.liubor != KENT) & (b.liubor != KENTU)] = 0.0
+    b.vbor[(b.livbor != KENT) & (b.livbor != KENTU)] = 0.0
+    return b
+
+
+def limpro_depth(b: Boundary) -> np.ndarray:
+    """Technical condition of the depth at each boundary point, as telemac2d
+    sets LIMPRO(K,1) for the propagation step
+    (sources/telemac2d/propin_telemac2d.f:246-258): prescribed value -> KDIR,
+    free exit -> KDDL, wall -> KNEU; any other code is an error."""
+    out = np.empty(b.nptfr, dtype=np.int32)
+    out[b.lihbor == KENT] = KDIR
+    out[b.lihbor == KSORT] = KDDL
+    out[b.lihbor == KLOG] = KNEU
+    bad = ~np.isin(b.lihbor, (KENT, KSORT, KLOG))
+    if bad.any():
+        k = int(np.nonzero(bad)[0][0])
+        raise ValueError(f"boundary point {k + 1}: depth code {int(b.lihbor[k])} "
+                         "is not a prescribed value, a free exit or a wall")
+    return out
+
+
+def depth_dirichlet(b: Boundary, npoin, values=None):
+    """Node arrays for DIRICH on the depth: (fbor, limdir) of length npoin,
+    LIMDIR = KDIR at the boundary nodes whose depth is prescribed and KDDL
+    elsewhere, FBOR the prescribed value there (values: per boundary point,
+    default HBOR) and 0 elsewhere. Pass kdir=KDIR to DIRICH with them."""
+    lim = limpro_depth(b)
+    fbor = np.zeros(npoin)
+    limdir = np.full(npoin, KDDL, dtype=np.int32)
+    d = lim == KDIR
+    v = b.hbor if values is None else np.broadcast_to(np.asarray(values, dtype=np.float64),
+                                                      (b.nptfr,))
+    fbor[b.nbor[d]] = v[d]
+    limdir[b.nbor[d]] = KDIR
+    return fbor, limdir

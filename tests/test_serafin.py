@@ -41,18 +41,69 @@ def test_gouttedo_geometry():
     assert np.array_equal(np.unique(s.ikle), np.arange(s.npoin))
 
 
-def test_gouttedo_boundary_matches_cli():
-    s = SF.read(SLF)
-    cli = np.loadtxt(CLI)
-    n = cli[:, 11].astype(np.int64)
-    k = cli[:, 12].astype(np.int64)
-    assert np.array_equal(k, np.arange(1, len(k) + 1))
-    assert np.array_equal(s.ipobo[n - 1], k)
-    assert np.count_nonzero(s.ipobo) == len(n)
-    ik = s.ikle
+def _single_edge_nodes(ik):
     edges = np.sort(np.concatenate([ik[[0, 1]], ik[[1, 2]], ik[[2, 0]]], axis=1), axis=0)
     uniq, count = np.unique(edges, axis=1, return_counts=True)
-    assert np.array_equal(np.unique(uniq[:, count == 1]), np.sort(n - 1))
+    return np.unique(uniq[:, count == 1])
+
+
+def test_gouttedo_boundary_matches_cli():
+    s = SF.read(SLF)
+    b = SF.read_cli(CLI, s.npoin)
+    k = b.colour
+    assert np.array_equal(k, np.arange(1, len(k) + 1))
+    assert np.array_equal(s.ipobo[b.nbor], k)
+    assert np.count_nonzero(s.ipobo) == b.nptfr
+    assert np.array_equal(_single_edge_nodes(s.ikle), np.sort(b.nbor))
+    # a closed basin: walls everywhere, no prescribed depth
+    assert np.all(SF.limpro_depth(b) == SF.KNEU)
+    fbor, limdir = SF.depth_dirichlet(b, s.npoin)
+    assert not np.any(limdir == SF.KDIR) and not np.any(fbor)
+
+
+def test_estimation_boundary_file():
+    """The channel of the reference's friction-estimation example
+    (examples/telemac2d/estimation): 85 boundary points, the velocity
+    prescribed on the 11 upstream points (code 4 5 5: free depth), the depth
+    prescribed on the 6 downstream points (5 4 4), walls elsewhere; the
+    boundary nodes are those of the mesh's single-triangle edges and IPOBO
+    numbers them in file order."""
+    s = SF.read(os.path.join(GOLD, "geo_estimation.slf"))
+    b = SF.read_cli(os.path.join(GOLD, "geo_estimation.cli"), s.npoin)
+    assert b.nptfr == 85 and s.npoin == 319
+    assert np.array_equal(s.ipobo[b.nbor], np.arange(1, 86))
+    assert np.array_equal(_single_edge_nodes(s.ikle), np.sort(b.nbor))
+    lim = SF.limpro_depth(b)
+    assert np.count_nonzero(lim == SF.KDIR) == 6
+    assert np.count_nonzero(lim == SF.KDDL) == 11
+    assert np.count_nonzero(lim == SF.KNEU) == 85 - 17
+    assert set(b.lihbor[lim == SF.KDIR]) == {SF.KENT}
+    assert set(b.liubor[lim == SF.KDIR]) == {SF.KSORT}
+    # the prescribed-depth nodes lie on one cross-section of the channel
+    xd = s.x[b.nbor[lim == SF.KDIR]]
+    assert np.ptp(xd) < 1e-3 * np.ptp(s.x)
+    fbor, limdir = SF.depth_dirichlet(b, s.npoin, values=1.5)
+    assert np.array_equal(np.nonzero(limdir == SF.KDIR)[0], np.sort(b.nbor[lim == SF.KDIR]))
+    assert np.all(fbor[limdir == SF.KDIR] == 1.5) and np.count_nonzero(fbor) == 6
+
+
+def test_cli_corrections_and_errors(tmp_path):
+    """LECLIM's corrections (leclim.f:158-185) and the reader's refusals."""
+    line = "{} {} {}  0.000  1.000  2.000  0.0   2  0.000  0.000  0.000 {} {}  extra text\n"
+    p = tmp_path / "a.cli"
+    p.write_text(line.format(0, 0, 5, 1, 1) + line.format(1, 6, 4, 2, 2) + "\n")
+    b = SF.read_cli(p, 2)
+    assert list(b.lihbor) == [SF.KLOG, SF.KSORT]   # no-slip -> wall, incident -> free exit
+    assert list(b.ubor) == [0.0, 1.0] and list(b.vbor) == [2.0, 0.0]
+    assert list(b.nbor) == [0, 1]
+    with pytest.raises(ValueError):
+        SF.read_cli(p, 1)                            # node 2 of a 1-node mesh
+    (tmp_path / "s.cli").write_text("2 2 2 0 0 0 0 2 0 0 0 1\n")
+    with pytest.raises(ValueError):
+        SF.read_cli(tmp_path / "s.cli")
+    (tmp_path / "c.cli").write_text(line.format(3, 2, 2, 1, 1))
+    with pytest.raises(ValueError):
+        SF.limpro_depth(SF.read_cli(tmp_path / "c.cli"))
 
 
 def test_writer_reproduces_file_bytes(tmp_path):
