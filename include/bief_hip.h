@@ -68,17 +68,20 @@ typedef struct {
   int nitmax;     /* CG/CGSTAB: iterations; GMRES: restart cycles            */
   double eps;     /* stop: ||r'||^2 <= max(||b'||^2,1) eps^2 (CG, CGSTAB);
                      ||r'|| <= max(||b'||,1) eps (GMRES)                    */
-  int precision;  /* BHIP_PREC_FP64 or BHIP_PREC_MIXED (CG only)           */
-  double inner_eps;  /* mixed: relative reduction per inner FP32 solve       */
-  int inner_nitmax;  /* mixed: iteration cap of one inner solve              */
+  int precision;  /* BHIP_PREC_FP64 or BHIP_PREC_MIXED (CG and GMRES)      */
+  double inner_eps;  /* mixed CG: relative reduction per inner FP32 solve    */
+  int inner_nitmax;  /* mixed CG: iteration cap of one inner solve (NITMAX
+                        caps the inner iterations of the whole solve)        */
 } bhip_solver_cfg;
 
 /* Outcome of a solve (BIEF SLVCFG%NIT / OK are never set by the reference;
  * its listing line carries the same numbers, gracjg.F:426-435). */
 typedef struct {
   int niter;        /* iterations M (GMRES: restart cycles)                  */
-  int inner_iters;  /* mixed: total FP32 inner CG iterations                 */
-  int outer_iters;  /* mixed: FP64 refinement sweeps                         */
+  int inner_iters;  /* mixed CG: total FP32 inner CG iterations; mixed
+                       GMRES: FP32 Arnoldi products                          */
+  int outer_iters;  /* mixed: FP64 refinement sweeps (FP64 residuals after
+                       the first)                                            */
   int relat;        /* 1: relative precision, 0: absolute (||b'|| < 1)       */
   double relres;    /* sqrt(RMRM/XL) on the preconditioned system            */
   int converged;
@@ -119,9 +122,10 @@ int bhip_layout_check(int npoin, int nelem, int ndp, const int32_t* ikle,
  * reference, which solves one mesh per run): every entry point then works on
  * the union of the meshes -- vectors are the meshes' vectors one after the
  * other, xa the meshes' element terms one after the other (nelem = the sum)
- * -- and one launch covers all meshes. bhip_solve_dev runs FP64 CG (SLV 1,
- * PRECON 0/2/5) with the scalars, stop test and iteration count of each
- * mesh its own; bhip_batch_info returns each mesh's outcome. ikle holds the
+ * -- and one launch covers all meshes. bhip_solve_dev runs CG (SLV 1,
+ * PRECON 0/2/5), FP64 or mixed, with the scalars, stop tests, refinement
+ * sweeps and iteration counts of each mesh its own; bhip_batch_info returns
+ * each mesh's outcome. ikle holds the
  * meshes' connectivities one after the other (mesh s: ndp x nelem[s], SoA,
  * its own 0-based node ids); x, y (may be NULL) the coordinates. ndp 3
  * (triangles) or 2 (segment storage). */
@@ -129,6 +133,10 @@ int bhip_batch_create(int nsys, const int* npoin, const int* nelem, int ndp,
                       const int32_t* ikle, const double* x, const double* y,
                       int block_nodes, bhip_ctx** out);
 int bhip_batch_info(const bhip_ctx* ctx, int nsys, bhip_solve_info* infos);
+/* Per-mesh completion time of the last CG solve of an ensemble (ms, device
+ * clock): from the solve's set-up kernel to the kernel that stopped that
+ * mesh (converged, NITMAX, or a zero right-hand side: 0). */
+int bhip_batch_latency(const bhip_ctx* ctx, int nsys, double* ms);
 
 /* ---- matrix ------------------------------------------------------------ */
 

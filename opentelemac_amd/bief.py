@@ -105,6 +105,7 @@ def lib():
         L.bhip_matrix_mt02aa.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _I]
         L.bhip_batch_create.argtypes = [_I, _P, _P, _I, _P, _P, _P, _I, ctypes.POINTER(_P)]
         L.bhip_batch_info.argtypes = [_P, _I, ctypes.POINTER(SolveInfo)]
+        L.bhip_batch_latency.argtypes = [_P, _I, _P]
         L.bhip_matrix_mt01pp.argtypes = [_P, _D, _P, _P, _I]
         L.bhip_matrix_mt02pp.argtypes = [_P, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I]
         L.bhip_solve_dev_internal.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
@@ -570,8 +571,8 @@ class WaveStep:
 class Ensemble(Context):
     """Several independent meshes in one context (bhip_batch_create): the
     vectors and element terms of all meshes one after the other; one launch
-    covers all meshes, and FP64 CG solves every system with its own scalars
-    and stop test."""
+    covers all meshes, and CG (FP64 or mixed) solves every system with its
+    own scalars, stop tests and refinement sweeps."""
 
     def __init__(self, meshes, block_nodes: int = 0):
         ndp = meshes[0].ndp
@@ -610,3 +611,11 @@ class Ensemble(Context):
         arr = (SolveInfo * self.nsys)()
         _check(lib().bhip_batch_info(self._h, self.nsys, arr), "bhip_batch_info")
         return [a.as_dict() for a in arr]
+
+    def latencies_ms(self):
+        """Completion time of each mesh in the last solve (ms of device
+        clock from the solve's set-up kernel to the kernel that stopped it)."""
+        out = np.zeros(self.nsys)
+        _check(lib().bhip_batch_latency(self._h, self.nsys, out.ctypes.data),
+               "bhip_batch_latency")
+        return out

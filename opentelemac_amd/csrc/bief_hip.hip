@@ -77,18 +77,23 @@ struct SolveState {
   double eps2_outer;   // outer stop threshold on the FP64 ||r'||^2
   double rr_outer;     // last FP64 ||r'||^2
   double inner_eps2;   // inner relative threshold
-  int inner_nitmax;
-  int outer_done;      // 1 when the FP64 residual met eps2_outer
-  int stalled;         // an outer test found its inner solve unfinished
+  int inner_nitmax;    // iteration cap of one inner solve
+  int nitmax_total;    // NITMAX of the whole mixed solve (inner iterations)
+  int outer_done;      // the outer loop has stopped (outer_status says why)
+  int outer_status;    // ST_CONVERGED or ST_MAXITER once outer_done
   int outer_count;     // inner solves started
   int inner_total;     // inner iterations of finished inner solves
   int done1;           // inner solve finished: (M << 2) | status, set by a kernel
                        // after the one that applied the last update
-  int stop[2];         // single-kernel inner CG: set by the kernel of iteration M
-                       // that applied the last update, (M << 2) | status, by parity
+  int fresh;           // an outer test started an inner solve whose first
+                       // kernel has not run yet (set by k_outer_check,
+                       // cleared by the first update kernel of that solve)
+  int xpar;            // which of the two FP64 solution buffers holds x
   int dpar;            // reliable updates: parity of the direction the next inner
                        // start continues from, -1 for a fresh start
   int reliable;        // mixed path: inner CG continued across outer steps
+  long long t_start;   // device clock at the set-up of the solve
+  long long t_done;    // device clock when this system stopped
 };
 
 // ---------------------------------------------------------------------------
@@ -111,51 +116,6 @@ __device__ inline double block_sum(double v, double* sh) {
   }
   __syncthreads();
   return s;
-}
-
-// four fixed-order block sums at once; results valid in thread 0
-__device__ inline void block_sum4(double (&v)[4], double* sh /* [kWaves * 4] */) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sh[w * 4 + k] = v[k];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      double s = 0.0;
-      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i * 4 + k];
-      v[k] = s;
-    }
-  }
-  __syncthreads();
-}
-
-// Fixed-order sums of n partial quadruples ([n][4], 32-byte aligned) by every
-// workgroup that calls it (identical results everywhere); valid in all threads.
-__device__ inline void reduce_partials4(const double* __restrict__ part, int n, double* sh,
-                                        double* bcast /* [4] */, double (&out)[4]) {
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  const double2* p2 = reinterpret_cast<const double2*>(part);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double2 lo = p2[2 * i], hi = p2[2 * i + 1];
-    v[0] += lo.x;
-    v[1] += lo.y;
-    v[2] += hi.x;
-    v[3] += hi.y;
-  }
-  block_sum4(v, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) bcast[k] = v[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 4; ++k) out[k] = bcast[k];
-  __syncthreads();
 }
 
 // Fixed-order sum of n partials computed by every workgroup that calls it
@@ -219,12 +179,12 @@ __device__ inline void store_relaxed(int* p, int v) {
 // ---------------------------------------------------------------------------
 enum : int { DIAG_ZERO = 0, DIAG_ONE = 1, DIAG_Q = 2 };
 // how the multiplied vector is obtained for each local node
-enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2, STAGE_CG1 = 3 };
+enum : int { STAGE_PLAIN = 0, STAGE_CGDIR = 1, STAGE_XUPD = 2 };
 // what happens at owned nodes after the element loop
 enum : int { EPI_OP = 0, EPI_RESID = 1, EPI_CGAD = 2, EPI_RESID_MIXED = 3, EPI_DOTW = 4,
-             EPI_DOT2 = 5, EPI_CG1 = 6 };
+             EPI_DOT2 = 5 };
 // whether the kernel writes a partial sum per workgroup / runs the CG control
-enum : int { FIN_NONE = 0, FIN_PARTIAL = 1, FIN_CG = 2, FIN_CG1 = 3 };
+enum : int { FIN_NONE = 0, FIN_PARTIAL = 1, FIN_CG = 2 };
 
 template <typename T, typename TA>
 struct EbeArgs {
@@ -244,9 +204,12 @@ struct EbeArgs {
   const T* r;                 // CGDIR: residual
   const T* dold;              // CGDIR: previous direction
   T* dnew;                    // CGDIR: new direction (owned nodes written)
-  const double* xd;           // XUPD: x (FP64)
+  T* xzero;                   // CGDIR (first kernel of an inner solve): its
+                              // solution, zeroed at owned nodes
+  double* xb0;                // XUPD: the two FP64 solution buffers; the
+  double* xb1;                // system's xpar names the current one, x - delta
+                              // is written to the other (owned nodes)
   const float* delta;         // XUPD: inner correction
-  double* xnew;               // XUPD: x - delta (owned nodes written)
   // epilogue
   T* out;
   const T* xin;
@@ -256,23 +219,17 @@ struct EbeArgs {
   const T* w;                 // DOTW: out = A y, partial w.out
   double* partial2;           // DOT2: out = A y, partials out.y and out.out
   float* r32;                 // RESID_MIXED: inner residual -(A x - b)
-  float* zero32;              // RESID_MIXED: inner solution reset
-  // single-kernel CG iteration (STAGE_CG1 / EPI_CG1 / FIN_CG1): r, dold and
-  // ad_in are R, D, AD of iteration M - 1; r_out, dnew and out receive those
-  // of iteration M; xs is updated in place
-  const T* ad_in;
-  T* r_out;
-  T* xs;
-  int first;                  // iteration 0: D = R, no update
   // reductions / state
   double* partial;            // this kernel's partials (one per workgroup)
   const double* part_in;      // FIN_CG: partials of the preceding update kernel
   int n_part_in;
   SolveState* st;
   int parity;                 // FIN_CG: parity of the iteration
-  int skip_if_stalled;        // first direction kernel of a scheduled inner solve
+  int skip_unless_fresh;      // first direction kernel of an inner solve: runs
+                              // only for systems whose outer test started one
   int skip_if_done;           // kernels of a running solve after it finished
-  int skip_if_outer_done;     // mixed outer residual: skip once the solve is done
+  int skip_if_outer_done;     // mixed outer residual: skip systems that stopped
+                              // or whose inner solve is still running
   // ensembles: system of each block and block range of each system (NULL
   // for one system: all blocks, partials [0, n_part_in))
   const int32_t* block_sys;
@@ -341,7 +298,7 @@ __device__ inline void load_phase(PhaseSlot<T, NDP, NC>& ps, int c, const int* s
 template <typename T>
 struct NodeIn {
   int g;
-  T r, d, ad, x;
+  T r, d;
 };
 // CG staging: the first node of each thread is loaded ahead of the scalar
 // reduction; the others kStageUnroll at a time (their loads issued together)
@@ -368,8 +325,6 @@ k_ebe(EbeArgs<T, TA> a) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
   __shared__ int sh_co[kMaxColors + 1];
-  __shared__ double sh_red4[FIN == FIN_CG1 ? kWaves * 4 : 1];
-  __shared__ double sh_b4[4];
   constexpr int npair = NDP * (NDP - 1) / 2;
   constexpr int ncoef = SYMQ ? 2 * npair : npair;
 
@@ -382,7 +337,7 @@ k_ebe(EbeArgs<T, TA> a) {
   const int sys = a.block_sys ? a.block_sys[b] : 0;
   const int pb0 = a.sys_blocks ? a.sys_blocks[sys] : 0;
   const int pb1 = a.sys_blocks ? a.sys_blocks[sys + 1] : a.n_part_in;
-  if (a.skip_if_outer_done && outer_blocked(a.st)) return;
+  if (a.skip_if_outer_done && outer_blocked(a.st + sys)) return;
   if (a.skip_if_done && load_relaxed(&a.st->done)) return;
   const int32_t* co = a.color_off + (size_t)b * (kMaxColors + 1);
   if (threadIdx.x <= kMaxColors) sh_co[threadIdx.x] = co[threadIdx.x];
@@ -411,36 +366,23 @@ k_ebe(EbeArgs<T, TA> a) {
   // ---- early exits of the CG kernels (state written by earlier kernels) --
   if constexpr (FIN == FIN_CG) {
     SolveState* st = a.st + sys;
-    if (a.skip_if_stalled && st->stalled) return;
+    if (a.skip_unless_fresh && !st->fresh) return;
     if (load_relaxed(&st->done)) return;
     if (load_relaxed(&st->final_flag)) {
       // the update kernel before this one applied the last X update
       if (b == pb0 && threadIdx.x == 0) {
+        st->t_done = wall_clock64();
         store_relaxed(&st->done, ST_MAXITER);
         store_relaxed(&st->done1, (st->final_m << 2) | ST_MAXITER);
       }
       return;
     }
   }
-  if constexpr (FIN == FIN_CG1) {
-    SolveState* st = a.st;
-    if (a.skip_if_stalled && st->stalled) return;
-    if (load_relaxed(&st->done)) return;
-    if (load_relaxed(&st->done1)) return;
-    if (!a.first) {
-      const int prev_stop = st->stop[1 - a.parity];
-      if (prev_stop) {
-        // the previous kernel finished the solve: publish it, no work
-        if (blockIdx.x == 0 && threadIdx.x == 0) store_relaxed(&st->done1, prev_stop);
-        return;
-      }
-    }
-  }
 
   // ---- node values of the staging, loaded before the scalar reduction ----
   // (the first kStagePre nodes of each thread) so that their HBM latency
   // overlaps the reduction of the previous kernel's partial sums
-  constexpr bool kPre = STAGE == STAGE_CGDIR || STAGE == STAGE_CG1;
+  constexpr bool kPre = STAGE == STAGE_CGDIR;
   constexpr int KS = kPre ? (sizeof(T) == 4 ? kStagePreF32 : kStagePre) : 1;
   NodeIn<T> pre[KS];
   auto load_node = [&](int l, NodeIn<T>& v) {
@@ -449,13 +391,6 @@ k_ebe(EbeArgs<T, TA> a) {
     if constexpr (STAGE == STAGE_CGDIR) {
       v.r = a.r[v.g];
       v.d = a.dold[v.g];
-    } else if constexpr (STAGE == STAGE_CG1) {
-      v.r = a.r[v.g];
-      if (!a.first) {
-        v.d = a.dold[v.g];
-        v.ad = a.ad_in[v.g];
-        if (owned) v.x = a.xs[v.g];
-      }
     }
   };
   if constexpr (kPre) {
@@ -480,6 +415,7 @@ k_ebe(EbeArgs<T, TA> a) {
       if (b == pb0 && threadIdx.x == 0) {
         st->final_m = m;
         st->final_rr = rr;
+        st->t_done = wall_clock64();
         store_relaxed(&st->done, ST_CONVERGED);
         store_relaxed(&st->done1, (m << 2) | ST_CONVERGED);
       }
@@ -493,54 +429,8 @@ k_ebe(EbeArgs<T, TA> a) {
     }
   }
 
-  // Single-kernel CG iteration M (inner solves of the mixed path): the
-  // updates of iteration M - 1 are applied while staging,
-  //   RO = RR / D.AD, X -= RO D, R -= RO AD, D = R + BETA D,
-  // with the new ||R||^2 = RR - 2 RO R.AD + RO^2 AD.AD from the dots of
-  // iteration M - 1 (one reduction point per iteration); the kernel that
-  // finds the stop applies the last X update and records it in stop[parity].
-  T cg1_rho = (T)0, cg1_beta = (T)0;
-  if constexpr (FIN == FIN_CG1) {
-    SolveState* st = a.st;
-    const int p = a.parity;
-    const int m = st->m[1 - p] + 1;
-    if (!a.first) {
-      double s4[4];
-      reduce_partials4(a.part_in, a.n_part_in, sh_red4, sh_b4, s4);
-      const double dad = s4[0], rad = s4[1], adad = s4[2], rrp = s4[3];
-      const double rho = rrp / dad;
-      double rr = rrp - 2.0 * rho * rad + rho * rho * adad;
-      if (rr < 0.0) rr = 0.0;
-      const double thr = st->use_rr0 ? st->eps2xl * st->rr0 : st->eps2xl;
-      const bool conv = rr <= thr;
-      const bool fin = m >= st->nitmax;
-      cg1_rho = (T)rho;
-      if (conv || fin) {
-        for (int l = threadIdx.x; l < nown; l += blockDim.x) {
-          const int g = n0 + l;
-          a.xs[g] = a.xs[g] - cg1_rho * a.dold[g];
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-          st->final_m = m;
-          st->final_rr = rr;
-          st->m[p] = m;
-          st->stop[p] = (m << 2) | (conv ? ST_CONVERGED : ST_MAXITER);
-        }
-        return;
-      }
-      cg1_beta = (T)(rr / rrp);
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->rr[p] = rr;
-        st->m[p] = m;
-      }
-    } else {
-      if (blockIdx.x == 0 && threadIdx.x == 0) st->m[p] = m;
-    }
-  }
-
   T* ys = reinterpret_cast<T*>(smem_raw);
   T* acc = ys + ((nloc + 1) & ~1);
-  T* rs = acc + ((nown + 1) & ~1);  // EPI_CG1: R of the owned nodes
 
   // ---- stage the multiplied vector (owned + halo) into LDS ---------------
   auto put = [&](int l, int g, T v) {
@@ -558,24 +448,10 @@ k_ebe(EbeArgs<T, TA> a) {
     auto stage_node = [&](int l, const NodeIn<T>& v) {
       const bool owned = l < nown;
       const int g = v.g;
-      T val;
-      if constexpr (STAGE == STAGE_CGDIR) {
-        val = v.r + (T)cg_beta * v.d;
-        if (owned) a.dnew[g] = val;
-      } else {  // STAGE_CG1
-        T rv = v.r;
-        if (a.first) {
-          val = rv;
-        } else {
-          rv = rv - cg1_rho * v.ad;
-          val = rv + cg1_beta * v.d;
-          if (owned) a.xs[g] = v.x - cg1_rho * v.d;
-        }
-        if (owned) {
-          a.r_out[g] = rv;
-          a.dnew[g] = val;
-          rs[l] = rv;
-        }
+      const T val = v.r + (T)cg_beta * v.d;
+      if (owned) {
+        a.dnew[g] = val;
+        if (a.xzero) a.xzero[g] = (T)0;
       }
       put(l, g, val);
     };
@@ -598,6 +474,16 @@ k_ebe(EbeArgs<T, TA> a) {
       }
     }
   } else {
+    // STAGE_XUPD: x - delta from the system's current FP64 buffer, written
+    // to the other one (halo nodes of other blocks read the current buffer,
+    // which no workgroup of this kernel writes)
+    const double* xcur = nullptr;
+    double* xnxt = nullptr;
+    if constexpr (STAGE == STAGE_XUPD) {
+      const int xp = a.st[sys].xpar;
+      xcur = xp ? a.xb1 : a.xb0;
+      xnxt = xp ? a.xb0 : a.xb1;
+    }
     for (int l = threadIdx.x; l < nloc; l += blockDim.x) {
       const bool owned = l < nown;
       const int g = owned ? n0 + l : a.halo[h0 + l - nown];
@@ -605,9 +491,9 @@ k_ebe(EbeArgs<T, TA> a) {
       if constexpr (STAGE == STAGE_PLAIN) {
         v = a.y[g];
       } else {  // STAGE_XUPD
-        const double xv = a.xd[g] - (double)a.delta[g];
+        const double xv = xcur[g] - (double)a.delta[g];
         v = (T)xv;
-        if (owned) a.xnew[g] = xv;
+        if (owned) xnxt[g] = xv;
       }
       put(l, g, v);
     }
@@ -656,7 +542,7 @@ k_ebe(EbeArgs<T, TA> a) {
     }
   }
   // ---- epilogue at owned nodes -------------------------------------------
-  double part = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
+  double part = 0.0, part2 = 0.0;
   for (int l = threadIdx.x; l < nown; l += blockDim.x) {
     const int g = n0 + l;
     const T v = acc[l];
@@ -685,30 +571,13 @@ k_ebe(EbeArgs<T, TA> a) {
       a.out[g] = v;
       part += (double)v * (double)ys[l];
       part2 += (double)v * (double)v;
-    } else if constexpr (EPI == EPI_CG1) {
-      // D.AD, R.AD, AD.AD, R.R of this iteration
-      a.out[g] = v;
-      const double dv = (double)v, rv = (double)rs[l];
-      part += (double)ys[l] * dv;
-      part2 += rv * dv;
-      part3 += dv * dv;
-      part4 += rv * rv;
     } else {  // EPI_RESID_MIXED: FP64 residual, inner RHS in FP32
       const double rv = (double)v - a.b[g];
       a.r32[g] = (float)(-rv);
-      a.zero32[g] = 0.0f;
       part += rv * rv;
     }
   }
-  if constexpr (FIN == FIN_CG1) {
-    double v4[4] = {part, part2, part3, part4};
-    block_sum4(v4, sh_red4);
-    if (threadIdx.x == 0) {
-      double2* o = reinterpret_cast<double2*>(a.partial + (size_t)4 * b);
-      o[0] = make_double2(v4[0], v4[1]);
-      o[1] = make_double2(v4[2], v4[3]);
-    }
-  } else if constexpr (FIN != FIN_NONE) {
+  if constexpr (FIN != FIN_NONE) {
     double s = block_sum(part, sh_red);
     if (threadIdx.x == 0) a.partial[b] = s;
     if constexpr (EPI == EPI_DOT2) {
@@ -742,6 +611,9 @@ k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
   const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
   SolveState* st = st0 + sys;
   if (load_relaxed(&st->done)) return;
+  // the first update of an inner solve retires its start (no workgroup of
+  // this kernel reads the flag)
+  if (b == pb0 && threadIdx.x == 0 && st->fresh) st->fresh = 0;
   const int n0 = node_start[b], n1 = node_start[b + 1];
   const int q = 1 - parity;  // slot of iteration m - 1
   const int status = st->status[q];
@@ -838,14 +710,12 @@ __global__ void k_reduce_into(const double* __restrict__ partial, int n, double*
   if (threadIdx.x == 0) *dst = s;
 }
 
-// an inner solve has applied its last update (either CG form)
-__device__ inline bool inner_finished(const SolveState* st) {
-  return st->done1 != 0 || st->stop[0] != 0 || st->stop[1] != 0;
-}
+// an inner solve has applied its last update
+__device__ inline bool inner_finished(const SolveState* st) { return st->done1 != 0; }
 
-// the mixed outer step must wait: solved, stalled, or its inner solve still running
+// the mixed outer step of a system waits: stopped, or its inner solve still running
 __device__ inline bool outer_blocked(const SolveState* st) {
-  return st->outer_done || st->stalled || (st->outer_count > 0 && !inner_finished(st));
+  return st->outer_done || (st->outer_count > 0 && !inner_finished(st));
 }
 
 __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int nitmax) {
@@ -860,7 +730,6 @@ __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int 
   st->final_rr = 0.0;
   st->done = 0;
   st->done1 = 0;
-  st->stop[0] = st->stop[1] = 0;
 }
 
 // GRACJG set-up (gracjg.F:189-209) from the partial sums of ||b'||^2: XL,
@@ -884,48 +753,65 @@ __global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
   st->xl = xl;
   st->zero_rhs = sqrt(xl_raw) < 1e-15;
   st->outer_done = 0;
-  st->stalled = 0;
+  st->outer_status = 0;
   st->outer_count = 0;
   st->inner_total = 0;
   st->rr_outer = 0.0;
   st->dpar = -1;
+  st->fresh = 0;
+  st->xpar = 0;
   st->reliable = mixed && reliable;
+  st->t_start = wall_clock64();
+  st->t_done = 0;
   if (!mixed) {
     reset_cg(st, eps * eps * xl, 0, nitmax);
   } else {
     st->eps2_outer = eps * eps * xl;
     st->inner_eps2 = inner_eps * inner_eps;
-    st->inner_nitmax = inner_nitmax;
+    st->inner_nitmax = inner_nitmax < 1 ? 1 : inner_nitmax;
+    st->nitmax_total = nitmax < 1 ? 1 : nitmax;
     reset_cg(st, inner_eps * inner_eps, 1, inner_nitmax);
     st->done = ST_CONVERGED;  // no inner solve until an outer test starts one
   }
   if (st->zero_rhs) {
     st->done = ST_CONVERGED;
     st->outer_done = 1;
+    st->outer_status = ST_CONVERGED;
     st->final_m = 0;
     st->final_rr = 0.0;
+    st->t_done = st->t_start;
   }
 }
 
-// Mixed precision outer test: FP64 ||r'||^2 from the residual kernel's
-// partials; converged -> stop everything, else start the next inner solve
-// (its state reset on the device, rr0 = ||r'||^2).
-__global__ void k_outer_check(const double* __restrict__ partial, int n, SolveState* st) {
+// Mixed precision outer test, one workgroup per system: FP64 ||r'||^2 from
+// the partials of the system's blocks in the residual kernel; converged or
+// out of iterations -> the system stops, else the next inner solve starts
+// (its state reset on the device, rr0 = ||r'||^2). A system whose inner
+// solve is still running had no residual this step and is left alone, as
+// the residual kernel left it (outer_blocked).
+constexpr int kMaxOuter = 100;  // refinement sweeps of one mixed solve
+__global__ void k_outer_check(const double* __restrict__ partial, int nblocks,
+                              const int32_t* __restrict__ sys_blocks, SolveState* st0) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
-  if (st->outer_done || st->stalled) return;
-  if (st->outer_count > 0 && !inner_finished(st)) {
-    if (threadIdx.x == 0) st->stalled = 1;  // inner solve still running
-    return;
-  }
-  const double rr = reduce_partials(partial, n, sh_red, &sh_b);
+  const int sys = blockIdx.x;
+  SolveState* st = st0 + sys;
+  if (outer_blocked(st)) return;
+  const int pb0 = sys_blocks ? sys_blocks[sys] : 0;
+  const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
+  const double rr = reduce_partials(partial + pb0, pb1 - pb0, sh_red, &sh_b);
   if (threadIdx.x != 0) return;
+  // the residual kernel wrote x - delta into the other solution buffer
+  st->xpar ^= 1;
   // a continued inner solve starts at M = 1 and counts one product fewer
   if (st->outer_count > 0) st->inner_total += st->final_m - (st->dpar >= 0 ? 1 : 0);
   st->rr_outer = rr;
-  if (rr <= st->eps2_outer) {
+  const int left = st->nitmax_total - st->inner_total;
+  if (rr <= st->eps2_outer || left <= 0 || st->outer_count >= kMaxOuter) {
+    st->outer_status = rr <= st->eps2_outer ? ST_CONVERGED : ST_MAXITER;
     st->outer_done = 1;
     st->done = ST_CONVERGED;
+    st->t_done = wall_clock64();
     return;
   }
   // reliable updates: the next inner solve keeps the last direction D of
@@ -940,7 +826,10 @@ __global__ void k_outer_check(const double* __restrict__ partial, int n, SolveSt
                     st->done == ST_CONVERGED;
   const double rprev = cont ? st->rr[slot] : 1.0;
   st->outer_count += 1;
-  reset_cg(st, st->inner_eps2, 1, st->inner_nitmax);
+  // the inner solve may use what is left of NITMAX (a continued solve's
+  // count starts at M = 1)
+  const int cap = (st->inner_nitmax < left ? st->inner_nitmax : left) + (cont ? 1 : 0);
+  reset_cg(st, st->inner_eps2, 1, cap);
   st->rr0 = rr;
   st->dpar = -1;
   if (cont) {
@@ -948,25 +837,34 @@ __global__ void k_outer_check(const double* __restrict__ partial, int n, SolveSt
     st->rr[1] = rprev;
     st->dpar = slot;
   }
+  st->fresh = 1;
 }
 
 // the direction a continued inner solve starts from, copied to dst (the
-// start kernel reads it while writing d0, so it cannot stay in d0)
+// start kernel reads it while writing d0, so it cannot stay in d0); one
+// workgroup per node block, for the systems whose inner solve starts
 __global__ void k_inner_dir(float* __restrict__ dst, const float* __restrict__ d0,
-                            const float* __restrict__ d1, int n, const SolveState* st) {
+                            const float* __restrict__ d1, const int32_t* __restrict__ node_start,
+                            const int32_t* __restrict__ block_sys, const SolveState* st0) {
+  const int b = blockIdx.x;
+  const SolveState* st = st0 + (block_sys ? block_sys[b] : 0);
   const int par = st->dpar;
-  if (par < 0 || st->outer_done) return;
+  if (!st->fresh || par < 0 || st->outer_done) return;
   const float* src = par ? d1 : d0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+  for (int i = node_start[b] + threadIdx.x; i < node_start[b + 1]; i += blockDim.x)
     dst[i] = src[i];
 }
 
-// mixed outer update x <- x - delta (owned nodes, in place), skipped once done
-__global__ void k_outer_xupd(double* __restrict__ x, const float* __restrict__ delta, int n,
-                             const SolveState* st) {
-  if (outer_blocked(st)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    x[i] = x[i] - (double)delta[i];
+// the solution into buffer 0 for the systems whose last outer step left it
+// in buffer 1 (one workgroup per node block)
+__global__ void k_xpar_gather(double* __restrict__ x0, const double* __restrict__ x1,
+                              const int32_t* __restrict__ node_start,
+                              const int32_t* __restrict__ block_sys, const SolveState* st0) {
+  const int b = blockIdx.x;
+  const SolveState* st = st0 + (block_sys ? block_sys[b] : 0);
+  if (!st->xpar) return;
+  for (int i = node_start[b] + threadIdx.x; i < node_start[b + 1]; i += blockDim.x)
+    x0[i] = x1[i];
 }
 
 // out[i] = in[idx[i]]
@@ -1161,13 +1059,12 @@ struct bhip_ctx {
   bhip::DevBuf<float> da32, xa32;
   // work vectors (internal order)
   bhip::DevBuf<double> vx, vb, vr, vd0, vd1, vad, vdz, vdinv, vio0, vio1;
-  bhip::DevBuf<float> fr, fd0, fd1, fad, fx0, fx1, fdz, fr1, fad1;
+  bhip::DevBuf<double> vx2;      // mixed path: second FP64 solution buffer
+  bhip::DevBuf<float> fr, fd0, fd1, fad, fx0, fdz;
   bhip::DevBuf<double> partial;  // 4 arrays of max_part: U[0], U[1], E[0], E[1]
   int max_part = 0;
-  bhip::DevBuf<double> part4;    // single-kernel CG: 2 parities x [nblocks][4]
   bhip::DevBuf<double> geo;      // element geometry per slot (bhip_ctx_set_geometry)
   bool have_geo = false;
-  bool inner_cg1 = false;        // mixed inner solver: single-kernel CG iterations
   bool reliable = true;          // mixed inner CG continued across outer steps
   bhip::DevBuf<bhip::SolveState> st;  // one per system
   // ensembles of independent meshes (nsys > 1): the system of each node
@@ -1177,6 +1074,8 @@ struct bhip_ctx {
   std::vector<int32_t> sys_blocks_h;
   std::vector<int64_t> sys_node_off, sys_elem_off;
   std::vector<bhip_solve_info> sys_info;  // per system, last solve
+  std::vector<double> sys_latency_ms;     // per system, last CG solve: device time
+                                          // from its set-up to its stop
   // captured CG iteration batches (key: precision, solution buffer)
   struct Graph {
     const void* key = nullptr;
@@ -1197,6 +1096,7 @@ struct bhip_ctx {
   int iter_hint[4] = {0, 0, 0, 0};  // FP64 CG, inner CG, inner solves, CGSTAB
   // BiCGSTAB scalars and extra partial sums; GMRES basis and scalars
   bhip::DevBuf<double> bicg_sc, bicg_part, gm, gm_scal;
+  bhip::DevBuf<float> gm32;      // mixed GMRES: FP32 basis V and AV
   // a saved copy of the matrix (OM 'M=N' target), slot layout
   bhip::DevBuf<double> da_saved, xa_saved;
   char typdia_saved = '0', typext_saved = '0';
@@ -1274,9 +1174,7 @@ static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
 template <typename T, typename TA, int STAGE, int EPI, int FIN>
 static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typdia,
                              char typext) {
-  // the single-kernel CG keeps R of the owned nodes in LDS as well
-  const size_t lds = lds_bytes(c->L, sizeof(T)) +
-                     (FIN == FIN_CG1 ? ((size_t)c->L.max_owned + 2) * sizeof(T) : 0);
+  const size_t lds = lds_bytes(c->L, sizeof(T));
   dim3 grid(c->L.nblocks), block(kThreads);
   const int diag = typdia == 'Q' ? DIAG_Q : (typdia == 'I' ? DIAG_ONE : DIAG_ZERO);
   // TYPEXT '0' is stored as zero symmetric coefficients
@@ -1391,18 +1289,10 @@ static int ctx_setup(std::unique_ptr<bhip_ctx>& c, bhip_ctx** out) {
                      c->stream, c->partial.p, 0.0, (long)4 * c->max_part);
   HIP_OK(hipGetLastError());
   if (const char* ev = getenv("BHIP_NO_GRAPH")) c->use_graphs = !(ev[0] == '1');
-  // mixed path inner CG: two kernels per iteration (GRACJG form, measured
-  // the faster) or BHIP_INNER_CG=1: one kernel per iteration
-  c->inner_cg1 = false;
-  if (const char* ev = getenv("BHIP_INNER_CG")) c->inner_cg1 = ev[0] == '1';
   // reliable updates in the two-kernel inner CG; BHIP_RELIABLE=0 restarts
   // every inner solve from D = R instead
   c->reliable = true;
   if (const char* ev = getenv("BHIP_RELIABLE")) c->reliable = ev[0] != '0';
-  HIP_OK(c->part4.alloc((size_t)8 * (L.nblocks + 16)));
-  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(8 * (L.nblocks + 16))), dim3(kThreads), 0,
-                     c->stream, c->part4.p, 0.0, (long)8 * (L.nblocks + 16));
-  HIP_OK(hipGetLastError());
   c->prefetch_j = 1;
   if (const char* ev = getenv("BHIP_PREFETCH")) {
     int v = atoi(ev);
@@ -1474,6 +1364,12 @@ int bhip_batch_info(const bhip_ctx* c, int nsys, bhip_solve_info* infos) {
   return BHIP_OK;
 }
 
+int bhip_batch_latency(const bhip_ctx* c, int nsys, double* ms) {
+  if (!c || !ms || nsys != c->nsys || (int)c->sys_latency_ms.size() != nsys) return BHIP_EBADARG;
+  for (int s = 0; s < nsys; ++s) ms[s] = c->sys_latency_ms[s];
+  return BHIP_OK;
+}
+
 
 int bhip_ctx_destroy(bhip_ctx* c) {
   if (!c) return BHIP_OK;
@@ -1484,12 +1380,11 @@ int bhip_ctx_destroy(bhip_ctx* c) {
   c->lid.release();
   for (auto* v : {&c->bicg_sc, &c->bicg_part, &c->gm, &c->gm_scal, &c->da_saved, &c->xa_saved,
                   &c->da, &c->xa, &c->vx, &c->vb, &c->vr, &c->vd0, &c->vd1, &c->vad, &c->vdz,
-                  &c->vdinv, &c->vio0, &c->vio1, &c->partial})
+                  &c->vdinv, &c->vio0, &c->vio1, &c->vx2, &c->partial})
     v->release();
-  for (auto* v : {&c->da32, &c->xa32, &c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1,
-                  &c->fdz, &c->fr1, &c->fad1})
+  for (auto* v : {&c->da32, &c->xa32, &c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fdz,
+                  &c->gm32})
     v->release();
-  c->part4.release();
   c->geo.release();
   c->st.release();
   for (auto* v : {&c->wv_geo, &c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_c, &c->wv_qx, &c->wv_qy,
@@ -1751,7 +1646,6 @@ static void init_cg_state(SolveState* sh, double eps2xl, int use_rr0, int nitmax
   sh->rr[0] = sh->rr[1] = 1.0;
   sh->done = 0;
   sh->done1 = 0;
-  sh->stop[0] = sh->stop[1] = 0;
   sh->final_flag = 0;
   sh->final_m = 0;
   sh->final_rr = 0.0;
@@ -1898,7 +1792,7 @@ static int enqueue_batches(bhip_ctx* c, hipGraphExec_t exec, int nb, TX* x, T* r
 // ||R||^2 are in part_u[0] (n_in of them)
 template <typename T, typename TA>
 static hipError_t enqueue_cg_start(bhip_ctx* c, T* r, T* d0, T* ad, const T* dz, const TA* xa,
-                                   const TA* da, int n_in, int skip_if_stalled) {
+                                   const TA* da, int n_in, T* xzero) {
   EbeArgs<T, TA> a = base_args<T, TA>(c, xa, da);
   a.r = r;
   a.dold = dz;
@@ -1908,7 +1802,10 @@ static hipError_t enqueue_cg_start(bhip_ctx* c, T* r, T* d0, T* ad, const T* dz,
   a.part_in = part_u(c, 0);
   a.n_part_in = n_in;
   a.parity = 0;
-  a.skip_if_stalled = skip_if_stalled;
+  // the first kernel of a mixed inner solve: only for systems starting one,
+  // and it zeroes their inner solution
+  a.skip_unless_fresh = xzero != nullptr;
+  a.xzero = xzero;
   return launch_ebe<T, TA, STAGE_CGDIR, EPI_CGAD, FIN_CG>(c, a, false, c->typdia, c->typext);
 }
 
@@ -1921,6 +1818,91 @@ static hipError_t enqueue_bb_setup(bhip_ctx* c, const double* b, double eps, int
                      c->L.nblocks, sys_ptr(c->sys_blocks), eps, nitmax, mixed, inner_eps,
                      inner_nitmax, reliable, c->st.p);
   return hipGetLastError();
+}
+
+static int ensure_f32(bhip_ctx* c) {
+  const Layout& L = c->L;
+  const int n = L.npoin;
+  for (auto* v : {&c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fdz}) HIP_OK(v->alloc(n));
+  HIP_OK(c->da32.alloc(n));
+  const int npair = L.ndp * (L.ndp - 1) / 2;
+  HIP_OK(c->xa32.alloc((size_t)2 * npair * L.nslot_pad));
+  hipLaunchKernelGGL(k_fill<float>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                     c->fdz.p, 0.0f, (long)n);
+  HIP_OK(hipGetLastError());
+  return BHIP_OK;
+}
+
+// device clock rate (kHz) of wall_clock64(), for the per-system latencies
+static double clock_khz(bhip_ctx* c) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess ||
+      khz <= 0)
+    return 0.0;
+  return (double)khz;
+}
+
+// per-system outcome of a CG solve (FP64 or mixed) from the device states;
+// fills c->sys_info and the ensemble summary in *info
+static int collect_infos(bhip_ctx* c, double* x, bool mixed, bhip_solve_info* info) {
+  SolveState* sh = c->st_host;
+  const int g = c->grid_nodes;
+  const double khz = clock_khz(c);
+  c->sys_latency_ms.assign(c->nsys, 0.0);
+  int maxm = 0, nconv = 0, nzero = 0, maxout = 0, maxin = 0;
+  double maxrel = 0.0;
+  for (int sy = 0; sy < c->nsys; ++sy) {
+    const SolveState& q = sh[sy];
+    bhip_solve_info& si = c->sys_info[sy];
+    std::memset(&si, 0, sizeof(si));
+    si.relat = q.xl_raw >= 1.0;
+    if (khz > 0 && q.t_done >= q.t_start) c->sys_latency_ms[sy] = (q.t_done - q.t_start) / khz;
+    if (q.zero_rhs) {
+      // X = 0 (gracjg.F:194-202) on this system's nodes
+      const int b0 = c->nsys > 1 ? c->sys_blocks_h[sy] : 0;
+      const int b1 = c->nsys > 1 ? c->sys_blocks_h[sy + 1] : c->L.nblocks;
+      const int n0 = c->L.node_start[b0], n1 = c->L.node_start[b1];
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x + n0, 0.0,
+                         (long)(n1 - n0));
+      HIP_OK(hipGetLastError());
+      si.converged = 1;
+      ++nzero;
+      ++nconv;
+      continue;
+    }
+    if (mixed) {
+      si.niter = q.inner_total;
+      si.inner_iters = q.inner_total;
+      si.outer_iters = q.outer_count;
+      si.relres = std::sqrt(q.rr_outer / q.xl);
+      si.converged = q.outer_status == ST_CONVERGED;
+    } else {
+      si.niter = q.final_m;
+      si.relres = std::sqrt(q.final_rr / q.xl);
+      si.converged = q.done == ST_CONVERGED;
+    }
+    nconv += si.converged;
+    maxm = std::max(maxm, si.niter);
+    maxin = std::max(maxin, si.inner_iters);
+    maxout = std::max(maxout, si.outer_iters);
+    maxrel = std::max(maxrel, si.relres);
+  }
+  if (nzero == c->nsys) {
+    *info = c->sys_info[0];
+    return BHIP_ZERO_RHS;
+  }
+  if (c->nsys == 1) {
+    *info = c->sys_info[0];
+  } else {
+    // the ensemble: largest counts and precision, converged if all did
+    info->niter = maxm;
+    info->inner_iters = maxin;
+    info->outer_iters = maxout;
+    info->relres = maxrel;
+    info->relat = c->sys_info[0].relat;
+    info->converged = nconv == c->nsys;
+  }
+  return info->converged ? BHIP_OK : BHIP_MAXITER;
 }
 
 // GRACJG (gracjg.F:184-420) on the device-resident preconditioned system,
@@ -1950,7 +1932,7 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
     HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
     HIP_OK((enqueue_cg_start<double, double>(c, c->vr.p, c->vd0.p, c->vad.p, c->vdz.p, c->xa.p,
-                                             c->da.p, c->L.nblocks, 0)));
+                                             c->da.p, c->L.nblocks, nullptr)));
     return enqueue_batches<double, double, double>(c, capturing ? nullptr : exec, nb, x, c->vr.p,
                                                    c->vd0.p, c->vd1.p, c->vad.p, c->xa.p,
                                                    c->da.p);
@@ -1970,115 +1952,24 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
                                                  c->vad.p, c->xa.p, c->da.p);
     if (rc) return rc;
   }
-  int maxm = 0, nconv = 0, nzero = 0;
-  double maxrel = 0.0;
-  for (int sy = 0; sy < c->nsys; ++sy) {
-    const SolveState& q = sh[sy];
-    bhip_solve_info& si = c->sys_info[sy];
-    std::memset(&si, 0, sizeof(si));
-    si.relat = q.xl_raw >= 1.0;
-    if (q.zero_rhs) {
-      // X = 0 (gracjg.F:194-202) on this system's nodes
-      const int b0 = c->nsys > 1 ? c->sys_blocks_h[sy] : 0;
-      const int b1 = c->nsys > 1 ? c->sys_blocks_h[sy + 1] : c->L.nblocks;
-      const int n0 = c->L.node_start[b0], n1 = c->L.node_start[b1];
-      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x + n0, 0.0,
-                         (long)(n1 - n0));
-      HIP_OK(hipGetLastError());
-      si.converged = 1;
-      ++nzero;
-      ++nconv;
-      continue;
-    }
-    si.niter = q.final_m;
-    si.relres = std::sqrt(q.final_rr / q.xl);
-    si.converged = q.done == ST_CONVERGED;
-    nconv += si.converged;
-    maxm = std::max(maxm, q.final_m);
-    maxrel = std::max(maxrel, si.relres);
-  }
-  (void)n;
-  if (nzero == c->nsys) {
-    *info = c->sys_info[0];
-    return BHIP_ZERO_RHS;
-  }
-  hint = maxm;
-  if (c->nsys == 1) {
-    *info = c->sys_info[0];
-  } else {
-    // the ensemble: largest count and precision, converged if all did
-    info->niter = maxm;
-    info->relres = maxrel;
-    info->relat = c->sys_info[0].relat;
-    info->converged = nconv == c->nsys;
-  }
-  return info->converged ? BHIP_OK : BHIP_MAXITER;
-}
-
-static int ensure_f32(bhip_ctx* c) {
-  const Layout& L = c->L;
-  const int n = L.npoin;
-  for (auto* v : {&c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fx1, &c->fdz, &c->fr1,
-                  &c->fad1})
-    HIP_OK(v->alloc(n));
-  HIP_OK(c->da32.alloc(n));
-  const int npair = L.ndp * (L.ndp - 1) / 2;
-  HIP_OK(c->xa32.alloc((size_t)2 * npair * L.nslot_pad));
-  hipLaunchKernelGGL(k_fill<float>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
-                     c->fdz.p, 0.0f, (long)n);
-  HIP_OK(hipGetLastError());
-  return BHIP_OK;
-}
-
-// ---- single-kernel FP32 CG (inner solves of the mixed path) --------------
-struct Cg1Bufs {
-  float* x;      // the inner solution (the correction), updated in place
-  float* r[2];   // R, D, AD by iteration parity
-  float* d[2];
-  float* ad[2];
-};
-
-static double* part4(bhip_ctx* c, int p) {
-  return c->part4.p + (size_t)p * 4 * (c->L.nblocks + 16);
-}
-
-// iteration of parity p; iteration 0 (first) takes R from r[1]
-static hipError_t launch_cg1(bhip_ctx* c, const Cg1Bufs& v, int p, int first,
-                             int skip_if_stalled) {
-  EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
-  const int q = first ? 1 : 1 - p;
-  a.r = v.r[q];
-  a.dold = v.d[q];
-  a.ad_in = v.ad[q];
-  a.r_out = v.r[p];
-  a.dnew = v.d[p];
-  a.out = v.ad[p];
-  a.xs = v.x;
-  a.partial = part4(c, p);
-  a.part_in = part4(c, 1 - p);
-  a.n_part_in = c->L.nblocks;
-  a.parity = p;
-  a.first = first;
-  a.skip_if_stalled = skip_if_stalled;
-  return launch_ebe<float, float, STAGE_CG1, EPI_CG1, FIN_CG1>(c, a, false, c->typdia,
-                                                                c->typext);
-}
-
-// iterations 1, 2, ... of a batch (parities 1, 0, 1, ...)
-static hipError_t enqueue_cg1_batch(bhip_ctx* c, int iters, const Cg1Bufs& v) {
-  for (int k = 0; k < iters; ++k) {
-    hipError_t e = launch_cg1(c, v, (k & 1) ? 0 : 1, 0, 0);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  int maxm = 0;
+  for (int sy = 0; sy < c->nsys; ++sy)
+    if (!sh[sy].zero_rhs) maxm = std::max(maxm, sh[sy].final_m);
+  if (maxm > 0) hint = maxm;
+  return collect_infos(c, x, false, info);
 }
 
 // Mixed precision CG: FP64 true residual r = A'x - b' and FP64 update
 // x <- x - delta, with delta from an FP32 EBE CG on the FP32 copy of A'
 // (inner stop ||R||^2 <= inner_eps^2 ||r||^2). Same stop rule as GRACJG on the
-// preconditioned system, evaluated with the FP64 residual. The outer loop is
-// driven on the device (k_outer_check), so the expected number of outer and
-// inner iterations is queued at once and the host waits once.
+// preconditioned system, evaluated with the FP64 residual; NITMAX caps the
+// inner iterations of the whole solve. The outer loop runs on the device,
+// per system of an ensemble: each outer step (x -= delta fused into the FP64
+// residual product, then k_outer_check) acts on the systems whose inner
+// solve has finished, each inner start on the systems an outer test has just
+// restarted, and the inner CG kernels on the systems still iterating. So
+// the expected number of outer and inner steps is queued at once and the
+// host waits once, then queues more rounds while a system has not stopped.
 static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
                     double inner_eps, int inner_nitmax, bool xa32_ready,
                     bhip_solve_info* info) {
@@ -2087,68 +1978,50 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   const int g = c->grid_nodes;
   SolveState* sh = c->st_host;
   if (ensure_f32(c)) return BHIP_EDEVICE;
+  HIP_OK(c->vx2.alloc(n));
   const int npair = L.ndp * (L.ndp - 1) / 2;
   const long ncf = (long)(c->typext == 'Q' ? 2 * npair : npair) * L.nslot_pad;
   float* delta = c->fx0.p;
-  const bool cg1 = c->inner_cg1;
-  const Cg1Bufs v1 = {delta, {c->fr.p, c->fr1.p}, {c->fd0.p, c->fd1.p}, {c->fad.p, c->fad1.p}};
   hipGraphExec_t exec;
-  int rc;
-  if (cg1)
-    rc = cached_graph(c, (const void*)delta, 2, [&]() { return enqueue_cg1_batch(c, kBatch, v1); },
-                      &exec);
-  else
-    rc = cg_batch_exec<float, float, float>(c, BHIP_PREC_MIXED, delta, c->fr.p, c->fd0.p,
-                                            c->fd1.p, c->fad.p, c->xa32.p, c->da32.p, &exec);
+  int rc = cg_batch_exec<float, float, float>(c, BHIP_PREC_MIXED, delta, c->fr.p, c->fd0.p,
+                                              c->fd1.p, c->fad.p, c->xa32.p, c->da32.p, &exec);
   if (rc) return rc;
-  // more iterations of the running inner solve (raw: plain launches, as
+  // more iterations of the running inner solves (raw: plain launches, as
   // inside a capture)
   auto more = [&](int nbat, bool raw) -> int {
-    hipGraphExec_t ex = raw ? nullptr : exec;
-    if (!cg1)
-      return enqueue_batches<float, float, float>(c, ex, nbat, delta, c->fr.p, c->fd0.p,
-                                                  c->fd1.p, c->fad.p, c->xa32.p, c->da32.p);
-    for (int k = 0; k < nbat; ++k) {
-      if (ex)
-        HIP_OK(hipGraphLaunch(ex, c->stream));
-      else
-        HIP_OK(enqueue_cg1_batch(c, kBatch, v1));
-    }
-    return BHIP_OK;
+    return enqueue_batches<float, float, float>(c, raw ? nullptr : exec, nbat, delta, c->fr.p,
+                                                c->fd0.p, c->fd1.p, c->fad.p, c->xa32.p,
+                                                c->da32.p);
   };
-  // outer step: x -= delta ; r = A x - b, inner RHS -r, delta = 0 ; test
+  // outer step: x - delta into the other solution buffer, r = A x - b with
+  // its partial sums, the inner right-hand side -r; then the test
   auto outer_step = [&]() -> int {
-    hipLaunchKernelGGL(k_outer_xupd, dim3(g), dim3(kThreads), 0, c->stream, x, delta, n,
-                       c->st.p);
-    HIP_OK(hipGetLastError());
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
-    a.y = x;
+    a.xb0 = x;
+    a.xb1 = c->vx2.p;
+    a.delta = delta;
     a.b = b;
-    a.r32 = cg1 ? v1.r[1] : c->fr.p;
-    a.zero32 = delta;
+    a.r32 = c->fr.p;
     a.partial = part_u(c, 0);
     a.skip_if_outer_done = 1;
-    HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID_MIXED, FIN_PARTIAL>(
+    HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
-    hipLaunchKernelGGL(k_outer_check, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0),
-                       L.nblocks, c->st.p);
+    hipLaunchKernelGGL(k_outer_check, dim3(c->nsys), dim3(kThreads), 0, c->stream,
+                       part_u(c, 0), L.nblocks, sys_ptr(c->sys_blocks), c->st.p);
     HIP_OK(hipGetLastError());
     return BHIP_OK;
   };
-  const bool reliable = !cg1 && c->reliable;
-  auto inner_solve = [&](int nbat, bool raw) -> int {
-    if (cg1) {
-      HIP_OK(launch_cg1(c, v1, 0, 1, 1));
-    } else {
-      // a continued solve reads its previous direction from fdz (zero for a
-      // fresh start, where BETA = 0)
-      if (reliable)
-        hipLaunchKernelGGL(k_inner_dir, dim3(g), dim3(kThreads), 0, c->stream, c->fdz.p,
-                           c->fd0.p, c->fd1.p, n, c->st.p);
-      HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
-                                             c->da32.p, L.nblocks, 1)));
-    }
-    return more(nbat, raw);
+  const bool reliable = c->reliable;
+  // the first kernel of the inner solves an outer test has just started
+  // (reliable updates: a continued solve reads its previous direction from
+  // fdz, zero for a fresh start, where BETA = 0)
+  auto inner_start = [&]() -> int {
+    if (reliable)
+      hipLaunchKernelGGL(k_inner_dir, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->fdz.p,
+                         c->fd0.p, c->fd1.p, c->node_start.p, sys_ptr(c->block_sys), c->st.p);
+    HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
+                                           c->da32.p, L.nblocks, delta)));
+    return BHIP_OK;
   };
   int& hint_in = c->iter_hint[1];   // iterations of one inner solve
   int& hint_out = c->iter_hint[2];  // inner solves of one mixed solve
@@ -2168,52 +2041,50 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax, reliable ? 1 : 0));
     int r = outer_step();
     for (int k = 0; k < nout && !r; ++k) {
-      if ((r = inner_solve(nbat, capturing))) break;
+      if ((r = inner_start())) break;
+      if ((r = more(nbat, capturing))) break;
       r = outer_step();
     }
     return r;
   };
   const uint64_t key[6] = {
-      1u | (cg1 ? 2u : 0u) | (xa32_ready ? 4u : 0u) | (reliable ? 8u : 0u) |
+      1u | (xa32_ready ? 4u : 0u) | (reliable ? 8u : 0u) |
           ((uint64_t)(uint32_t)inner_nitmax << 8),
       (uint64_t)nbat | ((uint64_t)nout << 16) | ((uint64_t)(uint32_t)nitmax << 32),
       matrix_tag(c), dbits(eps), dbits(inner_eps), (uint64_t)(uintptr_t)x};
   if ((rc = run_planned(c, key, plan))) return rc;
-  int guard = 0;
+  // further rounds while a system has not stopped: start the inner solves
+  // the last outer tests began, iterate the running ones, test again. Every
+  // round runs at least kBatch inner iterations of each running solve, and
+  // NITMAX (and kMaxOuter) stop every system, so the loop ends.
+  int nround = 1, rounds = 0;
   for (;;) {
     if (read_state(c)) return BHIP_EDEVICE;
-    if (sh->outer_done) break;
-    if (sh->inner_total >= nitmax || sh->outer_count > 100 || ++guard > 10000) break;
-    if (sh->stalled) {
-      // finish the running inner solve, then resume the outer loop
-      if (!(sh->done1 || sh->stop[0] || sh->stop[1])) {
-        if ((rc = more(1, false))) return rc;
-        continue;
-      }
-      sh->stalled = 0;
-      if (write_state(c)) return BHIP_EDEVICE;
-      if ((rc = outer_step())) return rc;
-      continue;
-    }
-    // the last outer test started an inner solve that was not queued
-    if ((rc = inner_solve(nbat, false))) return rc;
+    bool all = true;
+    for (int sy = 0; sy < c->nsys; ++sy) all = all && sh[sy].outer_done != 0;
+    if (all) break;
+    if (++rounds > 100000) return BHIP_EDEVICE;
+    if ((rc = inner_start())) return rc;
+    if ((rc = more(nround, false))) return rc;
     if ((rc = outer_step())) return rc;
+    nround = std::min(nround * 2, std::max(nbat, 1));
   }
-  info->relat = sh->xl_raw >= 1.0;
-  if (sh->zero_rhs) {
-    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, x, 0.0, (long)n);
-    HIP_OK(hipGetLastError());
-    info->converged = 1;
-    return BHIP_ZERO_RHS;
+  // the solution of every system into x
+  hipLaunchKernelGGL(k_xpar_gather, dim3(L.nblocks), dim3(kThreads), 0, c->stream, x, c->vx2.p,
+                     c->node_start.p, sys_ptr(c->block_sys), c->st.p);
+  HIP_OK(hipGetLastError());
+  int maxin = 0, maxout = 0;
+  long long sumin = 0, sumout = 0;
+  for (int sy = 0; sy < c->nsys; ++sy) {
+    maxin = std::max(maxin, sh[sy].inner_total);
+    maxout = std::max(maxout, sh[sy].outer_count);
+    sumin += sh[sy].inner_total;
+    sumout += sh[sy].outer_count;
   }
-  if (sh->outer_count > 0) hint_in = (sh->inner_total + sh->outer_count - 1) / sh->outer_count;
-  hint_out = sh->outer_count;
-  info->niter = sh->inner_total;
-  info->inner_iters = sh->inner_total;
-  info->outer_iters = sh->outer_count;
-  info->relres = std::sqrt(sh->rr_outer / sh->xl);
-  info->converged = sh->outer_done;
-  return sh->outer_done ? BHIP_OK : BHIP_MAXITER;
+  if (sumout > 0) hint_in = (int)((sumin + sumout - 1) / sumout);
+  hint_out = maxout;
+  (void)maxin;
+  return collect_infos(c, x, true, info);
 }
 
 }  // namespace bhip
@@ -2251,9 +2122,10 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
     return BHIP_EBADARG;
   if (cfg->slv != 1 && cfg->slv != 6 && cfg->slv != 7) return BHIP_EBADARG;
   if (cfg->slv == 7 && (cfg->krylov < 1 || cfg->krylov > 20)) return BHIP_EBADARG;
-  if (cfg->slv != 1 && cfg->precision == BHIP_PREC_MIXED) return BHIP_EBADARG;
-  // ensembles: FP64 CG (per-system scalars in its kernels)
-  if (c->nsys > 1 && (cfg->slv != 1 || cfg->precision != BHIP_PREC_FP64)) return BHIP_EBADARG;
+  if (cfg->slv == 6 && cfg->precision == BHIP_PREC_MIXED) return BHIP_EBADARG;
+  if (cfg->precision != BHIP_PREC_FP64 && cfg->precision != BHIP_PREC_MIXED) return BHIP_EBADARG;
+  // ensembles: CG, FP64 or mixed (per-system scalars in its kernels)
+  if (c->nsys > 1 && cfg->slv != 1) return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
@@ -2291,6 +2163,9 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
   int rc;
   if (cfg->slv == 6) {
     rc = bhip::cgstab_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
+  } else if (cfg->slv == 7 && mixed) {
+    rc = bhip::gmres_mixed(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, cfg->krylov, xa32_ready,
+                           info);
   } else if (cfg->slv == 7) {
     rc = bhip::gmres_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, cfg->krylov, info);
   } else if (cfg->precision == BHIP_PREC_MIXED) {
@@ -2367,13 +2242,7 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
     return BHIP_OK;
   };
   int rc;
-  if (precision == BHIP_PREC_MIXED && c->inner_cg1) {
-    // the single-kernel CG iteration (it includes the vector updates)
-    const Cg1Bufs v1 = {c->fx0.p, {c->fr.p, c->fr1.p}, {c->fd0.p, c->fd1.p},
-                        {c->fad.p, c->fad1.p}};
-    rc = time_it([&]() { return launch_cg1(c, v1, 1, 0, 0); }, &t_dir);
-    if (rc) return rc;
-  } else if (precision == BHIP_PREC_MIXED) {
+  if (precision == BHIP_PREC_MIXED) {
     EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
     a.r = c->fr.p;
     a.dold = c->fd0.p;

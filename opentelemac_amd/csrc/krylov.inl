@@ -252,16 +252,23 @@ __global__ void k_scale_into(double* __restrict__ x, const double* __restrict__ 
     x[i] = cst * y[i];
 }
 
+// x = c * y in the basis precision (the mixed GMRES's V1 from the FP64 residual)
+__global__ void k_scale_into_f32(float* __restrict__ x, const double* __restrict__ y, double cst,
+                                 int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] = (float)(cst * y[i]);
+}
+
 // V <- V / sqrt(nrm2[0]) unless the norm is below the Krylov-reduction
 // threshold of gmres.f:274 (the host then sees the breakdown in the
 // Hessenberg column and ignores what follows)
-__global__ void k_scale_inv_norm(double* __restrict__ v, const double* __restrict__ nrm2,
-                                 int n) {
+template <typename T>
+__global__ void k_scale_inv_norm(T* __restrict__ v, const double* __restrict__ nrm2, int n) {
   const double r = sqrt(nrm2[0]);
   if (!(r >= 1e-20)) return;
   const double s = 1.0 / r;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    v[i] = s * v[i];
+    v[i] = (T)(s * (double)v[i]);
 }
 
 // One modified Gram-Schmidt step: H(i,j) = reduced partials of w.V_i;
@@ -269,19 +276,22 @@ __global__ void k_scale_inv_norm(double* __restrict__ v, const double* __restric
 // next basis vector, or w itself for the norm). w and src may be the same
 // buffer (in-place steps after the first): each element is read before it is
 // written at the same index, so neither is declared __restrict__.
-__global__ void k_mgs(double* w, const double* src,
-                      const double* __restrict__ vi, const double* __restrict__ next, int n,
-                      const double* __restrict__ part_in, int npart_in,
+// The basis precision T is FP64, or FP32 in the mixed GMRES (H and the
+// dots stay FP64).
+template <typename T>
+__global__ void k_mgs(T* w, const T* src, const T* __restrict__ vi, const T* __restrict__ next,
+                      int n, const double* __restrict__ part_in, int npart_in,
                       double* __restrict__ part_out, double* __restrict__ hij) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
   const double h = reduce_partials(part_in, npart_in, sh_red, &sh_b);
+  const T ht = (T)h;
   double acc = 0.0;
   const bool self = (next == nullptr);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double wv = src[i] + (-h) * vi[i];
+    const T wv = src[i] + (-ht) * vi[i];
     w[i] = wv;
-    acc += wv * (self ? wv : next[i]);
+    acc += (double)wv * (double)(self ? wv : next[i]);
   }
   const double s = block_sum(acc, sh_red);
   if (threadIdx.x == 0) part_out[blockIdx.x] = s;
@@ -289,19 +299,19 @@ __global__ void k_mgs(double* w, const double* src,
 }
 
 // partial sums of av.v_i for i < k and of av.av (last column, gmres.f:285-291)
-template <int KMAX>
-__global__ void k_multidot(const double* __restrict__ av, const double* __restrict__ vbase,
-                           int k, int n, double* __restrict__ part, int stride) {
+template <int KMAX, typename T>
+__global__ void k_multidot(const T* __restrict__ av, const T* __restrict__ vbase, int k, int n,
+                           double* __restrict__ part, int stride) {
   __shared__ double sh_red[kWaves];
   double acc[KMAX + 1];
 #pragma unroll
   for (int q = 0; q <= KMAX; ++q) acc[q] = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double a = av[i];
+    const double a = (double)av[i];
     acc[0] += a * a;
 #pragma unroll
     for (int q = 0; q < KMAX; ++q)
-      if (q < k) acc[q + 1] += a * vbase[(size_t)q * n + i];
+      if (q < k) acc[q + 1] += a * (double)vbase[(size_t)q * n + i];
   }
 #pragma unroll
   for (int q = 0; q <= KMAX; ++q) {
@@ -339,14 +349,52 @@ __global__ void k_gmres_update(double* __restrict__ x, double* __restrict__ r0,
     r0[i] = rv;
   }
 }
+// the mixed GMRES's correction: X = X + E(L) V(L) in FP64 with the FP32
+// basis (the residual is then recomputed in FP64, not updated)
+__global__ void k_gmres_xupd(double* __restrict__ x, const float* __restrict__ vbase, Coef20 e,
+                             int k, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double xv = x[i];
+    for (int l = 0; l < k; ++l) xv = xv + e.e[l] * (double)vbase[(size_t)l * n + i];
+    x[i] = xv;
+  }
+}
 
-static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
-                      int krylov, bhip_solve_info* info) {
+// Restarted GMRES(K), gmres.f:171-372. T is the precision of the Krylov
+// basis and of the Arnoldi products:
+//  * double: the reference's arithmetic (R0 updated with the AV vectors,
+//    gmres.f:354-359, the stop on |E(K+1)|);
+//  * float (mixed): the Arnoldi products use the FP32 copy of the matrix
+//    and the basis V, AV is FP32 (Hessenberg entries and dots FP64); every
+//    restart is an FP64 refinement step -- X += V y in FP64, then the true
+//    residual R0 = A X - B by the FP64 product -- and the stop rule
+//    ||R0|| / max(||B||, 1) < EPS is tested on that FP64 residual.
+template <typename T>
+static int gmres_impl(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
+                      int krylov, bool xa32_ready, bhip_solve_info* info) {
+  constexpr bool kMixed = sizeof(T) == 4;
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
   const int K0 = krylov;
-  if (c->gm.alloc((size_t)2 * K0 * n)) return BHIP_EDEVICE;
+  T* Vb;
+  if constexpr (kMixed) {
+    if (ensure_f32(c)) return BHIP_EDEVICE;
+    if (c->gm32.alloc((size_t)2 * K0 * n)) return BHIP_EDEVICE;
+    Vb = c->gm32.p;
+    const int npair = L.ndp * (L.ndp - 1) / 2;
+    const long ncf = (long)(c->typext == 'Q' ? 2 * npair : npair) * L.nslot_pad;
+    if (!xa32_ready)
+      hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
+                         ncf);
+    hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
+                       (long)n);
+    HIP_OK(hipGetLastError());
+  } else {
+    (void)xa32_ready;
+    if (c->gm.alloc((size_t)2 * K0 * n)) return BHIP_EDEVICE;
+    Vb = c->gm.p;
+  }
   // device scalars: ||B||^2, ||R0||^2, then one Hessenberg column per
   // Arnoldi step (H(0..j, j) and ||w||^2), then the last column
   const int colw = K0 + 1;
@@ -354,8 +402,7 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
   const int lastoff = col(K0 - 1);
   if (c->gm_scal.alloc((size_t)lastoff + K0 + 2)) return BHIP_EDEVICE;
   if (c->bicg_part.alloc((size_t)(K0 + 2) * c->max_part)) return BHIP_EDEVICE;
-  double* Vb = c->gm.p;
-  double* AVb = c->gm.p + (size_t)K0 * n;
+  T* AVb = Vb + (size_t)K0 * n;
   auto Vp = [&](int j) { return Vb + (size_t)j * n; };
   auto AVp = [&](int j) { return AVb + (size_t)j * n; };
   double* R0 = c->vr.p;
@@ -374,8 +421,8 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
                      part_u(c, 0));
   hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 0), g,
                      ds);
-  // R0 = A X - B ; ||R0||^2 (gmres.f:198-207)
-  {
+  // R0 = A X - B ; ||R0||^2 (gmres.f:198-207), FP64
+  auto residual = [&]() -> int {
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.y = x;
     a.b = b;
@@ -383,10 +430,12 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
     a.partial = part_u(c, 1);
     HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
-  }
-  hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 1),
-                     L.nblocks, ds + 1);
-  HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream, part_u(c, 1),
+                       L.nblocks, ds + 1);
+    HIP_OK(hipGetLastError());
+    return BHIP_OK;
+  };
+  if (residual()) return BHIP_EDEVICE;
   if (fetch(hs, 2)) return BHIP_EDEVICE;
   double nb = std::sqrt(hs[0]);
   info->relat = 1;
@@ -396,27 +445,39 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
   }
   double nr0 = std::sqrt(hs[1]);
   double prec = nr0 / nb;
-  int m = 0;
+  int m = 0, products = 0;
   double H[21][20];
   double cg[20], sg[20], e[21];
   std::memset(H, 0, sizeof(H));
+  // EBE product of the Arnoldi steps: FP64 matrix, or its FP32 copy
+  auto ebe_args = [&]() {
+    if constexpr (kMixed)
+      return base_args<T, T>(c, c->xa32.p, c->da32.p);
+    else
+      return base_args<T, T>(c, c->xa.p, c->da.p);
+  };
   if (!(prec <= eps)) {
     for (;;) {
       m = m + 1;
       int k = K0;
       // V1 = -R0 / ||R0|| (gmres.f:243)
-      hipLaunchKernelGGL(k_scale_into, dim3(g), dim3(kThreads), 0, c->stream, Vp(0), R0,
-                         -1.0 / nr0, n);
+      if constexpr (kMixed)
+        hipLaunchKernelGGL(k_scale_into_f32, dim3(g), dim3(kThreads), 0, c->stream, Vp(0), R0,
+                           -1.0 / nr0, n);
+      else
+        hipLaunchKernelGGL(k_scale_into, dim3(g), dim3(kThreads), 0, c->stream, Vp(0), R0,
+                           -1.0 / nr0, n);
       HIP_OK(hipGetLastError());
-      auto matvec_w = [&](int j, const double* w0) -> int {
+      auto matvec_w = [&](int j, const T* w0) -> int {
         // AV_j = A V_j with partial sums of AV_j . w0
-        EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+        EbeArgs<T, T> a = ebe_args();
         a.y = Vp(j);
         a.out = AVp(j);
         a.w = w0;
         a.partial = mp;
-        HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_DOTW, FIN_PARTIAL>(
-            c, a, false, c->typdia, c->typext)));
+        HIP_OK((launch_ebe<T, T, STAGE_PLAIN, EPI_DOTW, FIN_PARTIAL>(c, a, false, c->typdia,
+                                                                      c->typext)));
+        ++products;
         return BHIP_OK;
       };
       // the Arnoldi steps are queued back to back: each normalises its new
@@ -427,30 +488,31 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
         // V_{j+1} from AV_j by modified Gram-Schmidt over V_0..V_j
         int npin = L.nblocks;
         for (int i = 0; i <= j; ++i) {
-          const double* src = (i == 0) ? AVp(j) : Vp(j + 1);
-          const double* next = (i < j) ? Vp(i + 1) : nullptr;
+          const T* src = (i == 0) ? AVp(j) : Vp(j + 1);
+          const T* next = (i < j) ? Vp(i + 1) : nullptr;
           double* pin = mp + (size_t)(i & 1) * stride;
           double* pout = mp + (size_t)((i + 1) & 1) * stride;
-          hipLaunchKernelGGL(k_mgs, dim3(g), dim3(kThreads), 0, c->stream, Vp(j + 1), src,
+          hipLaunchKernelGGL(k_mgs<T>, dim3(g), dim3(kThreads), 0, c->stream, Vp(j + 1), src,
                              Vp(i), next, n, pin, npin, pout, ds + col(j) + i);
           npin = g;
         }
         hipLaunchKernelGGL(k_reduce_into, dim3(1), dim3(kThreads), 0, c->stream,
                            mp + (size_t)((j + 1) & 1) * stride, g, ds + col(j) + j + 1);
-        hipLaunchKernelGGL(k_scale_inv_norm, dim3(g), dim3(kThreads), 0, c->stream, Vp(j + 1),
-                           ds + col(j) + j + 1, n);
+        hipLaunchKernelGGL(k_scale_inv_norm<T>, dim3(g), dim3(kThreads), 0, c->stream,
+                           Vp(j + 1), ds + col(j) + j + 1, n);
         HIP_OK(hipGetLastError());
       }
       // K-th column (gmres.f:285-296): AV_K, H(I,K) = AV_K.V_I,
       // H(K+1,K) = sqrt(max(||AV_K||^2 - sum H(I,K)^2, 0))
       auto last_column = [&](int kk) -> int {
-        EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+        EbeArgs<T, T> a = ebe_args();
         a.y = Vp(kk - 1);
         a.out = AVp(kk - 1);
         a.op = BHIP_OP_AY;
-        HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_OP, FIN_NONE>(
-            c, a, false, c->typdia, c->typext)));
-        hipLaunchKernelGGL((k_multidot<20>), dim3(g), dim3(kThreads), 0, c->stream,
+        HIP_OK((launch_ebe<T, T, STAGE_PLAIN, EPI_OP, FIN_NONE>(c, a, false, c->typdia,
+                                                                 c->typext)));
+        ++products;
+        hipLaunchKernelGGL((k_multidot<20, T>), dim3(g), dim3(kThreads), 0, c->stream,
                            AVp(kk - 1), Vb, kk, n, mp, stride);
         hipLaunchKernelGGL(k_reduce_many, dim3(1), dim3(kThreads), 0, c->stream, mp, g, stride,
                            kk + 1, ds + lastoff);
@@ -506,19 +568,43 @@ static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int n
       }
       Coef20 ce;
       for (int l = 0; l < 20; ++l) ce.e[l] = l < k ? e[l] : 0.0;
-      hipLaunchKernelGGL(k_gmres_update, dim3(g), dim3(kThreads), 0, c->stream, x, R0, Vb, AVb,
-                         ce, k, n);
-      HIP_OK(hipGetLastError());
-      nr0 = std::fabs(e[k]);
+      if constexpr (kMixed) {
+        // refinement: X += V y (FP64), then the true FP64 residual
+        hipLaunchKernelGGL(k_gmres_xupd, dim3(g), dim3(kThreads), 0, c->stream, x, Vb, ce, k,
+                           n);
+        HIP_OK(hipGetLastError());
+        if (residual()) return BHIP_EDEVICE;
+        if (fetch(hs, 2)) return BHIP_EDEVICE;
+        nr0 = std::sqrt(hs[1]);
+      } else {
+        hipLaunchKernelGGL(k_gmres_update, dim3(g), dim3(kThreads), 0, c->stream, x, R0, Vb,
+                           AVb, ce, k, n);
+        HIP_OK(hipGetLastError());
+        nr0 = std::fabs(e[k]);
+      }
       prec = nr0 / nb;
       if (!(prec >= eps && m < nitmax)) break;
     }
   }
   HIP_OK(hipStreamSynchronize(c->stream));
   info->niter = m;
+  if (kMixed) {
+    info->inner_iters = products;
+    info->outer_iters = m;
+  }
   info->relres = prec;
   info->converged = prec < eps;
   return prec < eps ? BHIP_OK : BHIP_MAXITER;
+}
+
+static int gmres_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
+                      int krylov, bhip_solve_info* info) {
+  return gmres_impl<double>(c, x, b, eps, nitmax, krylov, false, info);
+}
+
+static int gmres_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
+                       int krylov, bool xa32_ready, bhip_solve_info* info) {
+  return gmres_impl<float>(c, x, b, eps, nitmax, krylov, xa32_ready, info);
 }
 
 }  // namespace bhip

@@ -76,30 +76,46 @@ long meshgen_poisson_disk(const double* px, const double* py, int npoly,
                tx * ty * fgrid[(long)(iy + 1) * gnx + ix + 1];
     return r0 * f;
   };
-  // coarse inside/outside raster: cells farther than their diagonal from
-  // every polygon edge take the status of their centre, the others are
-  // tested exactly
+  // coarse inside/outside raster. Cells an edge passes through, and their
+  // eight neighbours, are tested exactly (2); every other cell lies wholly
+  // on one side of the outline and takes the status of its centre, found by
+  // a scan of each row's edge crossings.
   const int mnx = 1024, mny = 1024;
   const double mdx = (xmax - xmin) / mnx, mdy = (ymax - ymin) / mny;
-  std::vector<signed char> mask((size_t)mnx * mny);
+  std::vector<signed char> mask((size_t)mnx * mny, 0);
   {
-    const double diag = std::sqrt(mdx * mdx + mdy * mdy);
-    for (int j = 0; j < mny; ++j)
-      for (int i = 0; i < mnx; ++i) {
-        double cx = xmin + (i + 0.5) * mdx, cy = ymin + (j + 0.5) * mdy;
-        double dmin = 1e300;
-        for (int a = 0, b = npoly - 1; a < npoly; b = a++) {
-          double ex = px[a] - px[b], ey = py[a] - py[b];
-          double l2 = ex * ex + ey * ey;
-          double t = l2 > 0 ? ((cx - px[b]) * ex + (cy - py[b]) * ey) / l2 : 0.0;
-          t = std::fmin(1.0, std::fmax(0.0, t));
-          double qx = px[b] + t * ex - cx, qy = py[b] + t * ey - cy;
-          dmin = std::fmin(dmin, qx * qx + qy * qy);
-        }
-        signed char v = 2;  // 2: test exactly
-        if (std::sqrt(dmin) > diag) v = inside(px, py, npoly, cx, cy) ? 1 : 0;
-        mask[(size_t)j * mnx + i] = v;
+    auto mark = [&](double x, double y) {
+      const int i = (int)std::floor((x - xmin) / mdx), j = (int)std::floor((y - ymin) / mdy);
+      for (int jj = j - 1; jj <= j + 1; ++jj)
+        for (int ii = i - 1; ii <= i + 1; ++ii)
+          if (ii >= 0 && jj >= 0 && ii < mnx && jj < mny) mask[(size_t)jj * mnx + ii] = 2;
+    };
+    const double step = 0.25 * std::fmin(mdx, mdy);
+    for (int a = 0, b = npoly - 1; a < npoly; b = a++) {
+      const double ex = px[a] - px[b], ey = py[a] - py[b];
+      const long ns = (long)std::ceil(std::sqrt(ex * ex + ey * ey) / step) + 1;
+      for (long k = 0; k <= ns; ++k) {
+        const double t = (double)k / (double)ns;
+        mark(px[b] + t * ex, py[b] + t * ey);
       }
+    }
+    std::vector<double> xs;
+    for (int j = 0; j < mny; ++j) {
+      const double cy = ymin + (j + 0.5) * mdy;
+      xs.clear();
+      for (int a = 0, b = npoly - 1; a < npoly; b = a++)
+        if ((py[a] > cy) != (py[b] > cy))
+          xs.push_back((px[b] - px[a]) * (cy - py[a]) / (py[b] - py[a]) + px[a]);
+      for (int i = 0; i < mnx; ++i) {
+        signed char& v = mask[(size_t)j * mnx + i];
+        if (v == 2) continue;
+        const double cx = xmin + (i + 0.5) * mdx;
+        // even-odd: the number of crossings right of the centre
+        int cnt = 0;
+        for (double xc : xs) cnt += cx < xc;
+        v = (cnt & 1) ? 1 : 0;
+      }
+    }
   }
   auto in_domain = [&](double x, double y) {
     int i = (int)((x - xmin) / mdx), j = (int)((y - ymin) / mdy);

@@ -313,7 +313,7 @@ SUBROUTINE SOLVE(X, A, B, TB, CFG, INFOGR, MESH, AUX)
   SC%NITMAX = CFG%NITMAX
   SC%EPS = CFG%EPS
   SC%PRECISION = BHIP_PREC_FP64
-  IF (CFG%SLV == 1) SC%PRECISION = BHIP_PRECISION_MODE()
+  IF (CFG%SLV == 1 .OR. CFG%SLV == 7) SC%PRECISION = BHIP_PRECISION_MODE()
   SC%INNER_EPS = BHIP_INNER_EPS
   SC%INNER_NITMAX = 0
   RC = BHIP_SOLVE(H, SC, C_LOC(PX%R(1)), C_LOC(PB%R(1)), INFO)

@@ -195,3 +195,46 @@ def wave_system(mesh: Mesh, depth, dt=1.0, theta=1.0, inflow=None, amplitude=0.2
     f, da, xa, b = dirich_host(da, xa, mesh.ikle, np.zeros(mesh.npoin), fbor, limdir)
     return {"da": da, "xa": xa, "typdia": td, "typext": te, "b": b, "x0": f,
             "fbor": fbor, "limdir": limdir}
+
+
+def sea_boundary(mesh: Mesh):
+    """Boundary nodes on the seaward edge (largest x) of an estuary mesh."""
+    bnd = boundary_nodes(mesh)
+    return bnd[mesh.x[bnd] > mesh.x.max() - 1.0]
+
+
+def ensemble_sizes(nsys, seed=0, lo=5000, hi=200000):
+    """Member sizes drawn log-uniformly from [lo, hi] nodes (seeded)."""
+    rng = np.random.default_rng(seed + 3)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), nsys)).astype(int)
+
+
+def estuary_member(npoin, seed, dt=1.0):
+    """One member of an ensemble of estuaries: the unstructured estuary mesh
+    generator at npoin points with its own seed (outline, points, depth), a
+    wave-equation step (dt, theta = 1) with a 0.2 m tide on its sea boundary.
+    Returns (mesh, system dict)."""
+    from . import mesh as M
+    m = M.estuary_mesh(int(npoin), seed=int(seed))
+    depth = M.smooth_random_depth(m, seed=int(seed))
+    return m, wave_system(m, depth, dt=dt, theta=1.0, inflow=sea_boundary(m))
+
+
+def estuary_ensemble(sizes, seed=0, dt=1.0, workers=None):
+    """Members estuary_member(sizes[k], seed * 1000 + k) built by a pool of
+    threads (the sampler, the triangulation and numpy release the GIL; no
+    process is forked, so this is safe once a GPU is in use)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+    n = len(sizes)
+    if workers is None:
+        workers = max(1, min(n, os.cpu_count() or 1, 16))
+    args = [(int(s), seed * 1000 + k) for k, s in enumerate(sizes)]
+    # largest members first, so the pool does not end on one big mesh
+    order = np.argsort([-a[0] for a in args], kind="stable")
+    out = [None] * n
+    with ThreadPoolExecutor(workers) as ex:
+        futs = {int(k): ex.submit(estuary_member, args[k][0], args[k][1], dt) for k in order}
+        for k, f in futs.items():
+            out[k] = f.result()
+    return [o[0] for o in out], [o[1] for o in out]

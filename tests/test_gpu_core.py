@@ -122,6 +122,30 @@ def test_solve_nitmax_matches_oracle(gpu, nitmax):
     assert abs(info["relres"] - info_ref.relres) <= 1e-9 * info_ref.relres
 
 
+def test_mixed_nitmax_is_a_hard_cap(gpu):
+    """NITMAX caps the inner iterations of a whole mixed solve, also when the
+    context's earlier solve queued more inner solves than NITMAX allows: a
+    converged solve first, then the same system with small NITMAX values on
+    the same context."""
+    m = M.random_small_mesh(12, 9, seed=11)
+    sysd = S.wave_system(m, 2.0 + m.x / 10.0, dt=5.0)
+    ctx = bief.Context(m, block_nodes=32)
+    ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+    ctx.save_matrix()
+    _, info = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-12,
+                        nitmax=1000, precision=bief.PREC_MIXED)
+    assert info["converged"] and info["outer_iters"] >= 2
+    for nitmax in (1, 4, info["niter"] - 1):
+        ctx.restore_matrix()
+        x, inf2 = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-12,
+                            nitmax=nitmax, precision=bief.PREC_MIXED)
+        assert 1 <= inf2["niter"] <= nitmax, (nitmax, inf2)
+        assert (inf2["status"] == "maxiter") == (not inf2["converged"]), inf2
+        if nitmax < 5:
+            assert not inf2["converged"], (nitmax, inf2)
+        assert np.all(np.isfinite(x))
+
+
 def test_solve_zero_rhs(gpu):
     m = M.random_small_mesh(6, 5, seed=2)
     sysd = S.wave_system(m, np.full(m.npoin, 3.0))

@@ -1,7 +1,11 @@
 """Ensembles of independent meshes in one context (bhip_batch_create): one
-launch covers every mesh, and FP64 CG runs each system with its own scalars,
-stop test and iteration count. Each system is checked against the oracle's
-solve of that system alone."""
+launch covers every mesh, and CG runs each system with its own scalars,
+stop test and iteration count. FP64: each system is checked against the
+oracle's solve of that system alone. Mixed precision: each system's
+outer loop (FP64 residual, refinement sweeps, NITMAX) is its own, so every
+member equals, to the bit, the mixed solve of that system in a context of
+its own (same kernels, same block layout, same reduction order), and its
+FP64 residual meets GRACJG's stop rule."""
 import numpy as np
 import pytest
 
@@ -85,3 +89,64 @@ def test_ensemble_nitmax_per_system(gpu):
                                sysd["x0"], slv=1, precon=2, eps=1e-14, nitmax=7)
         assert si["niter"] == ir.niter == 7
         assert np.linalg.norm(xs - xr) <= 1e-10 * np.linalg.norm(xr)
+
+
+def _split_info(ens, infos):
+    return [(si["niter"], si["outer_iters"], si["converged"]) for si in infos]
+
+
+@pytest.mark.parametrize("bn", [0, 300])
+def test_ensemble_mixed_equals_each_system_alone(gpu, bn):
+    sizes = [600, 2500, 150, 4000, 1200]
+    meshes, systems = _members(sizes, seed=30)
+    ens = bief.Ensemble(meshes, block_nodes=bn)
+    ens.set_matrix("Q", "S", _cat(systems, "da"), _cat(systems, "xa"))
+    eps = 1e-9
+    x, info = ens.solve(_cat(systems, "x0"), _cat(systems, "b"), slv=1, precon=2, eps=eps,
+                        nitmax=2000, precision=bief.PREC_MIXED)
+    infos = ens.infos()
+    lat = ens.latencies_ms()
+    assert info["converged"] and len(infos) == len(meshes)
+    assert np.all(lat > 0) and np.all(np.isfinite(lat))
+    for m, sysd, xs, si in zip(meshes, systems, ens.split(x), infos):
+        ctx = bief.Context(m, block_nodes=bn)
+        ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+        xa, ia = ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=eps,
+                           nitmax=2000, precision=bief.PREC_MIXED)
+        assert si["converged"] and si["relres"] <= eps
+        assert (si["niter"], si["outer_iters"]) == (ia["niter"], ia["outer_iters"]), (si, ia)
+        assert np.array_equal(xs, xa)
+        # and against the oracle's FP64 solve of the system alone
+        A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+        xr, ir, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2, eps=eps,
+                               nitmax=2000)
+        assert np.linalg.norm(xs - xr) <= 1e-7 * np.linalg.norm(xr)
+        ctx.close()
+    assert info["niter"] == max(si["niter"] for si in infos)
+
+
+def test_ensemble_mixed_zero_rhs_and_nitmax(gpu):
+    """A zero right-hand side takes the X = 0 exit in its own member; NITMAX
+    caps each member's inner iterations (the whole mixed solve), and a
+    member that needs more stops unconverged while the others converge."""
+    meshes, systems = _members([800, 300, 1500], seed=40)
+    ens = bief.Ensemble(meshes)
+    ens.set_matrix("Q", "S", _cat(systems, "da"), _cat(systems, "xa"))
+    b = _cat(systems, "b").copy()
+    b[ens.node_off[1]:ens.node_off[2]] = 0.0
+    rng = np.random.default_rng(5)
+    x0 = rng.normal(size=ens.npoin)
+    x, info = ens.solve(x0.copy(), b, slv=1, precon=2, eps=1e-10, nitmax=2000,
+                        precision=bief.PREC_MIXED)
+    infos = ens.infos()
+    assert all(si["converged"] for si in infos)
+    assert np.all(ens.split(x)[1] == 0.0) and infos[1]["niter"] == 0
+    for nitmax in (3, 9):
+        ens.set_matrix("Q", "S", _cat(systems, "da"), _cat(systems, "xa"))
+        x, info = ens.solve(_cat(systems, "x0"), _cat(systems, "b"), slv=1, precon=2,
+                            eps=1e-14, nitmax=nitmax, precision=bief.PREC_MIXED)
+        infos = ens.infos()
+        assert not info["converged"]
+        for si in infos:
+            assert 1 <= si["niter"] <= nitmax and not si["converged"], (nitmax, si)
+        assert np.all(np.isfinite(x))

@@ -153,9 +153,17 @@ def test_fullsize_supg_system_and_gmres(gpu, estuary):
     limdir = np.full(m.npoin, 2, dtype=np.int32)
     limdir[sea] = 5
     f, sm2 = ctx.dirich(fn, sm, np.zeros(m.npoin), limdir)
-    x, info = ctx.solve(f, sm2, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000)
+    ctx.save_matrix()
+    x, info = ctx.solve(f.copy(), sm2.copy(), slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000)
     assert info["converged"] and abs(info["niter"] - 3) <= 1, info
     assert abs(info["relres"] - 2.229581e-09) <= 1e-3 * 2.229581e-09, info
+    # mixed precision: FP32 Arnoldi inside FP64 restarts, the stop rule on
+    # the FP64 residual; restarts within one of the FP64 count
+    ctx.restore_matrix()
+    xm, im = ctx.solve(f.copy(), sm2.copy(), slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000,
+                       precision=bief.PREC_MIXED)
+    assert im["converged"] and im["relres"] < 1e-8 and abs(im["niter"] - info["niter"]) <= 1, im
+    assert np.linalg.norm(xm - x) <= 1e-6 * np.linalg.norm(x)
 
 
 def test_fullsize_adjoint_gradient(gpu):
@@ -199,3 +207,75 @@ def test_fullsize_adjoint_gradient(gpu):
         p.copy_(p0)
         fd = (jp - jm) / (2 * h)
         assert abs(fd - float(g @ d)) <= 1e-5 * float((g * d).abs().sum()), (fd, float(g @ d))
+
+
+@pytest.fixture(scope="module")
+def ensemble256():
+    sizes = S.ensemble_sizes(256, seed=0)
+    return S.estuary_ensemble(sizes, seed=0)
+
+
+def _scaled_residuals(ens, systems, x):
+    """Per member: the residual GRACJG's stop rule tests, recomputed in FP64
+    from the solution (D (b - A x) over D b, D = diag(A)^-1/2, with the
+    ensemble's FP64 product of the original matrix), and the unscaled
+    ||b - A x|| / ||b||."""
+    b = np.concatenate([s["b"] for s in systems])
+    r = b - ens.matvec("X=AY", x)
+    d = 1.0 / np.sqrt(np.concatenate([s["da"] for s in systems]))
+    out = []
+    for rs, bs, ds in zip(ens.split(r), ens.split(b), ens.split(d)):
+        out.append((np.linalg.norm(ds * rs) / max(np.linalg.norm(ds * bs), 1.0),
+                    np.linalg.norm(rs) / np.linalg.norm(bs)))
+    return np.array(out)
+
+
+def test_fullsize_ensemble_256_estuaries(gpu, ensemble256):
+    """The batched-ensemble workload at its size: 256 unstructured estuary
+    meshes of 5k-200k nodes (log-uniform, seeded; each its own outline,
+    points, depth field and tide), solved as one ensemble by CG + PRECON 2 to
+    1e-8, FP64 and mixed. Every member: converged, its stop-rule residual
+    recomputed in FP64 from the solution at most 1e-8 (mixed; FP64 CG tests
+    its recursive residual, so 1.05e-8 there), and the unscaled relative
+    residual at most 2e-8 (the diagonal scaling moves it by up to 1.3x on
+    these members). Eight members (the smallest, the largest and six drawn
+    at random) against the oracle's solve of that system alone: FP64 within
+    1e-10 and +-1 iteration, mixed within 1e-6."""
+    meshes, systems = ensemble256
+    sizes = np.array([m.npoin for m in meshes])
+    assert len(meshes) == 256 and sizes.min() < 5500 and sizes.max() > 190_000
+    assert all(m.meta.get("kind") == "estuary" for m in meshes)
+    ens = bief.Ensemble(meshes)
+    da = np.concatenate([s["da"] for s in systems])
+    xa = np.ascontiguousarray(np.concatenate([s["xa"] for s in systems], axis=1))
+    x0 = np.concatenate([s["x0"] for s in systems])
+    b = np.concatenate([s["b"] for s in systems])
+    rng = np.random.default_rng(7)
+    pick = [int(np.argmin(sizes)), int(np.argmax(sizes))]
+    pick += [int(k) for k in rng.choice(np.setdiff1d(np.arange(256), pick), 6, replace=False)]
+    sols = {}
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ens.set_matrix("Q", "S", da, xa)
+        x, info = ens.solve(x0.copy(), b.copy(), slv=1, precon=2, eps=1e-8, nitmax=5000,
+                            precision=prec)
+        infos = ens.infos()
+        lat = ens.latencies_ms()
+        assert info["converged"] and all(si["converged"] for si in infos), prec
+        assert all(si["relres"] <= 1e-8 for si in infos), prec
+        assert np.all(lat > 0) and np.all(np.isfinite(lat))
+        ens.set_matrix("Q", "S", da, xa)
+        res = _scaled_residuals(ens, systems, x)
+        lim = 1e-8 if prec == bief.PREC_MIXED else 1.05e-8
+        assert res[:, 0].max() <= lim, (prec, res[:, 0].max(), int(np.argmax(res[:, 0])))
+        assert res[:, 1].max() <= 2e-8, (prec, res[:, 1].max())
+        sols[prec] = (ens.split(x), infos)
+    for k in pick:
+        m, sd = meshes[k], systems[k]
+        A = O.Matrix(sd["da"], sd["xa"], "Q", "S")
+        xr, ir, _, _ = O.solve(A, m.ikle, sd["b"], sd["x0"], slv=1, precon=2, eps=1e-8,
+                               nitmax=5000)
+        xs, infos = sols[bief.PREC_FP64]
+        assert abs(infos[k]["niter"] - ir.niter) <= 1, (k, infos[k], ir.niter)
+        assert np.linalg.norm(xs[k] - xr) <= 1e-10 * np.linalg.norm(xr), k
+        xm, _ = sols[bief.PREC_MIXED]
+        assert np.linalg.norm(xm[k] - xr) <= 1e-6 * np.linalg.norm(xr), k

@@ -141,6 +141,35 @@ def test_supg_solves_match_reference(gpu, drop, slv):
     assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
 
 
+def test_supg_mixed_gmres(gpu, drop):
+    """GMRES(5) in mixed precision on the reference's CVDFTR system of the
+    gouttedo mesh: FP32 Arnoldi products and basis inside FP64 restarts,
+    each restart an FP64 refinement step whose true residual carries the
+    reference's stop rule. It stops by that rule (checked again here from
+    the solution with the oracle's FP64 product), in the reference's number
+    of restarts +-1, at the reference's solution within what the tolerance
+    leaves (1e-8 relative)."""
+    g, gu, gv, fn, z = drop
+    eps = 1e-10
+    ctx = bief.Context(g, block_nodes=600)
+    sm = tracer.supg_system(ctx, g, gu, gv, np.full(g.npoin, 0.1), 0.5, fn)
+    td, da, xa = ctx.get_matrix()
+    b0 = sm.copy()
+    x, info = ctx.solve(fn.copy(), sm, slv=7, precon=2, krylov=5, eps=eps, nitmax=500,
+                        precision=bief.PREC_MIXED)
+    xr = z["solve_7_x"]
+    nr = int(z["solve_7_niter"])
+    assert info["converged"] and info["relres"] < eps, info
+    assert abs(info["niter"] - nr) <= 1, (info, nr)
+    assert info["outer_iters"] == info["niter"] and info["inner_iters"] >= 5 * info["niter"]
+    # the stop rule recomputed: ||D (A x - b)|| / max(||D b||, 1) < eps
+    A = O.Matrix(da, xa, td, "Q")
+    d = 1.0 / np.sqrt(np.abs(da))
+    r = d * (O.matvec("X=AY", A, g.ikle, x) - b0)
+    assert np.linalg.norm(r) / max(np.linalg.norm(d * b0), 1.0) < 1.01 * eps
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+
+
 def test_device_geometry_matches_caller_arrays(gpu, small):
     """Element matrices from the geometry kept on the device (set_geometry,
     then xel = yel = surfac = None) are the same bits as from the caller's
