@@ -185,6 +185,19 @@ int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
  * permutation at the boundary. */
 int bhip_solve_dev_internal(bhip_ctx* ctx, const bhip_solver_cfg* cfg,
                             double* x_dev, double* b_dev, bhip_solve_info* info);
+/* bhip_solve_dev for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. It calls no hipMalloc/hipFree: the partial
+ * sums and solver states live in the library's static device scratch (about
+ * 2.1 MB per GPU, held until the process ends) instead of the context's
+ * buffers; the context's own work vectors (allocated by bhip_ctx_create) are
+ * used as they are, and no graph is captured. Limits: FP64 CG (slv 1) only,
+ * at most 65,600 partial sums per array (node blocks) and 64 systems;
+ * anything beyond returns BHIP_EBADARG with a message. All calls on one GPU
+ * share the scratch: calls that overlap in time on one GPU race. Results
+ * equal bhip_solve_dev's to the bit. */
+int bhip_solve_dev_noalloc(bhip_ctx* ctx, const bhip_solver_cfg* cfg,
+                           double* x_dev, double* b_dev, bhip_solve_info* info);
 /* The context's node order: perm[i] is the caller's node of internal node
  * i (npoin entries, host memory). */
 int bhip_ctx_perm(const bhip_ctx* ctx, int32_t* perm);

@@ -110,6 +110,8 @@ def lib():
         L.bhip_matrix_mt02pp.argtypes = [_P, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I]
         L.bhip_solve_dev_internal.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
                                               ctypes.POINTER(SolveInfo)]
+        L.bhip_solve_dev_noalloc.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
+                                             ctypes.POINTER(SolveInfo)]
         L.bhip_ctx_perm.argtypes = [_P, _P]
         L.bhip_matrix_supg.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I]
         L.bhip_matrix_om.argtypes = [_P, ctypes.c_char_p, _D]

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1067,6 +1068,11 @@ struct bhip_ctx {
   bool have_geo = false;
   bool reliable = true;          // mixed inner CG continued across outer steps
   bhip::DevBuf<bhip::SolveState> st;  // one per system
+  // the partial sums and solver states the kernels use: the context's own
+  // buffers above, or the library's static scratch during a call of
+  // bhip_solve_dev_noalloc
+  double* part_p = nullptr;
+  bhip::SolveState* st_p = nullptr;
   // ensembles of independent meshes (nsys > 1): the system of each node
   // block, the block range and node/element offsets of each system
   int nsys = 1;
@@ -1163,8 +1169,8 @@ static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
   a.nblocks = c->L.nblocks;
   a.xa = xa;
   a.da = da;
-  a.partial = c->partial.p;
-  a.st = c->st.p;
+  a.partial = c->part_p;
+  a.st = c->st_p;
   a.block_sys = sys_ptr(c->block_sys);
   a.sys_blocks = sys_ptr(c->sys_blocks);
   return a;
@@ -1238,6 +1244,42 @@ static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typ
 
 using namespace bhip;
 
+// ---------------------------------------------------------------------------
+// static scratch of bhip_solve_dev_noalloc: the per-workgroup partial sums
+// (4 arrays of up to kScratchParts doubles) and the solver states of up to
+// kScratchSys systems. Every GPU the library runs on holds its own copy until
+// the process ends; all calls on one GPU share it.
+// ---------------------------------------------------------------------------
+constexpr int kScratchParts = 65536 + 64;  // node blocks (and node-kernel grid) + 64
+constexpr int kScratchSys = 64;
+constexpr size_t kScratchPartBytes = (size_t)4 * kScratchParts * sizeof(double);
+constexpr size_t kScratchBytes = kScratchPartBytes + (size_t)kScratchSys * sizeof(SolveState);
+static_assert(kScratchPartBytes % 256 == 0, "states start 256-byte aligned");
+static_assert(sizeof(SolveState) % sizeof(double) == 0, "states are cleared as doubles");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on device dev (the array's name is only a
+// host-side handle), looked up once per device with that device current.
+static void* scratch_addr(int dev) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  if (dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != dev) return nullptr;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "libbiefhip: hipGetSymbolAddress(scratch) failed: %s\n",
+              hipGetErrorString(e));
+      return nullptr;
+    }
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1302,6 +1344,10 @@ static int ctx_setup(std::unique_ptr<bhip_ctx>& c, bhip_ctx** out) {
   HIP_OK(hipHostMalloc((void**)&c->st_host, sizeof(SolveState) * c->nsys, hipHostMallocDefault));
   std::memset(c->st_host, 0, sizeof(SolveState) * c->nsys);
   HIP_OK(up(c->st.p, c->st_host, sizeof(SolveState) * c->nsys));
+  c->part_p = c->partial.p;
+  c->st_p = c->st.p;
+  // the static scratch of bhip_solve_dev_noalloc, located once per device
+  if (scratch_addr(c->device) == nullptr) return BHIP_EDEVICE;
   c->sys_info.assign(c->nsys, bhip_solve_info());
   if (c->nsys > 1) {
     std::vector<int32_t> bs(L.nblocks);
@@ -1619,20 +1665,20 @@ int bhip_matvec(bhip_ctx* c, int op, double cst, const double* y, double* x) {
 namespace bhip {
 
 static int read_state(bhip_ctx* c) {
-  HIP_OK(hipMemcpyAsync(c->st_host, c->st.p, sizeof(SolveState) * c->nsys, hipMemcpyDeviceToHost,
+  HIP_OK(hipMemcpyAsync(c->st_host, c->st_p, sizeof(SolveState) * c->nsys, hipMemcpyDeviceToHost,
                         c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return BHIP_OK;
 }
 
 static int write_state(bhip_ctx* c) {
-  HIP_OK(hipMemcpyAsync(c->st.p, c->st_host, sizeof(SolveState) * c->nsys, hipMemcpyHostToDevice,
+  HIP_OK(hipMemcpyAsync(c->st_p, c->st_host, sizeof(SolveState) * c->nsys, hipMemcpyHostToDevice,
                         c->stream));
   return BHIP_OK;
 }
 
-static double* part_u(bhip_ctx* c, int p) { return c->partial.p + (size_t)p * c->max_part; }
-static double* part_e(bhip_ctx* c, int p) { return c->partial.p + (size_t)(2 + p) * c->max_part; }
+static double* part_u(bhip_ctx* c, int p) { return c->part_p + (size_t)p * c->max_part; }
+static double* part_e(bhip_ctx* c, int p) { return c->part_p + (size_t)(2 + p) * c->max_part; }
 
 // host copy of the state -> device (the device never writes a field the
 // host sets here while no kernel is in flight)
@@ -1659,7 +1705,7 @@ template <typename T, typename TX>
 static hipError_t launch_update(bhip_ctx* c, TX* x, T* r, const T* d, const T* ad, int p) {
   hipLaunchKernelGGL((k_cg_update<T, TX>), dim3(c->L.nblocks), dim3(kThreads), 0, c->stream, x,
                      r, d, ad, c->node_start.p, c->L.nblocks, sys_ptr(c->block_sys),
-                     sys_ptr(c->sys_blocks), part_e(c, 1 - p), part_u(c, p), c->st.p, p);
+                     sys_ptr(c->sys_blocks), part_e(c, 1 - p), part_u(c, p), c->st_p, p);
   return hipGetLastError();
 }
 
@@ -1816,7 +1862,7 @@ static hipError_t enqueue_bb_setup(bhip_ctx* c, const double* b, double eps, int
                      c->node_start.p, part_u(c, 1));
   hipLaunchKernelGGL(k_cg_setup, dim3(c->nsys), dim3(kThreads), 0, c->stream, part_u(c, 1),
                      c->L.nblocks, sys_ptr(c->sys_blocks), eps, nitmax, mixed, inner_eps,
-                     inner_nitmax, reliable, c->st.p);
+                     inner_nitmax, reliable, c->st_p);
   return hipGetLastError();
 }
 
@@ -2007,7 +2053,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_PARTIAL>(
         c, a, false, c->typdia, c->typext)));
     hipLaunchKernelGGL(k_outer_check, dim3(c->nsys), dim3(kThreads), 0, c->stream,
-                       part_u(c, 0), L.nblocks, sys_ptr(c->sys_blocks), c->st.p);
+                       part_u(c, 0), L.nblocks, sys_ptr(c->sys_blocks), c->st_p);
     HIP_OK(hipGetLastError());
     return BHIP_OK;
   };
@@ -2018,7 +2064,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   auto inner_start = [&]() -> int {
     if (reliable)
       hipLaunchKernelGGL(k_inner_dir, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->fdz.p,
-                         c->fd0.p, c->fd1.p, c->node_start.p, sys_ptr(c->block_sys), c->st.p);
+                         c->fd0.p, c->fd1.p, c->node_start.p, sys_ptr(c->block_sys), c->st_p);
     HIP_OK((enqueue_cg_start<float, float>(c, c->fr.p, c->fd0.p, c->fad.p, c->fdz.p, c->xa32.p,
                                            c->da32.p, L.nblocks, delta)));
     return BHIP_OK;
@@ -2071,7 +2117,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   }
   // the solution of every system into x
   hipLaunchKernelGGL(k_xpar_gather, dim3(L.nblocks), dim3(kThreads), 0, c->stream, x, c->vx2.p,
-                     c->node_start.p, sys_ptr(c->block_sys), c->st.p);
+                     c->node_start.p, sys_ptr(c->block_sys), c->st_p);
   HIP_OK(hipGetLastError());
   int maxin = 0, maxout = 0;
   long long sumin = 0, sumout = 0;
@@ -2207,6 +2253,49 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
 int bhip_solve_dev_internal(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev,
                             double* b_dev, bhip_solve_info* info) {
   return solve_dev(c, cfg, x_dev, b_dev, info, true);
+}
+
+int bhip_solve_dev_noalloc(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev,
+                           double* b_dev, bhip_solve_info* info) {
+  if (!c || !cfg) return BHIP_EBADARG;
+  if (cfg->slv != 1 || cfg->precision != BHIP_PREC_FP64) {
+    fprintf(stderr, "libbiefhip: bhip_solve_dev_noalloc runs FP64 CG (slv 1) only\n");
+    return BHIP_EBADARG;
+  }
+  if (c->max_part > kScratchParts || c->nsys > kScratchSys) {
+    fprintf(stderr,
+            "libbiefhip: bhip_solve_dev_noalloc: %d partial sums and %d systems exceed the "
+            "static scratch (%d and %d)\n",
+            c->max_part, c->nsys, kScratchParts, kScratchSys);
+    return BHIP_EBADARG;
+  }
+  unsigned char* base = static_cast<unsigned char*>(scratch_addr(c->device));
+  if (!base) return BHIP_EDEVICE;
+  // this call's partial sums and states in the scratch, graphs off (a graph
+  // holds the addresses it was captured with)
+  struct Restore {
+    bhip_ctx* c;
+    double* part;
+    SolveState* st;
+    bool graphs;
+    ~Restore() {
+      c->part_p = part;
+      c->st_p = st;
+      c->use_graphs = graphs;
+    }
+  } restore{c, c->part_p, c->st_p, c->use_graphs};
+  c->part_p = reinterpret_cast<double*>(base);
+  c->st_p = reinterpret_cast<SolveState*>(base + kScratchPartBytes);
+  c->use_graphs = false;
+  // the scratch keeps the previous call's contents: clear what this call uses
+  const long nd = (long)4 * c->max_part;
+  const long ns = (long)(sizeof(SolveState) * c->nsys / sizeof(double));
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for((int)nd)), dim3(kThreads), 0, c->stream,
+                     c->part_p, 0.0, nd);
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kThreads), 0, c->stream,
+                     reinterpret_cast<double*>(c->st_p), 0.0, ns);
+  HIP_OK(hipGetLastError());
+  return solve_dev(c, cfg, x_dev, b_dev, info, false);
 }
 
 int bhip_ctx_perm(const bhip_ctx* c, int32_t* perm) {

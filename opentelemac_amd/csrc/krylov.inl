@@ -180,7 +180,7 @@ static int cgstab_fp64(bhip_ctx* c, double* x, const double* b, double eps, int 
   hipLaunchKernelGGL(k_bicg_init, dim3(g), dim3(kThreads), 0, c->stream, R, P, Q, V, b, n,
                      prr[0], prp[0]);
   hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(kThreads), 0, c->stream, prr[0], g, eps,
-                     nitmax, c->st.p, sc);
+                     nitmax, c->st_p, sc);
   HIP_OK(hipGetLastError());
   // the two EBE kernels of an iteration write their partials past the
   // four node-kernel arrays
@@ -193,7 +193,7 @@ static int cgstab_fp64(bhip_ctx* c, double* x, const double* b, double eps, int 
   auto enqueue_iter = [&]() -> int {
     const int p = (it & 1) ? 0 : 1;  // iteration it+1
     hipLaunchKernelGGL(k_bicg_a, dim3(g), dim3(kThreads), 0, c->stream, Q, R, V, n, prr[1 - p],
-                       prp[1 - p], g, c->st.p, sc, p);
+                       prp[1 - p], g, c->st_p, sc, p);
     HIP_OK(hipGetLastError());
     {
       EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
@@ -206,7 +206,7 @@ static int cgstab_fp64(bhip_ctx* c, double* x, const double* b, double eps, int 
           c, a, false, c->typdia, c->typext)));
     }
     hipLaunchKernelGGL(k_bicg_b, dim3(g), dim3(kThreads), 0, c->stream, S, R, V, n, ppv,
-                       L.nblocks, c->st.p, sc);
+                       L.nblocks, c->st_p, sc);
     HIP_OK(hipGetLastError());
     {
       EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
@@ -219,7 +219,7 @@ static int cgstab_fp64(bhip_ctx* c, double* x, const double* b, double eps, int 
           c, a, false, c->typdia, c->typext)));
     }
     hipLaunchKernelGGL(k_bicg_c, dim3(g), dim3(kThreads), 0, c->stream, x, R, Q, S, T, P, n,
-                       pts, ptt, L.nblocks, prr[p], prp[p], c->st.p, sc);
+                       pts, ptt, L.nblocks, prr[p], prp[p], c->st_p, sc);
     HIP_OK(hipGetLastError());
     ++it;
     return BHIP_OK;
