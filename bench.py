@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the BIEF linear-system hot path on one MI355X.
 
-Workload (the unstructured 2M-node configuration): seeded Poisson-disk
+Headline (the unstructured 2M-node configuration): seeded Poisson-disk
 points in a 50 km x 30 km estuary polygon with a 4x denser 2 km channel,
 Delaunay triangulated (~2.0 M nodes, ~4.0 M P1 triangles); depth 0.5-40 m
 from a seeded sum of 32 Gaussians; one telemac2d wave-equation step
@@ -15,7 +15,15 @@ SOLVE, data resident in HBM.
 `value` is mixed-precision solves/s (FP32 EBE CG inside FP64 iterative
 refinement); the FP64 path is timed the same way and reported beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+The default run (--workload all) measures every other configuration in the
+same process and adds them to the one JSON line under "configs": the 20k
+channel against the reference's CPU SOLVE, the 2M SUPG tracer system by
+GMRES(5) (FP64 and mixed), the 500k-node adjoint (gradient/forward cost,
+finite-difference and tangent-linear checks), the 256-member ensemble
+(systems/s, per-system latency) and the 5M-node prism systems; each with
+the roofline figure of its dominant kernel.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|...]
 """
 from __future__ import annotations
 
@@ -109,11 +117,11 @@ def main():
     ap.add_argument("--cache", default=None, help="npz file to reuse the synthetic workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (profiler counter scale)")
-    ap.add_argument("--workload", default="estuary",
-                    choices=["estuary", "prisms", "adjoint", "ensemble", "supg"],
-                    help="estuary: the 2M-node wave-equation CG (the headline line); prisms: "
-                         "the 5M-node sigma-mesh diffusion and Poisson systems (parity "
-                         "configuration, measured the same way)")
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "estuary", "channel", "prisms", "adjoint", "ensemble", "supg"],
+                    help="all (default): the headline 2M-node estuary CG line with every other "
+                         "configuration measured in the same run as sub-objects of 'configs'; "
+                         "the others run one configuration alone")
     ap.add_argument("--adj-fd", type=int, default=20,
                     help="adjoint workload: nodes per parameter checked by central differences")
     args = ap.parse_args()
@@ -138,18 +146,44 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": src.numel() * 8,
                           "launches": 5}), flush=True)
         return
-    if args.workload == "prisms":
-        return bench_prisms(args, dev)
-    if args.workload == "ensemble":
-        print(json.dumps(bench_ensemble(args)), flush=True)
+    single = {"channel": lambda: bench_channel(args),
+              "prisms": lambda: bench_prisms(args),
+              "ensemble": lambda: bench_ensemble(args),
+              "adjoint": lambda: bench_adjoint(args, n_fd=args.adj_fd),
+              "supg": lambda: bench_supg(args)}
+    if args.workload in single:
+        print(json.dumps(single[args.workload]()), flush=True)
         return
-    if args.workload == "adjoint":
-        print(json.dumps(bench_adjoint(args, n_fd=args.adj_fd)), flush=True)
-        return
-    if args.workload == "supg":
-        print(json.dumps(bench_supg(args)), flush=True)
-        return
+    t_all = time.time()
     m, sysd = build_workload(args.npoin, args.seed, args.cache)
+    line = bench_estuary(args, dev, m, sysd)
+    if args.workload == "all":
+        # every other configuration in the same run, each reported (or its
+        # error), never silently left out
+        configs = {}
+        for name, fn in (("channel_20k", lambda: bench_channel(args)),
+                         ("supg_2m", lambda: bench_supg(args, mesh=m)),
+                         ("adjoint_500k", lambda: bench_adjoint(args, n_fd=args.adj_fd)),
+                         ("ensemble_256", lambda: bench_ensemble(args)),
+                         ("prisms_5m", lambda: bench_prisms(args))):
+            t = time.time()
+            try:
+                configs[name] = fn()
+            except Exception as e:  # reported in the line
+                configs[name] = {"error": repr(e)}
+            configs[name]["wall_s"] = round(time.time() - t, 1)
+            log(f"{name} done in {time.time() - t:.1f}s")
+            torch.cuda.empty_cache()
+        line["configs"] = configs
+    line["wall_s"] = round(time.time() - t_all, 1)
+    print(json.dumps(line), flush=True)
+
+
+def bench_estuary(args, dev, m, sysd):
+    """The headline line: the 2M-node estuary wave-equation CG, FP64 and
+    mixed, its roofline and the CPU baseline."""
+    import torch
+    from opentelemac_amd import bief
     t = time.time()
     ctx = bief.Context(m, block_nodes=args.block_nodes)
     st = ctx.stats()
@@ -245,12 +279,9 @@ def main():
     # algorithmic bytes per launch of the dominant kernels (int32
     # connectivity, 3 coefficients per element, node vectors):
     #  * FP64 CG direction + EBE mat-vec + D.AD: R, D_old read; D, AD written
-    #  * mixed inner CG, one kernel per iteration (ms_upd == 0): R, D, AD, X
-    #    read and written (the vector updates are fused into the mat-vec)
-    #  * mixed inner CG, two kernels: as FP64, in FP32
+    #  * mixed inner CG: the same kernel in FP32
     bytes_f64 = ne * (3 * 8 + 12) + nprime * 4 * 8
-    fused = results["mixed"]["ms_upd_kernel"] == 0.0
-    bytes_f32 = ne * (3 * 4 + 12) + nprime * (8 if fused else 4) * 4
+    bytes_f32 = ne * (3 * 4 + 12) + nprime * 4 * 4
     ach32 = bytes_f32 / (results["mixed"]["ms_dir_kernel"] * 1e-3) / 1e9
     ach64 = bytes_f64 / (results["fp64"]["ms_dir_kernel"] * 1e-3) / 1e9
     traffic = None
@@ -259,8 +290,7 @@ def main():
         with open(tpath) as fh:
             tj = json.load(fh)
         traffic = tj.get("mixed_bytes_per_launch")
-    kname = ("k_ebe<float,...,STAGE_CG1,EPI_CG1,FIN_CG1> (mixed inner CG iteration)" if fused
-             else "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG direction)")
+    kname = "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG direction)"
     roofline = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kname,
@@ -313,7 +343,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    ctx.close()
+    return line
+
+
 
 
 def _copy_rate(dev, nbytes, reps=50):
@@ -336,7 +369,28 @@ def _copy_rate(dev, nbytes, reps=50):
     return {"bytes": moved, "avg_launch_ms": ms, "GBps": round(moved / (ms * 1e-3) / 1e9, 1)}
 
 
-def bench_prisms(args, dev):
+_MESHES = {}
+
+
+def estuary(npoin, seed):
+    """The estuary mesh generator's mesh, built once per run and size."""
+    from opentelemac_amd import mesh as M
+    key = (int(npoin), int(seed))
+    if key not in _MESHES:
+        _MESHES[key] = M.estuary_mesh(int(npoin), seed=int(seed))
+    return _MESHES[key]
+
+
+def _roof(nbytes, ms):
+    """Algorithmic rate of a kernel moving nbytes per launch in ms, and its
+    fraction of the 8 TB/s HBM peak."""
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "algorithmic_bytes_per_launch": int(nbytes), "avg_launch_ms": ms,
+            "achieved": round(gbps, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbps / HBM_PEAK_GBS, 4)}
+
+
+def bench_prisms(args):
     """5,014,910-node prism mesh: a 501,491-node estuary base (the 2M
     generator at 500k points) extruded to 10 sigma planes over the seeded
     depth; 8,996,184 prisms. Two systems, assembled on the device:
@@ -348,8 +402,9 @@ def bench_prisms(args, dev):
     preconditioning to eps = 1e-8, FP64 and mixed."""
     import torch
     from opentelemac_amd import bief, mesh as M
+    dev = torch.device("cuda:0")
     t = time.time()
-    base = M.estuary_mesh(500_000, seed=args.seed)
+    base = estuary(500_000, args.seed)
     depth = M.smooth_random_depth(base, seed=args.seed)
     nplan = 10
     p = M.extrude_prisms(base, nplan, -depth, np.zeros(base.npoin))
@@ -432,10 +487,14 @@ def bench_prisms(args, dev):
                           "converged": bool(infos[-1]["converged"]),
                           "ms_dir_kernel": ms_dir, "ms_upd_kernel": ms_upd}
             log(name, pname, json.dumps(res[pname]))
-        # algorithmic bytes of the FP64 CG direction + mat-vec kernel: 15
+        # algorithmic bytes of the CG direction + mat-vec kernel: 15
         # coefficients and 6 int32 node ids per prism, 4 node vectors
         b64 = p.nelem * (15 * 8 + 24) + p.npoin * 4 * 8
+        b32 = p.nelem * (15 * 4 + 24) + p.npoin * 4 * 4
         res["fp64_kernel_GBps"] = round(b64 / (res["fp64"]["ms_dir_kernel"] * 1e-3) / 1e9, 1)
+        res["roofline"] = _roof(b32, res["mixed"]["ms_dir_kernel"])
+        res["roofline"]["kernel"] = "k_ebe<float,...,6,...,STAGE_CGDIR,EPI_CGAD,FIN_CG>"
+        res["roofline_fp64"] = _roof(b64, res["fp64"]["ms_dir_kernel"])
         res["mixed_speedup_vs_fp64"] = round(res["fp64"]["ms_per_solve"] /
                                              res["mixed"]["ms_per_solve"], 3)
         if asm:
@@ -443,7 +502,8 @@ def bench_prisms(args, dev):
             res["assembly"] = ("MT01PP / dt + MT02PP ('M=M+N') on the device from the geometry "
                                "kept there, host node elevations and viscosities in")
         out["systems"][name] = res
-    print(json.dumps(out), flush=True)
+    ctx.close()
+    return out
 
 
 def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
@@ -467,10 +527,9 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     import torch.autograd.forward_ad as fwAD
     from opentelemac_amd import bief, mesh as M, system as S, wave as W
     t = time.time()
-    m = M.estuary_mesh(npoin, seed=args.seed)
+    m = estuary(npoin, args.seed)
     depth = M.smooth_random_depth(m, seed=args.seed)
-    bnd = S.boundary_nodes(m)
-    sea = bnd[m.x[bnd] > m.x.max() - 1.0]
+    sea = S.sea_boundary(m)
     log(f"adjoint mesh {m.npoin} nodes, built in {time.time() - t:.1f}s")
     kw = dict(dt=2.0, amp=0.2, period=600.0, precision=bief.PREC_MIXED, eps=1e-10)
     model = W.DeviceWaveModel(m, sea, **kw)
@@ -544,7 +603,11 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
         jd = tl_model.run(fwAD.make_dual(k, dk), fwAD.make_dual(zb, dz), nsteps, gauges, obs)
         tl = float(fwAD.unpack_dual(jd).tangent)
     adj = float((gk * dk).sum() + (gz * dz).sum())
-    return {"workload": "wave-equation adjoint, 500k nodes, 50 steps, checkpoints every 10",
+    # the dominant kernel: the inner CG's direction + EBE product (FP32)
+    ms_dir, _ = model.ctx.time_cg_kernels(bief.PREC_MIXED, reps=50)
+    roof = _roof(m.nelem * (3 * 4 + 12) + m.npoin * 4 * 4, ms_dir)
+    roof["kernel"] = "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG direction)"
+    return {"roofline": roof,"workload": "wave-equation adjoint, 500k nodes, 50 steps, checkpoints every 10",
             "npoin": m.npoin, "nelem": m.nelem, "steps": nsteps, "cost": j0,
             "forward_s": t_fwd, "gradient_s": t_grad,
             "adjoint_over_forward": round(t_grad / t_fwd, 3),
@@ -559,21 +622,21 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
             "solver": "mixed (FP32 EBE CG, FP64 refinement), eps 1e-10"}
 
 
-def bench_supg(args, dt=10.0, nu=0.1):
+def bench_supg(args, dt=10.0, nu=0.1, mesh=None):
     """The non-symmetric system of the 2M-node estuary: CVDFTR's SUPG tracer
     system (classical SUPG, implicit P1 diffusion nu = 0.1 m^2/s, dt = 10 s)
     with a seeded divergence-free velocity field of at most 2 m/s (a
     streamfunction of 16 Gaussians), a zero Dirichlet value on the sea
     boundary and a tracer of Gaussian patches, assembled on the device
     (MT01AA, MT05AA, MT03AA with KSUPG in place, MT02AA, OM) and solved by
-    GMRES(5) with PRECON 2 to eps 1e-8 (FP64). A step restores the
-    assembled matrix and solves."""
+    GMRES(5) with PRECON 2 to eps 1e-8, FP64 and mixed (FP32 Arnoldi inside
+    FP64 restarts). A step restores the assembled matrix and solves."""
     import torch
-    from opentelemac_amd import bief, mesh as M, system as S, tracer
+    from opentelemac_amd import bief, tracer
     t = time.time()
-    m = M.estuary_mesh(args.npoin, seed=args.seed)
+    m = mesh if mesh is not None else estuary(args.npoin, args.seed)
     u, v, fn, sea = tracer.estuary_flow(m, seed=args.seed)
-    log(f"supg mesh {m.npoin} nodes, built in {time.time() - t:.1f}s")
+    log(f"supg mesh {m.npoin} nodes, ready in {time.time() - t:.1f}s")
     ctx = bief.Context(m)
     t = time.time()
     sm = tracer.supg_system(ctx, m, u, v, np.full(m.npoin, nu), dt, fn)
@@ -585,67 +648,84 @@ def bench_supg(args, dt=10.0, nu=0.1):
         t1 = time.perf_counter()
         sm = tracer.supg_system(ctx, m, u, v, np.full(m.npoin, nu), dt, fn)
         asm.append(time.perf_counter() - t1)
-    t = time.time()
     limdir = np.full(m.npoin, 2, dtype=np.int32)
     limdir[sea] = 5
     f, sm = ctx.dirich(fn, sm, np.zeros(m.npoin), limdir)
-    log(f"system assembled on the device in {time.time() - t:.1f}s")
     ctx.save_matrix()
     dev = torch.device("cuda:0")
     x0 = torch.from_numpy(f).to(dev)
     b0 = torch.from_numpy(sm).to(dev)
     xx, bb = torch.empty_like(x0), torch.empty_like(b0)
+    out = {"workload": "SUPG tracer system (CVDFTR), 2M-node estuary, GMRES(5) + PRECON 2",
+           "npoin": m.npoin, "nelem": m.nelem, "dt_s": dt, "nu": nu, "eps": 1e-8,
+           "umax": 2.0,
+           "ms_assembly_median": 1e3 * float(np.median(asm)),
+           "ms_first_assembly_with_geometry_upload": 1e3 * t_first,
+           "assembly": "tracer.supg_system: MT05AA, MT03AA + KSUPG, MT01AA, OM, MT02AA on the "
+                       "device, 2 mat-vecs, host node arrays in, right-hand side out"}
+    sols = {}
+    for name, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
+        def step():
+            ctx.restore_matrix()
+            xx.copy_(x0)
+            bb.copy_(b0)
+            return ctx.solve(xx, bb, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000,
+                             precision=prec)[1]
 
-    def step():
-        ctx.restore_matrix()
-        xx.copy_(x0)
-        bb.copy_(b0)
-        return ctx.solve(xx, bb, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000)[1]
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    lat = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        info = step()
+        for _ in range(args.warmup):
+            step()
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
-    ms = float(np.median(lat)) * 1e3
-    return {"workload": "SUPG tracer system (CVDFTR), 2M-node estuary, GMRES(5) + PRECON 2",
-            "npoin": m.npoin, "nelem": m.nelem, "dt_s": dt, "nu": nu, "eps": 1e-8,
-            "umax": 2.0, "ms_per_solve": ms, "solves_per_s": 1e3 / ms,
-            "gmres_restarts": info["niter"], "converged": bool(info["converged"]),
-            "relres": info["relres"], "dtype": "f64",
-            "ms_assembly_median": 1e3 * float(np.median(asm)),
-            "ms_first_assembly_with_geometry_upload": 1e3 * t_first,
-            "assembly": "tracer.supg_system: MT05AA, MT03AA + KSUPG, MT01AA, OM, MT02AA on the "
-                        "device, 2 mat-vecs, host node arrays in, right-hand side out"}
+        lat = []
+        for _ in range(max(args.steps, 5)):
+            t0 = time.perf_counter()
+            info = step()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        sols[name] = xx.cpu().numpy()
+        ms = float(np.median(lat)) * 1e3
+        out[name] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms,
+                     "gmres_restarts": info["niter"], "ebe_products": info["inner_iters"] or
+                     info["niter"] * 5, "converged": bool(info["converged"]),
+                     "relres": info["relres"],
+                     "ms_ebe_product": ctx.time_matvec(prec, reps=20)}
+        log("supg", name, json.dumps(out[name]))
+    out["mixed_speedup_vs_fp64"] = round(out["fp64"]["ms_per_solve"] /
+                                         out["mixed"]["ms_per_solve"], 3)
+    out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) /
+                                        np.linalg.norm(sols["fp64"]))
+    # the Arnoldi steps' EBE product on the 'Q' (non-symmetric) matrix after
+    # PRECON 2 (unit diagonal): 6 coefficients and 3 int32 ids per element,
+    # Y read and X written
+    ne, npo = m.nelem, m.npoin
+    out["roofline"] = _roof(ne * (6 * 4 + 12) + npo * 2 * 4, out["mixed"]["ms_ebe_product"])
+    out["roofline"]["kernel"] = "k_ebe<float,...,SYMQ=false,...,STAGE_PLAIN,EPI_OP> (FP32 'Q')"
+    out["roofline_fp64"] = _roof(ne * (6 * 8 + 12) + npo * 2 * 8, out["fp64"]["ms_ebe_product"])
+    ctx.close()
+    return out
 
 
 def bench_ensemble(args, nsys=256):
-    """256 channel meshes (2000 m x 200 m, the 20,451-node channel's shape)
-    of 5k to 200k nodes, log-uniform (seeded), each with its own wave-equation
-    step (depth 1 -> 5 m with a bump, 0.2 m tide at x = 0, dt = 1 s); all
-    solved as one ensemble (one context, one launch per kernel covering every
-    mesh, FP64 CG + PRECON 2 to 1e-8 with per-mesh scalars), its node-block
-    size chosen by the autotuner on the first 32 meshes. A step restores the
-    assembled matrices and solves every mesh; p50/p99 of the step time over
-    the timed steps. Beside it: the same meshes one context each, solved one
-    after the other."""
+    """The batched ensemble: 256 unstructured estuary meshes (the estuary
+    generator, each member its own seed, outline, points and depth field),
+    5k to 200k nodes drawn log-uniformly (seeded), each with a wave-equation
+    step (dt = 1 s, theta = 1, a 0.2 m tide on its sea boundary). All
+    solved as one ensemble -- one context, every kernel launch covering every
+    mesh, each mesh with its own scalars, stop tests and refinement sweeps --
+    by CG + PRECON 2 to 1e-8, mixed precision (the configuration) and FP64,
+    the node-block size chosen by the autotuner on the first 32 members. A
+    step restores the assembled matrices and solves all members. Reported:
+    systems/s, and per-system completion latency (device clock from the
+    solve's set-up kernel to the kernel that stopped the system) p50/p99 over
+    all members of the timed steps. Beside it: the same meshes one context
+    each, solved one after the other."""
     import torch
-    from opentelemac_amd import bief, mesh as M, system as S, tune
-    rng = np.random.default_rng(args.seed + 3)
-    sizes = np.exp(rng.uniform(np.log(5000), np.log(200000), nsys)).astype(int)
+    from opentelemac_amd import bief, system as S, tune
+    sizes = S.ensemble_sizes(nsys, seed=args.seed)
     t = time.time()
-    meshes, systems = [], []
-    for k, n in enumerate(sizes):
-        m = M.channel_mesh_n(int(n), seed=args.seed * 1000 + k)
-        meshes.append(m)
-        systems.append(S.wave_system(m, M.channel_depth(m), dt=1.0))
+    meshes, systems = S.estuary_ensemble(sizes, seed=args.seed)
     total = sum(m.npoin for m in meshes)
     log(f"ensemble of {nsys} meshes, {total} nodes, built in {time.time() - t:.1f}s")
-    cat = lambda key, sel=None: np.ascontiguousarray(np.concatenate(
+    cat = lambda key, sel=None: np.ascontiguousarray(np.concatenate(  # noqa: E731
         [s[key] for s in (systems if sel is None else systems[:sel])], axis=-1))
 
     def make(bn, sel=32):
@@ -653,10 +733,11 @@ def bench_ensemble(args, nsys=256):
 
     def load(ctx, sel=32):
         ctx.set_matrix("Q", "S", cat("da", sel), cat("xa", sel))
-        ctx.solve(cat("x0", sel), cat("b", sel).copy(), slv=1, precon=2, eps=1e-8, nitmax=5000)
-        ctx.set_matrix("Q", "S", cat("da", sel), cat("xa", sel))
+        ctx.solve(cat("x0", sel), cat("b", sel).copy(), slv=1, precon=2, eps=1e-8, nitmax=5000,
+                  precision=bief.PREC_MIXED)
 
-    best, table = tune.block_nodes(make, load, candidates=(256, 512, 1024, 2048))
+    best, table = tune.block_nodes(make, load, candidates=(512, 1024, 2048),
+                                   precision=bief.PREC_MIXED)
     log(f"block size {best}: {table}")
     t = time.time()
     ens = bief.Ensemble(meshes, block_nodes=best)
@@ -668,50 +749,135 @@ def bench_ensemble(args, nsys=256):
     x0 = torch.from_numpy(cat("x0")).to(dev)
     b0 = torch.from_numpy(cat("b")).to(dev)
     x, b = torch.empty_like(x0), torch.empty_like(b0)
+    ne = sum(m.nelem for m in meshes)
+    out = {"workload": f"ensemble of {nsys} unstructured estuary meshes, 5k-200k nodes "
+                       "(log-uniform)",
+           "systems": nsys, "total_nodes": int(total), "total_elements": int(ne),
+           "block_nodes": best, "autotune_ms_per_iteration": table,
+           "workgroup_threads": 256, "eps": 1e-8}
+    for name, prec in (("mixed", bief.PREC_MIXED), ("fp64", bief.PREC_FP64)):
+        def step():
+            ens.restore_matrix()
+            x.copy_(x0)
+            b.copy_(b0)
+            return ens.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec)[1]
 
-    def step():
-        ens.restore_matrix()
-        x.copy_(x0)
-        b.copy_(b0)
-        return ens.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000)[1]
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    lat = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        info = step()
+        for _ in range(args.warmup):
+            step()
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
-    infos = ens.infos()
-    lat_ms = np.array(lat) * 1e3
-    # the same systems, one context each, one after the other
+        steps, lat_sys = [], []
+        for _ in range(max(args.steps, 5)):
+            t0 = time.perf_counter()
+            info = step()
+            torch.cuda.synchronize()
+            steps.append(time.perf_counter() - t0)
+            lat_sys.append(ens.latencies_ms())
+        infos = ens.infos()
+        lat_sys = np.concatenate(lat_sys)
+        st_ms = np.array(steps) * 1e3
+        ms_dir, ms_upd = ens.time_cg_kernels(prec, reps=20)
+        sv = 4 if prec == bief.PREC_MIXED else 8
+        roof = _roof(ne * (3 * sv + 12) + total * 4 * sv, ms_dir)
+        roof["kernel"] = ("k_ebe<%s,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> over all members"
+                          % ("float" if sv == 4 else "double"))
+        out[name] = {"ms_per_ensemble_step_median": float(np.median(st_ms)),
+                     "systems_per_s": float(nsys * 1e3 / st_ms.mean()),
+                     "per_system_latency_ms_p50": float(np.percentile(lat_sys, 50)),
+                     "per_system_latency_ms_p99": float(np.percentile(lat_sys, 99)),
+                     "per_system_latency_ms_max": float(lat_sys.max()),
+                     "iterations_max": int(info["niter"]),
+                     "iterations_median": float(np.median([i["niter"] for i in infos])),
+                     "outer_max": int(info["outer_iters"]),
+                     "all_converged": bool(all(i["converged"] for i in infos)),
+                     "ms_dir_kernel": ms_dir, "ms_upd_kernel": ms_upd, "roofline": roof}
+        log("ensemble", name, json.dumps(out[name]))
+    out["roofline"] = out["mixed"]["roofline"]
+    # the same systems, one context each, one after the other (mixed)
     ctxs = [bief.Context(m, block_nodes=best) for m in meshes]
-    dsys = [(torch.from_numpy(sd["da"]).to(dev), torch.from_numpy(np.ascontiguousarray(sd["xa"])).to(dev),
-             torch.from_numpy(sd["x0"]).to(dev), torch.from_numpy(sd["b"]).to(dev)) for sd in systems]
+    dsys = [(torch.from_numpy(sd["da"]).to(dev),
+             torch.from_numpy(np.ascontiguousarray(sd["xa"])).to(dev),
+             torch.from_numpy(sd["x0"]).to(dev), torch.from_numpy(sd["b"]).to(dev))
+            for sd in systems]
     seq = []
     for rep in range(2):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for ctx, (da, xa, xx, bb) in zip(ctxs, dsys):
             ctx.set_matrix("Q", "S", da, xa)
-            ctx.solve(xx.clone(), bb.clone(), slv=1, precon=2, eps=1e-8, nitmax=5000)
+            ctx.solve(xx.clone(), bb.clone(), slv=1, precon=2, eps=1e-8, nitmax=5000,
+                      precision=bief.PREC_MIXED)
         torch.cuda.synchronize()
         seq.append(time.perf_counter() - t0)
-    return {"workload": f"ensemble of {nsys} channel meshes, 5k-200k nodes (log-uniform)",
-            "systems": nsys, "total_nodes": int(total),
-            "total_elements": int(sum(m.nelem for m in meshes)),
-            "block_nodes": best, "autotune_ms_per_iteration": table,
-            "ms_per_ensemble_step_p50": float(np.percentile(lat_ms, 50)),
-            "ms_per_ensemble_step_p99": float(np.percentile(lat_ms, 99)),
-            "ensemble_steps_per_s": float(1e3 / lat_ms.mean()),
-            "systems_per_s": float(nsys * 1e3 / lat_ms.mean()),
-            "iterations_max": int(info["niter"]),
-            "iterations_median": float(np.median([i["niter"] for i in infos])),
-            "all_converged": bool(all(i["converged"] for i in infos)),
-            "sequential_ms_per_round": float(min(seq) * 1e3),
-            "speedup_vs_sequential": float(min(seq) * 1e3 / lat_ms.mean())}
+    for ctx in ctxs:
+        ctx.close()
+    ens.close()
+    out["sequential_ms_per_round_mixed"] = float(min(seq) * 1e3)
+    out["speedup_vs_sequential"] = float(min(seq) * 1e3 / out["mixed"]
+                                         ["ms_per_ensemble_step_median"])
+    return out
+
+
+def bench_channel(args):
+    """The CPU-runnable configuration: the 2000 m x 200 m channel (400 x 50
+    cells, 20,451 nodes, 40,000 triangles), depth 1-5 m with a 1.5 m bump,
+    one wave-equation step (dt = 1 s) with a 0.2 m tide at the inflow, CG +
+    PRECON 2 to 1e-8: GPU FP64 and mixed, and the reference's own SOLVE on
+    one host core, on the same system."""
+    import torch
+    from opentelemac_amd import bief, mesh as M, system as S
+    from oracle import oracle as O
+    m = M.channel_mesh()
+    sysd = S.wave_system(m, M.channel_depth(m), dt=1.0)
+    dev = torch.device("cuda:0")
+    ctx = bief.Context(m)
+    ctx.set_matrix("Q", "S", torch.from_numpy(sysd["da"]).to(dev),
+                   torch.from_numpy(np.ascontiguousarray(sysd["xa"])).to(dev))
+    ctx.save_matrix()
+    x0 = torch.from_numpy(sysd["x0"]).to(dev)
+    b0 = torch.from_numpy(sysd["b"]).to(dev)
+    x, b = torch.empty_like(x0), torch.empty_like(b0)
+    out = {"workload": "structured channel, 20,451 nodes, wave-equation CG",
+           "npoin": m.npoin, "nelem": m.nelem}
+    sols = {}
+    for name, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
+        def step():
+            ctx.restore_matrix()
+            x.copy_(x0)
+            b.copy_(b0)
+            return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec)[1]
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            info = step()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        sols[name] = x.cpu().numpy()
+        ms = 1e3 * float(np.median(lat))
+        out[name] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms, "iterations": info["niter"],
+                     "relres": info["relres"], "converged": bool(info["converged"])}
+    ctx.close()
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    if O.have_ref():
+        rm = O.RefMesh(m.ikle, m.npoin, m.nelem)
+        one = lambda: O.ref_solve(rm, A, sysd["b"], sysd["x0"], slv=1, precon=2,  # noqa: E731
+                                  eps=1e-8, nitmax=5000)
+        kind = "reference"
+    else:
+        one = lambda: O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2,  # noqa: E731
+                              eps=1e-8, nitmax=5000)
+        kind = "port"
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        xr = one()[0]
+        n += 1
+    dt = time.perf_counter() - t0
+    out["cpu_1core"] = {"kind": kind, "solves_per_s": n / dt, "sample": f"{n} solves"}
+    out["fp64_rel_l2_vs_cpu"] = float(np.linalg.norm(sols["fp64"] - xr) / np.linalg.norm(xr))
+    out["mixed_rel_l2_vs_cpu"] = float(np.linalg.norm(sols["mixed"] - xr) / np.linalg.norm(xr))
+    return out
 
 
 def _cpu_model():

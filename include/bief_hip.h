@@ -206,13 +206,18 @@ int bhip_ctx_perm(const bhip_ctx* ctx, int32_t* perm);
  * the kernels of one CG iteration on the context's current (preconditioned)
  * matrix. FP64: the fused direction update + EBE mat-vec + D.AD kernel
  * (ms_dir) and the X/R update + R.R kernel (ms_upd). Mixed (after a mixed
- * solve): the same two kernels of the FP32 inner CG, or, when
- * BHIP_INNER_CG=1 selects the single-kernel inner CG, that kernel (ms_dir;
- * ms_upd is 0). Scratch vectors are reused, so
+ * solve): the same two kernels of the FP32 inner CG. Scratch vectors are
+ * reused, so
  * call it after the solves whose results are needed. No counterpart in the
  * reference (it has no timers below the whole run, homere_telemac2d.F). */
 int bhip_time_cg_kernels(bhip_ctx* ctx, int precision, int reps,
                          double* ms_dir, double* ms_upd);
+/* Measurement: average duration (ms, HIP events on the context stream) of
+ * the plain EBE product X = A Y (MV0303/MV0606 + ASSVEC, the kernel of
+ * GMRES's Arnoldi steps) on the context's current matrix, FP64 or (after a
+ * mixed solve) its FP32 copy. Scratch vectors are overwritten. No
+ * counterpart in the reference. */
+int bhip_time_matvec(bhip_ctx* ctx, int precision, int reps, double* ms);
 
 /* ---- operand preparation before SOLVE ----------------------------------- */
 
@@ -299,18 +304,23 @@ int bhip_matrix_supg(bhip_ctx* ctx, int kind, double xmul, const double* xel,
 int bhip_matrix_om(bhip_ctx* ctx, const char* op, double c);
 
 /* ---- the time-stepped wave equation and its adjoint ---------------------- *
- * One time step of telemac2d's wave-equation treatment (PROPAG,
- * propag.F:664-716, 1369-1408, 1634-1756; theta = 1, no advection) with
- * Strickler friction (friction_calc.f:130-134) and a prescribed free surface
- * on sea nodes, as the device harness runs it, and the transposed kernels of
- * its hand-written adjoint. No single routine of the reference corresponds:
- * PROPAG builds the same system from MATMAS (MT01AA), MATDIFUV (MT02AA 'UV'),
- * VECTOR and DIRICH calls. All node vectors are device pointers in the
- * context's internal order (bhip_ctx_perm); triangles only. */
+ * One time step of telemac2d's wave-equation treatment (PROPAG with
+ * TREATMENT OF THE LINEAR SYSTEM = 2, propag.F:664-716, 1003-1012,
+ * 1150-1170, 1369-1465, 1634-1710) for implicitation 1, no advection or
+ * diffusion of velocity, no sources, no tidal flats, Strickler friction
+ * (friction_unif.f:124-127, friction_calc.f:130-134, fricti.f:123-135, with
+ * the bottom-slope factor of slop10.f), the free surface prescribed on sea
+ * nodes (free velocity there) and walls elsewhere, and the transposed kernels
+ * of its hand-written adjoint. No single routine of the reference
+ * corresponds: PROPAG builds the same operands from MATMAS (MT01AA),
+ * MATDIFUV (MT02AA 'UV'), MATFGR (MT08AA), VECTOR 'GRADF'/'MASBAS', OS and
+ * DIRICH calls. All node vectors are device pointers in the context's
+ * internal order (bhip_ctx_perm); triangles only. */
 typedef struct {
   double dt;     /* time step (s) */
   double grav;   /* gravity */
-  double eps_u;  /* speed floor in |u| = sqrt(u^2 + v^2 + eps_u^2) */
+  double eps_u;  /* speed floor in |u| = sqrt(u^2 + v^2 + eps_u^2); the
+                    reference's |u| is eps_u = 0 */
   double tide;   /* free surface prescribed on sea nodes at the new time */
 } bhip_wave_params;
 
@@ -320,15 +330,16 @@ typedef struct {
 int bhip_wave_setup(bhip_ctx* ctx, const double* xel, const double* yel,
                     const double* surfac, const int32_t* sea);
 /* The system of the step from the state (eta, u, v), Strickler kstr and
- * bottom zb: A = M/dt + MATDIFUV(g c, h) into the context's matrix (TYPDIA
- * 'Q', TYPEXT 'S', rows and columns of sea nodes cleared off the diagonal)
- * and its right-hand side rhs (D fb on sea nodes). */
+ * bottom zb: AM1 = M/dt + MATDIFUV(-1, DM1, max(h, 0)) into the context's
+ * matrix (TYPDIA 'Q', TYPEXT 'S', rows and columns of sea nodes cleared off
+ * the diagonal) and its right-hand side rhs = CV1 = -BM1 T2 - BM2 T3 with
+ * DIRICH's terms (D fb on sea nodes, fb = tide - eta). */
 int bhip_wave_assemble(bhip_ctx* ctx, const bhip_wave_params* p,
                        const double* eta, const double* u, const double* v,
                        const double* kstr, const double* zb, double* rhs);
 /* The new state from the increment deta = A^-1 rhs: eta' = eta + deta and
- * the momentum update with the lumped gradient of eta'. Uses the friction
- * factor of the last bhip_wave_assemble. */
+ * the momentum update u' = (GRADF(-g eta') + VOL/dt u) / AM2D. Uses the
+ * friction of the last bhip_wave_assemble. */
 int bhip_wave_momentum(bhip_ctx* ctx, const bhip_wave_params* p,
                        const double* eta, const double* deta, const double* u,
                        const double* v, double* eta_n, double* u_n,

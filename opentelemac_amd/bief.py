@@ -98,6 +98,7 @@ def lib():
         L.bhip_matrix_save.argtypes = [_P]
         L.bhip_matrix_restore.argtypes = [_P]
         L.bhip_time_cg_kernels.argtypes = [_P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]
+        L.bhip_time_matvec.argtypes = [_P, _I, _I, ctypes.POINTER(_D)]
         L.bhip_layout_check.argtypes = [_I, _I, _I, _P, _P, _P, _I, _I, _P]
         L.bhip_dirich.argtypes = [_P, _P, _P, _P, _P, _I]
         L.bhip_lump.argtypes = [_P, _D, _P]
@@ -483,6 +484,14 @@ class Context:
         _check(lib().bhip_time_cg_kernels(self._h, precision, reps, ctypes.byref(a),
                                           ctypes.byref(b)), "bhip_time_cg_kernels")
         return a.value, b.value
+
+    def time_matvec(self, precision=PREC_FP64, reps=20):
+        """Average ms of the plain EBE product on the current matrix (FP64, or
+        its FP32 copy after a mixed solve): bhip_time_matvec."""
+        ms = ctypes.c_double()
+        _check(lib().bhip_time_matvec(self._h, precision, reps, ctypes.byref(ms)),
+               "bhip_time_matvec")
+        return ms.value
 
     # ---- SOLVE ----------------------------------------------------------
     def solve(self, x, b, slv=1, precon=2, krylov=5, nitmax=1000, eps=1e-8,

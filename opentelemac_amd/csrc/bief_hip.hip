@@ -1114,7 +1114,8 @@ struct bhip_ctx {
   // [5][nslot_pad] (x2, x3, y2, y3, S of the slot's element) and node fields
   // in internal order (1 - sea mask, basis integrals, h, c, qx, qy, y, and
   // the assembled diagonal before preconditioning)
-  bhip::DevBuf<double> wv_geo, wv_w, wv_vol, wv_h, wv_c, wv_qx, wv_qy, wv_y, wv_da;
+  bhip::DevBuf<double> wv_geo, wv_w, wv_vol, wv_h, wv_zb, wv_f, wv_vs, wv_c, wv_usx, wv_usy,
+      wv_fb, wv_da, wv_sbx, wv_sby, wv_svs, wv_seb;
   bool wv_ready = false;
 };
 
@@ -1433,8 +1434,9 @@ int bhip_ctx_destroy(bhip_ctx* c) {
     v->release();
   c->geo.release();
   c->st.release();
-  for (auto* v : {&c->wv_geo, &c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_c, &c->wv_qx, &c->wv_qy,
-                  &c->wv_y, &c->wv_da})
+  for (auto* v : {&c->wv_geo, &c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_zb, &c->wv_f, &c->wv_vs,
+                  &c->wv_c, &c->wv_usx, &c->wv_usy, &c->wv_fb, &c->wv_da, &c->wv_sbx,
+                  &c->wv_sby, &c->wv_svs, &c->wv_seb})
     v->release();
   for (auto& gr : c->graphs)
     if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
@@ -2374,6 +2376,40 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
   (void)hipEventDestroy(e1);
   if (ms_dir) *ms_dir = t_dir / reps;
   if (ms_upd) *ms_upd = t_upd / reps;
+  return BHIP_OK;
+}
+
+int bhip_time_matvec(bhip_ctx* c, int precision, int reps, double* ms) {
+  if (!c || !c->have_matrix || reps < 1 || !ms) return BHIP_EBADARG;
+  if (precision == BHIP_PREC_MIXED && !c->fr.p) return BHIP_EBADARG;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i) {
+    if (precision == BHIP_PREC_MIXED) {
+      EbeArgs<float, float> a = base_args<float, float>(c, c->xa32.p, c->da32.p);
+      a.y = c->fr.p;
+      a.out = c->fad.p;
+      a.op = BHIP_OP_AY;
+      HIP_OK((launch_ebe<float, float, STAGE_PLAIN, EPI_OP, FIN_NONE>(c, a, false, c->typdia,
+                                                                      c->typext)));
+    } else {
+      EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
+      a.y = c->vio0.p;
+      a.out = c->vio1.p;
+      a.op = BHIP_OP_AY;
+      HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_OP, FIN_NONE>(c, a, false, c->typdia,
+                                                                        c->typext)));
+    }
+  }
+  HIP_OK(hipEventRecord(e1, c->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_OK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / reps;
   return BHIP_OK;
 }
 

@@ -2,28 +2,44 @@
 // its hand-written adjoint. Included at the end of bief_hip.hip.
 //
 // One step (eta, u, v) -> (eta', u', v') at the new time, with Strickler
-// friction K and bottom zb, restates the wave-equation treatment of telemac2d
-// PROPAG (propag.F:664-716, 1369-1408, 1634-1756) for a theta = 1 scheme
-// without advection:
-//   node      h = eta - zb, CF = 2 g / (K^2 h^(1/3)) (friction_calc.f:130-134),
-//             f = CF |u|_e / (2 h), c = 1 / (1/dt + f), q = h c u / dt,
-//             y = eta on inner nodes, the tide on sea nodes
-//   element   A = M/dt + MATDIFUV(U = g c, V = h) (mt01aa.f:83-106,
-//             mt02aa.f:200-240), rhs = int grad(phi).q_mean - A y + M eta/dt,
-//             sea rows: rhs = D fb, rows and columns of sea nodes cleared in the
-//             off-diagonal terms (DIRICH, diri01.f:94-139)
+// friction K and bottom zb, is telemac2d PROPAG's wave-equation treatment
+// (TREATMENT OF THE LINEAR SYSTEM = 2; propag.F:664-716, 1003-1012,
+// 1150-1170, 1369-1465, 1634-1710) for these options: P1 depth and velocity,
+// implicitation 1 (TETAH = TETAU = TETAZCOMP = 1), no advection or diffusion
+// of velocity, no sources, no tidal flats, one sub-iteration, Strickler law
+// (KFROT 3, HFROT 1), the free surface prescribed on sea nodes with free
+// velocity there, walls elsewhere (no boundary flux term then):
+//   node      h = eta - zb; CF = 2 g / K^2 / max(h, 1e-4)^(1/3)
+//             (friction_unif.f:124-127, friction_calc.f:130-134);
+//             f = CF |u| / (2 max(h, 1e-9)), |u| raised to the shallow-water
+//             speed below 3 cm (fricti.f:123-135); the sea nodes' prescribed
+//             increment fb = tide - eta (HBOR - HN)
+//   element   GX = GRADF(eta) (vc13aa.f), VS = MASBAS weighted by the bottom
+//             slope factor (slop10.f; propag.F:1155-1161)
+//   node      AM2D = VOL/dt + f VS, c = VOL / AM2D (so DM1 = -g c),
+//             u* = c (u/dt - g GX/VOL)  (T2 = CV2 / AM2D, propag.F:1413)
+//   element   A = M/dt + MATDIFUV(-1, DM1, max(h,0)) (mt01aa.f:83-106,
+//             mt02aa.f:200-240); rhs = -MATFGR(h)u* (mt08aa.f; CV1 =
+//             -BM1 T2 - BM2 T3, propag.F:1464-1465) - A fb; sea rows
+//             rhs = D fb and their rows and columns cleared off the diagonal
+//             (DIRICH, diri01.f:94-139)
 //   solve     A deta = rhs (SOLVE, CG + PRECON 2 on the device)
-//   element   u' = c (u/dt - g grad(eta')), grad lumped: sum over the elements
-//             of S/3 grad(eta') divided by the basis integral
+//   element   u' = c (u/dt - g GRADF(eta + deta)/VOL)  (propag.F:1685-1707)
 // All node vectors are in the context's internal order; the matrix is written
 // straight into the context (slot order), so the solve needs no upload.
 //
 // The adjoint kernels are the transposes of these, derived by hand: the
 // element terms of each kernel are differentiated with respect to the node
 // values they gather, and the results are assembled onto owned nodes with the
-// same colour-by-colour LDS scheme (deterministic, no atomics).
+// same colour-by-colour LDS scheme (deterministic, no atomics). The floors
+// max(h, .) pass the derivative where h is above them and nothing below.
 
 namespace bhip {
+
+constexpr double kWaveHO = 3e-2;      // fricti.f: HO, the shallow-water threshold
+constexpr double kWaveGHO = 9.81;     // fricti.f: the gravity of its shallow speed
+constexpr double kWaveMinHF = 1e-9;   // fricti.f: MAX(HHN, 1.D-9)
+constexpr double kWaveMinHC = 1e-4;   // friction_unif.f: MINH
 
 struct WaveArgs {
   // layout
@@ -39,13 +55,21 @@ struct WaveArgs {
   double dt, grav, eps_u, tide;
   // node fields (internal order)
   const double* w;    // 1 inner node, 0 sea node
-  const double* vol;  // basis integrals
-  double* h;
-  double* c;
-  double* qx;
-  double* qy;
-  double* y;
-  double* da_keep;
+  const double* vol;  // basis integrals (VOLU2D)
+  double* h;          // eta - zb
+  double* zb;         // the bottom of the step
+  double* f;          // friction factor 0.5 CF |u| / h
+  double* vs;         // slope-weighted basis integrals
+  double* c;          // VOL / AM2D
+  double* usx;        // u* (T2), v* (T3)
+  double* usy;
+  double* fb;         // prescribed increment on sea nodes, 0 elsewhere
+  double* da_keep;    // assembled diagonal of the step's system
+  // adjoint scratch: adjoints of u*, v*, VS and the partial adjoint of eta
+  double* sbx;
+  double* sby;
+  double* svs;
+  double* seb;
 };
 
 __device__ inline int wave_block(const WaveArgs& a) { return xcd_block(blockIdx.x, a.nblocks); }
@@ -76,15 +100,19 @@ __device__ inline void colour_walk(const WaveArgs& a, int b, F&& f) {
 }
 
 // the MT02AA 'UV' geometric pair factors and the P1 gradient factors of the
-// slot's element (node 1 at the origin)
+// slot's element (node 1 at the origin): dphi_k/dx = bx_k / (2S)
 struct ElemGeo {
-  double s, p12, p13, p23, bx[3], by[3];
+  double s, x2, x3, y2, y3, p12, p13, p23, bx[3], by[3];
 };
 __device__ inline ElemGeo elem_geo(const WaveArgs& a, int s) {
   const size_t np = a.nslot_pad;
-  const double x2 = a.geo[s], x3 = a.geo[np + s], y2 = a.geo[2 * np + s], y3 = a.geo[3 * np + s];
   ElemGeo e;
+  e.x2 = a.geo[s];
+  e.x3 = a.geo[np + s];
+  e.y2 = a.geo[2 * np + s];
+  e.y3 = a.geo[3 * np + s];
   e.s = a.geo[4 * np + s];
+  const double x2 = e.x2, x3 = e.x3, y2 = e.y2, y3 = e.y3;
   e.p12 = y3 * (y2 - y3) + x3 * (x2 - x3);
   e.p13 = -(y2 * (y2 - y3) + x2 * (x2 - x3));
   e.p23 = -(y2 * y3 + x2 * x3);
@@ -97,22 +125,95 @@ __device__ inline ElemGeo elem_geo(const WaveArgs& a, int s) {
   return e;
 }
 
-// node quantities of the step
+// GRADF of a P1 field on the element, the same value for its three nodes
+// (vc13aa.f, XMUL = 1)
+__device__ inline double gradf_x(const ElemGeo& e, const double (&q)[3]) {
+  return (e.y2 * (q[0] - q[2]) + e.y3 * (q[1] - q[0])) / 6.0;
+}
+__device__ inline double gradf_y(const ElemGeo& e, const double (&q)[3]) {
+  return (e.x2 * (q[2] - q[0]) + e.x3 * (q[0] - q[1])) / 6.0;
+}
+
+// SLOP10 (slop10.f): sqrt((A + B + C) / A), the area factor of the inclined
+// bottom, with its partial derivatives along the bottom differences z2 - z1,
+// z3 - z1 when asked
+__device__ inline double elem_slope(const ElemGeo& e, double z1, double z2, double z3,
+                                    double* d2 = nullptr, double* d3 = nullptr) {
+  const double Z2 = z2 - z1, Z3 = z3 - z1;
+  const double ta = e.x2 * e.y3 - e.x3 * e.y2;
+  const double tb = e.y2 * Z3 - Z2 * e.y3;
+  const double tc = e.x3 * Z2 - Z3 * e.x2;
+  const double A = ta * ta, B = tb * tb, C = tc * tc;
+  const double sl = sqrt((A + B + C) / A);
+  if (d2) {
+    const double k = 1.0 / (2.0 * sl * A);
+    *d2 = k * (-2.0 * e.y3 * tb + 2.0 * e.x3 * tc);
+    *d3 = k * (2.0 * e.y2 * tb - 2.0 * e.x2 * tc);
+  }
+  return sl;
+}
+
+// node quantities of the step: depth, friction factor, prescribed increment
 __global__ void k_wave_nodes(WaveArgs a, int n, const double* __restrict__ eta,
                              const double* __restrict__ u, const double* __restrict__ v,
                              const double* __restrict__ kstr, const double* __restrict__ zb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double h = eta[i] - zb[i];
     const double k = kstr[i];
-    const double cf = 2.0 * a.grav / (k * k * cbrt(h));
-    const double sp = sqrt(u[i] * u[i] + v[i] * v[i] + a.eps_u * a.eps_u);
-    const double f = 0.5 * cf * sp / h;
-    const double c = 1.0 / (1.0 / a.dt + f);
+    const double hc = fmax(h, kWaveMinHC);
+    const double cf = 2.0 * a.grav / (k * k) / pow(hc, 1.0 / 3.0);
+    const double hf = fmax(h, kWaveMinHF);
+    double sp = sqrt(u[i] * u[i] + v[i] * v[i] + a.eps_u * a.eps_u);
+    if (hf < kWaveHO) sp = fmax(sp, sqrt(kWaveGHO * (kWaveHO - hf) * hf / kWaveHO));
     a.h[i] = h;
+    a.zb[i] = zb[i];
+    a.f[i] = 0.5 * cf * sp / hf;
+    a.fb[i] = a.w[i] != 0.0 ? 0.0 : a.tide - eta[i];
+  }
+}
+
+// GRADF(eta) and the slope-weighted basis integrals at owned nodes, then
+// c = VOL / (VOL/dt + f VS) and the velocity predictor u* = c (u/dt - g GX/VOL)
+__global__ void __launch_bounds__(kThreads)
+k_wave_grad(WaveArgs a, const double* __restrict__ eta, const double* __restrict__ u,
+            const double* __restrict__ v) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b = wave_block(a);
+  const int n0 = a.node_start[b], nown = a.node_start[b + 1] - n0;
+  const int hb = a.halo_start[b];
+  double* acc_x = reinterpret_cast<double*>(smem_raw);
+  double* acc_y = acc_x + nown;
+  double* acc_v = acc_y + nown;
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) acc_x[l] = acc_y[l] = acc_v[l] = 0.0;
+  __syncthreads();
+  colour_walk(a, b, [&](int s) {
+    int l[3], g[3];
+    slot_nodes(a, s, n0, nown, hb, l, g);
+    const ElemGeo e = elem_geo(a, s);
+    double ek[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ek[k] = eta[g[k]];
+    const double gx = gradf_x(e, ek), gy = gradf_y(e, ek);
+    const double vs = e.s / 3.0 * elem_slope(e, a.zb[g[0]], a.zb[g[1]], a.zb[g[2]]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (l[k] < nown) {
+        acc_x[l[k]] += gx;
+        acc_y[l[k]] += gy;
+        acc_v[l[k]] += vs;
+      }
+    }
+  });
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+    const int i = n0 + l;
+    const double vol = a.vol[i];
+    const double am = vol / a.dt + a.f[i] * acc_v[l];
+    // OS 'X=1/Y' guarded at 1e-6 (propag.F:1384)
+    const double c = fabs(am) > 1e-6 ? vol / am : 0.0;
+    a.vs[i] = acc_v[l];
     a.c[i] = c;
-    a.qx[i] = h * c * u[i] / a.dt;
-    a.qy[i] = h * c * v[i] / a.dt;
-    a.y[i] = a.w[i] != 0.0 ? eta[i] : a.tide;
+    a.usx[i] = c * (u[i] / a.dt - a.grav * (acc_x[l] / vol));
+    a.usy[i] = c * (v[i] / a.dt - a.grav * (acc_y[l] / vol));
   }
 }
 
@@ -120,8 +221,8 @@ __global__ void k_wave_nodes(WaveArgs a, int n, const double* __restrict__ eta,
 // terms with the sea rows and columns cleared into xa, slot order) and its
 // right-hand side
 __global__ void __launch_bounds__(kThreads)
-k_wave_assemble(WaveArgs a, const double* __restrict__ eta, double* __restrict__ da,
-                double* __restrict__ xa, double* __restrict__ rhs) {
+k_wave_assemble(WaveArgs a, double* __restrict__ da, double* __restrict__ xa,
+                double* __restrict__ rhs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = wave_block(a);
   const int n0 = a.node_start[b], nown = a.node_start[b + 1] - n0;
@@ -135,32 +236,33 @@ k_wave_assemble(WaveArgs a, const double* __restrict__ eta, double* __restrict__
     int l[3], g[3];
     slot_nodes(a, s, n0, nown, hb, l, g);
     const ElemGeo e = elem_geo(a, s);
-    double hh[3], uu[3], qxk[3], qyk[3], yk[3], ek[3], wk[3];
+    double hp[3], uu[3], sx[3], sy[3], fb[3], wk[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      hh[k] = a.h[g[k]];
+      hp[k] = fmax(a.h[g[k]], 0.0);
       uu[k] = a.grav * a.c[g[k]];
-      qxk[k] = a.qx[g[k]];
-      qyk[k] = a.qy[g[k]];
-      yk[k] = a.y[g[k]];
-      ek[k] = eta[g[k]];
+      sx[k] = a.usx[g[k]];
+      sy[k] = a.usy[g[k]];
+      fb[k] = a.fb[g[k]];
       wk[k] = a.w[g[k]];
     }
     const double m_off = e.s * mdt;
-    const double g123 = hh[0] + hh[1] + hh[2];
-    const double coef = (uu[0] * (hh[0] + g123) + uu[1] * (hh[1] + g123) +
-                         uu[2] * (hh[2] + g123)) / (48.0 * e.s);
+    const double g123 = hp[0] + hp[1] + hp[2];
+    const double coef = (uu[0] * (hp[0] + g123) + uu[1] * (hp[1] + g123) +
+                         uu[2] * (hp[2] + g123)) / (48.0 * e.s);
     const double a12 = e.p12 * coef, a13 = e.p13 * coef, a23 = e.p23 * coef;
     const double x12 = a12 + m_off, x13 = a13 + m_off, x23 = a23 + m_off;
     const double t[3] = {-a12 - a13 + 2 * m_off, -a12 - a23 + 2 * m_off,
                          -a13 - a23 + 2 * m_off};
-    const double qxm = (qxk[0] + qxk[1] + qxk[2]) / 3.0;
-    const double qym = (qyk[0] + qyk[1] + qyk[2]) / 3.0;
-    const double ay[3] = {t[0] * yk[0] + x12 * yk[1] + x13 * yk[2],
-                          x12 * yk[0] + t[1] * yk[1] + x23 * yk[2],
-                          x13 * yk[0] + x23 * yk[1] + t[2] * yk[2]};
-    const double me[3] = {m_off * (2 * ek[0] + ek[1] + ek[2]), m_off * (ek[0] + 2 * ek[1] + ek[2]),
-                          m_off * (ek[0] + ek[1] + 2 * ek[2])};
+    // -MATFGR(h) u*: r_k = bx_k Qx + by_k Qy with F = h/24 (mt08aa.f)
+    const double F[3] = {hp[0] / 24.0, hp[1] / 24.0, hp[2] / 24.0};
+    const double F123 = F[0] + F[1] + F[2];
+    const double qx = F123 * (sx[0] + sx[1] + sx[2]) + F[0] * sx[0] + F[1] * sx[1] + F[2] * sx[2];
+    const double qy = F123 * (sy[0] + sy[1] + sy[2]) + F[0] * sy[0] + F[1] * sy[1] + F[2] * sy[2];
+    // DIRICH: b -= A fb
+    const double af[3] = {t[0] * fb[0] + x12 * fb[1] + x13 * fb[2],
+                          x12 * fb[0] + t[1] * fb[1] + x23 * fb[2],
+                          x13 * fb[0] + x23 * fb[1] + t[2] * fb[2]};
     const size_t np = a.nslot_pad;
     xa[s] = x12 * (wk[0] * wk[1]);
     xa[np + s] = x13 * (wk[0] * wk[2]);
@@ -169,7 +271,7 @@ k_wave_assemble(WaveArgs a, const double* __restrict__ eta, double* __restrict__
     for (int k = 0; k < 3; ++k) {
       if (l[k] < nown) {
         acc_d[l[k]] += t[k];
-        acc_r[l[k]] += 0.5 * (e.bx[k] * qxm + e.by[k] * qym) - ay[k] + me[k];
+        acc_r[l[k]] += e.bx[k] * qx + e.by[k] * qy - af[k];
       }
     }
   });
@@ -178,12 +280,12 @@ k_wave_assemble(WaveArgs a, const double* __restrict__ eta, double* __restrict__
     const double d = acc_d[l];
     da[i] = d;
     a.da_keep[i] = d;
-    rhs[i] = a.w[i] != 0.0 ? acc_r[l] : d * (a.tide - eta[i]);
+    rhs[i] = a.w[i] != 0.0 ? acc_r[l] : d * a.fb[i];
   }
 }
 
-// the new state from the increment: eta' = eta + deta, the lumped gradient of
-// eta' and the momentum update
+// the new state from the increment: eta' = eta + deta, GRADF(eta') and the
+// momentum update u' = c (u/dt - g GRADF(eta')/VOL)
 __global__ void __launch_bounds__(kThreads)
 k_wave_momentum(WaveArgs a, const double* __restrict__ eta, const double* __restrict__ deta,
                 const double* __restrict__ u, const double* __restrict__ v,
@@ -203,8 +305,7 @@ k_wave_momentum(WaveArgs a, const double* __restrict__ eta, const double* __rest
     double en[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) en[k] = eta[g[k]] + deta[g[k]];
-    const double gx = (e.bx[0] * en[0] + e.bx[1] * en[1] + e.bx[2] * en[2]) / 6.0;
-    const double gy = (e.by[0] * en[0] + e.by[1] * en[1] + e.by[2] * en[2]) / 6.0;
+    const double gx = gradf_x(e, en), gy = gradf_y(e, en);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (l[k] < nown) {
@@ -222,8 +323,9 @@ k_wave_momentum(WaveArgs a, const double* __restrict__ eta, const double* __rest
   }
 }
 
-// adjoint of the momentum update with respect to eta' (the c and u parts are
-// node-local and taken in k_wave_assemble_adj): etot = ebar + grad^T(...)
+// transpose of GRADF: etot = ebar + GRADF^T(-g c ubar / VOL, -g c vbar / VOL)
+// (the momentum update's dependence on eta'; the c and u parts are
+// node-local and taken in k_wave_assemble_adj)
 __global__ void __launch_bounds__(kThreads)
 k_wave_momentum_adj(WaveArgs a, const double* __restrict__ ubar_n,
                     const double* __restrict__ vbar_n, const double* __restrict__ ebar_n,
@@ -239,7 +341,6 @@ k_wave_momentum_adj(WaveArgs a, const double* __restrict__ ubar_n,
     int l[3], g[3];
     slot_nodes(a, s, n0, nown, hb, l, g);
     const ElemGeo e = elem_geo(a, s);
-    // adjoint of the lumped gradient: -g c ubar / vol at each node, summed
     double sx = 0.0, sy = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -259,7 +360,9 @@ k_wave_momentum_adj(WaveArgs a, const double* __restrict__ ubar_n,
 
 // adjoint of the system, its right-hand side and the node quantities, given
 // lam = A^-1 etot (the adjoint solve) and the increment deta. Writes the
-// adjoint state of step n (ebar, ubar, vbar) and adds to those of K and zb.
+// adjoint state of step n (ebar without the GRADF and slope transposes, which
+// k_wave_grad_adj adds; ubar, vbar), adds to those of K and zb, and leaves
+// the adjoints of u*, v* and VS in the scratch vectors.
 struct WaveAdjIn {
   const double* lam;
   const double* deta;
@@ -289,75 +392,75 @@ k_wave_assemble_adj(WaveArgs a, WaveAdjIn in, WaveAdjOut out) {
   const int hb = a.halo_start[b];
   double* acc_c = reinterpret_cast<double*>(smem_raw);
   double* acc_h = acc_c + nown;
-  double* acc_qx = acc_h + nown;
-  double* acc_qy = acc_qx + nown;
-  double* acc_y = acc_qy + nown;
-  double* acc_m = acc_y + nown;
+  double* acc_sx = acc_h + nown;
+  double* acc_sy = acc_sx + nown;
+  double* acc_fb = acc_sy + nown;
   for (int l = threadIdx.x; l < nown; l += blockDim.x)
-    acc_c[l] = acc_h[l] = acc_qx[l] = acc_qy[l] = acc_y[l] = acc_m[l] = 0.0;
+    acc_c[l] = acc_h[l] = acc_sx[l] = acc_sy[l] = acc_fb[l] = 0.0;
   __syncthreads();
   const double mdt = 1.0 / (12.0 * a.dt);
   colour_walk(a, b, [&](int s) {
     int l[3], g[3];
     slot_nodes(a, s, n0, nown, hb, l, g);
     const ElemGeo e = elem_geo(a, s);
-    double hh[3], uu[3], yk[3], wk[3], lam[3], de[3], rho[3], dabar[3];
+    double hp[3], uu[3], sx[3], sy[3], fb[3], wk[3], lam[3], de[3], rho[3], dabar[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int i = g[k];
-      hh[k] = a.h[i];
+      hp[k] = fmax(a.h[i], 0.0);
       uu[k] = a.grav * a.c[i];
-      yk[k] = a.y[i];
+      sx[k] = a.usx[i];
+      sy[k] = a.usy[i];
+      fb[k] = a.fb[i];
       wk[k] = a.w[i];
       lam[k] = in.lam[i];
       de[k] = in.deta[i];
       rho[k] = wk[k] * lam[k];
-      // adjoint of the assembled diagonal: the solve (-lam deta), the sea
-      // rows D fb (lam fb), the product -A y (-rho y)
-      const double fb = wk[k] != 0.0 ? 0.0 : a.tide - in.eta[i];
-      dabar[k] = -lam[k] * de[k] + (wk[k] != 0.0 ? 0.0 : lam[k] * fb) - rho[k] * yk[k];
+      // adjoint of the element's diagonal term: the solve (-lam deta) and
+      // the sea rows D fb (lam fb)
+      dabar[k] = -lam[k] * de[k] + (1.0 - wk[k]) * lam[k] * fb[k];
     }
     const double m_off = e.s * mdt;
-    const double g123 = hh[0] + hh[1] + hh[2];
+    const double g123 = hp[0] + hp[1] + hp[2];
     const double u123 = uu[0] + uu[1] + uu[2];
-    const double coef = (uu[0] * (hh[0] + g123) + uu[1] * (hh[1] + g123) +
-                         uu[2] * (hh[2] + g123)) / (48.0 * e.s);
+    const double coef = (uu[0] * (hp[0] + g123) + uu[1] * (hp[1] + g123) +
+                         uu[2] * (hp[2] + g123)) / (48.0 * e.s);
     const double a12 = e.p12 * coef, a13 = e.p13 * coef, a23 = e.p23 * coef;
     const double x12 = a12 + m_off, x13 = a13 + m_off, x23 = a23 + m_off;
     const double t[3] = {-a12 - a13 + 2 * m_off, -a12 - a23 + 2 * m_off,
                          -a13 - a23 + 2 * m_off};
     // adjoints of the off-diagonal terms: the solve's -(lam_k de_l + lam_l
-    // de_k) through the cleared rows/columns, the product's -(rho_k y_l + rho_l y_k)
+    // de_k) through the cleared rows/columns, DIRICH's -(rho_k fb_l + rho_l fb_k)
     const double xb12 = -(lam[0] * de[1] + lam[1] * de[0]) * (wk[0] * wk[1]) -
-                        (rho[0] * yk[1] + rho[1] * yk[0]);
+                        (rho[0] * fb[1] + rho[1] * fb[0]);
     const double xb13 = -(lam[0] * de[2] + lam[2] * de[0]) * (wk[0] * wk[2]) -
-                        (rho[0] * yk[2] + rho[2] * yk[0]);
+                        (rho[0] * fb[2] + rho[2] * fb[0]);
     const double xb23 = -(lam[1] * de[2] + lam[2] * de[1]) * (wk[1] * wk[2]) -
-                        (rho[1] * yk[2] + rho[2] * yk[1]);
+                        (rho[1] * fb[2] + rho[2] * fb[1]);
     // through X = a + m_off and T_k = -(sum of a_kl) + 2 m_off
     const double ab12 = xb12 - dabar[0] - dabar[1];
     const double ab13 = xb13 - dabar[0] - dabar[2];
     const double ab23 = xb23 - dabar[1] - dabar[2];
     const double cb2 = (ab12 * e.p12 + ab13 * e.p13 + ab23 * e.p23) / (48.0 * e.s);
-    // -A rho (A symmetric) and M rho / dt
+    // DIRICH's -A fb: its transpose -A rho onto fb (A symmetric)
     const double arho[3] = {t[0] * rho[0] + x12 * rho[1] + x13 * rho[2],
                             x12 * rho[0] + t[1] * rho[1] + x23 * rho[2],
                             x13 * rho[0] + x23 * rho[1] + t[2] * rho[2]};
-    const double mr[3] = {m_off * (2 * rho[0] + rho[1] + rho[2]),
-                          m_off * (rho[0] + 2 * rho[1] + rho[2]),
-                          m_off * (rho[0] + rho[1] + 2 * rho[2])};
-    const double qxb = 0.5 * (e.bx[0] * rho[0] + e.bx[1] * rho[1] + e.bx[2] * rho[2]) / 3.0;
-    const double qyb = 0.5 * (e.by[0] * rho[0] + e.by[1] * rho[1] + e.by[2] * rho[2]) / 3.0;
+    // -MATFGR(h) u*: r_k = bx_k Qx + by_k Qy, Qx = F123 sum(u*) + sum(F u*)
+    const double rx = e.bx[0] * rho[0] + e.bx[1] * rho[1] + e.bx[2] * rho[2];
+    const double ry = e.by[0] * rho[0] + e.by[1] * rho[1] + e.by[2] * rho[2];
+    const double F[3] = {hp[0] / 24.0, hp[1] / 24.0, hp[2] / 24.0};
+    const double F123 = F[0] + F[1] + F[2];
+    const double Sx = sx[0] + sx[1] + sx[2], Sy = sy[0] + sy[1] + sy[2];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (l[k] < nown) {
         const int q = l[k];
-        acc_c[q] += a.grav * cb2 * (hh[k] + g123);
-        acc_h[q] += cb2 * (uu[k] + u123);
-        acc_qx[q] += qxb;
-        acc_qy[q] += qyb;
-        acc_y[q] -= arho[k];
-        acc_m[q] += mr[k];
+        acc_c[q] += a.grav * cb2 * (hp[k] + g123);
+        acc_h[q] += cb2 * (uu[k] + u123) + (rx * (Sx + sx[k]) + ry * (Sy + sy[k])) / 24.0;
+        acc_sx[q] += rx * (F123 + F[k]);
+        acc_sy[q] += ry * (F123 + F[k]);
+        acc_fb[q] -= arho[k];
       }
     }
   });
@@ -366,32 +469,99 @@ k_wave_assemble_adj(WaveArgs a, WaveAdjIn in, WaveAdjOut out) {
     const double dt = a.dt;
     const double w = a.w[i];
     const double lam = in.lam[i];
-    const double ybar = acc_y[l];
-    const double fbbar = ybar + (w != 0.0 ? 0.0 : lam * a.da_keep[i]);
-    double ebar = in.etot[i] + ybar + acc_m[l] - (w != 0.0 ? 0.0 : fbbar);
+    const double fbbar = acc_fb[l] + (w != 0.0 ? 0.0 : lam * a.da_keep[i]);
+    // eta' = eta + deta; fb = tide - eta on sea nodes
+    double ebar = in.etot[i] - (w != 0.0 ? 0.0 : fbbar);
     // the node quantities again
-    const double h = a.h[i], c = a.c[i];
+    const double h = a.h[i], c = a.c[i], f = a.f[i], vs = a.vs[i], vol = a.vol[i];
     const double uu = in.u[i], vv = in.v[i];
     const double k = in.kstr[i];
-    const double cf = 2.0 * a.grav / (k * k * cbrt(h));
-    const double sp = sqrt(uu * uu + vv * vv + a.eps_u * a.eps_u);
-    const double f = 0.5 * cf * sp / h;
-    const double qxbar = acc_qx[l], qybar = acc_qy[l];
+    const double hc = fmax(h, kWaveMinHC);
+    const double cf = 2.0 * a.grav / (k * k) / pow(hc, 1.0 / 3.0);
+    const double hf = fmax(h, kWaveMinHF);
+    const double sr = sqrt(uu * uu + vv * vv + a.eps_u * a.eps_u);
+    double sh = 0.0;
+    if (hf < kWaveHO) sh = sqrt(kWaveGHO * (kWaveHO - hf) * hf / kWaveHO);
+    const bool shallow = hf < kWaveHO && sh > sr;
+    const double sp = shallow ? sh : sr;
+    const double sbx = acc_sx[l], sby = acc_sy[l];
     const double ubn = in.ubar_n[i], vbn = in.vbar_n[i];
-    // u' = c (u/dt - g gx): du'/dc = u'/c
-    double cbar = acc_c[l] + (ubn * in.u_n[i] + vbn * in.v_n[i]) / c +
-                  (qxbar * uu + qybar * vv) * h / dt;
-    double hbar = acc_h[l] + (qxbar * uu + qybar * vv) * c / dt;
-    const double fbar = -c * c * cbar;
-    const double cfbar = fbar * 0.5 * sp / h;
-    const double spbar = fbar * 0.5 * cf / h;
-    hbar += -fbar * f / h - cfbar * cf / (3.0 * h);
-    out.ubar[i] = ubn * c / dt + qxbar * h * c / dt + spbar * uu / sp;
-    out.vbar[i] = vbn * c / dt + qybar * h * c / dt + spbar * vv / sp;
+    // u' = c (u/dt - g GX'/VOL) and u* = c (u/dt - g GX/VOL): du/dc = u/c
+    double cbar = acc_c[l];
+    if (c != 0.0)
+      cbar += (ubn * in.u_n[i] + vbn * in.v_n[i] + sbx * a.usx[i] + sby * a.usy[i]) / c;
+    // c = VOL / (VOL/dt + f VS)
+    const double fbar = -c * c * vs / vol * cbar;
+    const double vsbar = -c * c * f / vol * cbar;
+    // f = 0.5 CF sp / hf
+    const double cfbar = fbar * 0.5 * sp / hf;
+    const double spbar = fbar * 0.5 * cf / hf;
+    double hfbar = -fbar * f / hf;
+    double ub = (ubn + sbx) * c / dt, vb = (vbn + sby) * c / dt;
+    if (shallow) {
+      hfbar += spbar * kWaveGHO * (kWaveHO - 2.0 * hf) / (2.0 * kWaveHO * sh);
+    } else if (sr > 0.0) {
+      ub += spbar * uu / sr;
+      vb += spbar * vv / sr;
+    }
+    // CF = 2 g / K^2 / hc^(1/3)
+    const double hcbar = -cfbar * cf / (3.0 * hc);
     out.kbar[i] += cfbar * (-2.0 * cf / k);
+    const double hbar = (h > 0.0 ? acc_h[l] : 0.0) + (h > kWaveMinHC ? hcbar : 0.0) +
+                        (h > kWaveMinHF ? hfbar : 0.0);
     ebar += hbar;
     out.zbbar[i] -= hbar;
-    out.ebar[i] = ebar;
+    out.ubar[i] = ub;
+    out.vbar[i] = vb;
+    a.seb[i] = ebar;
+    a.sbx[i] = sbx;
+    a.sby[i] = sby;
+    a.svs[i] = vsbar;
+  }
+}
+
+// the transposes of GRADF (u* = c (u/dt - g GX/VOL): eta gets GRADF^T of
+// -g c u*bar / VOL) and of the slope factor of VS (zb), from the scratch
+// vectors k_wave_assemble_adj left; completes ebar and zbbar
+__global__ void __launch_bounds__(kThreads)
+k_wave_grad_adj(WaveArgs a, double* __restrict__ ebar, double* __restrict__ zbbar) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b = wave_block(a);
+  const int n0 = a.node_start[b], nown = a.node_start[b + 1] - n0;
+  const int hb = a.halo_start[b];
+  double* acc_e = reinterpret_cast<double*>(smem_raw);
+  double* acc_z = acc_e + nown;
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) acc_e[l] = acc_z[l] = 0.0;
+  __syncthreads();
+  colour_walk(a, b, [&](int s) {
+    int l[3], g[3];
+    slot_nodes(a, s, n0, nown, hb, l, g);
+    const ElemGeo e = elem_geo(a, s);
+    double sx = 0.0, sy = 0.0, sv = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int i = g[k];
+      const double f = -a.grav * a.c[i] / a.vol[i];
+      sx += f * a.sbx[i];
+      sy += f * a.sby[i];
+      sv += a.svs[i];
+    }
+    double d2, d3;
+    (void)elem_slope(e, a.zb[g[0]], a.zb[g[1]], a.zb[g[2]], &d2, &d3);
+    const double slb = sv * e.s / 3.0;
+    const double zb3[3] = {-(d2 + d3) * slb, d2 * slb, d3 * slb};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (l[k] < nown) {
+        acc_e[l[k]] += (e.bx[k] * sx + e.by[k] * sy) / 6.0;
+        acc_z[l[k]] += zb3[k];
+      }
+    }
+  });
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) {
+    const int i = n0 + l;
+    ebar[i] = a.seb[i] + acc_e[l];
+    zbbar[i] += acc_z[l];
   }
 }
 
@@ -417,11 +587,18 @@ WaveArgs wave_args(bhip_ctx* c, const bhip_wave_params* p) {
   a.w = c->wv_w.p;
   a.vol = c->wv_vol.p;
   a.h = c->wv_h.p;
+  a.zb = c->wv_zb.p;
+  a.f = c->wv_f.p;
+  a.vs = c->wv_vs.p;
   a.c = c->wv_c.p;
-  a.qx = c->wv_qx.p;
-  a.qy = c->wv_qy.p;
-  a.y = c->wv_y.p;
+  a.usx = c->wv_usx.p;
+  a.usy = c->wv_usy.p;
+  a.fb = c->wv_fb.p;
   a.da_keep = c->wv_da.p;
+  a.sbx = c->wv_sbx.p;
+  a.sby = c->wv_sby.p;
+  a.svs = c->wv_svs.p;
+  a.seb = c->wv_seb.p;
   return a;
 }
 
@@ -439,16 +616,17 @@ int bhip_wave_setup(bhip_ctx* c, const double* xel, const double* yel, const dou
     return BHIP_EBADARG;
   const Layout& L = c->L;
   const int n = L.npoin, ne = L.nelem;
-  // k_wave_assemble_adj holds six accumulators per owned node in LDS
-  if ((size_t)6 * L.max_owned * sizeof(double) > 64 * 1024) {
+  // k_wave_assemble_adj holds five accumulators per owned node in LDS
+  if ((size_t)5 * L.max_owned * sizeof(double) > 64 * 1024) {
     fprintf(stderr, "libbiefhip: bhip_wave_setup: blocks of %d nodes need more than 64 KiB "
-            "of LDS; use blocks of at most 1365 nodes\n", L.max_owned);
+            "of LDS; use blocks of at most 1638 nodes\n", L.max_owned);
     return BHIP_EBADARG;
   }
   for (int e = 0; e < ne; ++e)
     if (!(surfac[e] > 0.0)) return BHIP_EBADARG;
-  for (auto* v : {&c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_c, &c->wv_qx, &c->wv_qy, &c->wv_y,
-                  &c->wv_da})
+  for (auto* v : {&c->wv_w, &c->wv_vol, &c->wv_h, &c->wv_zb, &c->wv_f, &c->wv_vs, &c->wv_c,
+                  &c->wv_usx, &c->wv_usy, &c->wv_fb, &c->wv_da, &c->wv_sbx, &c->wv_sby,
+                  &c->wv_svs, &c->wv_seb})
     HIP_OK(v->alloc(n));
   HIP_OK(c->wv_geo.alloc((size_t)5 * L.nslot_pad));
   // geometry per slot, from the caller's element arrays
@@ -498,9 +676,11 @@ int bhip_wave_assemble(bhip_ctx* c, const bhip_wave_params* p, const double* eta
   WaveArgs a = wave_args(c, p);
   hipLaunchKernelGGL(k_wave_nodes, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, a,
                      L.npoin, eta, u, v, kstr, zb);
+  hipLaunchKernelGGL(k_wave_grad, dim3(L.nblocks), dim3(kThreads),
+                     3 * (size_t)L.max_owned * sizeof(double), c->stream, a, eta, u, v);
   hipLaunchKernelGGL(k_wave_assemble, dim3(L.nblocks), dim3(kThreads),
-                     2 * (size_t)L.max_owned * sizeof(double), c->stream, a, eta, c->da.p,
-                     c->xa.p, rhs);
+                     2 * (size_t)L.max_owned * sizeof(double), c->stream, a, c->da.p, c->xa.p,
+                     rhs);
   HIP_OK(hipGetLastError());
   // MATRIY's storage of the assembled system (matriy.f:375-376)
   c->typdia = 'Q';
@@ -549,9 +729,11 @@ int bhip_wave_assemble_adj(bhip_ctx* c, const bhip_wave_params* p, const double*
   const Layout& L = c->L;
   WaveAdjIn in = {lam, deta, eta, u, v, u_n, v_n, ubar_n, vbar_n, etot, kstr};
   WaveAdjOut out = {ebar, ubar, vbar, kbar, zbbar};
+  const WaveArgs a = wave_args(c, p);
   hipLaunchKernelGGL(k_wave_assemble_adj, dim3(L.nblocks), dim3(kThreads),
-                     6 * (size_t)L.max_owned * sizeof(double), c->stream, wave_args(c, p), in,
-                     out);
+                     5 * (size_t)L.max_owned * sizeof(double), c->stream, a, in, out);
+  hipLaunchKernelGGL(k_wave_grad_adj, dim3(L.nblocks), dim3(kThreads),
+                     2 * (size_t)L.max_owned * sizeof(double), c->stream, a, ebar, zbbar);
   HIP_OK(hipGetLastError());
   return BHIP_OK;
 }

@@ -1,27 +1,34 @@
 """Time-stepped telemac2d wave equation on the GPU, with its adjoint.
 
 The forward step is the wave-equation treatment of PROPAG (sources/telemac2d/
-propag.F, TREATMENT OF THE LINEAR SYSTEM = 2: the depth increment solved
-from the system AM1 = M/dt + MATDIFUV(-1, DM1, HPROP), propag.F:664-716,
-1369-1408, 1634-1756), restated for a theta = 1 scheme without advection:
+propag.F, TREATMENT OF THE LINEAR SYSTEM = 2) for implicitation 1
+(TETAH = TETAU = TETAZCOMP = 1), no advection or diffusion of velocity, no
+sources, no tidal flats, one sub-iteration, Strickler friction (KFROT 3,
+HFROT 1), the free surface prescribed on sea nodes (free velocity there) and
+walls elsewhere:
 
-  friction  CF = 2 g / (K^2 h^(1/3))            (friction_calc.f:130-134)
-            f  = CF |u| / (2 h)                  (implicit coefficient, fricti.f)
-            c  = 1 / (1/dt + f)
-  momentum  u' = c (u/dt - g grad(eta'))         (nodal, lumped projection)
-  continuity, weak form with P1 test functions, integrated by parts:
-            (M/dt + K_w) (eta' - eta) = G - K_w eta
-            K_w = int g c h grad(phi_i).grad(phi_j)   (MATDIFUV 'UV', mt02aa.f:200-240)
-            G_i = int grad(phi_i) . (h c u) / dt
+  friction  CF = 2 g / K^2 / max(h, 1e-4)^(1/3)   (friction_unif.f:124-127,
+                                                    friction_calc.f:130-134)
+            f  = CF |u| / (2 max(h, 1e-9)), |u| raised to the shallow-water
+                 speed below 3 cm                  (fricti.f:123-135)
+            AM2D = VOL/dt + f VS, VS the basis integrals weighted by the
+                 bottom-slope factor               (slop10.f, propag.F:1155-1170)
+            c = VOL / AM2D                         (DM1 = -g c, propag.F:1397)
+  momentum  u* = c (u/dt - g GRADF(eta)/VOL)       (T2 = CV2/AM2D, propag.F:1413)
+            u' = c (u/dt - g GRADF(eta')/VOL)      (propag.F:1685-1707)
+  continuity, for the increment deta = eta' - eta:
+            (M/dt + MATDIFUV(-1, DM1, max(h,0))) deta = -MATFGR(h) u*
+            (mt01aa.f, mt02aa.f:200-240, mt08aa.f; propag.F:679, 1407, 1464)
   Dirichlet values of eta' on the sea boundary enter through DIRICH
   (dirich.f: b -= A f, rows/columns of prescribed nodes cleared).
 
 with h = eta - zb. The linear system is solved by the HIP CG behind SOLVE
 (bief.Context, diagonal preconditioning), its matrix-vector products by the
-HIP EBE mat-vec. The model has no counterpart to compare against: the
-reference differentiates telemac2d with the external dco/fortran tool
-(COMPAD, ad_solve.F), which is absent. It is checked against central finite
-differences and a tangent-linear run instead.
+HIP EBE mat-vec. The step's operands are pinned against the reference's own
+routines run in that sequence (the PROPAG golden vectors); the reference
+differentiates telemac2d with the external dco/fortran tool (COMPAD,
+ad_solve.F), which is absent, so the gradients are checked against central
+finite differences and a tangent-linear run.
 
 Differentiation: the linear solve and the EBE mat-vec are torch.autograd
 Functions with hand-written adjoints (adjoint CG solve with the same
@@ -217,16 +224,8 @@ class WaveModel:
             out = out.index_add(0, self.ikle[k], w[k])
         return out
 
-    def mass_matvec(self, y, xmul):
-        """M y for the P1 mass matrix times xmul (MT01AA)."""
-        i1, i2, i3 = self.ikle
-        c = self.s * (xmul / 12.0)
-        y1, y2, y3 = y[i1], y[i2], y[i3]
-        return self.scatter(torch.stack([c * (2 * y1 + y2 + y3), c * (y1 + 2 * y2 + y3),
-                                         c * (y1 + y2 + 2 * y3)]))
-
     def system(self, h, cnode):
-        """(da, xa) of AM1 = M/dt + MATDIFUV(U = g c, V = h)."""
+        """(da, xa) of AM1 = M/dt + MATDIFUV(-1, DM1 = -g c, h) (h = max(eta - zb, 0))."""
         i1, i2, i3 = self.ikle
         x2, x3, y2, y3, s = self.x2, self.x3, self.y2, self.y3, self.s
         # MT01AA (mt01aa.f:83-106)
@@ -248,39 +247,64 @@ class WaveModel:
     def tide(self, t):
         return self.amp * np.sin(2.0 * np.pi * t / self.period)
 
+    def gradf(self, q):
+        """GRADF of a node field, assembled (VECTOR 'GRADF', vc13aa.f, XMUL 1)."""
+        i1, i2, i3 = self.ikle
+        q1, q2, q3 = q[i1], q[i2], q[i3]
+        gx = (self.y2 * (q1 - q3) + self.y3 * (q2 - q1)) / 6.0
+        gy = (self.x2 * (q3 - q1) + self.x3 * (q1 - q2)) / 6.0
+        return self.scatter(torch.stack([gx, gx, gx])), self.scatter(torch.stack([gy, gy, gy]))
+
+    def slope_mass(self, zb):
+        """Basis integrals weighted by the bottom-slope factor of each element
+        (SLOP10, slop10.f; the masked VECTOR 'MASBAS' of propag.F:1155-1161)."""
+        i1, i2, i3 = self.ikle
+        z2, z3 = zb[i2] - zb[i1], zb[i3] - zb[i1]
+        a = (self.x2 * self.y3 - self.x3 * self.y2) ** 2
+        b = (self.y2 * z3 - z2 * self.y3) ** 2
+        c = (self.x3 * z2 - z3 * self.x2) ** 2
+        w = self.s / 3.0 * torch.sqrt((a + b + c) / a)
+        return self.scatter(torch.stack([w, w, w]))
+
     def step(self, eta, u, v, k_str, zb, t_new):
         """One time step (eta, u, v) -> (eta', u', v') at time t_new."""
         dt = self.dt
         h = eta - zb
-        cf = 2.0 * GRAV / (k_str * k_str * h.pow(1.0 / 3.0))
-        speed = torch.sqrt(u * u + v * v + self.eps_u * self.eps_u)
-        f = 0.5 * cf * speed / h
-        c = 1.0 / (1.0 / dt + f)
-        da, xa = self.system(h, c)
-        # right-hand side of the increment: G - K_w eta = G - (A eta - M eta/dt)
-        qx, qy = h * c * u / dt, h * c * v / dt
+        hp = torch.clamp(h, min=0.0)
+        hc = torch.clamp(h, min=1e-4)
+        cf = 2.0 * GRAV / (k_str * k_str) / hc.pow(1.0 / 3.0)
+        hf = torch.clamp(h, min=1e-9)
+        sp = torch.sqrt(u * u + v * v + self.eps_u * self.eps_u)
+        shallow = hf < 3e-2
+        arg = torch.where(shallow, 9.81 * (3e-2 - hf) * hf / 3e-2, torch.ones_like(hf))
+        sp = torch.where(shallow, torch.maximum(sp, torch.sqrt(arg)), sp)
+        f = 0.5 * cf * sp / hf
+        c = self.vol / (self.vol / dt + f * self.slope_mass(zb))
+        da, xa = self.system(hp, c)
+        gx, gy = self.gradf(eta)
+        usx = c * (u / dt - GRAV * gx / self.vol)
+        usy = c * (v / dt - GRAV * gy / self.vol)
+        # -MATFGR(h) u*: r_k = bx_k Qx + by_k Qy, Qx = F123 sum(u*) + sum(F u*), F = h/24
         i1, i2, i3 = self.ikle
-        qxm = (qx[i1] + qx[i2] + qx[i3]) / 3.0
-        qym = (qy[i1] + qy[i2] + qy[i3]) / 3.0
-        g_el = 0.5 * (self.bx * qxm + self.by * qym)  # S * grad(phi_k).q_mean
+        f1, f2, f3 = hp[i1] / 24.0, hp[i2] / 24.0, hp[i3] / 24.0
+        f123 = f1 + f2 + f3
+        sx1, sx2, sx3 = usx[i1], usx[i2], usx[i3]
+        sy1, sy2, sy3 = usy[i1], usy[i2], usy[i3]
+        qx = f123 * (sx1 + sx2 + sx3) + f1 * sx1 + f2 * sx2 + f3 * sx3
+        qy = f123 * (sy1 + sy2 + sy3) + f1 * sy1 + f2 * sy2 + f3 * sy3
+        r_el = self.bx * qx + self.by * qy
         # DIRICH (diri01.f:94-139): prescribed increment fb on the sea
-        # boundary, b -= A fb; one product A (eta + fb) serves both terms
+        # boundary, b -= A fb, sea rows D fb, their rows and columns cleared
         w = (~self.sea).to(eta.dtype)
         fb = torch.where(self.sea, self.tide(t_new) - eta, torch.zeros_like(eta))
-        rhs = self.scatter(g_el) - EbeMatvec.apply(self.ops, da, xa, eta + fb) + \
-            self.mass_matvec(eta, 1.0 / dt)
+        rhs = self.scatter(r_el) - EbeMatvec.apply(self.ops, da, xa, fb)
         rhs = torch.where(self.sea, da * fb, rhs)
         xa_d = xa * torch.stack([w[i1] * w[i2], w[i1] * w[i3], w[i2] * w[i3]])
         deta = EbeSolve.apply(self.ops, da, xa_d, rhs)
         eta_n = eta + deta
-        # momentum with the new free surface: lumped projection of grad(eta')
-        e1, e2, e3 = eta_n[i1], eta_n[i2], eta_n[i3]
-        gx_el = (self.bx[0] * e1 + self.bx[1] * e2 + self.bx[2] * e3) / 6.0  # S/3 * deta/dx
-        gy_el = (self.by[0] * e1 + self.by[1] * e2 + self.by[2] * e3) / 6.0
-        gxn = self.scatter(torch.stack([gx_el, gx_el, gx_el])) / self.vol
-        gyn = self.scatter(torch.stack([gy_el, gy_el, gy_el])) / self.vol
-        u_n = c * (u / dt - GRAV * gxn)
-        v_n = c * (v / dt - GRAV * gyn)
+        gxn, gyn = self.gradf(eta_n)
+        u_n = c * (u / dt - GRAV * gxn / self.vol)
+        v_n = c * (v / dt - GRAV * gyn / self.vol)
         return eta_n, u_n, v_n
 
     def initial_state(self, zb):

@@ -296,6 +296,9 @@ def _bind_driver(L):
     L.ref_mt03aa.argtypes = [_I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.ref_mt04aa.argtypes = [_I, _D, _P, _P, _P, _P, _P, _P, _P, _P]
     L.ref_mt05aa.argtypes = [_I, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P]
+    if hasattr(L, "ref_propag"):
+        L.ref_propag.argtypes = [_I, _I] + [_P] * 10 + [_P, _I, _D, _D, _I, _D] + [_P] * 15 + \
+            [ctypes.POINTER(_I)]
     return L
 
 
@@ -532,3 +535,162 @@ def supg_system(mesh, u, v, nu, dt, fn, tetat=1.0, reference=False):
     am1_d = am1_d + assemble_diag(td, ik, n)
     am1_x = am1_x + _om_xm(xd)
     return Matrix(am1_d, am1_x, "Q", "Q"), sm
+
+
+# --------------------------------------------------------------------------
+# telemac2d PROPAG: one wave-equation step's operands
+# --------------------------------------------------------------------------
+
+def _assemble3(w, ikle, npoin, mask=None):
+    """ASSVEC of three element columns (assvec.f: column k, elements in
+    order), MASKEL multiplying each term when given (assve1.f:66-84)."""
+    x = np.zeros(npoin)
+    for k in range(3):
+        wk = w[k] if mask is None else w[k] * mask
+        np.add.at(x, ikle[k], wk)
+    return x
+
+
+def _vc13aa(xel, yel, ikle, f, xmul):
+    """GRADF element terms (vc13aa.f, P1 F): (x, y) components, the same
+    value for the three nodes."""
+    f1, f2, f3 = f[ikle[0]], f[ikle[1]], f[ikle[2]]
+    xs6 = xmul / 6.0
+    wx = (yel[1] * (f1 - f3) + yel[2] * (f2 - f1)) * xs6
+    wy = (xel[1] * (f3 - f1) + xel[2] * (f1 - f2)) * xs6
+    return np.stack([wx, wx, wx]), np.stack([wy, wy, wy])
+
+
+def _slop10(xel, yel, ikle, z):
+    """Bottom-slope area factor per element (slop10.f)."""
+    x2, x3, y2, y3 = xel[1], xel[2], yel[1], yel[2]
+    z2 = z[ikle[1]] - z[ikle[0]]
+    z3 = z[ikle[2]] - z[ikle[0]]
+    a = (x2 * y3 - x3 * y2) ** 2
+    b = (y2 * z3 - z2 * y3) ** 2
+    c = (x3 * z2 - z3 * x2) ** 2
+    return np.sqrt((a + b + c) / a)
+
+
+def _mt08aa(xel, yel, ikle, f, xmul, icoord):
+    """MATFGR (mt08aa.f): A(i,j) = -XMUL int PSI2(j) F dPSI1(i)/dx (or /dy),
+    as (diagonal (3, ne), off-diagonal in the 'Q' order 12,13,23,21,31,32)."""
+    s24 = xmul / 24.0
+    f1, f2, f3 = f[ikle[0]] * s24, f[ikle[1]] * s24, f[ikle[2]] * s24
+    f123 = f1 + f2 + f3
+    if icoord == 1:
+        y2, y3 = yel[1], yel[2]
+        a12 = (y3 - y2) * (f123 + f2)
+        a13 = (y3 - y2) * (f123 + f3)
+        a23 = y3 * (-f123 - f3)
+        a21 = y3 * (-f123 - f1)
+        a31 = y2 * (f123 + f1)
+        a32 = y2 * (f123 + f2)
+    else:
+        x2, x3 = xel[1], xel[2]
+        a12 = (x2 - x3) * (f123 + f2)
+        a13 = (x2 - x3) * (f123 + f3)
+        a23 = x3 * (f123 + f3)
+        a21 = x3 * (f123 + f1)
+        a31 = x2 * (-f123 - f1)
+        a32 = x2 * (-f123 - f2)
+    t = np.stack([-a21 - a31, -a12 - a32, -a13 - a23])
+    return t, np.stack([a12, a13, a23, a21, a31, a32])
+
+
+def propag_step(mesh, hn, un, vn, zf, chestr, hbor, limdir, kdir=2, dt=1.0, grav=9.81,
+                nitmax=1000, eps=1e-12):
+    """One wave-equation step of telemac2d PROPAG (TREATMENT OF THE LINEAR
+    SYSTEM = 2) for the options the driver's ref_propag runs (implicitation
+    1, no advection or diffusion of velocity, no sources, no tidal flats, one
+    sub-iteration, Strickler friction with HFROT 1, the depth hbor prescribed
+    where limdir == kdir, walls elsewhere), in PROPAG's sequence of
+    operations (propag.F:664-716, 1003-1012, 1150-1170, 1369-1465, 1634-1710):
+
+      VOLU2D = MASBAS; HC = max(HN, 1e-4) (friction_unif.f:124-127);
+      CF = 2 GRAV / K^2 / HC^(1/3) (friction_calc.f:133-134);
+      FU = -0.5 CF |U| / max(HN, 1e-9), |U| raised below 3 cm (fricti.f:123-135);
+      T = FU * MASBAS(-1, slope mask) (slop10.f); AM2D = VOLU2D/dt + T;
+      CV2 = GRADF(-GRAV, HN + ZF) + (VOLU2D/dt) UN; AM2D^-1 (guard 1e-6);
+      DM1 = -GRAV VOLU2D AM2D^-1; HPROP = max(HN, 0);
+      AM1 = MATMAS(1/dt) + MATDIFUV(-1, DM1, HPROP);
+      T2 = CV2 AM2D^-1; CV1 = -MATFGR_X(HPROP) T2 - MATFGR_Y(HPROP) T3;
+      DIRICH(DH, AM1, CV1, HBOR - HN); SOLVE (CG, PRECON 2);
+      U = (CV2 + GRADF(-GRAV, DH)) AM2D^-1.
+
+    The element routines are the oracle's (MT01AA, MT02AA, the EBE product)
+    or restated here (VC00AA, VC13AA, SLOP10, MT08AA); sums follow ASSVEC's
+    order. Returns a dict of node vectors (and AM1's da, xa (3, ne))."""
+    xe, ye = mesh.xel_yel()
+    s = mesh.surfac()
+    ik = mesh.ikle
+    n = mesh.npoin
+    hn, un, vn, zf, chestr = (_f64(a) for a in (hn, un, vn, zf, chestr))
+    third = np.stack([s / 3.0] * 3)
+    volu2d = _assemble3(third, ik, n)
+    hc = np.maximum(hn, 1e-4)
+    cf = 2.0 * grav / chestr ** 2 / hc ** (1.0 / 3.0)
+    unorm = np.sqrt(un ** 2 + vn ** 2)
+    h = np.maximum(hn, 1e-9)
+    ho = 3e-2
+    shallow = h < ho
+    unorm = np.where(shallow, np.maximum(unorm, np.sqrt(9.81 * np.maximum(ho - h, 0.0) * h / ho)),
+                     unorm)
+    fu = -0.5 * cf * unorm / h
+    te3 = _slop10(xe, ye, ik, zf)
+    tm = _assemble3(-third, ik, n, mask=te3) * fu
+    am3d = volu2d * (1.0 / dt)
+    am2d = am3d + tm
+    t8 = hn + zf
+    wx, wy = _vc13aa(xe, ye, ik, t8, -grav)
+    cv2 = _assemble3(wx, ik, n) + am3d * un
+    cv3 = _assemble3(wy, ik, n) + am3d * vn
+    inv = np.where(np.abs(am2d) > 1e-6, 1.0 / np.where(am2d == 0.0, 1.0, am2d), 0.0)
+    dm1 = -grav * volu2d * inv
+    hprop = np.maximum(hn, 0.0)
+    t1, x1 = mt01aa(s, 1.0 / dt)
+    t2e, x2e = mt02aa(2, xe, ye, s, ik, dm1, hprop, -1.0)
+    da = assemble_diag(t1, ik, n) + assemble_diag(t2e, ik, n)
+    xa = x1 + x2e
+    t2 = cv2 * inv
+    t3 = cv3 * inv
+    cv1 = np.zeros(n)
+    for icoord, tt in ((1, t2), (2, t3)):
+        tb, xb = _mt08aa(xe, ye, ik, hprop, 1.0, icoord)
+        B = Matrix(assemble_diag(tb, ik, n), xb, "Q", "Q")
+        cv1 = matvec("X=X-AY", B, ik, tt, x=cv1)
+    fbor = np.where(_i32(limdir) == kdir, _f64(hbor) - hn, 0.0)
+    dh0, A1, cv1 = dirich(Matrix(da, xa, "Q", "S"), ik, cv1, fbor, limdir, kdir=kdir,
+                          f=np.zeros(n))
+    dh, info, _, _ = solve(A1, ik, cv1, dh0, slv=1, precon=2, eps=eps, nitmax=nitmax)
+    gx, gy = _vc13aa(xe, ye, ik, dh, -grav)
+    u = (cv2 + _assemble3(gx, ik, n)) * inv
+    v = (cv3 + _assemble3(gy, ik, n)) * inv
+    return {"volu2d": volu2d, "cf": cf, "fu": fu, "am2d": inv, "dm1": dm1, "hprop": hprop,
+            "t2": t2, "t3": t3, "da": A1.da, "xa": A1.xa, "cv1": cv1, "dh0": dh0, "dh": dh,
+            "u": u, "v": v, "niter": info.niter}
+
+
+def ref_propag(mesh, hn, un, vn, zf, chestr, hbor, limdir, kdir=2, dt=1.0, grav=9.81,
+               nitmax=1000, eps=1e-12):
+    """The same step computed by the reference's own routines in the
+    driver (ref_driver.f90: ref_propag). Same dict (niter not reported)."""
+    rm = RefMesh(mesh.ikle, mesh.npoin, mesh.nelem)
+    rm.bind()
+    xe, ye = mesh.xel_yel()
+    s = mesh.surfac()
+    n, ne = mesh.npoin, mesh.nelem
+    out = {k: np.zeros(n) for k in ("volu2d", "cf", "fu", "am2d", "dm1", "hprop", "t2", "t3",
+                                    "da", "cv1", "dh0", "dh", "u", "v")}
+    xa = np.zeros((3, ne))
+    niter = ctypes.c_int(0)
+    ik = _i32(mesh.ikle)
+    args = [_f64(a) for a in (hn, un, vn, zf, chestr, hbor)]
+    rm.lib.ref_propag(ne, n, _ptr(ik), _ptr(_f64(xe)), _ptr(_f64(ye)), _ptr(_f64(s)),
+                      *[_ptr(a) for a in args], _ptr(_i32(limdir)), kdir, dt, grav, nitmax, eps,
+                      *[_ptr(out[k]) for k in ("volu2d", "cf", "fu", "am2d", "dm1", "hprop",
+                                               "t2", "t3", "da")],
+                      _ptr(xa), _ptr(out["cv1"]), _ptr(out["dh0"]), _ptr(out["dh"]),
+                      _ptr(out["u"]), _ptr(out["v"]), ctypes.byref(niter))
+    out["xa"] = xa
+    return out

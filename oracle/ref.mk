@@ -13,6 +13,9 @@
 #   make -f oracle/ref.mk            (from the repository root)
 #
 REFSRC  ?= /root/reference/sources/utils
+# telemac2d's friction routines (FRICTION_CALC, FRICTI) and the modules their
+# interfaces need, for the PROPAG operand sequence of the driver
+REFTOP  ?= $(REFSRC)/..
 OUT     ?= oracle/_ref
 ifeq ($(origin FC),default)
 FC      := flang
@@ -62,7 +65,18 @@ $(OBJ)/%.o: $(OUT)/stamp_mods
 	$(FC) $(FFLAGS) -module-dir $(OBJ)/m_$* -I$(MOD) -c $$f -o $@ 2>$(OBJ)/$*.err || (echo "skip $$s"; rm -f $@); \
 	rm -rf $(OBJ)/m_$*
 
-$(OUT)/libbief_ref.a: $(OUT)/stamp_mods $(RESTOBJ)
+# module files in dependency order, then the routines (DRAGFO: FRICTI names it)
+T2DMODS := waqtel/declarations_waqtel.f telemac2d/friction_def.f \
+           telemac2d/declarations_telemac2d.f telemac2d/interface_telemac2d.f
+T2DSRCS := telemac2d/friction_calc.f telemac2d/fricti.f telemac2d/dragfo.f
+
+$(OUT)/stamp_t2d: $(OUT)/stamp_mods
+	@set -e; for s in $(T2DMODS) $(T2DSRCS); do \
+	  o=$(OBJ)/t2d__$$(basename $${s%.*}).o; \
+	  $(FC) $(FFLAGS) -module-dir $(MOD) -I$(MOD) -c $(REFTOP)/$$s -o $$o; \
+	done; touch $@
+
+$(OUT)/libbief_ref.a: $(OUT)/stamp_mods $(OUT)/stamp_t2d $(RESTOBJ)
 	@rm -f $@; ar rcs $@ $(sort $(MODOBJ) $(BMODOBJ) $(wildcard $(OBJ)/*.o))
 
 $(OBJ) $(MOD):

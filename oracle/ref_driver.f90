@@ -2,7 +2,10 @@
 ! oracle/_ref/libbief_ref.a). It is written for this project: it fills a
 ! BIEF_MESH with the fields the linear-system path reads, then calls the
 ! reference routines SOLVE, MATVEC, DIRICH, LUMP, MT01AA, MT02AA, MT01PP,
-! MT02PP, and the SUPG routines KSPG11, MT03AA, MT04AA, MT05AA unchanged.
+! MT02PP, and the SUPG routines KSPG11, MT03AA, MT04AA, MT05AA unchanged, and
+! runs telemac2d PROPAG's wave-equation operand sequence with the reference's
+! element routines, ASSVEC, OV/OVD, MATVEC, DIRICH, SOLVE and telemac2d's
+! FRICTION_CALC and FRICTI (ref_propag).
 !
 ! TEST INFRASTRUCTURE ONLY (checker + "reference" CPU baseline). Entry points
 ! are bind(C) so tests/ and bench.py's cpu_baseline leg can load the shared
@@ -10,6 +13,7 @@
 module ref_driver
   use bief
   use declarations_special
+  use interface_telemac2d, only: fricti, friction_calc
   use iso_c_binding
   implicit none
   type(bief_mesh), target, save :: mesh
@@ -450,6 +454,219 @@ contains
     call mt05aa(t(1,1), xm(1,1), xm(1,2), xm(1,4), t(1,2), xm(1,3), xm(1,5), xm(1,6), &
          t(1,3), xmul, su, sv, u, v, xel, yel, ik, nelem, nelem, formul)
     deallocate(ik)
+  end subroutine
+
+  ! One wave-equation step of telemac2d PROPAG (TREATMENT OF THE LINEAR
+  ! SYSTEM = 2, propag.F:664-716, 1003-1012, 1150-1170, 1369-1465, 1634-1710)
+  ! for these options: P1 depth and velocity, implicitation 1 for depth and
+  ! velocity (TETAH = TETAU = TETAZCOMP = 1), no advection of velocity, no
+  ! diffusion of velocity, no sources, no tidal flats (OPTBAN 0), one
+  ! sub-iteration (HPROP = max(HN, 0), hpropa.f), Strickler friction (KFROT 3,
+  ! HFROT 1), no vertical structures, sequential, MODASS 1, IORDRH 0 (DH
+  ! starts at 0), depth prescribed (KDIR) where limdir == kdir with the depth
+  ! hbor, walls elsewhere (no boundary flux term: FLBOR vanishes on wall
+  ! segments, and free-exit segments join prescribed-depth points only).
+  ! Every operand is computed by the reference's routine for it:
+  !   VOLU2D  VC00AA + ASSVEC (VECTOR 'MASBAS')
+  !   CF      OV 'X=+(Y,C)' on HN (friction_unif.f:124-127), FRICTION_CALC
+  !   FU      FRICTI; SLOP10 (SLOPES) and the masked MASBAS of propag.F:1155-1161
+  !   AM2%D   OV 'X=CY', 'X=X+Y' (OM 'M=M+D'), OVD 'X=1/Y' guarded at 1e-6
+  !   DM1     OV 'X=CYZ' with C = -GRAV
+  !   AM1     MT01AA (SL1 = 1/dt) 'M=N', MT02AA 'MATDIFUV' (-1, DM1, HPROP)
+  !           'M=M+N' (assembled diagonals added, off-diagonal terms added)
+  !   CV2,CV3 VC13AA + ASSVEC (VECTOR 'GRADF', -GRAV, T8 = HN + ZF),
+  !           OV 'X=X+YZ' with AM3%D = VOLU2D/dt and UN, VN
+  !   T2,T3   OV 'X=YZ' with the inverted AM2%D (= AM3%D)
+  !   CV1     0, MATVEC 'X=X-AY' with BM1, BM2 = MT08AA (MATFGR X/Y, TETAU,
+  !           HPROP) and T2, T3
+  !   DIRICH  on DH with HBOR - HN; then SOLVE (CG, PRECON 2);
+  !   U, V    CV2 += VC13AA(-GRAV DH) assembled (VECTOR 'GRADF' '+'), OV 'X=YZ'
+  ! Outputs are node vectors (npoin) unless stated: volu2d, cf, fu, am2d (the
+  ! inverted diagonal), dm1, hprop, t2, t3, da and xa(nelem, 3) of AM1 after
+  ! DIRICH, cv1 after DIRICH, dh0 (DH after DIRICH), dh (after SOLVE), u, v.
+  subroutine ref_propag(nelem, npoin, ikle0, xel, yel, surfac, hn, un, vn, zf, chestr, hbor, &
+       limdir, kdir, dt, grav, nitmax, eps, volu2d, cf, fu, am2d, dm1, hprop, t2, t3, da, &
+       xa, cv1, dh0, dh, u, v, niter) bind(c, name='ref_propag')
+    integer(c_int), value :: nelem, npoin, kdir, nitmax
+    integer(c_int), intent(in) :: ikle0(nelem, 3), limdir(*)
+    real(c_double), intent(in) :: xel(nelem, 3), yel(nelem, 3), surfac(nelem)
+    real(c_double), intent(in) :: hn(npoin), un(npoin), vn(npoin), zf(npoin), chestr(npoin)
+    real(c_double), intent(in) :: hbor(npoin)
+    real(c_double), value :: dt, grav, eps
+    real(c_double), intent(out) :: volu2d(npoin), cf(npoin), fu(npoin), am2d(npoin)
+    real(c_double), intent(out) :: dm1(npoin), hprop(npoin), t2(npoin), t3(npoin)
+    real(c_double), intent(out) :: da(npoin), xa(nelem, 3), cv1(npoin), dh0(npoin)
+    real(c_double), intent(out) :: dh(npoin), u(npoin), v(npoin)
+    integer(c_int), intent(out) :: niter
+    integer, allocatable :: ik(:,:)
+    real(c_double), allocatable :: w(:,:), te3(:), tm(:), am3d(:), d1(:), d2(:)
+    real(c_double), allocatable :: tmass(:,:), xmass(:,:), tdif(:,:), xdif(:,:)
+    real(c_double), allocatable :: a(:,:), t8(:), cv2(:), cv3(:), hc(:)
+    real(c_double), allocatable :: xbm(:,:), dbm(:)
+    type(bief_obj) :: sf, sdm1, shp, maskpt
+    type(bief_obj), save :: oun, ovn, ohn, ocf, ofu, ofv, ofud, ofvd, ot1, ot2, ounsv, omsk, &
+         ochestr, ohc, ovres
+    type(slvcfg) :: cfg
+    real(c_double) :: dummy(1)
+    integer :: icoord
+    character(len=16) :: formul
+    allocate(ik(nelem, 3), w(nelem, 3), te3(nelem), tm(npoin), am3d(npoin), d1(npoin), d2(npoin))
+    allocate(tmass(nelem, 3), xmass(nelem, 3), tdif(nelem, 3), xdif(nelem, 3), a(nelem, 9))
+    allocate(t8(npoin), cv2(npoin), cv3(npoin), hc(npoin), xbm(nelem, 6), dbm(npoin))
+    ik = ikle0 + 1
+    sf%elm = 11
+    sf%type = 2
+    sf%dim2 = 1
+    ! VOLU2D: VECTOR 'MASBAS' (vc00aa.f + assvec.f)
+    call vc00aa(1.d0, surfac, nelem, nelem, w(1,1), w(1,2), w(1,3))
+    call assvec(volu2d, ik, npoin, nelem, nelem, 11, w, .true., 1, .false., dummy, 3)
+    ! friction coefficient (friction_unif.f:120-146; friction_calc.f:130-134)
+    call bief_allvec(1, ochestr, 'CHESTR', 11, 1, 2, mesh)
+    call bief_allvec(1, ohc, 'HC    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ovres, 'VRES  ', 11, 1, 2, mesh)
+    call bief_allvec(1, ocf, 'CF    ', 11, 1, 2, mesh)
+    ochestr%r(1:npoin) = chestr
+    call ov('X=Y     ', hc, hn, hn, 0.d0, npoin)
+    call ov('X=+(Y,C)', hc, hc, hc, 1.d-4, npoin)
+    ohc%r(1:npoin) = hc
+    ovres%r = 0.d0
+    call friction_calc(1, npoin, 3, 0.d0, 0.d0, grav, 0.4d0, ochestr, ohc, ohc, ovres, ocf)
+    cf = ocf%r(1:npoin)
+    ! implicit friction FU_IMP (fricti.f, HFROT 1)
+    call bief_allvec(1, oun, 'UN    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ovn, 'VN    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ohn, 'HN    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ofu, 'FU    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ofv, 'FV    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ofud, 'FUD   ', 11, 1, 2, mesh)
+    call bief_allvec(1, ofvd, 'FVD   ', 11, 1, 2, mesh)
+    call bief_allvec(1, ot1, 'T1    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ot2, 'T2    ', 11, 1, 2, mesh)
+    call bief_allvec(1, ounsv, 'UNSV2D', 11, 1, 2, mesh)
+    call bief_allvec(1, omsk, 'MASKEL', 10, 1, 2, mesh)
+    oun%r(1:npoin) = un
+    ovn%r(1:npoin) = vn
+    ohn%r(1:npoin) = hn
+    call fricti(ofu, ofv, ofud, ofvd, oun, ovn, ohn, ocf, mesh, ot1, ot2, .false., ounsv, &
+         .false., omsk, 1)
+    fu = ofu%r(1:npoin)
+    ! T2 = -MASBAS with the slope mask TE3 (SLOPES), T3 = FU T2 (propag.F:1155-1161)
+    call slop10(te3, xel, yel, zf, ik, nelem, nelem)
+    call vc00aa(-1.d0, surfac, nelem, nelem, w(1,1), w(1,2), w(1,3))
+    call assvec(tm, ik, npoin, nelem, nelem, 11, w, .true., 1, .true., te3, 3)
+    call ov('X=XY    ', tm, fu, fu, 0.d0, npoin)
+    ! AM2%D = VOLU2D / DT (propag.F:707), AM3%D the same, + friction (1169)
+    call ov('X=CY    ', am2d, volu2d, volu2d, 1.d0 / dt, npoin)
+    am3d = am2d
+    call ov('X=X+Y   ', am2d, tm, tm, 0.d0, npoin)
+    ! CV2, CV3 (propag.F:581, 602-606, 1010-1011), before the friction is added to AM3
+    call ov('X=Y+Z   ', t8, hn, zf, 0.d0, npoin)
+    do icoord = 1, 2
+      call vc13aa(-grav, sf, t8, xel, yel, ik(1,1), ik(1,2), ik(1,3), ik(1,1), nelem, nelem, &
+           w(1,1), w(1,2), w(1,3), icoord)
+      if (icoord == 1) then
+        call assvec(cv2, ik, npoin, nelem, nelem, 11, w, .true., 1, .false., dummy, 3)
+        call ov('X=X+YZ  ', cv2, am3d, un, 0.d0, npoin)
+      else
+        call assvec(cv3, ik, npoin, nelem, nelem, 11, w, .true., 1, .false., dummy, 3)
+        call ov('X=X+YZ  ', cv3, am3d, vn, 0.d0, npoin)
+      end if
+    end do
+    ! inversion (propag.F:1384) and DM1 (1397)
+    call ovd('X=1/Y   ', am2d, am2d, am2d, 0.d0, npoin, 2, 0.d0, 1.d-6)
+    call ov('X=CYZ   ', dm1, volu2d, am2d, -grav, npoin)
+    ! HPROP = max(HN, 0) (hpropa.f)
+    call ov('X=Y     ', hprop, hn, hn, 0.d0, npoin)
+    call ov('X=+(Y,C)', hprop, hprop, hprop, 0.d0, npoin)
+    ! AM1 = MATMAS / DT (propag.F:679) + MATDIFUV (-1, DM1, HPROP) (1407)
+    call mt01aa(tmass(1,1), xmass(1,1), xmass(1,2), tmass(1,2), xmass(1,3), tmass(1,3), &
+         1.d0 / dt, surfac, nelem, nelem)
+    sdm1%elm = 11
+    sdm1%type = 2
+    sdm1%dim2 = 1
+    sdm1%maxdim1 = npoin
+    shp = sdm1
+    formul = 'MATDIFUV        '
+    call mt02aa(tdif(1,1), xdif(1,1), xdif(1,2), tdif(1,2), xdif(1,3), tdif(1,3), -1.d0, &
+         sdm1, dm1, shp, hprop, xel, yel, surfac, ik(1,1), ik(1,2), ik(1,3), nelem, nelem, &
+         formul)
+    call assvec(d1, ik, npoin, nelem, nelem, 11, tmass, .true., 1, .false., dummy, 3)
+    call assvec(d2, ik, npoin, nelem, nelem, 11, tdif, .true., 1, .false., dummy, 3)
+    call ov('X=Y+Z   ', da, d1, d2, 0.d0, npoin)
+    xa = xmass + xdif
+    ! T2, T3 (propag.F:1413-1414)
+    call ov('X=YZ    ', t2, cv2, am2d, 0.d0, npoin)
+    call ov('X=YZ    ', t3, cv3, am2d, 0.d0, npoin)
+    ! CV1 = - BM1 T2 - BM2 T3 (propag.F:745-754, 466-499, 1464-1465)
+    cv1 = 0.d0
+    do icoord = 1, 2
+      call mt08aa(a(1,1), a(1,2), a(1,3), a(1,4), a(1,5), a(1,6), a(1,7), a(1,8), a(1,9), &
+           1.d0, sf, hprop, xel, yel, ik(1,1), ik(1,2), ik(1,3), nelem, nelem, icoord)
+      ! the diagonal assembled (A11, A22, A33), the off-diagonal terms in the
+      ! 'Q' order 12, 13, 23, 21, 31, 32
+      w(:,1) = a(:,1)
+      w(:,2) = a(:,5)
+      w(:,3) = a(:,9)
+      call assvec(dbm, ik, npoin, nelem, nelem, 11, w, .true., 1, .false., dummy, 3)
+      xbm(:,1) = a(:,2)
+      xbm(:,2) = a(:,3)
+      xbm(:,3) = a(:,6)
+      xbm(:,4) = a(:,4)
+      xbm(:,5) = a(:,7)
+      xbm(:,6) = a(:,8)
+      call load_matrix('Q', 'Q', dbm, xbm)
+      vx%r(1:npoin) = cv1
+      if (icoord == 1) then
+        vy%r(1:npoin) = t2
+      else
+        vy%r(1:npoin) = t3
+      end if
+      call matvec('X=X-AY  ', vx, ma, vy, 0.d0, mesh)
+      cv1 = vx%r(1:npoin)
+    end do
+    ! DIRICH on DH with HBOR - HN (propag.F:462, 1647), DH = 0 first (IORDRH 0)
+    call load_matrix('Q', 'S', da, xa)
+    vb%r(1:npoin) = cv1
+    vf%r(1:npoin) = 0.d0
+    ! HBOR - HN on the prescribed points (OSBD, propag.F:462): a domain-sized
+    ! FBOR must be 0 elsewhere (diri01.f:85-87; a boundary-sized HBOR gets
+    ! the same through OSDBIF)
+    vfbor%r(1:npoin) = hbor
+    call ov('X=X-Y   ', vfbor%r, hn, hn, 0.d0, npoin)
+    where (limdir(1:npoin) /= kdir) vfbor%r(1:npoin) = 0.d0
+    call dirich(vf, ma, vb, vfbor, limdir, vwork, mesh, kdir, .false., maskpt)
+    cv1 = vb%r(1:npoin)
+    dh0 = vf%r(1:npoin)
+    da = ma%d%r(1:npoin)
+    xa = reshape(ma%x%r(1:nelem * 3), (/nelem, 3/))
+    ! SOLVE (CG, diagonal preconditioning)
+    cfg%slv = 1
+    cfg%precon = 2
+    cfg%krylov = 5
+    cfg%nitmax = nitmax
+    cfg%eps = eps
+    cfg%zero = 1.d-12
+    cfg%ok = .true.
+    cfg%nit = 0
+    vx%r(1:npoin) = dh0
+    call solve(vx, ma, vb, tb, cfg, .false., mesh, ma)
+    dh = vx%r(1:npoin)
+    niter = cfg%nit
+    ! momentum (propag.F:1685-1707): CV2 += GRADF(-GRAV DH), U = CV2 AM2%D
+    do icoord = 1, 2
+      call vc13aa(-grav, sf, dh, xel, yel, ik(1,1), ik(1,2), ik(1,3), ik(1,1), nelem, nelem, &
+           w(1,1), w(1,2), w(1,3), icoord)
+      call assvec(tm, ik, npoin, nelem, nelem, 11, w, .true., 1, .false., dummy, 3)
+      if (icoord == 1) then
+        call ov('X=X+Y   ', cv2, tm, tm, 0.d0, npoin)
+        call ov('X=YZ    ', u, cv2, am2d, 0.d0, npoin)
+      else
+        call ov('X=X+Y   ', cv3, tm, tm, 0.d0, npoin)
+        call ov('X=YZ    ', v, cv3, am2d, 0.d0, npoin)
+      end if
+    end do
+    deallocate(ik, w, te3, tm, am3d, d1, d2, tmass, xmass, tdif, xdif, a, t8, cv2, cv3, hc)
+    deallocate(xbm, dbm)
   end subroutine
 
 end module ref_driver

@@ -263,9 +263,72 @@ def supg_case():
     return out
 
 
+def propag_state(m, seed, shallow=False, dry=False):
+    """A state of the wave equation on mesh m (seeded): a sloping bottom with
+    a cross-channel ripple, a free surface of a cosine wave and noise, random
+    velocities and Strickler coefficients; shallow: 5 % of the nodes at 1-30 mm
+    depth (the shallow-water friction speed of fricti.f); dry: 2 % at or
+    below the bottom (the depth floors)."""
+    rng = np.random.default_rng(seed)
+    lx = m.x.max() - m.x.min()
+    ly = m.y.max() - m.y.min()
+    zb = -(1.0 + 4.0 * (m.x - m.x.min()) / lx) + 0.3 * np.sin(m.y / (0.3 * ly + 1.0))
+    eta = 0.05 * np.cos(2 * np.pi * (m.x - m.x.min()) / lx) + 0.01 * rng.normal(size=m.npoin)
+    if shallow:
+        pick = rng.uniform(size=m.npoin) < 0.05
+        zb = np.where(pick, eta - rng.uniform(1e-3, 0.03, m.npoin), zb)
+    if dry:
+        pick = rng.uniform(size=m.npoin) < 0.02
+        zb = np.where(pick, eta + rng.uniform(0.0, 1e-3, m.npoin), zb)
+    un = 0.3 * rng.normal(size=m.npoin)
+    vn = 0.1 * rng.normal(size=m.npoin)
+    k = rng.uniform(20, 60, m.npoin)
+    return eta - zb, un, vn, zb, k
+
+
+def propag_case():
+    """One wave-equation step of telemac2d PROPAG computed by the reference's
+    own routines (driver ref_propag: MT01AA, MT02AA 'MATDIFUV', MT08AA,
+    VC00AA, VC13AA, SLOP10, ASSVEC, OV/OVD, FRICTION_CALC, FRICTI, MATVEC,
+    DIRICH, SOLVE) on three meshes:
+      chan   a 400 m x 80 m channel (60 x 12 cells), the free surface 0.2 m
+             prescribed at x = 0, dt = 1 s, shallow nodes;
+      goutte the reference's gouttedo basin (closed), dt = 0.04 s, dry nodes;
+      estim  the reference's estimation channel with its boundary file: the
+             depth prescribed where the file prescribes it, walls elsewhere
+             (the file's prescribed-velocity inflow is a wall here), dt = 2 s.
+    Inputs and outputs are stored per case, prefixed by the case name."""
+    from opentelemac_amd import serafin as SF
+    out = {}
+    cases = []
+    m = M.channel_mesh(60, 12, 400.0, 80.0)
+    sea = np.nonzero(m.x <= m.x.min() + 1e-9)[0]
+    cases.append(("chan", m, sea, 1.0, dict(shallow=True)))
+    g = SF.read(os.path.join(HERE, "geo_gouttedo.slf")).mesh()
+    cases.append(("goutte", g, np.zeros(0, dtype=np.int64), 0.04, dict(dry=True)))
+    e = SF.read(os.path.join(HERE, "geo_estimation.slf")).mesh()
+    b = SF.read_cli(os.path.join(HERE, "geo_estimation.cli"), e.npoin)
+    _, lim = SF.depth_dirichlet(b, e.npoin)
+    cases.append(("estim", e, np.nonzero(lim == SF.KDIR)[0], 2.0, {}))
+    for seed, (name, mesh, sea, dt, kw) in enumerate(cases):
+        hn, un, vn, zb, k = propag_state(mesh, 40 + seed, **kw)
+        limdir = np.full(mesh.npoin, SF.KNEU, dtype=np.int32)
+        limdir[sea] = SF.KDIR
+        hbor = np.where(limdir == SF.KDIR, 0.2 - zb, 0.0)
+        r = O.ref_propag(mesh, hn, un, vn, zb, k, hbor, limdir, kdir=SF.KDIR, dt=dt,
+                         nitmax=2000, eps=1e-13)
+        out.update({f"{name}_{key}": val for key, val in
+                    dict(hn=hn, un=un, vn=vn, zb=zb, k=k, limdir=limdir, hbor=hbor,
+                         dt=np.array(dt)).items()})
+        for key in ("cf", "fu", "am2d", "dm1", "t2", "t3", "da", "xa", "cv1", "dh", "u", "v"):
+            out[f"{name}_{key}"] = r[key]
+    out["chan_x"], out["chan_y"], out["chan_ikle"] = cases[0][1].x, cases[0][1].y, cases[0][1].ikle
+    return out
+
+
 CASES = {"ref_prism": prism_case, "ref_small": small_case, "ref_channel": channel_case,
          "ref_mt02pp": mt02pp_case, "ref_seg": seg_case, "ref_gouttedo": gouttedo_case,
-         "ref_supg": supg_case}
+         "ref_supg": supg_case, "ref_propag": propag_case}
 
 
 def main(names):

@@ -97,3 +97,41 @@ def test_rejects_bad_vectors(setup):
     z = dev._zeros()
     with pytest.raises(bief.BiefError):
         dev.kern.assemble(p, z, z, z, bad, z, z)
+
+
+def test_shallow_and_dry_nodes(gpu):
+    """Nodes at 5-25 mm (the shallow-water friction speed of fricti.f) and
+    nodes above the water (negative depth: the depth floors of
+    friction_unif.f and fricti.f, HPROP = 0): the run stays finite, the device
+    step equals the torch model and so does its gradient (the floors pass no
+    derivative below them)."""
+    base = M.random_small_mesh(14, 8, seed=73, jitter=0.2)
+    m = M.Mesh(x=base.x * 50.0, y=base.y * 50.0, ikle=base.ikle, npoin2=base.npoin,
+               nelem2=base.nelem, meta={})
+    sea = np.nonzero(m.x <= 1e-9)[0]
+    kw = dict(dt=1.0, amp=0.05, period=60.0, eps=1e-13)
+    ref = W.WaveModel(m, sea, **kw)
+    dev = W.DeviceWaveModel(m, sea, **kw)
+    rng = np.random.default_rng(9)
+    d = ref.dev
+    zb = -(0.5 + m.x / m.x.max())
+    inner = np.setdiff1d(np.arange(m.npoin), sea)
+    pick = rng.choice(inner, 12, replace=False)
+    zb[pick[:8]] = -rng.uniform(0.005, 0.025, 8)     # shallow
+    zb[pick[8:]] = rng.uniform(1e-5, 1e-3, 4)         # dry
+    zb = torch.tensor(zb, dtype=torch.float64, device=d)
+    k = torch.tensor(rng.uniform(20, 60, m.npoin), dtype=torch.float64, device=d)
+    gauges = torch.tensor(rng.choice(m.npoin, 10, replace=False), device=d)
+    n = 8
+    with torch.no_grad():
+        obs = ref.run(k * 0.8, zb, n, gauges)
+        s_ref = ref.run(k, zb, n, gauges)
+    s_dev = dev.run(k, zb, n, gauges)
+    assert torch.isfinite(s_dev).all()
+    assert _rel(s_dev, s_ref) <= 1e-10
+    j_ref, gk_ref, gz_ref = W.cost_and_gradient(ref, k, zb, n, gauges, obs, checkpoint=4)
+    j, gk, gz = dev.cost_and_gradient(k, zb, n, gauges, obs, checkpoint=4)
+    assert torch.isfinite(gk).all() and torch.isfinite(gz).all()
+    assert abs(float(j) - float(j_ref)) <= 1e-9 * float(j_ref)
+    assert _rel(gk, gk_ref) <= 1e-7, _rel(gk, gk_ref)
+    assert _rel(gz, gz_ref) <= 1e-7, _rel(gz, gz_ref)

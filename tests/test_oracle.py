@@ -523,3 +523,55 @@ def test_supg_planted_transpose_is_caught():
     wrong = _full(t, np.concatenate([xm[3:], xm[:3]]))
     ug = (1.0 * b + 0.5 * c) / (2 * sf)
     assert not np.allclose(wrong[0, 1], sf / 3 * ug[1], rtol=1e-6)
+
+
+def _propag_meshes():
+    from opentelemac_amd import serafin as SF
+    return {"chan": M.channel_mesh(60, 12, 400.0, 80.0),
+            "goutte": SF.read(os.path.join(GOLD, "geo_gouttedo.slf")).mesh(),
+            "estim": SF.read(os.path.join(GOLD, "geo_estimation.slf")).mesh()}
+
+
+@pytest.mark.parametrize("case", ["chan", "goutte", "estim"])
+def test_propag_step_matches_reference(case):
+    """The oracle's PROPAG step (oracle.propag_step) against the reference's
+    own routines run in PROPAG's sequence (tests/golden/ref_propag.npz, from
+    the driver's ref_propag): friction coefficient, implicit friction,
+    inverted AM2%D, DM1, the velocity predictor T2/T3, AM1 and CV1 after
+    DIRICH, the solved increment and the new velocities, to 1e-12 of each
+    vector's scale (the two differ only in summation and rounding order)."""
+    z = np.load(os.path.join(GOLD, "ref_propag.npz"))
+    m = _propag_meshes()[case]
+    g = lambda k: z[f"{case}_{k}"]  # noqa: E731
+    if case == "chan":
+        assert np.array_equal(m.ikle, z["chan_ikle"]) and np.array_equal(m.x, z["chan_x"])
+    r = O.propag_step(m, g("hn"), g("un"), g("vn"), g("zb"), g("k"), g("hbor"), g("limdir"),
+                      kdir=2, dt=float(g("dt")), nitmax=2000, eps=1e-13)
+    for key in ("cf", "fu", "am2d", "dm1", "t2", "t3", "da", "xa", "cv1", "dh", "u", "v"):
+        ref = g(key)
+        assert np.abs(r[key] - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1e-30), key
+    # the cases exercise what they were built for
+    h = g("hn")
+    if case == "chan":
+        assert np.any((h > 1e-4) & (h < 3e-2)) and np.count_nonzero(g("limdir") == 2) == 13
+    if case == "goutte":
+        assert np.any(h <= 0.0) and not np.any(g("limdir") == 2)
+    if case == "estim":
+        assert np.count_nonzero(g("limdir") == 2) == 6
+
+
+def test_propag_planted_error_fails():
+    """A planted mistake in the restated step (the slope factor of the
+    friction mass left out) moves CV1 beyond the tolerance."""
+    z = np.load(os.path.join(GOLD, "ref_propag.npz"))
+    m = _propag_meshes()["chan"]
+    g = lambda k: z[f"chan_{k}"]  # noqa: E731
+    saved = O._slop10
+    try:
+        O._slop10 = lambda xel, yel, ikle, zz: np.ones(ikle.shape[1])
+        r = O.propag_step(m, g("hn"), g("un"), g("vn"), g("zb"), g("k"), g("hbor"),
+                          g("limdir"), kdir=2, dt=float(g("dt")), nitmax=2000, eps=1e-13)
+    finally:
+        O._slop10 = saved
+    ref = g("cv1")
+    assert np.abs(r["cv1"] - ref).max() > 1e-9 * np.abs(ref).max()
