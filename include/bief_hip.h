@@ -230,6 +230,11 @@ int bhip_dirich(bhip_ctx* ctx, double* f, double* sm, const double* fbor,
 
 /* LUMP: DIAG = A * (XMUL everywhere) (sources/utils/bief/lump.f) */
 int bhip_lump(bhip_ctx* ctx, double xmul, double* diag);
+/* The same on device arrays in the caller's order (f, sm updated in place;
+ * limdir int32), queued on the context stream: no host copy, no wait. */
+int bhip_dirich_dev(bhip_ctx* ctx, double* f_dev, double* sm_dev,
+                    const double* fbor_dev, const int32_t* limdir_dev, int kdir);
+int bhip_lump_dev(bhip_ctx* ctx, double xmul, double* diag_dev);
 
 /* The mesh's element geometry, kept on the device in the context's element
  * order: XEL, YEL [3*nelem] (node 1 at the origin) and SURFAC [nelem], as
@@ -280,6 +285,21 @@ int bhip_matrix_mt02pp(bhip_ctx* ctx, double xmul, const double* xel,
                        int h_zero, int npou0, int inchyd, int mon,
                        int accumulate);
 
+/* The element matrices above from device node fields in the caller's order
+ * and the context's geometry (bhip_ctx_set_geometry, required), queued on
+ * the context stream: the per-time-step assembly of MATRIX without host
+ * copies or waits (matrix.f, mt01aa.f, mt02aa.f, mt01pp.f, mt02pp.f). */
+int bhip_matrix_mt01aa_dev(bhip_ctx* ctx, double xmul, int accumulate);
+int bhip_matrix_mt02aa_dev(bhip_ctx* ctx, int kind, double xmul,
+                           const double* u_dev, const double* v_dev,
+                           int accumulate);
+int bhip_matrix_mt01pp_dev(bhip_ctx* ctx, double xmul, const double* z_dev,
+                           int accumulate);
+int bhip_matrix_mt02pp_dev(bhip_ctx* ctx, double xmul, const double* z_dev,
+                           const double* f_dev, const double* g_dev,
+                           const double* h_dev, int h_zero, int npou0,
+                           int inchyd, int mon, int accumulate);
+
 /* SUPG element matrices of CVDFTR (cvdftr.F:398-560) on the device, P1
  * triangles, assembled into the context's matrix as MATRIX does (host
  * inputs in the caller's numbering; xel, yel [3*nelem], surfac [nelem]):
@@ -298,6 +318,13 @@ int bhip_matrix_supg(bhip_ctx* ctx, int kind, double xmul, const double* xel,
                      const double* yel, const double* surfac, const double* f,
                      const double* g, const double* u, const double* v, int flags,
                      int op);
+/* The same from device arrays (f, g per element in the caller's element
+ * order; u, v per node in the caller's order, or per element with
+ * BHIP_SUPG_U_P0) and the context's geometry, queued on the context stream. */
+int bhip_matrix_supg_dev(bhip_ctx* ctx, int kind, double xmul,
+                         const double* f_dev, const double* g_dev,
+                         const double* u_dev, const double* v_dev, int flags,
+                         int op);
 /* OM on the context's matrix M, with N its saved copy (bhip_matrix_save)
  * (sources/utils/bief/om.f -> om1111.f): "M=X(M)" (the non-symmetric form
  * of M), "M=CN" (M = c M), "M=M+CN", "M=M+N", "M=M+TN" (N transposed). */

@@ -116,6 +116,13 @@ def lib():
         L.bhip_ctx_perm.argtypes = [_P, _P]
         L.bhip_matrix_supg.argtypes = [_P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I]
         L.bhip_matrix_om.argtypes = [_P, ctypes.c_char_p, _D]
+        L.bhip_matrix_supg_dev.argtypes = [_P, _I, _D, _P, _P, _P, _P, _I, _I]
+        L.bhip_matrix_mt01aa_dev.argtypes = [_P, _D, _I]
+        L.bhip_matrix_mt02aa_dev.argtypes = [_P, _I, _D, _P, _P, _I]
+        L.bhip_matrix_mt01pp_dev.argtypes = [_P, _D, _P, _I]
+        L.bhip_matrix_mt02pp_dev.argtypes = [_P, _D, _P, _P, _P, _P, _I, _I, _I, _I, _I]
+        L.bhip_dirich_dev.argtypes = [_P, _P, _P, _P, _P, _I]
+        L.bhip_lump_dev.argtypes = [_P, _D, _P]
         L.bhip_ctx_set_geometry.argtypes = [_P, _P, _P, _P]
         _W = ctypes.POINTER(WaveParams)
         L.bhip_wave_setup.argtypes = [_P, _P, _P, _P, _P]
@@ -305,8 +312,22 @@ class Context:
     def dirich(self, f, sm, fbor, limdir, kdir=5):
         """DIRICH with domain-sized FBOR/LIMDIR (dirich.f -> diri01.f ->
         diraux.f). Returns (f, sm); the context's matrix gets the rows and
-        columns of prescribed nodes zeroed."""
+        columns of prescribed nodes zeroed. Torch CUDA tensors (limdir
+        int32): f and sm are updated in place on the context stream."""
         n = self.npoin
+        if _is_torch(f):
+            import torch
+            for t, w in ((f, "f"), (sm, "sm"), (fbor, "fbor")):
+                _dev_n(t, n, "dirich: " + w)
+            if not (_is_torch(limdir) and limdir.is_cuda and limdir.dtype == torch.int32 and
+                    limdir.is_contiguous() and limdir.numel() == n):
+                raise BiefError("dirich: limdir: expected a contiguous int32 CUDA tensor of "
+                                f"{n} values")
+            self._sync_torch()
+            _check(lib().bhip_dirich_dev(self._h, f.data_ptr(), sm.data_ptr(), fbor.data_ptr(),
+                                         limdir.data_ptr(), int(kdir)), "bhip_dirich_dev")
+            self._join_torch()
+            return f, sm
         f = _host_n(f, n, "dirich: f").copy()
         sm = _host_n(sm, n, "dirich: sm").copy()
         fb = _host_n(fbor, n, "dirich: fbor")
@@ -317,8 +338,15 @@ class Context:
                                  ld.ctypes.data, int(kdir)), "bhip_dirich")
         return f, sm
 
-    def lump(self, xmul=1.0):
-        """LUMP: A * (XMUL everywhere) (lump.f)."""
+    def lump(self, xmul=1.0, out=None):
+        """LUMP: A * (XMUL everywhere) (lump.f). out: a torch CUDA tensor
+        receives it on the context stream."""
+        if out is not None:
+            _dev_n(out, self.npoin, "lump: out")
+            self._sync_torch()
+            _check(lib().bhip_lump_dev(self._h, float(xmul), out.data_ptr()), "bhip_lump_dev")
+            self._join_torch()
+            return out
         d = np.empty(self.npoin)
         _check(lib().bhip_lump(self._h, float(xmul), d.ctypes.data), "bhip_lump")
         return d
@@ -337,21 +365,41 @@ class Context:
     def matrix_mt01aa(self, xmul, surfac, accumulate=False):
         """MATRIX(... 'MATMAS' ...) with OM 'M=N' / 'M=M+N' (mt01aa.f).
         surfac None: the geometry of set_geometry."""
-        s = _host_n_or_none(surfac, self.nelem, "surfac")
-        _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), _ptr(s), int(accumulate)),
-               "bhip_matrix_mt01aa")
+        if surfac is None and getattr(self, "has_geometry", False):
+            # no host array at all: queued on the context stream
+            self._sync_torch()
+            _check(lib().bhip_matrix_mt01aa_dev(self._h, float(xmul), int(accumulate)),
+                   "bhip_matrix_mt01aa_dev")
+            self._join_torch()
+        else:
+            s = _host_n_or_none(surfac, self.nelem, "surfac")
+            _check(lib().bhip_matrix_mt01aa(self._h, float(xmul), _ptr(s), int(accumulate)),
+                   "bhip_matrix_mt01aa")
         self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     def matrix_mt02aa(self, kind, xmul, xel, yel, surfac, u, v=None, accumulate=False):
         """MATRIX(... 'MATDIF' / 'MATDIFUV' ...): kind 1 P1 isotropic
         viscosity U, kind 2 coefficient U*V (mt02aa.f)."""
         ne, n = self.nelem, self.npoin
-        xel, yel = _host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel")
-        s = _host_n_or_none(surfac, ne, "surfac")
-        u, v = _host_n(u, n, "u"), _host_n_or_none(v, n, "v")
-        _check(lib().bhip_matrix_mt02aa(self._h, int(kind), float(xmul), _ptr(xel), _ptr(yel),
-                                        _ptr(s), u.ctypes.data, _ptr(v), int(accumulate)),
-               "bhip_matrix_mt02aa")
+        if _is_torch(u):
+            # device node fields, the context's geometry
+            if xel is not None or yel is not None or surfac is not None:
+                raise BiefError("matrix_mt02aa: device fields take the context's geometry")
+            _dev_n(u, n, "u")
+            if v is not None:
+                _dev_n(v, n, "v")
+            self._sync_torch()
+            _check(lib().bhip_matrix_mt02aa_dev(self._h, int(kind), float(xmul), u.data_ptr(),
+                                                None if v is None else v.data_ptr(),
+                                                int(accumulate)), "bhip_matrix_mt02aa_dev")
+            self._join_torch()
+        else:
+            xel, yel = _host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel")
+            s = _host_n_or_none(surfac, ne, "surfac")
+            u, v = _host_n(u, n, "u"), _host_n_or_none(v, n, "v")
+            _check(lib().bhip_matrix_mt02aa(self._h, int(kind), float(xmul), _ptr(xel),
+                                            _ptr(yel), _ptr(s), u.ctypes.data, _ptr(v),
+                                            int(accumulate)), "bhip_matrix_mt02aa")
         self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     SUPG_U_P0, SUPG_UPWIND, SUPG_KSUPG = 1, 2, 4
@@ -363,14 +411,31 @@ class Context:
         flags: SUPG_U_P0 (u, v per element), SUPG_UPWIND (MATVGR's 'N'
         variant), SUPG_KSUPG (the element vectors from KSUPG(1, u, v))."""
         ne, n = self.nelem, self.npoin
-        geo = [_host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel"),
-               _host_n_or_none(surfac, ne, "surfac")]
-        fg = [_host_n_or_none(a, ne, "f, g") for a in (f, g)]
         nu = ne if (flags & self.SUPG_U_P0) and not (flags & self.SUPG_KSUPG) else n
-        u, v = _host_n(u, nu, "u"), _host_n(v, nu, "v")
-        _check(lib().bhip_matrix_supg(self._h, int(kind), float(xmul), *(_ptr(a) for a in geo),
-                                      *(_ptr(a) for a in fg), u.ctypes.data, v.ctypes.data,
-                                      int(flags), self._OM_OPS[op]), "bhip_matrix_supg")
+        if _is_torch(u):
+            # device fields, the context's geometry
+            if xel is not None or yel is not None or surfac is not None:
+                raise BiefError("matrix_supg: device fields take the context's geometry")
+            _dev_n(u, nu, "u")
+            _dev_n(v, nu, "v")
+            for a in (f, g):
+                if a is not None:
+                    _dev_n(a, ne, "f, g")
+            self._sync_torch()
+            _check(lib().bhip_matrix_supg_dev(
+                self._h, int(kind), float(xmul), None if f is None else f.data_ptr(),
+                None if g is None else g.data_ptr(), u.data_ptr(), v.data_ptr(), int(flags),
+                self._OM_OPS[op]), "bhip_matrix_supg_dev")
+            self._join_torch()
+        else:
+            geo = [_host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel"),
+                   _host_n_or_none(surfac, ne, "surfac")]
+            fg = [_host_n_or_none(a, ne, "f, g") for a in (f, g)]
+            u, v = _host_n(u, nu, "u"), _host_n(v, nu, "v")
+            _check(lib().bhip_matrix_supg(self._h, int(kind), float(xmul),
+                                          *(_ptr(a) for a in geo), *(_ptr(a) for a in fg),
+                                          u.ctypes.data, v.ctypes.data, int(flags),
+                                          self._OM_OPS[op]), "bhip_matrix_supg")
         sym = kind == 4 and (op == "M=N" or (op == "M=M+N" and self.typext == "S"))
         self.typdia, self.typext = "Q", ("S" if sym else "Q")
 
@@ -385,10 +450,19 @@ class Context:
     def matrix_mt01pp(self, xmul, surfac, z, accumulate=False):
         """MATRIX(... 'MATMAS' ...) on prisms (mt01pp.f): surfac is the base
         triangle area of each prism, z the node elevations."""
-        s = _host_n_or_none(surfac, self.nelem, "surfac")
-        z = _host_n(z, self.npoin, "z")
-        _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), _ptr(s), z.ctypes.data,
-                                        int(accumulate)), "bhip_matrix_mt01pp")
+        if _is_torch(z):
+            if surfac is not None:
+                raise BiefError("matrix_mt01pp: device fields take the context's geometry")
+            _dev_n(z, self.npoin, "z")
+            self._sync_torch()
+            _check(lib().bhip_matrix_mt01pp_dev(self._h, float(xmul), z.data_ptr(),
+                                                int(accumulate)), "bhip_matrix_mt01pp_dev")
+            self._join_torch()
+        else:
+            s = _host_n_or_none(surfac, self.nelem, "surfac")
+            z = _host_n(z, self.npoin, "z")
+            _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), _ptr(s), z.ctypes.data,
+                                            int(accumulate)), "bhip_matrix_mt01pp")
         self.typdia, self.typext = "Q", "S"
 
     def matrix_mt02pp(self, xmul, xel, yel, surfac, z, f, g, h, npou0, h_zero=False,
@@ -397,12 +471,24 @@ class Context:
         tensor (f, g, h) per node; npou0 = NPOIN2 (P1 vertical viscosity) or
         0 (P0 on the vertical)."""
         ne, n = self.nelem, self.npoin
-        arrs = [_host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel"),
-                _host_n_or_none(surfac, ne, "surfac")] + \
-            [_host_n(a, n, "z, f, g, h") for a in (z, f, g, h)]
-        _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[_ptr(a) for a in arrs],
-                                        int(h_zero), int(npou0), int(inchyd), int(mon),
-                                        int(accumulate)), "bhip_matrix_mt02pp")
+        if _is_torch(z):
+            if xel is not None or yel is not None or surfac is not None:
+                raise BiefError("matrix_mt02pp: device fields take the context's geometry")
+            for a in (z, f, g, h):
+                _dev_n(a, n, "z, f, g, h")
+            self._sync_torch()
+            _check(lib().bhip_matrix_mt02pp_dev(self._h, float(xmul), z.data_ptr(), f.data_ptr(),
+                                                g.data_ptr(), h.data_ptr(), int(h_zero),
+                                                int(npou0), int(inchyd), int(mon),
+                                                int(accumulate)), "bhip_matrix_mt02pp_dev")
+            self._join_torch()
+        else:
+            arrs = [_host_n_or_none(xel, 3 * ne, "xel"), _host_n_or_none(yel, 3 * ne, "yel"),
+                    _host_n_or_none(surfac, ne, "surfac")] + \
+                [_host_n(a, n, "z, f, g, h") for a in (z, f, g, h)]
+            _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[_ptr(a) for a in arrs],
+                                            int(h_zero), int(npou0), int(inchyd), int(mon),
+                                            int(accumulate)), "bhip_matrix_mt02pp")
         self.typdia, self.typext = "Q", "S"
 
     # ---- MATVEC ---------------------------------------------------------

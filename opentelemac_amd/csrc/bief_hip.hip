@@ -1066,6 +1066,14 @@ struct bhip_ctx {
   int max_part = 0;
   bhip::DevBuf<double> geo;      // element geometry per slot (bhip_ctx_set_geometry)
   bool have_geo = false;
+  // inputs of the operand-preparation entry points in the context's order:
+  // node fields (7 slots of npoin: six inputs and a host staging area) and
+  // element fields (4 slots of nelem); in_dev: the current call's arrays are
+  // device pointers (the _dev entry points), so nothing is copied from the
+  // host and nothing waits
+  bhip::DevBuf<double> nsc, esc;
+  bhip::DevBuf<int32_t> isc;
+  bool in_dev = false;
   bool reliable = true;          // mixed inner CG continued across outer steps
   bhip::DevBuf<bhip::SolveState> st;  // one per system
   // the partial sums and solver states the kernels use: the context's own
@@ -2794,56 +2802,130 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
 
 extern "C" {
 
-int bhip_dirich(bhip_ctx* c, double* f, double* sm, const double* fbor, const int32_t* limdir,
-                int kdir) {
-  if (!c || !c->have_matrix || !f || !sm || !fbor || !limdir) return BHIP_EBADARG;
+}  // extern "C"
+
+namespace bhip {
+// out[i] = in[perm[i] + shift] (0 past the end): a node field of the
+// caller shifted by `shift` nodes, into the context's order
+__global__ void k_gather_shift(double* __restrict__ out, const double* __restrict__ in,
+                               const int32_t* __restrict__ perm, int shift, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int j = perm[i] + shift;
+    out[i] = j < n ? in[j] : 0.0;
+  }
+}
+
+// The operand-preparation entry points run with host arrays (synchronous:
+// copied in, waited for) or, in their _dev forms, with device arrays in the
+// caller's order (queued on the context stream only). Marks the call.
+struct InputMode {
+  bhip_ctx* c;
+  InputMode(bhip_ctx* cc, bool dev) : c(cc) { c->in_dev = dev; }
+  ~InputMode() { c->in_dev = false; }
+};
+
+// node field src (caller order, host or device per the call) into node
+// scratch slot k in the context's order, shifted by `shift` caller nodes;
+// returns the slot's address (nullptr on error)
+static double* node_in(bhip_ctx* c, int k, const double* src, int shift = 0) {
+  const int n = c->L.npoin;
+  if (c->nsc.alloc((size_t)7 * n) != hipSuccess) return nullptr;
+  double* dst = c->nsc.p + (size_t)k * n;
+  const double* from = src;
+  if (!c->in_dev) {
+    double* stage = c->nsc.p + (size_t)6 * n;
+    if (hipMemcpyAsync(stage, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream) !=
+        hipSuccess)
+      return nullptr;
+    from = stage;
+  }
+  hipLaunchKernelGGL(k_gather_shift, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dst, from,
+                     c->perm.p, shift, n);
+  if (hipGetLastError() != hipSuccess) return nullptr;
+  // the staging area is reused by the next host field
+  if (!c->in_dev && hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+  return dst;
+}
+
+// element field src (caller element order, read per slot through
+// slot_elem): a device array is used as it is, a host one copied to element
+// scratch slot k
+static const double* elem_in(bhip_ctx* c, int k, const double* src) {
+  if (c->in_dev) return src;
+  const int ne = c->L.nelem;
+  if (c->esc.alloc((size_t)4 * ne) != hipSuccess) return nullptr;
+  double* dst = c->esc.p + (size_t)k * ne;
+  if (hipMemcpyAsync(dst, src, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream) !=
+      hipSuccess)
+    return nullptr;
+  return dst;
+}
+
+// the end of an operand-preparation call: host calls wait for their work
+static int input_done(bhip_ctx* c) {
+  if (!c->in_dev) HIP_OK(hipStreamSynchronize(c->stream));
+  return BHIP_OK;
+}
+
+// DIRICH (dirich.f:83 -> diri01.f:94-139 -> diraux.f) on f, sm, fbor, limdir in
+// the caller's order, host or device arrays per the call (InputMode)
+static int dirich_impl(bhip_ctx* c, double* f, double* sm, const double* fbor,
+                       const int32_t* limdir, int kdir) {
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
-  DevBuf<double> hf, hs, hb;
-  DevBuf<int32_t> hl, il;
-  HIP_OK(hf.alloc(n));
-  HIP_OK(hs.alloc(n));
-  HIP_OK(hb.alloc(n));
-  HIP_OK(hl.alloc(n));
-  HIP_OK(il.alloc(n));
-  HIP_OK(hipMemcpyAsync(hf.p, f, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(hs.p, sm, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(hb.p, fbor, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(hl.p, limdir, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  const double* nb = node_in(c, 0, fbor);
+  const double* ns = nb ? node_in(c, 1, sm) : nullptr;
+  const double* nf = ns ? node_in(c, 2, f) : nullptr;
+  if (!nf) return BHIP_EDEVICE;
+  HIP_OK(c->isc.alloc((size_t)2 * n));
+  int32_t* il = c->isc.p;
+  const int32_t* lsrc = limdir;
+  if (!c->in_dev) {
+    HIP_OK(hipMemcpyAsync(c->isc.p + n, limdir, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                          c->stream));
+    lsrc = c->isc.p + n;
+  }
   // internal order: vio0 = FBOR, vd0 = SM, vd1 = F, il = LIMDIR
-  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio0.p, hb.p,
-                     c->perm.p, n);
-  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vd0.p, hs.p,
-                     c->perm.p, n);
-  hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vd1.p, hf.p,
-                     c->perm.p, n);
-  hipLaunchKernelGGL(k_gather_any<int32_t>, dim3(g), dim3(kThreads), 0, c->stream, il.p, hl.p,
+  HIP_OK(hipMemcpyAsync(c->vio0.p, nb, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->vd0.p, ns, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->vd1.p, nf, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_gather_any<int32_t>, dim3(g), dim3(kThreads), 0, c->stream, il, lsrc,
                      c->perm.p, n);
   HIP_OK(hipGetLastError());
   // WORK2 = A FBOR (diri01.f:103-110), SM -= WORK2, DIRAUX
   int rc = matvec_internal(c, BHIP_OP_AY, 0.0, c->vio0.p, c->vio0.p, c->vr.p);
   if (rc) return rc;
   hipLaunchKernelGGL(k_diraux, dim3(g), dim3(kThreads), 0, c->stream, c->vd0.p, c->vr.p,
-                     c->da.p, c->vio0.p, il.p, kdir, c->vdinv.p, c->vd1.p, n);
+                     c->da.p, c->vio0.p, il, kdir, c->vdinv.p, c->vd1.p, n);
   HIP_OK(hipGetLastError());
   // OM 'M=DMD' with W and the diagonal declared '0' (diri01.f:131-136)
   if (c->ncoef > 0) {
     HIP_OK(launch_dmd(c, (float*)nullptr, c->vdinv.p));
   }
-  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, hs.p, c->vd0.p,
-                     c->perm.p, n);
-  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, hf.p, c->vd1.p,
-                     c->perm.p, n);
+  if (c->in_dev) {
+    hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, sm, c->vd0.p,
+                       c->perm.p, n);
+    hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, f, c->vd1.p,
+                       c->perm.p, n);
+    HIP_OK(hipGetLastError());
+    return BHIP_OK;
+  }
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p,
+                     c->vd0.p, c->perm.p, n);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(sm, hs.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipMemcpyAsync(f, hf.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(sm, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p,
+                     c->vd1.p, c->perm.p, n);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(f, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return BHIP_OK;
 }
 
-int bhip_lump(bhip_ctx* c, double xmul, double* diag) {
-  if (!c || !c->have_matrix || !diag) return BHIP_EBADARG;
+// LUMP (lump.f): DIAG = A * XMUL, caller order
+static int lump_impl(bhip_ctx* c, double xmul, double* diag) {
   const int n = c->L.npoin;
   const int g = c->grid_nodes;
   hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio0.p, xmul,
@@ -2851,14 +2933,49 @@ int bhip_lump(bhip_ctx* c, double xmul, double* diag) {
   HIP_OK(hipGetLastError());
   int rc = matvec_internal(c, BHIP_OP_AY, 0.0, c->vio0.p, c->vio0.p, c->vr.p);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p,
-                     c->vr.p, c->perm.p, n);
+  double* dst = c->in_dev ? diag : c->vio1.p;
+  hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, dst, c->vr.p,
+                     c->perm.p, n);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(diag, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
+  if (!c->in_dev) {
+    HIP_OK(hipMemcpyAsync(diag, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
   return BHIP_OK;
 }
+}  // namespace bhip
 
+extern "C" {
+
+int bhip_dirich(bhip_ctx* c, double* f, double* sm, const double* fbor, const int32_t* limdir,
+                int kdir) {
+  if (!c || !c->have_matrix || !f || !sm || !fbor || !limdir) return BHIP_EBADARG;
+  InputMode mode(c, false);
+  return dirich_impl(c, f, sm, fbor, limdir, kdir);
+}
+
+int bhip_dirich_dev(bhip_ctx* c, double* f, double* sm, const double* fbor,
+                    const int32_t* limdir, int kdir) {
+  if (!c || !c->have_matrix || !f || !sm || !fbor || !limdir) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return dirich_impl(c, f, sm, fbor, limdir, kdir);
+}
+
+int bhip_lump(bhip_ctx* c, double xmul, double* diag) {
+  if (!c || !c->have_matrix || !diag) return BHIP_EBADARG;
+  InputMode mode(c, false);
+  return lump_impl(c, xmul, diag);
+}
+
+int bhip_lump_dev(bhip_ctx* c, double xmul, double* diag) {
+  if (!c || !c->have_matrix || !diag) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return lump_impl(c, xmul, diag);
+}
+
+}  // extern "C"
+
+namespace bhip {
 static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
                        const double* surfac, const double* u, const double* v,
                        int accumulate) {
@@ -2878,13 +2995,12 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
     HIP_OK(c->da.alloc(n));
     HIP_OK(c->xa.alloc((size_t)6 * L.nslot_pad));
   }
-  DevBuf<double> dS, dX, dY, dU, dV;
+  // the caller's geometry arrays (host calls only): temporaries
+  DevBuf<double> dS, dX, dY;
   if (!use_geo) {
     HIP_OK(dS.alloc(ne));
     HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
-  }
-  if (kind != 100) {
-    if (!use_geo) {
+    if (kind != 100) {
       HIP_OK(dX.alloc((size_t)3 * ne));
       HIP_OK(dY.alloc((size_t)3 * ne));
       HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
@@ -2892,26 +3008,23 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
       HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
                             c->stream));
     }
-    HIP_OK(dU.alloc(n));
-    HIP_OK(dV.alloc(n));
-    HIP_OK(hipMemcpyAsync(c->vio0.p, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dU.p,
-                       c->vio0.p, c->perm.p, n);
-    if (v) {
-      HIP_OK(hipMemcpyAsync(c->vio1.p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-      hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
-                         dV.p, c->vio1.p, c->perm.p, n);
-    }
-    HIP_OK(hipGetLastError());
+  }
+  const double* dU = nullptr;
+  const double* dV = nullptr;
+  if (kind != 100) {
+    if (!(dU = node_in(c, 0, u))) return BHIP_EDEVICE;
+    if (v && !(dV = node_in(c, 1, v))) return BHIP_EDEVICE;
   }
   const size_t lds = (size_t)L.max_owned * sizeof(double);
   hipLaunchKernelGGL(k_elem_tri, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
-                     dS.p, dX.p, dY.p, ne, dU.p, v ? dV.p : dU.p, c->slot_elem.p, c->lid.p,
+                     dS.p, dX.p, dY.p, ne, dU, v ? dV : dU, c->slot_elem.p, c->lid.p,
                      L.nslot_pad, c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p,
                      L.nblocks, c->xa.p, c->da.p, accumulate, target_q,
                      use_geo ? c->geo.p : nullptr);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipStreamSynchronize(c->stream));
+  // host calls, and calls with temporaries to free, wait for the kernel
+  if (!use_geo) HIP_OK(hipStreamSynchronize(c->stream));
+  if (input_done(c)) return BHIP_EDEVICE;
   // MATRIY sets TYPDIA 'Q', TYPEXT 'S' (matriy.f:375-376); 'M=M+N' of two
   // symmetric matrices stays symmetric, into a 'Q' one stays 'Q'
   c->typdia = 'Q';
@@ -2942,26 +3055,12 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
     HIP_OK(c->da.alloc(n));
     HIP_OK(c->xa.alloc((size_t)30 * L.nslot_pad));
   }
-  DevBuf<double> dS, dX, dY, dZ, dF, dG, dH, dHu;
-  std::vector<double> hup;
-  auto up_nodes = [&](DevBuf<double>& dst, const double* src) -> int {
-    HIP_OK(dst.alloc(n));
-    HIP_OK(hipMemcpyAsync(c->vio0.p, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dst.p,
-                       c->vio0.p, c->perm.p, n);
-    HIP_OK(hipGetLastError());
-    // vio0 is reused by the next upload: wait for this gather
-    HIP_OK(hipStreamSynchronize(c->stream));
-    return BHIP_OK;
-  };
+  // the caller's geometry arrays (host calls only): temporaries
+  DevBuf<double> dS, dX, dY;
   if (!use_geo) {
     HIP_OK(dS.alloc(ne));
     HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
-  }
-  int rc = up_nodes(dZ, z);
-  if (rc) return rc;
-  if (kind == 201) {
-    if (!use_geo) {
+    if (kind == 201) {
       HIP_OK(dX.alloc((size_t)3 * ne));
       HIP_OK(dY.alloc((size_t)3 * ne));
       HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
@@ -2969,28 +3068,36 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
       HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
                             c->stream));
     }
+  }
+  const double* dZ = node_in(c, 0, z);
+  if (!dZ) return BHIP_EDEVICE;
+  const double *dF = nullptr, *dG = nullptr, *dH = nullptr, *dHu = nullptr;
+  if (kind == 201) {
     // H at node i + NPOU0 (mt02pp.f:233-235: II4 = I1 + NPOU0); the nodes of
     // the top plane are never a prism's lower node
-    hup.assign(n, 0.0);
-    for (int i = 0; i + npou0 < n; ++i) hup[i] = h[i + npou0];
-    if ((rc = up_nodes(dF, f)) || (rc = up_nodes(dG, g)) || (rc = up_nodes(dH, h)) ||
-        (rc = up_nodes(dHu, hup.data())))
-      return rc;
+    if (!(dF = node_in(c, 1, f)) || !(dG = node_in(c, 2, g)) || !(dH = node_in(c, 3, h)) ||
+        !(dHu = node_in(c, 4, h, npou0)))
+      return BHIP_EDEVICE;
   }
   const size_t lds = (size_t)L.max_owned * sizeof(double);
   hipLaunchKernelGGL(k_elem_prism, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
-                     dS.p, dX.p, dY.p, ne, dZ.p, dF.p, dG.p, dH.p, dHu.p, h_zero, inchyd, mon,
+                     dS.p, dX.p, dY.p, ne, dZ, dF, dG, dH, dHu, h_zero, inchyd, mon,
                      c->slot_elem.p, c->lid.p, L.nslot_pad, c->node_start.p, c->halo_start.p,
                      c->halo.p, c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate,
                      use_geo ? c->geo.p : nullptr);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipStreamSynchronize(c->stream));
+  if (!use_geo) HIP_OK(hipStreamSynchronize(c->stream));
+  if (input_done(c)) return BHIP_EDEVICE;
   c->typdia = 'Q';
   c->typext = 'S';
   c->ncoef = 15;
   c->have_matrix = true;
   return BHIP_OK;
 }
+
+}  // namespace bhip
+
+extern "C" {
 
 int bhip_ctx_set_geometry(bhip_ctx* c, const double* xel, const double* yel,
                           const double* surfac) {
@@ -3035,6 +3142,37 @@ int bhip_matrix_mt02aa(bhip_ctx* c, int kind, double xmul, const double* xel, co
                        const double* surfac, const double* u, const double* v, int accumulate) {
   if (kind != 1 && kind != 2) return BHIP_EBADARG;
   return elem_common(c, kind, xmul, xel, yel, surfac, u, v, accumulate);
+}
+
+// the same element matrices from device node fields (caller order) and the
+// context's geometry, queued on the context stream
+int bhip_matrix_mt01aa_dev(bhip_ctx* c, double xmul, int accumulate) {
+  if (!c) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return elem_common(c, 100, xmul, nullptr, nullptr, nullptr, nullptr, nullptr, accumulate);
+}
+
+int bhip_matrix_mt02aa_dev(bhip_ctx* c, int kind, double xmul, const double* u_dev,
+                           const double* v_dev, int accumulate) {
+  if (!c || (kind != 1 && kind != 2)) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return elem_common(c, kind, xmul, nullptr, nullptr, nullptr, u_dev, v_dev, accumulate);
+}
+
+int bhip_matrix_mt01pp_dev(bhip_ctx* c, double xmul, const double* z_dev, int accumulate) {
+  if (!c) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return elem_prism(c, 200, xmul, nullptr, nullptr, nullptr, z_dev, nullptr, nullptr, nullptr, 0,
+                    0, 0, 0, accumulate);
+}
+
+int bhip_matrix_mt02pp_dev(bhip_ctx* c, double xmul, const double* z_dev, const double* f_dev,
+                           const double* g_dev, const double* h_dev, int h_zero, int npou0,
+                           int inchyd, int mon, int accumulate) {
+  if (!c) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return elem_prism(c, 201, xmul, nullptr, nullptr, nullptr, z_dev, f_dev, g_dev, h_dev, h_zero,
+                    npou0, inchyd, mon, accumulate);
 }
 
 }  // extern "C"

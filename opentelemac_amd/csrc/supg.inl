@@ -278,7 +278,7 @@ __global__ void k_om_diag(int op, double c, double* __restrict__ da,
 
 extern "C" {
 
-int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
+static int supg_impl(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
                      const double* surfac, const double* f, const double* g, const double* u,
                      const double* v, int flags, int op) {
   if (!c || c->L.ndp != 3 || !u || !v) return BHIP_EBADARG;
@@ -313,7 +313,8 @@ int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, cons
     HIP_OK(hipGetLastError());
   }
   const bool u_p0 = (flags & SUPG_U_P0) && !(flags & SUPG_KSUPG);
-  DevBuf<double> dS, dX, dY, dF, dG, dU, dV;
+  // the caller's geometry arrays (host calls only): temporaries
+  DevBuf<double> dS, dX, dY;
   if (!use_geo) {
     HIP_OK(dS.alloc(ne));
     HIP_OK(dX.alloc((size_t)3 * ne));
@@ -322,28 +323,15 @@ int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, cons
     HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
   }
+  const double *dF = nullptr, *dG = nullptr, *dU = nullptr, *dV = nullptr;
   if (kind == 3 && !(flags & SUPG_KSUPG)) {
-    HIP_OK(dF.alloc(ne));
-    HIP_OK(dG.alloc(ne));
-    HIP_OK(hipMemcpyAsync(dF.p, f, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dG.p, g, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    if (!(dF = elem_in(c, 0, f)) || !(dG = elem_in(c, 1, g))) return BHIP_EDEVICE;
   }
   if (u_p0) {
-    HIP_OK(dU.alloc(ne));
-    HIP_OK(dV.alloc(ne));
-    HIP_OK(hipMemcpyAsync(dU.p, u, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dV.p, v, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    if (!(dU = elem_in(c, 2, u)) || !(dV = elem_in(c, 3, v))) return BHIP_EDEVICE;
   } else {
-    HIP_OK(dU.alloc(n));
-    HIP_OK(dV.alloc(n));
-    HIP_OK(hipMemcpyAsync(c->vio0.p, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(c->vio1.p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dU.p,
-                       c->vio0.p, c->perm.p, n);
-    hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream, dV.p,
-                       c->vio1.p, c->perm.p, n);
+    if (!(dU = node_in(c, 0, u)) || !(dV = node_in(c, 1, v))) return BHIP_EDEVICE;
   }
-  HIP_OK(hipGetLastError());
   SupgArgs a;
   a.kind = kind;
   a.flags = flags;
@@ -353,10 +341,10 @@ int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, cons
   a.yel = dY.p;
   a.geo = use_geo ? c->geo.p : nullptr;
   a.nelem = ne;
-  a.f = dF.p;
-  a.g = dG.p;
-  a.u = dU.p;
-  a.v = dV.p;
+  a.f = dF;
+  a.g = dG;
+  a.u = dU;
+  a.v = dV;
   a.slot_elem = c->slot_elem.p;
   a.lid = c->lid.p;
   a.nslot_pad = L.nslot_pad;
@@ -372,12 +360,30 @@ int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, cons
   hipLaunchKernelGGL(k_elem_supg, dim3(L.nblocks), dim3(kThreads),
                      (size_t)L.max_owned * sizeof(double), c->stream, a);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipStreamSynchronize(c->stream));
+  if (!use_geo) HIP_OK(hipStreamSynchronize(c->stream));
+  if (input_done(c)) return BHIP_EDEVICE;
   c->typdia = 'Q';
   c->typext = target_q ? 'Q' : 'S';
   c->ncoef = target_q ? 6 : 3;
   c->have_matrix = true;
   return BHIP_OK;
+}
+
+int bhip_matrix_supg(bhip_ctx* c, int kind, double xmul, const double* xel, const double* yel,
+                     const double* surfac, const double* f, const double* g, const double* u,
+                     const double* v, int flags, int op) {
+  if (!c) return BHIP_EBADARG;
+  InputMode mode(c, false);
+  return supg_impl(c, kind, xmul, xel, yel, surfac, f, g, u, v, flags, op);
+}
+
+int bhip_matrix_supg_dev(bhip_ctx* c, int kind, double xmul, const double* f_dev,
+                         const double* g_dev, const double* u_dev, const double* v_dev, int flags,
+                         int op) {
+  if (!c) return BHIP_EBADARG;
+  InputMode mode(c, true);
+  return supg_impl(c, kind, xmul, nullptr, nullptr, nullptr, f_dev, g_dev, u_dev, v_dev, flags,
+                   op);
 }
 
 int bhip_matrix_om(bhip_ctx* c, const char* op, double cc) {

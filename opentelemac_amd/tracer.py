@@ -26,13 +26,18 @@ from . import bief
 
 def supg_system(ctx: bief.Context, mesh, u, v, nu, dt, fn, tetat=1.0, sm0=None):
     """Assembles AM1 into the context (non-symmetric, 'Q') and returns the
-    right-hand side SM (host array). u, v, nu, fn: node arrays (caller
-    order); sm0: the source term SM before the step (zero by default)."""
+    right-hand side SM. u, v, nu, fn: node arrays (caller order); sm0: the
+    source term SM before the step (zero by default). Host arrays give a host
+    SM; float64 torch CUDA tensors keep every operand on the device (the
+    element matrices are built from them on the context stream) and give a
+    device SM."""
     if not getattr(ctx, "has_geometry", False):
         xe, ye = mesh.xel_yel()
         ctx.set_geometry(xe, ye, mesh.surfac())
     G = (None, None, None)  # the geometry kept on the device
-    fn = np.ascontiguousarray(fn, dtype=np.float64)
+    dev = bief._is_torch(u)
+    if not dev:
+        fn = np.ascontiguousarray(fn, dtype=np.float64)
     K = bief.Context.SUPG_KSUPG
     # AM2: centred advection plus the SUPG term (cvdftr.F:398-409)
     ctx.matrix_supg(5, 1.0, *G, u, v, op="M=N")
@@ -44,7 +49,7 @@ def supg_system(ctx: bief.Context, mesh, u, v, nu, dt, fn, tetat=1.0, sm0=None):
     ctx.matrix_mt01aa(1.0 / dt, None)
     ctx.matrix_om("M=X(M)")
     ctx.matrix_supg(5, 1.0 / dt, *G, u, v, flags=K, op="M=M+TN")
-    rhs = fn if sm0 is None else dt * np.asarray(sm0, dtype=np.float64) + fn
+    rhs = fn if sm0 is None else dt * (sm0 if dev else np.asarray(sm0, dtype=np.float64)) + fn
     sm = ctx.matvec("X=AY", rhs)
     sm = sm + (tetat - 1.0) * t2
     ctx.matrix_om("M=M+CN", tetat)
