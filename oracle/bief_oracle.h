@@ -167,6 +167,15 @@ void orc_lump(double* diag, const double* da, char typdia, const double* xa,
               char typext, double xmul, const int* ikle, int npoin, int nelem,
               int nelmax, int ndp);
 
+/* Multi-threaded SOLVE (CG, PRECON 1 or 2) for the multi-core CPU baseline
+ * (bief_oracle_omp.c, built into liboracle_omp.so): a plan holds the
+ * node-to-element incidences in ASSVEC's order of addition. */
+void* orc_omp_plan_create(const int* ikle, int npoin, int nelem, int ndp);
+void orc_omp_plan_free(void* plan);
+int orc_omp_solve_cg(void* plan, const int* ikle, double* da, char typdia, double* xa,
+                     char typext, double* b, double* x, double eps, int nitmax, int precon,
+                     int nthreads, orc_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -228,6 +228,61 @@ def solve(A: Matrix, ikle, b, x0=None, slv=1, precon=2, krylov=5, nitmax=1000, e
     return x, info, A2, b2
 
 
+OMP_SO = os.path.join(HERE, "liboracle_omp.so")
+_omp = None
+
+
+def omp_lib():
+    global _omp
+    if _omp is None:
+        if not os.path.exists(OMP_SO):
+            raise RuntimeError(f"{OMP_SO} missing: run `make -C oracle`")
+        L = ctypes.CDLL(OMP_SO)
+        L.orc_omp_plan_create.restype = _P
+        L.orc_omp_plan_create.argtypes = [_P, _I, _I, _I]
+        L.orc_omp_plan_free.argtypes = [_P]
+        L.orc_omp_solve_cg.restype = _I
+        L.orc_omp_solve_cg.argtypes = [_P, _P, _P, _C, _P, _C, _P, _P, _D, _I, _I, _I,
+                                       ctypes.POINTER(OrcInfo)]
+        _omp = L
+    return _omp
+
+
+class OmpPlan:
+    """The multi-threaded SOLVE's node-to-element incidence lists (built once
+    per mesh, as the reference's domain decomposition is)."""
+
+    def __init__(self, ikle, npoin):
+        self.ikle = _i32(ikle)
+        self.ndp, self.nelem = self.ikle.shape
+        self.npoin = int(npoin)
+        self._h = omp_lib().orc_omp_plan_create(_ptr(self.ikle), self.npoin, self.nelem,
+                                                self.ndp)
+        if not self._h:
+            raise ValueError("orc_omp_plan_create failed (node ids out of range?)")
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _omp is not None:
+            _omp.orc_omp_plan_free(h)
+
+
+def omp_solve(plan: OmpPlan, A: Matrix, b, x0=None, precon=2, nitmax=1000, eps=1e-8,
+              nthreads=0):
+    """SOLVE with SLV 1 (CG) and PRECON 1/2 on nthreads threads (0: OpenMP's
+    default). Copies A and b as solve() does; returns x, info."""
+    A2 = A.copy()
+    b2 = _f64(b).copy()
+    x = np.zeros(plan.npoin) if x0 is None else _f64(x0).copy()
+    info = OrcInfo()
+    rc = omp_lib().orc_omp_solve_cg(plan._h, _ptr(plan.ikle), _ptr(A2.da), A2.typdia.encode(),
+                                    _ptr(A2.xa), A2.typext.encode(), _ptr(b2), _ptr(x), eps,
+                                    nitmax, precon, int(nthreads), ctypes.byref(info))
+    if rc != 0:
+        raise ValueError(f"orc_omp_solve_cg returned {rc}")
+    return x, info
+
+
 def dirich(A: Matrix, ikle, sm, fbor, limdir, kdir=5, f=None):
     ikle = _i32(ikle)
     ndp, ne = ikle.shape

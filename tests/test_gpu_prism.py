@@ -157,6 +157,55 @@ def test_prism_vertical_diffusion_solve(gpu):
             assert rel <= 1e-8, rel
 
 
+def test_prism_pressure_poisson_solve(gpu):
+    """The non-hydrostatic pressure system of telemac3d (prediv.F:210 builds
+    it with MT02PP and unit coefficients; :384 prescribes the surface plane
+    by DIRICH): assembled on the device from device-resident fields, the
+    surface nodes set to zero by the device DIRICH, and solved by CG +
+    PRECON 2, FP64 and mixed, against the oracle's DIRICH and SOLVE of the
+    oracle's matrix. The operator alone annihilates constants; with the
+    surface prescribed it is definite and the solve converges."""
+    import torch
+
+    base = M.random_small_mesh(13, 9, seed=31)
+    rng = np.random.default_rng(32)
+    bot = -rng.uniform(2, 15, base.npoin)
+    p = M.extrude_prisms(base, 7, bot, rng.uniform(-0.2, 0.2, base.npoin))
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    xel, yel, s = np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb]), \
+        base.surfac()[eb]
+    one = np.ones(p.npoin)
+    t, xm = O.mt02pp(one, one, one, xel, yel, p.z, s, p.ikle, base.npoin, 1.0)
+    A = O.Matrix(O.assemble_diag(t, p.ikle, p.npoin), xm, "Q", "S")
+    assert np.abs(O.matvec("X=AY", A, p.ikle, one)).max() <= 1e-12 * np.abs(A.da).max()
+    limdir = np.full(p.npoin, 2, dtype=np.int32)
+    limdir[(p.nplan - 1) * base.npoin:] = 5
+    sm = rng.normal(size=p.npoin)
+    fbor = np.zeros(p.npoin)
+    f_o, A_o, sm_o = O.dirich(A, p.ikle, sm, fbor, limdir)
+    x_ref, info_ref, _, _ = O.solve(A_o, p.ikle, sm_o, f_o, slv=1, precon=2, eps=1e-10,
+                                    nitmax=2000)
+    assert info_ref.converged
+    ctx = bief.Context(p, block_nodes=64)
+    ctx.set_geometry(xel, yel, s)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    zd, oned = d(p.z), d(one)
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ctx.matrix_mt02pp(1.0, None, None, None, zd, oned, oned, oned, base.npoin)
+        f, b = ctx.dirich(torch.zeros_like(zd), d(sm), torch.zeros_like(zd), d(limdir))
+        x, info = ctx.solve(f, b, slv=1, precon=2, eps=1e-10, nitmax=2000, precision=prec)
+        x = x.cpu().numpy() if isinstance(x, torch.Tensor) else x
+        rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+        assert info["converged"], info
+        assert np.all(x[limdir == 5] == 0.0)
+        if prec == bief.PREC_FP64:
+            assert abs(info["niter"] - info_ref.niter) <= 1, (info, info_ref.niter)
+            assert rel <= 1e-10, rel
+        else:
+            assert rel <= 1e-8, rel
+
+
 def test_prism_device_geometry_matches_caller_arrays(gpu):
     """MT01PP / MT02PP from the geometry kept on the device (set_geometry,
     then xel = yel = surfac = None) give the same bits as from the caller's

@@ -575,3 +575,49 @@ def test_propag_planted_error_fails():
         O._slop10 = saved
     ref = g("cv1")
     assert np.abs(r["cv1"] - ref).max() > 1e-9 * np.abs(ref).max()
+
+
+def test_multithreaded_cg_equals_sequential():
+    """The multi-core CPU baseline's OpenMP SOLVE (CG, PRECON 2): on one
+    thread it is the sequential restatement bit for bit (its gather mat-vec
+    adds the element terms in ASSVEC's order, its one chunk sums the dots
+    left to right); on several threads only the dot products' order moves,
+    so the iteration count stays within one and the solution within 1e-12.
+    Triangles and prisms, the reference's prism golden system included."""
+    from opentelemac_amd import system as S
+    m = M.random_small_mesh(30, 22, seed=8)
+    sysd = S.wave_system(m, 1.0 + m.x / 5.0, dt=3.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    z = np.load(os.path.join(GOLD, "ref_prism.npz"))
+    _, p = _prism_mesh()
+    Ap = O.Matrix(z["spd_da"], z["spd_xa"], "Q", "S")
+    for ik, A_, b, x0, eps in ((m.ikle, A, sysd["b"], sysd["x0"], 1e-8),
+                              (p.ikle, Ap, z["spd_b"], None, 1e-10)):
+        xs, inf_s, _, _ = O.solve(A_, ik, b, x0, slv=1, precon=2, eps=eps)
+        plan = O.OmpPlan(ik, A_.da.size)
+        x1, inf1 = O.omp_solve(plan, A_, b, x0, eps=eps, nthreads=1)
+        assert inf1.niter == inf_s.niter and np.array_equal(x1, xs)
+        for nt in (3, 8):
+            xn, infn = O.omp_solve(plan, A_, b, x0, eps=eps, nthreads=nt)
+            assert abs(infn.niter - inf_s.niter) <= 1 and infn.converged
+            assert np.abs(xn - xs).max() <= 10 * eps * np.abs(xs).max()
+    # the reference's own prism solve count and solution
+    x1, inf1 = O.omp_solve(O.OmpPlan(p.ikle, p.npoin), Ap, z["spd_b"], eps=1e-10, nthreads=1)
+    assert inf1.niter == int(z["solve_1_niter"]) and np.array_equal(x1, z["solve_1_x"])
+
+
+def test_multithreaded_cg_planted_order_is_caught():
+    """Checks the check: a plan whose incidences are added in another order
+    than ASSVEC's (elements reversed) changes the product's last bits, which
+    the one-thread equality above would catch."""
+    from opentelemac_amd import system as S
+    m = M.random_small_mesh(30, 22, seed=8)
+    sysd = S.wave_system(m, 1.0 + m.x / 5.0, dt=3.0)
+    A = O.Matrix(sysd["da"], sysd["xa"], "Q", "S")
+    xs, _, _, _ = O.solve(A, m.ikle, sysd["b"], sysd["x0"], slv=1, precon=2, eps=1e-8)
+    rev = np.ascontiguousarray(m.ikle[:, ::-1])
+    perm_xa = np.ascontiguousarray(sysd["xa"][:, ::-1])
+    x1, _ = O.omp_solve(O.OmpPlan(rev, m.npoin), O.Matrix(sysd["da"], perm_xa, "Q", "S"),
+                        sysd["b"], sysd["x0"], eps=1e-8, nthreads=1)
+    assert np.abs(x1 - xs).max() <= 1e-8 * np.abs(xs).max()
+    assert not np.array_equal(x1, xs)
