@@ -2015,6 +2015,7 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
   return collect_infos(c, x, false, info);
 }
 
+
 // Mixed precision CG: FP64 true residual r = A'x - b' and FP64 update
 // x <- x - delta, with delta from an FP32 EBE CG on the FP32 copy of A'
 // (inner stop ||R||^2 <= inner_eps^2 ||r||^2). Same stop rule as GRACJG on the
@@ -2684,23 +2685,24 @@ __constant__ int kPrismRowPairs[6][5] = {{0, 1, 2, 3, 4},   {0, 5, 6, 7, 8},
                                         {3, 7, 10, 12, 14}, {4, 8, 11, 13, 14}};
 
 // Prism element matrices per slot, diagonal assembled in LDS colour by colour.
-//   kind 200: MT01PP mass (mt01pp.f: XMUL*SURFAC/360 combinations of the
-//             column heights floored at 1e-3)
-//   kind 201: MT02PP diffusion (mt02pp.f), P1 coefficients f, g, h; h_up is
-//             H at the node NPOU0 above each node (the caller's II4 = I1 +
-//             NPOU0), h_zero cancels the vertical term, inchyd the hydrostatic
-//             inconsistency test, mon the 'MON' clipping
+//   KIND 200: MT01PP mass (mt01pp.f: XMUL*SURFAC/360 combinations of the
+//             column heights floored at 1e-3); nd holds Z per node
+//   KIND 201: MT02PP diffusion (mt02pp.f), P1 coefficients F, G, H; nd holds
+//             (Z, F, G, H + H_up) per node, H_up being H at the node NPOU0
+//             above (the caller's II4 = I1 + NPOU0); h_zero cancels the
+//             vertical term, inchyd the hydrostatic inconsistency test, mon
+//             the 'MON' clipping
+template <int KIND>
 __global__ void __launch_bounds__(kThreads)
-k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
+k_elem_prism(double xmul, const double* __restrict__ surfac,
              const double* __restrict__ xel, const double* __restrict__ yel, int nelem,
-             const double* __restrict__ z, const double* __restrict__ f,
-             const double* __restrict__ gg, const double* __restrict__ h,
-             const double* __restrict__ h_up, int h_zero, int inchyd, int mon,
+             const double* __restrict__ nd, int h_zero, int inchyd, int mon,
              const int32_t* __restrict__ slot_elem, const uint16_t* __restrict__ lid,
              int nslot_pad, const int32_t* __restrict__ node_start,
              const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
              const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
-             double* __restrict__ da, int accumulate, const double* __restrict__ geo) {
+             double* __restrict__ da, int accumulate, int target_q,
+             const double* __restrict__ geo) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* acc = reinterpret_cast<double*>(smem_raw);
   const int b = blockIdx.x;
@@ -2716,18 +2718,19 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
     if (s0 >= send) break;
     const int s1 = co[c + 1];
     for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-      const int e = slot_elem[s];
+      const int e = geo ? 0 : slot_elem[s];
       int l[6], g[6];
-      double zz[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         l[k] = lid[(size_t)k * nslot_pad + s];
         g[k] = l[k] < nown ? n0 + l[k] : halo[h0 + l[k] - nown];
-        zz[k] = z[g[k]];
       }
-      const SlotGeo G = load_geo(geo, nslot_pad, s, xel, yel, surfac, nelem, e, kind != 200);
+      const SlotGeo G = load_geo(geo, nslot_pad, s, xel, yel, surfac, nelem, e, KIND != 200);
       double cf[15], t[6];
-      if (kind == 200) {
+      if constexpr (KIND == 200) {
+        double zz[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) zz[k] = nd[g[k]];
         const double coef = G.s * (xmul / 360.0);
         const double h1 = fmax(zz[3] - zz[0], 1e-3) * coef;
         const double h2 = fmax(zz[4] - zz[1], 1e-3) * coef;
@@ -2746,6 +2749,17 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
         t[1] = t[4] = cf[7] + cf[7];
         t[2] = t[5] = cf[11] + cf[11];
       } else {
+        // (Z, F, G, H + H_up) of the six nodes, two 16-byte loads each
+        double zz[6], fv[6], gv[6], hs[3];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const double2 a0 = *reinterpret_cast<const double2*>(nd + 4 * (size_t)g[k]);
+          const double2 a1 = *reinterpret_cast<const double2*>(nd + 4 * (size_t)g[k] + 2);
+          zz[k] = a0.x;
+          fv[k] = a0.y;
+          gv[k] = a1.x;
+          if (k < 3) hs[k] = a1.y;
+        }
         const double hh[3] = {zz[3] - zz[0], zz[4] - zz[1], zz[5] - zz[2]};
         bool dry = hh[0] < 1e-4 || hh[1] < 1e-4 || hh[2] < 1e-4;
         if (inchyd)
@@ -2753,11 +2767,11 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
         double nf[6], ng[6], nh[3];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          nf[k] = dry ? 0.0 : f[g[k]] / hh[k % 3];
-          ng[k] = dry ? 0.0 : gg[g[k]] / hh[k % 3];
+          nf[k] = dry ? 0.0 : fv[k] / hh[k % 3];
+          ng[k] = dry ? 0.0 : gv[k] / hh[k % 3];
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) nh[k] = dry ? 0.0 : (h[g[k]] + h_up[g[k]]) / hh[k];
+        for (int k = 0; k < 3; ++k) nh[k] = dry ? 0.0 : hs[k] / hh[k];
         const double dd = (xmul / 2880.0) / G.s;
         const double x2 = G.x2, x3 = G.x3;
         const double y2 = G.y2, y3 = G.y3;
@@ -2779,10 +2793,24 @@ k_elem_prism(int kind, double xmul, const double* __restrict__ surfac,
           t[k] = sum;
         }
       }
+      if (accumulate) {
 #pragma unroll
-      for (int q = 0; q < 15; ++q) {
-        double* p = xa + (size_t)q * nslot_pad + s;
-        *p = accumulate ? *p + cf[q] : cf[q];
+        for (int q = 0; q < 15; ++q) {
+          double* p = xa + (size_t)q * nslot_pad + s;
+          *p = *p + cf[q];
+        }
+        // a symmetric element matrix into a non-symmetric ('Q') target goes to
+        // both halves (OM 'M=M+N' of an 'S' N into a 'Q' M)
+        if (target_q) {
+#pragma unroll
+          for (int q = 0; q < 15; ++q) {
+            double* p = xa + (size_t)(15 + q) * nslot_pad + s;
+            *p = *p + cf[q];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 15; ++q) xa[(size_t)q * nslot_pad + s] = cf[q];
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k)
@@ -2859,6 +2887,45 @@ static const double* elem_in(bhip_ctx* c, int k, const double* src) {
       hipSuccess)
     return nullptr;
   return dst;
+}
+
+// MT02PP's node record in the context's order: (Z, F, G, H + H_up), H_up
+// being H at the caller's node NPOU0 above (mt02pp.f:233-235: II4 = I1 +
+// NPOU0; zero past the last node, which is never a prism's lower node)
+__global__ void k_gather_prism4(double* __restrict__ out, const double* __restrict__ z,
+                                const double* __restrict__ f, const double* __restrict__ g,
+                                const double* __restrict__ h, const int32_t* __restrict__ perm,
+                                int npou0, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int j = perm[i], ju = j + npou0;
+    const double hu = ju < n ? h[ju] : 0.0;
+    double2* o = reinterpret_cast<double2*>(out + 4 * (size_t)i);
+    o[0] = make_double2(z[j], f[j]);
+    o[1] = make_double2(g[j], h[j] + hu);
+  }
+}
+
+// the four MT02PP node fields (caller order, host or device per the call)
+// into the node record array; returns its address (nullptr on error)
+static double* node_in_prism4(bhip_ctx* c, const double* z, const double* f, const double* g,
+                              const double* h, int npou0) {
+  const int n = c->L.npoin;
+  if (c->nsc.alloc((size_t)8 * n) != hipSuccess) return nullptr;
+  const double* src[4] = {z, f, g, h};
+  if (!c->in_dev) {
+    for (int k = 0; k < 4; ++k) {
+      double* stage = c->nsc.p + (size_t)(4 + k) * n;
+      if (hipMemcpyAsync(stage, src[k], sizeof(double) * n, hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess)
+        return nullptr;
+      src[k] = stage;
+    }
+  }
+  hipLaunchKernelGGL(k_gather_prism4, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                     c->nsc.p, src[0], src[1], src[2], src[3], c->perm.p, npou0, n);
+  if (hipGetLastError() != hipSuccess) return nullptr;
+  if (!c->in_dev && hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+  return c->nsc.p;
 }
 
 // the end of an operand-preparation call: host calls wait for their work
@@ -3069,28 +3136,27 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
                             c->stream));
     }
   }
-  const double* dZ = node_in(c, 0, z);
-  if (!dZ) return BHIP_EDEVICE;
-  const double *dF = nullptr, *dG = nullptr, *dH = nullptr, *dHu = nullptr;
-  if (kind == 201) {
-    // H at node i + NPOU0 (mt02pp.f:233-235: II4 = I1 + NPOU0); the nodes of
-    // the top plane are never a prism's lower node
-    if (!(dF = node_in(c, 1, f)) || !(dG = node_in(c, 2, g)) || !(dH = node_in(c, 3, h)) ||
-        !(dHu = node_in(c, 4, h, npou0)))
-      return BHIP_EDEVICE;
-  }
+  // node fields in the context's order: Z alone (MT01PP) or MT02PP's record
+  const double* nd = kind == 200 ? node_in(c, 0, z) : node_in_prism4(c, z, f, g, h, npou0);
+  if (!nd) return BHIP_EDEVICE;
   const size_t lds = (size_t)L.max_owned * sizeof(double);
-  hipLaunchKernelGGL(k_elem_prism, dim3(L.nblocks), dim3(kThreads), lds, c->stream, kind, xmul,
-                     dS.p, dX.p, dY.p, ne, dZ, dF, dG, dH, dHu, h_zero, inchyd, mon,
-                     c->slot_elem.p, c->lid.p, L.nslot_pad, c->node_start.p, c->halo_start.p,
-                     c->halo.p, c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate,
-                     use_geo ? c->geo.p : nullptr);
+#define BHIP_PRISM(KIND)                                                                       \
+  hipLaunchKernelGGL(k_elem_prism<KIND>, dim3(L.nblocks), dim3(kThreads), lds, c->stream, xmul, \
+                     dS.p, dX.p, dY.p, ne, nd, h_zero, inchyd, mon, c->slot_elem.p, c->lid.p,   \
+                     L.nslot_pad, c->node_start.p, c->halo_start.p, c->halo.p,                  \
+                     c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate, target_q,         \
+                     use_geo ? c->geo.p : nullptr)
+  if (kind == 200)
+    BHIP_PRISM(200);
+  else
+    BHIP_PRISM(201);
+#undef BHIP_PRISM
   HIP_OK(hipGetLastError());
   if (!use_geo) HIP_OK(hipStreamSynchronize(c->stream));
   if (input_done(c)) return BHIP_EDEVICE;
   c->typdia = 'Q';
-  c->typext = 'S';
-  c->ncoef = 15;
+  c->typext = target_q ? 'Q' : 'S';
+  c->ncoef = target_q ? 30 : 15;
   c->have_matrix = true;
   return BHIP_OK;
 }
