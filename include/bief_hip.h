@@ -218,6 +218,15 @@ int bhip_time_cg_kernels(bhip_ctx* ctx, int precision, int reps,
  * mixed solve) its FP32 copy. Scratch vectors are overwritten. No
  * counterpart in the reference. */
 int bhip_time_matvec(bhip_ctx* ctx, int precision, int reps, double* ms);
+/* Measurement: average duration (ms, HIP events on the context stream) of
+ * the CG direction + EBE mat-vec kernel (as ms_dir of bhip_time_cg_kernels)
+ * when every launch follows a read pass over flush_dev (flush_n
+ * doubles of device memory, larger than the last-level cache): the kernel
+ * then finds none of its data in any cache, so the figure is its
+ * DRAM-bound time. Only the kernel is inside the events. No counterpart in
+ * the reference. */
+int bhip_time_cg_dir_cold(bhip_ctx* ctx, int precision, int reps, double* flush_dev,
+                          long flush_n, double* ms_dir);
 
 /* ---- operand preparation before SOLVE ----------------------------------- */
 

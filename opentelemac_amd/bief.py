@@ -99,6 +99,7 @@ def lib():
         L.bhip_matrix_restore.argtypes = [_P]
         L.bhip_time_cg_kernels.argtypes = [_P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]
         L.bhip_time_matvec.argtypes = [_P, _I, _I, ctypes.POINTER(_D)]
+        L.bhip_time_cg_dir_cold.argtypes = [_P, _I, _I, _P, ctypes.c_long, ctypes.POINTER(_D)]
         L.bhip_layout_check.argtypes = [_I, _I, _I, _P, _P, _P, _I, _I, _P]
         L.bhip_dirich.argtypes = [_P, _P, _P, _P, _P, _I]
         L.bhip_lump.argtypes = [_P, _D, _P]
@@ -570,6 +571,23 @@ class Context:
         _check(lib().bhip_time_cg_kernels(self._h, precision, reps, ctypes.byref(a),
                                           ctypes.byref(b)), "bhip_time_cg_kernels")
         return a.value, b.value
+
+    def time_cg_dir_cold(self, flush, precision=PREC_FP64, reps=20):
+        """Average ms of the CG direction/mat-vec kernel launched after a
+        read pass over `flush` (a float64 CUDA tensor larger than
+        the last-level cache), so that every launch starts with none of its
+        data cached: bhip_time_cg_dir_cold."""
+        import torch
+        if not (_is_torch(flush) and flush.is_cuda and flush.dtype == torch.float64
+                and flush.is_contiguous()):
+            raise BiefError("time_cg_dir_cold: flush must be a contiguous float64 CUDA tensor")
+        ms = ctypes.c_double()
+        self._sync_torch()
+        _check(lib().bhip_time_cg_dir_cold(self._h, precision, reps,
+                                           ctypes.c_void_p(flush.data_ptr()),
+                                           flush.numel(), ctypes.byref(ms)),
+               "bhip_time_cg_dir_cold")
+        return ms.value
 
     def time_matvec(self, precision=PREC_FP64, reps=20):
         """Average ms of the plain EBE product on the current matrix (FP64, or

@@ -93,6 +93,8 @@ struct SolveState {
   int dpar;            // reliable updates: parity of the direction the next inner
                        // start continues from, -1 for a fresh start
   int reliable;        // mixed path: inner CG continued across outer steps
+  int inner_pairs[8];  // CG iterations (update + direction launches) of the
+                       // first inner solves, the next solve's launch plan
   long long t_start;   // device clock at the set-up of the solve
   long long t_done;    // device clock when this system stopped
 };
@@ -712,11 +714,14 @@ __global__ void k_reduce_into(const double* __restrict__ partial, int n, double*
 }
 
 // an inner solve has applied its last update
-__device__ inline bool inner_finished(const SolveState* st) { return st->done1 != 0; }
+__device__ inline bool inner_finished(const SolveState* st) {
+  return load_relaxed(&st->done1) != 0;
+}
 
 // the mixed outer step of a system waits: stopped, or its inner solve still running
 __device__ inline bool outer_blocked(const SolveState* st) {
-  return st->outer_done || (st->outer_count > 0 && !inner_finished(st));
+  return load_relaxed(&st->outer_done) ||
+         (load_relaxed(&st->outer_count) > 0 && !inner_finished(st));
 }
 
 __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int nitmax) {
@@ -762,6 +767,7 @@ __global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
   st->fresh = 0;
   st->xpar = 0;
   st->reliable = mixed && reliable;
+  for (int k = 0; k < 8; ++k) st->inner_pairs[k] = 0;
   st->t_start = wall_clock64();
   st->t_done = 0;
   if (!mixed) {
@@ -792,7 +798,8 @@ __global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
 // the residual kernel left it (outer_blocked).
 constexpr int kMaxOuter = 100;  // refinement sweeps of one mixed solve
 __global__ void k_outer_check(const double* __restrict__ partial, int nblocks,
-                              const int32_t* __restrict__ sys_blocks, SolveState* st0) {
+                              const int32_t* __restrict__ sys_blocks, SolveState* st0,
+                              int flip) {
   __shared__ double sh_red[kWaves];
   __shared__ double sh_b;
   const int sys = blockIdx.x;
@@ -802,35 +809,45 @@ __global__ void k_outer_check(const double* __restrict__ partial, int nblocks,
   const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
   const double rr = reduce_partials(partial + pb0, pb1 - pb0, sh_red, &sh_b);
   if (threadIdx.x != 0) return;
-  // the residual kernel wrote x - delta into the other solution buffer
-  st->xpar ^= 1;
-  // a continued inner solve starts at M = 1 and counts one product fewer
-  if (st->outer_count > 0) st->inner_total += st->final_m - (st->dpar >= 0 ? 1 : 0);
+  // every field this kernel reads, read once before any store, with
+  // agent-scope loads: a plain load of a field written by an earlier kernel
+  // has returned a stale value here (depending on how the compiler issued it)
+  const int oc = load_relaxed(&st->outer_count);
+  const int fm = load_relaxed(&st->final_m);
+  const int dp = load_relaxed(&st->dpar);
+  const int dn = load_relaxed(&st->done);
+  const int rel = load_relaxed(&st->reliable);
+  const int itot0 = load_relaxed(&st->inner_total);
+  const int ntot = load_relaxed(&st->nitmax_total);
+  const int innmax = load_relaxed(&st->inner_nitmax);
+  const int xp = load_relaxed(&st->xpar);
+  const double eps2o = st->eps2_outer;
+  const double ie2 = st->inner_eps2;
+  const double rr_s0 = st->rr[0], rr_s1 = st->rr[1];
+  if (flip) store_relaxed(&st->xpar, xp ^ 1);
+  int itot = itot0;
+  if (oc > 0) {
+    const int used = fm - (dp >= 0 ? 1 : 0);
+    itot += used;
+    if (oc <= 8) st->inner_pairs[oc - 1] = used;
+  }
+  st->inner_total = itot;
   st->rr_outer = rr;
-  const int left = st->nitmax_total - st->inner_total;
-  if (rr <= st->eps2_outer || left <= 0 || st->outer_count >= kMaxOuter) {
-    st->outer_status = rr <= st->eps2_outer ? ST_CONVERGED : ST_MAXITER;
+  const int left = ntot - itot;
+  if (rr <= eps2o || left <= 0 || oc >= kMaxOuter) {
+    st->outer_status = rr <= eps2o ? ST_CONVERGED : ST_MAXITER;
     st->outer_done = 1;
     st->done = ST_CONVERGED;
     st->t_done = wall_clock64();
     return;
   }
-  // reliable updates: the next inner solve keeps the last direction D of
-  // this one, and its first BETA is the new residual's ||R||^2 over the
-  // ||R||^2 of the iteration that built that direction
-  // Iteration M of a solve that started at M0 (0 fresh, 1 continued) keeps
-  // its ||R||^2 and direction in slot (M - M0) & 1.
-  const int mf = st->final_m;
-  const int m0 = st->dpar >= 0 ? 1 : 0;
-  const int slot = (mf - 1 - m0) & 1;
-  const bool cont = st->reliable && st->outer_count > 0 && mf - 1 >= m0 &&
-                    st->done == ST_CONVERGED;
-  const double rprev = cont ? st->rr[slot] : 1.0;
-  st->outer_count += 1;
-  // the inner solve may use what is left of NITMAX (a continued solve's
-  // count starts at M = 1)
-  const int cap = (st->inner_nitmax < left ? st->inner_nitmax : left) + (cont ? 1 : 0);
-  reset_cg(st, st->inner_eps2, 1, cap);
+  const int m0 = dp >= 0 ? 1 : 0;
+  const int slot = (fm - 1 - m0) & 1;
+  const bool cont = rel && oc > 0 && fm - 1 >= m0 && dn == ST_CONVERGED;
+  const double rprev = cont ? (slot ? rr_s1 : rr_s0) : 1.0;
+  st->outer_count = oc + 1;
+  const int cap = (innmax < left ? innmax : left) + (cont ? 1 : 0);
+  reset_cg(st, ie2, 1, cap);
   st->rr0 = rr;
   st->dpar = -1;
   if (cont) {
@@ -849,8 +866,8 @@ __global__ void k_inner_dir(float* __restrict__ dst, const float* __restrict__ d
                             const int32_t* __restrict__ block_sys, const SolveState* st0) {
   const int b = blockIdx.x;
   const SolveState* st = st0 + (block_sys ? block_sys[b] : 0);
-  const int par = st->dpar;
-  if (!st->fresh || par < 0 || st->outer_done) return;
+  const int par = load_relaxed(&st->dpar);
+  if (!load_relaxed(&st->fresh) || par < 0 || load_relaxed(&st->outer_done)) return;
   const float* src = par ? d1 : d0;
   for (int i = node_start[b] + threadIdx.x; i < node_start[b + 1]; i += blockDim.x)
     dst[i] = src[i];
@@ -863,7 +880,7 @@ __global__ void k_xpar_gather(double* __restrict__ x0, const double* __restrict_
                               const int32_t* __restrict__ block_sys, const SolveState* st0) {
   const int b = blockIdx.x;
   const SolveState* st = st0 + (block_sys ? block_sys[b] : 0);
-  if (!st->xpar) return;
+  if (!load_relaxed(&st->xpar)) return;
   for (int i = node_start[b] + threadIdx.x; i < node_start[b + 1]; i += blockDim.x)
     x0[i] = x1[i];
 }
@@ -904,14 +921,20 @@ __global__ void k_to_f32(float* __restrict__ out, const double* __restrict__ in,
 // PRECD1 node part (precd1.f:90-149): D = sqrt(DA) (|DA| for PRECON 5);
 // X = X*D; D = 1/D guarded (|D| > 1e-10, else 1: ovd.f:92-102); B = B*D;
 // DA = DA*D*D ('Q') or D*D ('I') (om1111.f:559-575).
-__global__ void k_precd1_nodes(double* __restrict__ x, double* __restrict__ b,
+// X and B are read from the caller's vectors (xs, bs; through perm when it
+// is given, i.e. in the caller's node order) and written to the context's
+// internal-order copies x and b, so SOLVE needs no separate gather.
+__global__ void k_precd1_nodes(double* x, double* b, const double* xs, const double* bs,
+                               const int32_t* __restrict__ perm,
                                double* __restrict__ da, double* __restrict__ dout, int n,
                                int precon, int typdia_q, int typdia_i) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int j = perm ? perm[i] : i;
+    const double xv = xs[j], bv = bs[j];
     double d = da[i];
     if (precon == 5) d = fabs(d);
     d = sqrt(d);
-    x[i] = x[i] * d;
+    x[i] = xv * d;
     d = (fabs(d) > 1e-10) ? 1.0 / d : 1.0;
     dout[i] = d;
     if (typdia_q) {
@@ -920,7 +943,7 @@ __global__ void k_precd1_nodes(double* __restrict__ x, double* __restrict__ b,
     } else if (typdia_i) {
       da[i] = d * d;
     }
-    b[i] = b[i] * d;
+    b[i] = bv * d;
   }
 }
 
@@ -1101,13 +1124,14 @@ struct bhip_ctx {
   // the planned part of whole solves (set-up, expected iteration batches),
   // captured once per plan and replayed as one graph
   struct SolveGraph {
-    uint64_t key[6];
+    uint64_t key[8];
     hipGraphExec_t exec;
   };
   std::vector<SolveGraph> solve_graphs;
   bool use_graphs = true;
   int prefetch_j = 0;  // register prefetch depth of the element kernels
   int iter_hint[4] = {0, 0, 0, 0};  // FP64 CG, inner CG, inner solves, CGSTAB
+  int inner_hint[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // iterations of inner solve k (mixed CG)
   // BiCGSTAB scalars and extra partial sums; GMRES basis and scalars
   bhip::DevBuf<double> bicg_sc, bicg_part, gm, gm_scal;
   bhip::DevBuf<float> gm32;      // mixed GMRES: FP32 basis V and AV
@@ -1745,6 +1769,7 @@ static hipError_t enqueue_cg_batch(bhip_ctx* c, int iters, TX* x, T* r, T* d0, T
 
 // The graph (or plain launches) of one batch of kBatch CG iterations.
 constexpr int kBatch = 8;
+static int even_up(int n) { return n + (n & 1); }
 
 // A captured batch of CG iterations, cached per (solution buffer, kernel
 // family, TYPDIA/TYPEXT specialisation).
@@ -1795,7 +1820,7 @@ static int cg_batch_exec(bhip_ctx* c, int prec, TX* x, T* r, T* d0, T* d1, T* ad
 // graph captured on first use for this key and replayed afterwards, or
 // directly when graphs are off.
 template <typename F>
-static int run_planned(bhip_ctx* c, const uint64_t (&key)[6], F&& plan) {
+static int run_planned(bhip_ctx* c, const uint64_t (&key)[8], F&& plan) {
   if (!c->use_graphs) return plan(false);
   hipGraphExec_t exec = nullptr;
   for (auto& sg : c->solve_graphs)
@@ -1879,13 +1904,19 @@ static hipError_t enqueue_bb_setup(bhip_ctx* c, const double* b, double eps, int
 static int ensure_f32(bhip_ctx* c) {
   const Layout& L = c->L;
   const int n = L.npoin;
+  const bool fresh = !c->fdz.p || c->fdz.n < (size_t)n;
   for (auto* v : {&c->fr, &c->fd0, &c->fd1, &c->fad, &c->fx0, &c->fdz}) HIP_OK(v->alloc(n));
   HIP_OK(c->da32.alloc(n));
   const int npair = L.ndp * (L.ndp - 1) / 2;
   HIP_OK(c->xa32.alloc((size_t)2 * npair * L.nslot_pad));
-  hipLaunchKernelGGL(k_fill<float>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
-                     c->fdz.p, 0.0f, (long)n);
-  HIP_OK(hipGetLastError());
+  // a fresh inner start multiplies fdz by BETA = 0, a continued one reads it
+  // after k_inner_dir has written a direction there: it only ever needs to
+  // hold finite values, so it is zeroed once, when allocated
+  if (fresh) {
+    hipLaunchKernelGGL(k_fill<float>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
+                       c->fdz.p, 0.0f, (long)n);
+    HIP_OK(hipGetLastError());
+  }
   return BHIP_OK;
 }
 
@@ -1975,9 +2006,13 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
                                                  c->vd1.p, c->vad.p, c->xa.p, c->da.p, &exec);
   if (rc) return rc;
   int& hint = c->iter_hint[0];
-  int nb = std::max(1, (hint + 1 + kBatch - 1) / kBatch);
+  int nb = 1;  // batches of the first extra round (doubling up to 8)
+  // iterations queued up front: as many as the previous solve needed (each
+  // launched iteration past convergence still costs two launches), rounded
+  // up to even so that later batches start on the same parity
+  const int npl = hint > 0 ? even_up(hint) : kBatch;
   // set-up, R = A X - B (gracjg.F:213-215) with the partial sums of RMRM,
-  // E_0, and the iteration batches the previous solve needed
+  // E_0, and the iterations the previous solve needed
   auto plan = [&](bool capturing) -> int {
     HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 0, 0.0, 0));
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
@@ -1989,12 +2024,13 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
         c, a, false, c->typdia, c->typext)));
     HIP_OK((enqueue_cg_start<double, double>(c, c->vr.p, c->vd0.p, c->vad.p, c->vdz.p, c->xa.p,
                                              c->da.p, c->L.nblocks, nullptr)));
-    return enqueue_batches<double, double, double>(c, capturing ? nullptr : exec, nb, x, c->vr.p,
-                                                   c->vd0.p, c->vd1.p, c->vad.p, c->xa.p,
-                                                   c->da.p);
+    (void)capturing;
+    HIP_OK((enqueue_cg_batch<double, double, double>(c, npl, x, c->vr.p, c->vd0.p, c->vd1.p,
+                                                     c->vad.p, c->xa.p, c->da.p)));
+    return BHIP_OK;
   };
-  const uint64_t key[6] = {0, (uint64_t)nb, matrix_tag(c), dbits(eps), (uint64_t)nitmax,
-                           (uint64_t)(uintptr_t)x};
+  const uint64_t key[8] = {0, (uint64_t)npl, matrix_tag(c), dbits(eps), (uint64_t)nitmax,
+                           (uint64_t)(uintptr_t)x, 0, 0};
   if ((rc = run_planned(c, key, plan))) return rc;
   // every system of an ensemble runs until its own stop; batches go on
   // while one of them has not stopped
@@ -2003,9 +2039,9 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
     bool all = true;
     for (int sy = 0; sy < c->nsys; ++sy) all = all && sh[sy].done != 0;
     if (all) break;
-    nb = std::min(nb * 2, 8);
     rc = enqueue_batches<double, double, double>(c, exec, nb, x, c->vr.p, c->vd0.p, c->vd1.p,
                                                  c->vad.p, c->xa.p, c->da.p);
+    nb = std::min(nb * 2, 8);
     if (rc) return rc;
   }
   int maxm = 0;
@@ -2029,7 +2065,7 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
 // host waits once, then queues more rounds while a system has not stopped.
 static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
                     double inner_eps, int inner_nitmax, bool xa32_ready,
-                    bhip_solve_info* info) {
+                    bhip_solve_info* info, double** xsol) {
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
@@ -2052,19 +2088,28 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   };
   // outer step: x - delta into the other solution buffer, r = A x - b with
   // its partial sums, the inner right-hand side -r; then the test
-  auto outer_step = [&]() -> int {
+  // (the first one of a solve, first = true: r = A x - b with x as given,
+  // no delta to apply and no buffer switch)
+  auto outer_step = [&](bool first = false) -> int {
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
-    a.xb0 = x;
-    a.xb1 = c->vx2.p;
-    a.delta = delta;
     a.b = b;
     a.r32 = c->fr.p;
     a.partial = part_u(c, 0);
     a.skip_if_outer_done = 1;
-    HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_PARTIAL>(
-        c, a, false, c->typdia, c->typext)));
+    if (first) {
+      a.y = x;
+      HIP_OK((launch_ebe<double, double, STAGE_PLAIN, EPI_RESID_MIXED, FIN_PARTIAL>(
+          c, a, false, c->typdia, c->typext)));
+    } else {
+      a.xb0 = x;
+      a.xb1 = c->vx2.p;
+      a.delta = delta;
+      HIP_OK((launch_ebe<double, double, STAGE_XUPD, EPI_RESID_MIXED, FIN_PARTIAL>(
+          c, a, false, c->typdia, c->typext)));
+    }
     hipLaunchKernelGGL(k_outer_check, dim3(c->nsys), dim3(kThreads), 0, c->stream,
-                       part_u(c, 0), L.nblocks, sys_ptr(c->sys_blocks), c->st_p);
+                       part_u(c, 0), L.nblocks, sys_ptr(c->sys_blocks), c->st_p,
+                       first ? 0 : 1);
     HIP_OK(hipGetLastError());
     return BHIP_OK;
   };
@@ -2082,39 +2127,53 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   };
   int& hint_in = c->iter_hint[1];   // iterations of one inner solve
   int& hint_out = c->iter_hint[2];  // inner solves of one mixed solve
-  const int nbat = std::max(1, (hint_in + 1 + kBatch - 1) / kBatch);
-  const int nout = std::max(hint_out, 1);
+  const int nout = std::min(std::max(hint_out, 1), 8);
+  // iterations queued for inner solve k: what inner solve k of the previous
+  // solve needed, rounded up to even (later batches keep the parity), or the
+  // average inner count when that is unknown
+  int npl[8];
+  uint64_t plan_key[2] = {0, 0};
+  for (int k = 0; k < nout; ++k) {
+    const int h = c->inner_hint[k] > 0 ? c->inner_hint[k] : std::max(hint_in, kBatch);
+    npl[k] = std::min(even_up(h), 0xfffe);
+    plan_key[k / 4] |= (uint64_t)npl[k] << (16 * (k % 4));
+  }
+  const bool da32_needed = c->typdia == 'Q';
   // the planned part: FP32 copies, set-up, then the outer and inner steps
   // the previous solve needed (the device skips what is not needed)
   auto plan = [&](bool capturing) -> int {
+    (void)capturing;
     if (!xa32_ready)
       hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(kThreads), 0, c->stream, c->xa32.p, c->xa.p,
                          ncf);
-    hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
-                       (long)n);
-    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, c->stream, delta, 0.0f,
-                       (long)n);
+    if (da32_needed)
+      hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->da32.p, c->da.p,
+                         (long)n);
     HIP_OK(hipGetLastError());
     HIP_OK(enqueue_bb_setup(c, b, eps, nitmax, 1, inner_eps, inner_nitmax, reliable ? 1 : 0));
-    int r = outer_step();
+    int r = outer_step(true);
     for (int k = 0; k < nout && !r; ++k) {
       if ((r = inner_start())) break;
-      if ((r = more(nbat, capturing))) break;
+      HIP_OK((enqueue_cg_batch<float, float, float>(c, npl[k], delta, c->fr.p, c->fd0.p,
+                                                    c->fd1.p, c->fad.p, c->xa32.p,
+                                                    c->da32.p)));
       r = outer_step();
     }
     return r;
   };
-  const uint64_t key[6] = {
-      1u | (xa32_ready ? 4u : 0u) | (reliable ? 8u : 0u) |
+  const uint64_t key[8] = {
+      1u | (xa32_ready ? 4u : 0u) | (reliable ? 8u : 0u) | (da32_needed ? 16u : 0u) |
           ((uint64_t)(uint32_t)inner_nitmax << 8),
-      (uint64_t)nbat | ((uint64_t)nout << 16) | ((uint64_t)(uint32_t)nitmax << 32),
-      matrix_tag(c), dbits(eps), dbits(inner_eps), (uint64_t)(uintptr_t)x};
+      (uint64_t)nout | ((uint64_t)(uint32_t)nitmax << 32),
+      matrix_tag(c), dbits(eps), dbits(inner_eps), (uint64_t)(uintptr_t)x,
+      plan_key[0], plan_key[1]};
   if ((rc = run_planned(c, key, plan))) return rc;
   // further rounds while a system has not stopped: start the inner solves
   // the last outer tests began, iterate the running ones, test again. Every
   // round runs at least kBatch inner iterations of each running solve, and
   // NITMAX (and kMaxOuter) stop every system, so the loop ends.
   int nround = 1, rounds = 0;
+  const int nbat = std::max(1, (hint_in + 1 + kBatch - 1) / kBatch);
   for (;;) {
     if (read_state(c)) return BHIP_EDEVICE;
     bool all = true;
@@ -2124,12 +2183,18 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     if ((rc = inner_start())) return rc;
     if ((rc = more(nround, false))) return rc;
     if ((rc = outer_step())) return rc;
-    nround = std::min(nround * 2, std::max(nbat, 1));
+    nround = std::min(nround * 2, nbat);
   }
-  // the solution of every system into x
-  hipLaunchKernelGGL(k_xpar_gather, dim3(L.nblocks), dim3(kThreads), 0, c->stream, x, c->vx2.p,
-                     c->node_start.p, sys_ptr(c->block_sys), c->st_p);
-  HIP_OK(hipGetLastError());
+  // the solution: one system tells the caller which buffer holds it (the
+  // host has just read its state); an ensemble gathers every system's into x
+  *xsol = x;
+  if (c->nsys == 1) {
+    if (sh[0].xpar) *xsol = c->vx2.p;
+  } else {
+    hipLaunchKernelGGL(k_xpar_gather, dim3(L.nblocks), dim3(kThreads), 0, c->stream, x,
+                       c->vx2.p, c->node_start.p, sys_ptr(c->block_sys), c->st_p);
+    HIP_OK(hipGetLastError());
+  }
   int maxin = 0, maxout = 0;
   long long sumin = 0, sumout = 0;
   for (int sy = 0; sy < c->nsys; ++sy) {
@@ -2140,6 +2205,12 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   }
   if (sumout > 0) hint_in = (int)((sumin + sumout - 1) / sumout);
   hint_out = maxout;
+  for (int k = 0; k < 8; ++k) {
+    int h = 0;
+    for (int sy = 0; sy < c->nsys; ++sy)
+      if (!sh[sy].zero_rhs && k < sh[sy].outer_count) h = std::max(h, sh[sy].inner_pairs[k]);
+    c->inner_hint[k] = h;
+  }
   (void)maxin;
   return collect_infos(c, x, true, info);
 }
@@ -2186,27 +2257,29 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
-  // caller order -> internal order
-  if (internal) {
+  const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
+  // caller order -> internal order (with diagonal preconditioning the
+  // PRECD1 kernel below reads the caller's vectors itself)
+  if (!diag && internal) {
     hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, x_dev,
                        nullptr, n);
     hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, c->vb.p, b_dev,
                        nullptr, n);
-  } else {
+  } else if (!diag) {
     hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, x_dev,
                        c->perm.p, n);
     hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vb.p, b_dev,
                        c->perm.p, n);
   }
   HIP_OK(hipGetLastError());
-  const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
   const bool mixed = cfg->precision == BHIP_PREC_MIXED;
   if (mixed && ensure_f32(c)) return BHIP_EDEVICE;
   bool xa32_ready = false;
   if (diag) {
-    // PRECD1 (precd1.f:90-149)
+    // PRECD1 (precd1.f:90-149), X and B taken from the caller's vectors
     hipLaunchKernelGGL(k_precd1_nodes, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, c->vb.p,
-                       c->da.p, c->vdinv.p, n, precon, c->typdia == 'Q', c->typdia == 'I');
+                       x_dev, b_dev, internal ? nullptr : c->perm.p, c->da.p, c->vdinv.p, n,
+                       precon, c->typdia == 'Q', c->typdia == 'I');
     HIP_OK(hipGetLastError());
     if (c->typdia == 'I') c->typdia = 'Q';
     if (c->ncoef > 0) {
@@ -2218,6 +2291,7 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
     if (precon % 2 == 0 || precon % 3 == 0) c->typdia = 'I';
   }
   int rc;
+  double* xsol = c->vx.p;  // internal-order buffer holding the solution
   if (cfg->slv == 6) {
     rc = bhip::cgstab_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
   } else if (cfg->slv == 7 && mixed) {
@@ -2229,24 +2303,24 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
     double ie = cfg->inner_eps > 0 ? cfg->inner_eps : 1e-4;
     int inn = cfg->inner_nitmax > 0 ? cfg->inner_nitmax : std::max(cfg->nitmax, 1);
     rc = cg_mixed(c, c->vx.p, c->vb.p, cfg->eps, std::max(cfg->nitmax, 1), ie, inn, xa32_ready,
-                  info);
+                  info, &xsol);
   } else {
     rc = cg_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
   }
   if (rc != BHIP_OK && rc != BHIP_MAXITER && rc != BHIP_ZERO_RHS) return rc;
   // X <- X*D (solve.f:637-641), back to caller order; B <- D*B as the reference leaves it
   if (internal) {
-    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
+    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev, xsol,
                        diag ? c->vdinv.p : nullptr, n);
     hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, b_dev, c->vb.p,
                        nullptr, n);
   } else {
     if (diag)
       hipLaunchKernelGGL(k_scatter_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev,
-                         c->vx.p, c->vdinv.p, c->perm.p, n);
+                         xsol, c->vdinv.p, c->perm.p, n);
     else
       hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, x_dev,
-                         c->vx.p, c->perm.p, n);
+                         xsol, c->perm.p, n);
     hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, b_dev,
                        c->vb.p, c->perm.p, n);
   }
@@ -2315,12 +2389,31 @@ int bhip_ctx_perm(const bhip_ctx* c, int32_t* perm) {
   return BHIP_OK;
 }
 
-int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
-                         double* ms_upd) {
+}  // extern "C"
+
+namespace bhip {
+// Reads every line of a buffer larger than the last-level cache, so that the
+// kernel timed next finds none of its data cached (and no dirty lines of the
+// flush to write back); the sum is stored only if it is NaN, which keeps the
+// reads without writing anything for a finite buffer.
+__global__ void k_flush(double* __restrict__ x, long n) {
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    s += x[i];
+  if (s != s) x[0] = s;
+}
+}  // namespace bhip
+
+// The two kernels of one CG iteration, `reps` launches each, timed with HIP
+// events on the context stream: back to back (flush == nullptr), or each
+// launch after a pass over `flush` (cold caches; only the kernel is inside
+// the events).
+static int time_cg(bhip_ctx* c, int precision, int reps, double* flush, long flush_n,
+                   double* ms_dir, double* ms_upd) {
   if (!c || !c->have_matrix || reps < 1) return BHIP_EBADARG;
   if (precision == BHIP_PREC_MIXED && !c->fr.p) return BHIP_EBADARG;
-  const int n = c->L.npoin;
-  const int g = c->grid_nodes;
+  if (flush && flush_n < 1) return BHIP_EBADARG;
   // never converges, never reaches NITMAX: every launch does its full work
   for (int sy = 0; sy < c->nsys; ++sy) {
     SolveState* sh = c->st_host + sy;
@@ -2334,11 +2427,26 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
   HIP_OK(hipEventCreate(&e1));
   float t_dir = 0.f, t_upd = 0.f;
   auto time_it = [&](auto&& launch, float* out) -> int {
-    HIP_OK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; ++i) HIP_OK(launch());
-    HIP_OK(hipEventRecord(e1, c->stream));
-    HIP_OK(hipEventSynchronize(e1));
-    HIP_OK(hipEventElapsedTime(out, e0, e1));
+    if (!flush) {
+      HIP_OK(hipEventRecord(e0, c->stream));
+      for (int i = 0; i < reps; ++i) HIP_OK(launch());
+      HIP_OK(hipEventRecord(e1, c->stream));
+      HIP_OK(hipEventSynchronize(e1));
+      HIP_OK(hipEventElapsedTime(out, e0, e1));
+      return BHIP_OK;
+    }
+    *out = 0.f;
+    for (int i = 0; i < reps; ++i) {
+      hipLaunchKernelGGL(k_flush, dim3(4096), dim3(kThreads), 0, c->stream, flush, flush_n);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipEventRecord(e0, c->stream));
+      HIP_OK(launch());
+      HIP_OK(hipEventRecord(e1, c->stream));
+      HIP_OK(hipEventSynchronize(e1));
+      float t = 0.f;
+      HIP_OK(hipEventElapsedTime(&t, e0, e1));
+      *out += t;
+    }
     return BHIP_OK;
   };
   int rc;
@@ -2357,10 +2465,12 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
                                                                       c->typext);
     }, &t_dir);
     if (rc) return rc;
-    rc = time_it([&]() {
-      return launch_update<float, float>(c, c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, 1);
-    }, &t_upd);
-    if (rc) return rc;
+    if (ms_upd) {
+      rc = time_it([&]() {
+        return launch_update<float, float>(c, c->fx0.p, c->fr.p, c->fd1.p, c->fad.p, 1);
+      }, &t_upd);
+      if (rc) return rc;
+    }
   } else {
     EbeArgs<double, double> a = base_args<double, double>(c, c->xa.p, c->da.p);
     a.r = c->vr.p;
@@ -2376,16 +2486,34 @@ int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
                                                                         c->typext);
     }, &t_dir);
     if (rc) return rc;
-    rc = time_it([&]() {
-      return launch_update<double, double>(c, c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, 1);
-    }, &t_upd);
-    if (rc) return rc;
+    if (ms_upd) {
+      rc = time_it([&]() {
+        return launch_update<double, double>(c, c->vio0.p, c->vr.p, c->vd1.p, c->vad.p, 1);
+      }, &t_upd);
+      if (rc) return rc;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (ms_dir) *ms_dir = t_dir / reps;
   if (ms_upd) *ms_upd = t_upd / reps;
   return BHIP_OK;
+}
+
+extern "C" {
+
+int bhip_time_cg_kernels(bhip_ctx* c, int precision, int reps, double* ms_dir,
+                         double* ms_upd) {
+  double upd = 0.0;
+  const int rc = time_cg(c, precision, reps, nullptr, 0, ms_dir, &upd);
+  if (ms_upd) *ms_upd = upd;
+  return rc;
+}
+
+int bhip_time_cg_dir_cold(bhip_ctx* c, int precision, int reps, double* flush_dev,
+                          long flush_n, double* ms_dir) {
+  if (!flush_dev || !ms_dir) return BHIP_EBADARG;
+  return time_cg(c, precision, reps, flush_dev, flush_n, ms_dir, nullptr);
 }
 
 int bhip_time_matvec(bhip_ctx* c, int precision, int reps, double* ms) {
