@@ -232,15 +232,19 @@ def bench_estuary(args, dev, m, sysd):
 
     ctx.set_matrix("Q", "S", da0, xa0)
     ctx.save_matrix()
+    # the step's torch copies run on the context's stream, so SOLVE needs no
+    # event handshake with another stream
+    ts = ctx.torch_stream()
 
     def step(precision):
         # SOLVE consumes A (D A D in place): restore the assembled copy first,
         # as telemac2d copies AM1 before its SOLVE (propag.F:1643)
-        ctx.restore_matrix()
-        x.copy_(x00)
-        b.copy_(b0)
-        _, info = ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=precision,
-                            inner_eps=args.inner_eps)
+        with torch.cuda.stream(ts):
+            ctx.restore_matrix()
+            x.copy_(x00)
+            b.copy_(b0)
+            _, info = ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000,
+                                precision=precision, inner_eps=args.inner_eps)
         return info
 
     results = {}

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -95,6 +96,10 @@ struct SolveState {
   int reliable;        // mixed path: inner CG continued across outer steps
   int inner_pairs[8];  // CG iterations (update + direction launches) of the
                        // first inner solves, the next solve's launch plan
+  int stop;            // CG kernels of this solve return at once. Set by the
+                       // kernel after the one that set done, so that no
+                       // workgroup of a direction kernel skips its work on a
+                       // flag another workgroup of the same launch has set
   long long t_start;   // device clock at the set-up of the solve
   long long t_done;    // device clock when this system stopped
 };
@@ -197,6 +202,7 @@ struct EbeArgs {
   const int32_t* halo;
   const int32_t* color_off;
   const uint16_t* lid;
+  int col_ids;                // prisms numbered by column: ids 3..5 = ids 0..2 + 1
   int nslot_pad;
   int nblocks;
   // matrix
@@ -297,6 +303,25 @@ __device__ inline void load_phase(PhaseSlot<T, NDP, NC>& ps, int c, const int* s
   }
 }
 
+// local node ids of slot s; column-numbered prisms store all six (other
+// kernels read them) but the product reads the bottom three only
+template <int NDP>
+__device__ inline void load_ids(int (&l)[NDP], const uint16_t* __restrict__ lid, size_t stride,
+                                int s, int col_ids) {
+  if constexpr (NDP == 6) {
+    if (col_ids) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        l[k] = lid[(size_t)k * stride + s];
+        l[k + 3] = l[k] + 1;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NDP; ++k) l[k] = lid[(size_t)k * stride + s];
+}
+
 // node values read by the CG staging passes (global index, R, D, AD, X)
 template <typename T>
 struct NodeIn {
@@ -370,7 +395,7 @@ k_ebe(EbeArgs<T, TA> a) {
   if constexpr (FIN == FIN_CG) {
     SolveState* st = a.st + sys;
     if (a.skip_unless_fresh && !st->fresh) return;
-    if (load_relaxed(&st->done)) return;
+    if (load_relaxed(&st->stop)) return;
     if (load_relaxed(&st->final_flag)) {
       // the update kernel before this one applied the last X update
       if (b == pb0 && threadIdx.x == 0) {
@@ -516,8 +541,7 @@ k_ebe(EbeArgs<T, TA> a) {
                s += kThreads) {
             int l[NDP];
             T cf[ncoef];
-#pragma unroll
-            for (int k = 0; k < NDP; ++k) l[k] = a.lid[(size_t)k * a.nslot_pad + s];
+            load_ids<NDP>(l, a.lid, a.nslot_pad, s, a.col_ids);
 #pragma unroll
             for (int qq = 0; qq < ncoef; ++qq) cf[qq] = (T)a.xa[(size_t)qq * a.nslot_pad + s];
             slot_apply<T, NDP, SYMQ, TRANS>(l, cf, ys, acc, nown);
@@ -535,8 +559,7 @@ k_ebe(EbeArgs<T, TA> a) {
       for (int s = c0 + threadIdx.x; s < c1; s += blockDim.x) {
         int l[NDP];
         T cf[ncoef];
-#pragma unroll
-        for (int k = 0; k < NDP; ++k) l[k] = a.lid[(size_t)k * a.nslot_pad + s];
+        load_ids<NDP>(l, a.lid, a.nslot_pad, s, a.col_ids);
 #pragma unroll
         for (int q = 0; q < ncoef; ++q) cf[q] = (T)a.xa[(size_t)q * a.nslot_pad + s];
         slot_apply<T, NDP, SYMQ, TRANS>(l, cf, ys, acc, nown);
@@ -613,7 +636,11 @@ k_cg_update(TX* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
   const int pb0 = sys_blocks ? sys_blocks[sys] : 0;
   const int pb1 = sys_blocks ? sys_blocks[sys + 1] : nblocks;
   SolveState* st = st0 + sys;
-  if (load_relaxed(&st->done)) return;
+  if (load_relaxed(&st->done)) {
+    // the direction kernel before this one found the stop: later kernels skip
+    if (b == pb0 && threadIdx.x == 0) store_relaxed(&st->stop, 1);
+    return;
+  }
   // the first update of an inner solve retires its start (no workgroup of
   // this kernel reads the flag)
   if (b == pb0 && threadIdx.x == 0 && st->fresh) st->fresh = 0;
@@ -736,6 +763,7 @@ __device__ inline void reset_cg(SolveState* st, double eps2xl, int use_rr0, int 
   st->final_rr = 0.0;
   st->done = 0;
   st->done1 = 0;
+  st->stop = 0;
 }
 
 // GRACJG set-up (gracjg.F:189-209) from the partial sums of ||b'||^2: XL,
@@ -779,9 +807,11 @@ __global__ void k_cg_setup(const double* __restrict__ partial, int nblocks,
     st->nitmax_total = nitmax < 1 ? 1 : nitmax;
     reset_cg(st, inner_eps * inner_eps, 1, inner_nitmax);
     st->done = ST_CONVERGED;  // no inner solve until an outer test starts one
+    st->stop = 1;
   }
   if (st->zero_rhs) {
     st->done = ST_CONVERGED;
+    st->stop = 1;
     st->outer_done = 1;
     st->outer_status = ST_CONVERGED;
     st->final_m = 0;
@@ -838,6 +868,7 @@ __global__ void k_outer_check(const double* __restrict__ partial, int nblocks,
     st->outer_status = rr <= eps2o ? ST_CONVERGED : ST_MAXITER;
     st->outer_done = 1;
     st->done = ST_CONVERGED;
+    st->stop = 1;
     st->t_done = wall_clock64();
     return;
   }
@@ -899,12 +930,17 @@ __global__ void k_scatter(T* __restrict__ out, const T* __restrict__ in,
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     out[idx[i]] = in[i];
 }
-// out[idx[i]] = in[i] * s[i]  (undo of the variable change, solve.f:637-641)
-__global__ void k_scatter_scaled(double* __restrict__ out, const double* __restrict__ in,
-                                 const double* __restrict__ s, const int32_t* __restrict__ idx,
-                                 int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    out[idx[i]] = in[i] * s[i];
+// SOLVE's results back to the caller: out[idx[i]] (out[i] without idx) =
+// src[i] (* s[i]), src the solution buffer the system's state names
+// (st: in1 when xpar is set, else in0; no state: in0)
+__global__ void k_x_out(double* __restrict__ out, const double* __restrict__ in0,
+                        const double* __restrict__ in1, const SolveState* st,
+                        const double* __restrict__ s, const int32_t* __restrict__ idx, int n) {
+  const double* src = (st && load_relaxed(&st->xpar)) ? in1 : in0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double v = s ? src[i] * s[i] : src[i];
+    out[idx ? idx[i] : i] = v;
+  }
 }
 template <typename T>
 __global__ void k_fill(T* __restrict__ x, T v, long n) {
@@ -1198,6 +1234,7 @@ static EbeArgs<T, TA> base_args(bhip_ctx* c, const TA* xa, const TA* da) {
   a.halo = c->halo.p;
   a.color_off = c->color_off.p;
   a.lid = c->lid.p;
+  a.col_ids = c->L.col_ids ? 1 : 0;
   a.nslot_pad = c->L.nslot_pad;
   a.nblocks = c->L.nblocks;
   a.xa = xa;
@@ -1726,6 +1763,7 @@ static void init_cg_state(SolveState* sh, double eps2xl, int use_rr0, int nitmax
   sh->rr[0] = sh->rr[1] = 1.0;
   sh->done = 0;
   sh->done1 = 0;
+  sh->stop = 0;
   sh->final_flag = 0;
   sh->final_m = 0;
   sh->final_rr = 0.0;
@@ -1997,7 +2035,7 @@ static int collect_infos(bhip_ctx* c, double* x, bool mixed, bhip_solve_info* in
 // iteration batches as the previous solve needed); the host then waits once
 // and only queues more batches if the solve is not finished.
 static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
-                   bhip_solve_info* info) {
+                   bhip_solve_info* info, const std::function<int()>& tail) {
   const int n = c->L.npoin;
   const int g = c->grid_nodes;
   SolveState* sh = c->st_host;
@@ -2033,8 +2071,10 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
                            (uint64_t)(uintptr_t)x, 0, 0};
   if ((rc = run_planned(c, key, plan))) return rc;
   // every system of an ensemble runs until its own stop; batches go on
-  // while one of them has not stopped
+  // while one of them has not stopped. The caller's results are written
+  // before each wait, so a finished solve needs no launch after it
   for (;;) {
+    if ((rc = tail())) return rc;
     if (read_state(c)) return BHIP_EDEVICE;
     bool all = true;
     for (int sy = 0; sy < c->nsys; ++sy) all = all && sh[sy].done != 0;
@@ -2065,7 +2105,7 @@ static int cg_fp64(bhip_ctx* c, double* x, const double* b, double eps, int nitm
 // host waits once, then queues more rounds while a system has not stopped.
 static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nitmax,
                     double inner_eps, int inner_nitmax, bool xa32_ready,
-                    bhip_solve_info* info, double** xsol) {
+                    bhip_solve_info* info, const std::function<int()>& tail) {
   const Layout& L = c->L;
   const int n = L.npoin;
   const int g = c->grid_nodes;
@@ -2175,6 +2215,7 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
   int nround = 1, rounds = 0;
   const int nbat = std::max(1, (hint_in + 1 + kBatch - 1) / kBatch);
   for (;;) {
+    if ((rc = tail())) return rc;
     if (read_state(c)) return BHIP_EDEVICE;
     bool all = true;
     for (int sy = 0; sy < c->nsys; ++sy) all = all && sh[sy].outer_done != 0;
@@ -2184,16 +2225,6 @@ static int cg_mixed(bhip_ctx* c, double* x, const double* b, double eps, int nit
     if ((rc = more(nround, false))) return rc;
     if ((rc = outer_step())) return rc;
     nround = std::min(nround * 2, nbat);
-  }
-  // the solution: one system tells the caller which buffer holds it (the
-  // host has just read its state); an ensemble gathers every system's into x
-  *xsol = x;
-  if (c->nsys == 1) {
-    if (sh[0].xpar) *xsol = c->vx2.p;
-  } else {
-    hipLaunchKernelGGL(k_xpar_gather, dim3(L.nblocks), dim3(kThreads), 0, c->stream, x,
-                       c->vx2.p, c->node_start.p, sys_ptr(c->block_sys), c->st_p);
-    HIP_OK(hipGetLastError());
   }
   int maxin = 0, maxout = 0;
   long long sumin = 0, sumout = 0;
@@ -2291,7 +2322,27 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
     if (precon % 2 == 0 || precon % 3 == 0) c->typdia = 'I';
   }
   int rc;
-  double* xsol = c->vx.p;  // internal-order buffer holding the solution
+  // X <- X*D (solve.f:637-641) and B <- D*B (as the reference leaves it),
+  // back to the caller's order or copied as they are (internal). A mixed
+  // solve of one system reads X from the solution buffer its state names;
+  // an ensemble's first gathers each system's into vx.
+  const bool mixed_cg = cfg->slv == 1 && mixed;
+  auto tail = [&]() -> int {
+    const int32_t* idx = internal ? nullptr : c->perm.p;
+    const SolveState* stx = nullptr;
+    if (mixed_cg && c->nsys == 1) {
+      stx = c->st_p;
+    } else if (mixed_cg) {
+      hipLaunchKernelGGL(k_xpar_gather, dim3(L.nblocks), dim3(kThreads), 0, c->stream, c->vx.p,
+                         c->vx2.p, c->node_start.p, sys_ptr(c->block_sys), c->st_p);
+    }
+    hipLaunchKernelGGL(k_x_out, dim3(g), dim3(kThreads), 0, c->stream, x_dev, c->vx.p,
+                       c->vx2.p, stx, diag ? c->vdinv.p : nullptr, idx, n);
+    hipLaunchKernelGGL(k_x_out, dim3(g), dim3(kThreads), 0, c->stream, b_dev, c->vb.p,
+                       nullptr, nullptr, nullptr, idx, n);
+    HIP_OK(hipGetLastError());
+    return BHIP_OK;
+  };
   if (cfg->slv == 6) {
     rc = bhip::cgstab_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
   } else if (cfg->slv == 7 && mixed) {
@@ -2303,28 +2354,20 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
     double ie = cfg->inner_eps > 0 ? cfg->inner_eps : 1e-4;
     int inn = cfg->inner_nitmax > 0 ? cfg->inner_nitmax : std::max(cfg->nitmax, 1);
     rc = cg_mixed(c, c->vx.p, c->vb.p, cfg->eps, std::max(cfg->nitmax, 1), ie, inn, xa32_ready,
-                  info, &xsol);
+                  info, tail);
   } else {
-    rc = cg_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info);
+    rc = cg_fp64(c, c->vx.p, c->vb.p, cfg->eps, cfg->nitmax, info, tail);
   }
   if (rc != BHIP_OK && rc != BHIP_MAXITER && rc != BHIP_ZERO_RHS) return rc;
-  // X <- X*D (solve.f:637-641), back to caller order; B <- D*B as the reference leaves it
-  if (internal) {
-    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev, xsol,
-                       diag ? c->vdinv.p : nullptr, n);
-    hipLaunchKernelGGL(k_copy_scaled, dim3(g), dim3(kThreads), 0, c->stream, b_dev, c->vb.p,
-                       nullptr, n);
-  } else {
-    if (diag)
-      hipLaunchKernelGGL(k_scatter_scaled, dim3(g), dim3(kThreads), 0, c->stream, x_dev,
-                         xsol, c->vdinv.p, c->perm.p, n);
-    else
-      hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, x_dev,
-                         xsol, c->perm.p, n);
-    hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, b_dev,
-                       c->vb.p, c->perm.p, n);
+  // the CG drivers wrote the results before their last wait, unless a
+  // system with a zero right-hand side has since had its X set to 0
+  const bool cg = cfg->slv == 1;
+  bool zero_any = false;
+  for (int sy = 0; cg && sy < c->nsys; ++sy) zero_any = zero_any || c->st_host[sy].zero_rhs;
+  if (!cg || zero_any) {
+    const int rt = tail();
+    if (rt) return rt;
   }
-  HIP_OK(hipGetLastError());
   return rc;
 }
 

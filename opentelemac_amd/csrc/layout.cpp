@@ -90,6 +90,17 @@ std::string build_layout(int npoin, int nelem, int ndp, const int32_t* ikle,
     }
   for (int i = 0; i < npoin; ++i) L.iperm[L.perm[i]] = i;
 
+  // sigma-mesh prisms (top node k + 3 = bottom node k one plane up, caller
+  // node p * npoin2 + i): columns are numbered contiguously, so the EBE
+  // product reads the three bottom ids of a prism and derives the top ones
+  if (ndp == 6 && nplan > 1) {
+    bool cols = true;
+    for (int e = 0; e < nelem && cols; ++e)
+      for (int k = 0; k < 3; ++k)
+        if (ikle[(long)(k + 3) * nelem + e] != ikle[(long)k * nelem + e] + npoin2) cols = false;
+    L.col_ids = cols;
+  }
+
   int nb = block_nodes > 0 ? block_nodes : 1024;
   if (nplan > 1) nb = std::max(nplan, (nb / nplan) * nplan);
   L.block_nodes = nb;
@@ -138,15 +149,25 @@ std::string build_layout(int npoin, int nelem, int ndp, const int32_t* ikle,
         }
       }
     // local ids: owned first (internal order), halo after in first-seen order
+    // (whole columns at a time for column-numbered prisms, plane by plane)
     int nloc = nown;
     for (int e : belems)
       for (int k = 0; k < ndp; ++k) {
         int g = L.iperm[ikle[(long)k * nelem + e]];
         if (g >= n0 && g < n1) continue;
         if (node_stamp[g] != b) {
-          node_stamp[g] = b;
-          node_local[g] = nloc++;
-          L.halo.push_back(g);
+          if (L.col_ids) {
+            const int c0 = g - g % nplan;
+            for (int q = c0; q < c0 + nplan; ++q) {
+              node_stamp[q] = b;
+              node_local[q] = nloc++;
+              L.halo.push_back(q);
+            }
+          } else {
+            node_stamp[g] = b;
+            node_local[g] = nloc++;
+            L.halo.push_back(g);
+          }
         }
       }
     if (nloc > 65535) return "block local node count exceeds 16-bit ids";
@@ -312,6 +333,8 @@ std::string validate_layout(const Layout& L, const int32_t* ikle) {
           if (l >= nloc) return "local id out of block range";
           int g = (l < nown) ? n0 + l : L.halo[h0 + l - nown];
           if (L.perm[g] != ikle[(long)k * nelem + e]) return "local id maps to the wrong node";
+          if (L.col_ids && k >= 3 && l != L.lid[(size_t)(k - 3) * L.nslot_pad + s] + 1)
+            return "column ids: top node is not the bottom node's successor";
           if (l < nown) {
             // conflict check: an owned node at most once per colour
             int tag = b * kMaxColors + c;

@@ -32,6 +32,7 @@ struct Layout {
   int max_local = 0;                 // max owned + halo nodes of one block
   int max_owned = 0;
   int64_t halo_total = 0;
+  bool col_ids = false;              // prisms: lid[k + 3][s] == lid[k][s] + 1
   std::vector<int32_t> perm;         // internal node -> caller node
   std::vector<int32_t> iperm;        // caller node -> internal node
   std::vector<int32_t> node_start;   // [nblocks+1] internal node ranges
