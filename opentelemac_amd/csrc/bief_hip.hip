@@ -282,27 +282,6 @@ __device__ inline void slot_apply(const int (&l)[NDP], const T* cf, const T* ys,
 
 // Element data of one colour phase for one thread (its first slot of the
 // phase; phases larger than the workgroup are finished by a loop).
-template <typename T, int NDP, int NC>
-struct PhaseSlot {
-  int l[NDP];
-  T cf[NC];
-  bool valid;
-};
-
-template <typename T, typename TA, int NDP, int NC>
-__device__ inline void load_phase(PhaseSlot<T, NDP, NC>& ps, int c, const int* sh_co, int se,
-                                  const uint16_t* __restrict__ lid,
-                                  const TA* __restrict__ xa, int nslot_pad) {
-  const int s = (c < kMaxColors) ? sh_co[c] + (int)threadIdx.x : se;
-  ps.valid = (c < kMaxColors) && s < sh_co[c + 1] && s < se;
-  if (ps.valid) {
-#pragma unroll
-    for (int k = 0; k < NDP; ++k) ps.l[k] = lid[(size_t)k * nslot_pad + s];
-#pragma unroll
-    for (int q = 0; q < NC; ++q) ps.cf[q] = (T)xa[(size_t)q * nslot_pad + s];
-  }
-}
-
 // local node ids of slot s; column-numbered prisms store all six (other
 // kernels read them) but the product reads the bottom three only
 template <int NDP>
@@ -320,6 +299,26 @@ __device__ inline void load_ids(int (&l)[NDP], const uint16_t* __restrict__ lid,
   }
 #pragma unroll
   for (int k = 0; k < NDP; ++k) l[k] = lid[(size_t)k * stride + s];
+}
+
+template <typename T, int NDP, int NC>
+struct PhaseSlot {
+  int l[NDP];
+  T cf[NC];
+  bool valid;
+};
+
+template <typename T, typename TA, int NDP, int NC>
+__device__ inline void load_phase(PhaseSlot<T, NDP, NC>& ps, int c, const int* sh_co, int se,
+                                  const uint16_t* __restrict__ lid,
+                                  const TA* __restrict__ xa, int nslot_pad, int col_ids) {
+  const int s = (c < kMaxColors) ? sh_co[c] + (int)threadIdx.x : se;
+  ps.valid = (c < kMaxColors) && s < sh_co[c + 1] && s < se;
+  if (ps.valid) {
+    load_ids<NDP>(ps.l, lid, nslot_pad, s, col_ids);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) ps.cf[q] = (T)xa[(size_t)q * nslot_pad + s];
+  }
 }
 
 // node values read by the CG staging passes (global index, R, D, AD, X)
@@ -383,7 +382,8 @@ k_ebe(EbeArgs<T, TA> a) {
   if constexpr (PF > 0) {
 #pragma unroll
     for (int q = 0; q < PF; ++q)
-      load_phase<T, TA, NDP, ncoef>(pipe[q], q, sh_co, se, a.lid, a.xa, a.nslot_pad);
+      load_phase<T, TA, NDP, ncoef>(pipe[q], q, sh_co, se, a.lid, a.xa, a.nslot_pad,
+                                    a.col_ids);
   }
 
   const int n0 = a.node_start[b], n1 = a.node_start[b + 1];
@@ -547,7 +547,8 @@ k_ebe(EbeArgs<T, TA> a) {
             slot_apply<T, NDP, SYMQ, TRANS>(l, cf, ys, acc, nown);
           }
           // refill this pipeline stage with phase c + PF before the barrier
-          load_phase<T, TA, NDP, ncoef>(pipe[q], c + PF, sh_co, se, a.lid, a.xa, a.nslot_pad);
+          load_phase<T, TA, NDP, ncoef>(pipe[q], c + PF, sh_co, se, a.lid, a.xa, a.nslot_pad,
+                                        a.col_ids);
         }
         __syncthreads();
       }
@@ -1279,14 +1280,21 @@ static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typ
     else                                    \
       BHIP_MJ(NDP, SQ, TR, DIAG_ZERO);      \
   } while (0)
+#define BHIP_PF1(NDP, SQ, TR, DG)           \
+  do {                                      \
+    if (mj > 0)                             \
+      BHIP_LAUNCH(NDP, SQ, TR, DG, 1);      \
+    else                                    \
+      BHIP_LAUNCH(NDP, SQ, TR, DG, 0);      \
+  } while (0)
 #define BHIP_DIAG0(NDP, SQ, TR)              \
   do {                                       \
     if (diag == DIAG_Q)                      \
-      BHIP_LAUNCH(NDP, SQ, TR, DIAG_Q, 0);   \
+      BHIP_PF1(NDP, SQ, TR, DIAG_Q);         \
     else if (diag == DIAG_ONE)               \
-      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ONE, 0); \
+      BHIP_PF1(NDP, SQ, TR, DIAG_ONE);       \
     else                                     \
-      BHIP_LAUNCH(NDP, SQ, TR, DIAG_ZERO, 0); \
+      BHIP_PF1(NDP, SQ, TR, DIAG_ZERO);      \
   } while (0)
   if (c->L.ndp == 3) {
     if (!symq) BHIP_DIAG(3, false, false);
@@ -1298,12 +1306,13 @@ static hipError_t launch_ebe(bhip_ctx* c, EbeArgs<T, TA> a, bool trans, char typ
     else if (trans) BHIP_DIAG(2, true, true);
     else BHIP_DIAG(2, true, false);
   } else {
-    // prisms: 15-30 coefficients per element exceed the register budget
+    // prisms: 15-30 coefficients per element, so at most one phase ahead
     if (!symq) BHIP_DIAG0(6, false, false);
     else if (trans) BHIP_DIAG0(6, true, true);
     else BHIP_DIAG0(6, true, false);
   }
 #undef BHIP_DIAG0
+#undef BHIP_PF1
 #undef BHIP_DIAG
 #undef BHIP_MJ
 #undef BHIP_LAUNCH

@@ -101,6 +101,36 @@ def test_solve_channel_fp64_and_mixed(gpu):
             assert rel <= 1e-6
 
 
+def test_kernel_timers_leave_solves_unchanged(gpu):
+    """The measurement entry points (back-to-back and cold-cache timings of
+    the CG kernels) run on a solved context, refuse a flush buffer that is
+    not a float64 CUDA tensor, and leave the next solve bit-identical."""
+    import torch
+    m = M.channel_mesh()
+    sysd = S.wave_system(m, M.channel_depth(m), dt=1.0)
+    ctx = bief.Context(m)
+
+    def solve(prec):
+        ctx.set_matrix("Q", "S", sysd["da"], sysd["xa"])
+        return ctx.solve(sysd["x0"].copy(), sysd["b"].copy(), slv=1, precon=2, eps=1e-8,
+                         nitmax=5000, precision=prec)
+
+    before = {p: solve(p)[0] for p in (bief.PREC_FP64, bief.PREC_MIXED)}
+    flush = torch.ones(1 << 20, dtype=torch.float64, device=gpu)
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ms_dir, ms_upd = ctx.time_cg_kernels(prec, reps=3)
+        cold = ctx.time_cg_dir_cold(flush, prec, reps=3)
+        assert ms_dir > 0 and ms_upd > 0 and cold > 0, (prec, ms_dir, ms_upd, cold)
+    with pytest.raises(bief.BiefError):
+        ctx.time_cg_dir_cold(flush.float(), bief.PREC_FP64, reps=1)
+    with pytest.raises(bief.BiefError):
+        ctx.time_cg_dir_cold(flush.cpu(), bief.PREC_FP64, reps=1)
+    for p in (bief.PREC_FP64, bief.PREC_MIXED):
+        x, info = solve(p)
+        assert info["converged"] and np.array_equal(x, before[p]), p
+    ctx.close()
+
+
 @pytest.mark.parametrize("nitmax", [1, 2, 3, 7])
 def test_solve_nitmax_matches_oracle(gpu, nitmax):
     """Non-convergence (NITMAX reached): same iterate as the reference's
