@@ -17,7 +17,8 @@ refinement); the FP64 path is timed the same way and reported beside it.
 
 The default run (--workload all) measures every other configuration in the
 same process and adds them to the one JSON line under "configs": the 20k
-channel against the reference's CPU SOLVE, the 2M SUPG tracer system by
+channel against the reference's CPU SOLVE, the headline system in segment
+storage (telemac2d's default MATRIX STORAGE = 3), the 2M SUPG tracer system by
 GMRES(5) (FP64 and mixed), the 500k-node adjoint (gradient/forward cost,
 finite-difference and tangent-linear checks), the 256-member ensemble
 (systems/s, per-system latency) and the 5M-node prism systems; each with
@@ -197,6 +198,7 @@ def main():
         # error), never silently left out
         configs = {}
         for name, fn in (("channel_20k", lambda: bench_channel(args)),
+                         ("estuary_2m_segments", lambda: bench_segments(args, m, sysd)),
                          ("supg_2m", lambda: bench_supg(args, mesh=m)),
                          ("adjoint_500k", lambda: bench_adjoint(args, n_fd=args.adj_fd)),
                          ("ensemble_256", lambda: bench_ensemble(args)),
@@ -914,6 +916,57 @@ def bench_ensemble(args, nsys=256):
     out["sequential_ms_per_round_mixed"] = float(min(seq) * 1e3)
     out["speedup_vs_sequential"] = float(min(seq) * 1e3 / out["mixed"]
                                          ["ms_per_ensemble_step_median"])
+    return out
+
+
+def bench_segments(args, m, sysd):
+    """The headline system in segment storage (MATRIX STORAGE = 3, the
+    default of telemac2d): the element terms assembled per edge as ASSEX3
+    does, 6.0 M segments run as two-node elements of the same kernels; FP64
+    and mixed CG + PRECON 2 to 1e-8, same step as the headline line."""
+    import torch
+    from opentelemac_amd import bief, mesh as M, system as S
+    dev = torch.device("cuda:0")
+    t = time.time()
+    sm, es = M.segment_mesh(m)
+    xs = np.ascontiguousarray(S.ebe_to_segments(sysd["xa"], es, m.ikle, sm.ikle, "S"))
+    ctx = bief.Context(sm, block_nodes=args.block_nodes)
+    log(f"segments {sm.nelem} in {time.time() - t:.1f}s: {ctx.stats()}")
+    ctx.set_matrix("Q", "S", torch.from_numpy(sysd["da"]).to(dev), torch.from_numpy(xs).to(dev))
+    ctx.save_matrix()
+    x00 = torch.from_numpy(sysd["x0"]).to(dev)
+    b0 = torch.from_numpy(sysd["b"]).to(dev)
+    x, b = torch.empty_like(x00), torch.empty_like(b0)
+    ts = ctx.torch_stream()
+    out = {"workload": "the 2M-node estuary system in segment storage (MATRIX STORAGE = 3)",
+           "npoin": m.npoin, "nseg": sm.nelem, "layout": ctx.stats()}
+    nseg, npo = sm.nelem, m.npoin
+    alg = {"fp64": nseg * (8 + 8) + npo * 4 * 8, "mixed": nseg * (4 + 8) + npo * 4 * 4}
+    for name, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
+        def step():
+            with torch.cuda.stream(ts):
+                ctx.restore_matrix()
+                x.copy_(x00)
+                b.copy_(b0)
+                return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec,
+                                 inner_eps=args.inner_eps)[1]
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = [step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms_dir, ms_upd = ctx.time_cg_kernels(prec, reps=50)
+        out[name] = {"ms_per_solve": 1e3 * el / args.steps, "solves_per_s": args.steps / el,
+                     "iterations": infos[-1]["niter"], "outer": infos[-1]["outer_iters"],
+                     "relres": infos[-1]["relres"], "converged": bool(infos[-1]["converged"]),
+                     "ms_dir_kernel": ms_dir, "ms_upd_kernel": ms_upd,
+                     "roofline": _roof(alg[name], ms_dir)}
+        log("segments", name, json.dumps(out[name]))
+    out["mixed_speedup_vs_fp64"] = round(out["fp64"]["ms_per_solve"] /
+                                         out["mixed"]["ms_per_solve"], 3)
+    ctx.close()
     return out
 
 

@@ -35,6 +35,55 @@ def test_prism_matvec_all_ops(gpu, bn, typext, typdia):
         assert np.abs(got - ref).max() <= 1e-13 * (np.abs(ref).max() + 1), op
 
 
+@pytest.mark.parametrize("typext", ["S", "Q"])
+def test_prism_matvec_and_cg_without_column_order(gpu, typext):
+    """Prisms whose IKLE does not list the top nodes as the bottom ones one
+    plane up (planes swapped) take the six-id path of the product (with the
+    element stream prefetched a phase ahead, as in the column path): every
+    MATVEC op and a CG solve, FP64 and mixed, against the oracle."""
+    base, p = _prisms(8, 7, 5, seed=11)
+    sw = M.Mesh(x=p.x, y=p.y, ikle=np.ascontiguousarray(p.ikle[[3, 4, 5, 0, 1, 2]]), z=p.z,
+                nplan=p.nplan, npoin2=p.npoin2, nelem2=p.nelem2)
+    rng = np.random.default_rng(9)
+    nc = 30 if typext == "Q" else 15
+    da = rng.uniform(1, 2, sw.npoin)
+    xa = rng.normal(size=(nc, sw.nelem))
+    ctx = bief.Context(sw, block_nodes=40)
+    ctx.set_matrix("Q", typext, da, xa)
+    A = O.Matrix(da, xa, "Q", typext)
+    for op in bief.OPS:
+        y = rng.normal(size=sw.npoin)
+        x0 = rng.normal(size=sw.npoin)
+        got = ctx.matvec(op, y, x0, c=0.3)
+        ref = O.matvec(op, A, sw.ikle, y, x0, c=0.3)
+        assert np.abs(got - ref).max() <= 1e-13 * (np.abs(ref).max() + 1), op
+    # a symmetric definite system: the MT01PP mass matrix of the prisms, its
+    # pair columns renumbered for the swapped local order (node k -> k +- 3)
+    s2 = np.tile(base.surfac(), p.nplan - 1)
+    t, xp = O.mt01pp(s2, p.z, p.ikle, 1.0)
+    dm = O.assemble_diag(t, p.ikle, p.npoin)
+    pairs = [(i, j) for i in range(6) for j in range(i + 1, 6)]
+    col = {pr: c for c, pr in enumerate(pairs)}
+    xm = np.empty_like(xp)
+    for c, (i, j) in enumerate(pairs):
+        xm[col[tuple(sorted(((i + 3) % 6, (j + 3) % 6)))]] = xp[c]
+    b = rng.normal(size=sw.npoin)
+    Am = O.Matrix(dm, xm, "Q", "S")
+    xr, ir, _, _ = O.solve(Am, sw.ikle, b, np.zeros(sw.npoin), slv=1, precon=2, eps=1e-10,
+                           nitmax=2000)
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        ctx.set_matrix("Q", "S", dm, xm)
+        x, info = ctx.solve(np.zeros(sw.npoin), b.copy(), slv=1, precon=2, eps=1e-10,
+                            nitmax=2000, precision=prec)
+        rel = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+        assert info["converged"]
+        if prec == bief.PREC_FP64:
+            assert rel <= 1e-10 and abs(info["niter"] - ir.niter) <= 1, (rel, info, ir.niter)
+        else:
+            assert rel <= 1e-7, rel
+    ctx.close()
+
+
 @pytest.mark.parametrize("slv", [1, 6, 7])
 def test_prism_solve(gpu, slv):
     base, p = _prisms(9, 8, 6, seed=5)

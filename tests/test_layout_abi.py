@@ -66,3 +66,23 @@ def test_layout_prisms():
     p = M.extrude_prisms(base, 5, np.zeros(base.npoin), np.full(base.npoin, 10.0))
     st = bief.layout_check(p, 100)
     assert st["elem_instances"] >= p.nelem
+
+
+def test_layout_prisms_with_and_without_column_order():
+    """A sigma mesh (top node k + 3 = bottom node k one plane up) is laid out
+    with column-contiguous halo ids, which the product relies on; the same
+    prisms with their planes swapped in IKLE (top node k + 3 one plane down)
+    are not, and lay out with six ids per prism. The layout validator checks
+    each case (including the column property for every slot) before any
+    kernel could run."""
+    base = M.random_small_mesh(9, 7, seed=4)
+    p = M.extrude_prisms(base, 6, np.zeros(base.npoin), np.full(base.npoin, 10.0))
+    swapped = M.Mesh(x=p.x, y=p.y, ikle=np.ascontiguousarray(p.ikle[[3, 4, 5, 0, 1, 2]]),
+                     z=p.z, nplan=p.nplan, npoin2=p.npoin2, nelem2=p.nelem2)
+    for bn in (60, 0):
+        a = bief.layout_check(p, bn)
+        b = bief.layout_check(swapped, bn)
+        # the same blocks and element instances; only the local numbering differs
+        assert (a["nblocks"], a["elem_instances"]) == (b["nblocks"], b["elem_instances"])
+        # whole halo columns: the halo is a multiple of the plane count
+        assert a["halo_nodes"] % p.nplan == 0
