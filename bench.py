@@ -622,7 +622,21 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     sea = S.sea_boundary(m)
     log(f"adjoint mesh {m.npoin} nodes, built in {time.time() - t:.1f}s")
     kw = dict(dt=2.0, amp=0.2, period=600.0, precision=bief.PREC_MIXED, eps=1e-10)
-    model = W.DeviceWaveModel(m, sea, **kw)
+    # node-block size by the autotuner, on the wave system of this mesh (the
+    # mixed inner CG's kernels): 500k nodes make too few 1024-node blocks to
+    # fill 256 compute units
+    from opentelemac_amd import tune
+    wsys = S.wave_system(m, depth, dt=2.0, theta=1.0, inflow=sea)
+
+    def load(c):
+        c.set_matrix("Q", "S", wsys["da"], wsys["xa"])
+        c.solve(wsys["x0"].copy(), wsys["b"].copy(), slv=1, precon=2, eps=1e-10, nitmax=5000,
+                precision=bief.PREC_MIXED)
+    bn, table = tune.block_nodes(lambda b: bief.Context(m, block_nodes=b), load,
+                                 candidates=(256, 512, 1024), precision=bief.PREC_MIXED,
+                                 reps=20)
+    log(f"adjoint block size {bn}: {table}")
+    model = W.DeviceWaveModel(m, sea, block_nodes=bn, **kw)
     tmodel = W.WaveModel(m, sea, **kw)
     dev = model.dev
     rng = np.random.default_rng(args.seed + 2)
@@ -697,7 +711,8 @@ def bench_adjoint(args, npoin=500_000, nsteps=50, checkpoint=10, n_fd=20):
     ms_dir, _ = model.ctx.time_cg_kernels(bief.PREC_MIXED, reps=50)
     roof = _roof(m.nelem * (3 * 4 + 12) + m.npoin * 4 * 4, ms_dir)
     roof["kernel"] = "k_ebe<float,...,STAGE_CGDIR,EPI_CGAD,FIN_CG> (mixed inner CG direction)"
-    return {"roofline": roof,"workload": "wave-equation adjoint, 500k nodes, 50 steps, checkpoints every 10",
+    return {"roofline": roof, "block_nodes": bn, "autotune_ms_per_iteration": table,
+            "workload": "wave-equation adjoint, 500k nodes, 50 steps, checkpoints every 10",
             "npoin": m.npoin, "nelem": m.nelem, "steps": nsteps, "cost": j0,
             "forward_s": t_fwd, "gradient_s": t_grad,
             "adjoint_over_forward": round(t_grad / t_fwd, 3),

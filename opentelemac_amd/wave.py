@@ -450,7 +450,7 @@ class DeviceWaveModel:
     pulls the adjoint state through it step by step."""
 
     def __init__(self, mesh: Mesh, sea, dt=2.0, amp=0.2, period=600.0, hmin_speed=1e-3,
-                 precision=bief.PREC_FP64, eps=1e-10, device="cuda"):
+                 precision=bief.PREC_FP64, eps=1e-10, device="cuda", block_nodes=0):
         if not torch.cuda.is_available():
             raise bief.BiefError("the wave model runs on the GPU")
         self.mesh = mesh
@@ -458,7 +458,10 @@ class DeviceWaveModel:
         self.dt, self.amp, self.period = float(dt), float(amp), float(period)
         self.eps_u = float(hmin_speed)
         self.n = mesh.npoin
-        self.ctx = bief.Context(mesh)
+        # block_nodes: nodes per workgroup of the context (0: the library's
+        # default); a mesh of a few hundred thousand nodes fills the GPU
+        # better with smaller blocks (tune.block_nodes measures it)
+        self.ctx = bief.Context(mesh, block_nodes=block_nodes)
         xe, ye = mesh.xel_yel()
         sea_mask = np.zeros(mesh.npoin, dtype=np.int32)
         sea_mask[np.asarray(sea)] = 1
