@@ -2864,6 +2864,14 @@ __constant__ int kPrismRowPairs[6][5] = {{0, 1, 2, 3, 4},   {0, 5, 6, 7, 8},
                                         {1, 5, 9, 10, 11},  {2, 6, 9, 12, 13},
                                         {3, 7, 10, 12, 14}, {4, 8, 11, 13, 14}};
 
+// inputs of one prism slot, loaded ahead of its colour phase
+struct PrismIn {
+  int s;           // slot; -1 when the thread has none in the phase
+  int l[6];
+  SlotGeo G;
+  double old[15];  // target coefficients before 'M=M+N' (the first 15)
+};
+
 // Prism element matrices per slot, diagonal assembled in LDS colour by colour.
 //   KIND 200: MT01PP mass (mt01pp.f: XMUL*SURFAC/360 combinations of the
 //             column heights floored at 1e-3); nd holds Z per node
@@ -2872,7 +2880,17 @@ __constant__ int kPrismRowPairs[6][5] = {{0, 1, 2, 3, 4},   {0, 5, 6, 7, 8},
 //             above (the caller's II4 = I1 + NPOU0); h_zero cancels the
 //             vertical term, inchyd the hydrostatic inconsistency test, mon
 //             the 'MON' clipping
-template <int KIND>
+// ST 1 copies the records of the block's local nodes (owned, then halo) into
+// LDS once, after the diagonal accumulator (rec_off doubles in), and the
+// slots read them there; ST 0 gathers each slot's six records from global
+// memory. Both give the same bits. A workgroup of the staged MT02PP holds up
+// to 160 KB of LDS, so it may be alone on its CU: each thread loads the inputs
+// of its slot in the next colour (ids, geometry, and the target's current
+// coefficients for 'M=M+N') before it computes the current one, so that their
+// latency overlaps the computation and the barriers. ACC 0 writes the element
+// matrices, 1 adds them to a symmetric target ('M=M+N'), 2 to both halves of
+// a non-symmetric one.
+template <int KIND, int ST, int ACC>
 __global__ void __launch_bounds__(kThreads)
 k_elem_prism(double xmul, const double* __restrict__ surfac,
              const double* __restrict__ xel, const double* __restrict__ yel, int nelem,
@@ -2881,125 +2899,210 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
              int nslot_pad, const int32_t* __restrict__ node_start,
              const int32_t* __restrict__ halo_start, const int32_t* __restrict__ halo,
              const int32_t* __restrict__ color_off, int nblocks, double* __restrict__ xa,
-             double* __restrict__ da, int accumulate, int target_q,
-             const double* __restrict__ geo) {
+             double* __restrict__ da, const double* __restrict__ geo, int rec_off,
+             int col_ids) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* acc = reinterpret_cast<double*>(smem_raw);
+  double* rec = acc + rec_off;
+  constexpr int W = KIND == 200 ? 1 : 4;  // doubles per node record
   const int b = blockIdx.x;
   if (b >= nblocks) return;
   const int n0 = node_start[b], nown = node_start[b + 1] - n0;
   const int h0 = halo_start[b];
-  for (int l = threadIdx.x; l < nown; l += blockDim.x) acc[l] = 0.0;
-  __syncthreads();
   const int32_t* co = color_off + (size_t)b * (kMaxColors + 1);
   const int send = co[kMaxColors];
+  const size_t np = nslot_pad;
+
+  auto load_slot = [&](PrismIn& in, int s) {
+    in.s = s;
+    if (s < 0) return;
+    // MT01PP reads all six ids: its three-id form measured slower (445 against
+    // 382 us on 10.5 M slots), MT02PP's the same either way
+    load_ids<6>(in.l, lid, np, s, KIND == 201 ? col_ids : 0);
+    const int e = geo ? 0 : slot_elem[s];
+    in.G = load_geo(geo, np, s, xel, yel, surfac, nelem, e, KIND != 200);
+    if constexpr (ACC > 0) {
+#pragma unroll
+      for (int q = 0; q < 15; ++q) in.old[q] = xa[(size_t)q * np + s];
+    }
+  };
+  // the thread's first slot of colour c, or -1
+  auto first_slot = [&](int c) {
+    if (c >= kMaxColors || co[c] >= send) return -1;
+    const int s = co[c] + (int)threadIdx.x;
+    return s < co[c + 1] ? s : -1;
+  };
+
+  auto compute_slot = [&](const PrismIn& in) {
+    const int s = in.s;
+    const SlotGeo& G = in.G;
+    int g[6];
+    if constexpr (!ST) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        g[k] = in.l[k] < nown ? n0 + in.l[k] : halo[h0 + in.l[k] - nown];
+    }
+    double cf[15], t[6];
+    if constexpr (KIND == 200) {
+      double zz[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) zz[k] = ST ? rec[in.l[k]] : nd[g[k]];
+      const double coef = G.s * (xmul / 360.0);
+      const double h1 = fmax(zz[3] - zz[0], 1e-3) * coef;
+      const double h2 = fmax(zz[4] - zz[1], 1e-3) * coef;
+      const double h3 = fmax(zz[5] - zz[2], 1e-3) * coef;
+      const double ht = h1 + h2 + h3;
+      cf[3] = cf[6] = h1 + h2 + ht;
+      cf[4] = cf[9] = h1 + h3 + ht;
+      cf[8] = cf[10] = h2 + h3 + ht;
+      cf[2] = 4 * h1 + ht + ht;
+      cf[7] = 4 * h2 + ht + ht;
+      cf[11] = 4 * h3 + ht + ht;
+      cf[0] = cf[12] = cf[3] + cf[3];
+      cf[1] = cf[13] = cf[4] + cf[4];
+      cf[5] = cf[14] = cf[8] + cf[8];
+      t[0] = t[3] = cf[2] + cf[2];
+      t[1] = t[4] = cf[7] + cf[7];
+      t[2] = t[5] = cf[11] + cf[11];
+    } else {
+      // (Z, F, G, H + H_up) of the six nodes, two 16-byte loads each
+      double zz[6], fv[6], gv[6], hs[3];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        double2 a0, a1;
+        if constexpr (ST) {
+          a0 = *reinterpret_cast<const double2*>(rec + 4 * in.l[k]);
+          a1 = *reinterpret_cast<const double2*>(rec + 4 * in.l[k] + 2);
+        } else {
+          a0 = *reinterpret_cast<const double2*>(nd + 4 * (size_t)g[k]);
+          a1 = *reinterpret_cast<const double2*>(nd + 4 * (size_t)g[k] + 2);
+        }
+        zz[k] = a0.x;
+        fv[k] = a0.y;
+        gv[k] = a1.x;
+        if (k < 3) hs[k] = a1.y;
+      }
+      const double hh[3] = {zz[3] - zz[0], zz[4] - zz[1], zz[5] - zz[2]};
+      bool dry = hh[0] < 1e-4 || hh[1] < 1e-4 || hh[2] < 1e-4;
+      if (inchyd)
+        dry = dry || fmax(fmax(zz[0], zz[1]), zz[2]) > fmin(fmin(zz[3], zz[4]), zz[5]);
+      double nf[6], ng[6], nh[3];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        nf[k] = dry ? 0.0 : fv[k] / hh[k % 3];
+        ng[k] = dry ? 0.0 : gv[k] / hh[k % 3];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) nh[k] = dry ? 0.0 : hs[k] / hh[k];
+      const double dd = (xmul / 2880.0) / G.s;
+      const double x2 = G.x2, x3 = G.x3;
+      const double y2 = G.y2, y3 = G.y3;
+      mt02pp_dir(y2 - y3, y3, -y2, nf, hh, zz, dd, cf, false);
+      mt02pp_dir(x3 - x2, -x3, x2, ng, hh, zz, dd, cf, true);
+      const double dz = h_zero ? 0.0 : G.s * (xmul / 6.0);
+      cf[2] = cf[2] - dz * nh[0];
+      cf[7] = cf[7] - dz * nh[1];
+      cf[11] = cf[11] - dz * nh[2];
+      if (mon) {
+#pragma unroll
+        for (int q = 0; q < 15; ++q) cf[q] = xmul > 0.0 ? fmin(cf[q], 0.0) : fmax(cf[q], 0.0);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        double sum = -cf[kPrismRowPairs[k][0]];
+#pragma unroll
+        for (int j = 1; j < 5; ++j) sum = sum - cf[kPrismRowPairs[k][j]];
+        t[k] = sum;
+      }
+    }
+    if constexpr (ACC > 0) {
+#pragma unroll
+      for (int q = 0; q < 15; ++q) xa[(size_t)q * np + s] = in.old[q] + cf[q];
+      // a symmetric element matrix into a non-symmetric ('Q') target goes to
+      // both halves (OM 'M=M+N' of an 'S' N into a 'Q' M)
+      if constexpr (ACC == 2) {
+#pragma unroll
+        for (int q = 0; q < 15; ++q) {
+          double* p = xa + (size_t)(15 + q) * np + s;
+          *p = *p + cf[q];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 15; ++q) xa[(size_t)q * np + s] = cf[q];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (in.l[k] < nown) acc[in.l[k]] += t[k];
+  };
+
+  // prefetch depth in colours (MT01PP's smaller blocks share their CU instead)
+  constexpr int PD = ST && KIND == 201 ? 1 : 0;
+  PrismIn p0, p1;
+  if constexpr (PD >= 1) load_slot(p0, first_slot(0));
+  if constexpr (PD >= 2) load_slot(p1, first_slot(1));
+  for (int l = threadIdx.x; l < nown; l += blockDim.x) acc[l] = 0.0;
+  if constexpr (ST) {
+    // records of the local nodes, kStage per thread in flight at once
+    constexpr int kStage = 4;
+    const int nloc = nown + halo_start[b + 1] - h0;
+    for (int l0 = threadIdx.x; l0 < nloc; l0 += kStage * kThreads) {
+      double2 va[kStage], vb[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int l = l0 + u * kThreads;
+        const int lc = l < nloc ? l : nloc - 1;
+        const size_t gl = lc < nown ? n0 + lc : halo[h0 + lc - nown];
+        if constexpr (W == 1) {
+          va[u].x = nd[gl];
+        } else {
+          const double2* src = reinterpret_cast<const double2*>(nd + 4 * gl);
+          va[u] = src[0];
+          vb[u] = src[1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int l = l0 + u * kThreads;
+        if (l < nloc) {
+          if constexpr (W == 1) {
+            rec[l] = va[u].x;
+          } else {
+            double2* dst = reinterpret_cast<double2*>(rec + 4 * (size_t)l);
+            dst[0] = va[u];
+            dst[1] = vb[u];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
   for (int c = 0; c < kMaxColors; ++c) {
     const int s0 = co[c];
     if (s0 >= send) break;
     const int s1 = co[c + 1];
-    for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-      const int e = geo ? 0 : slot_elem[s];
-      int l[6], g[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        l[k] = lid[(size_t)k * nslot_pad + s];
-        g[k] = l[k] < nown ? n0 + l[k] : halo[h0 + l[k] - nown];
-      }
-      const SlotGeo G = load_geo(geo, nslot_pad, s, xel, yel, surfac, nelem, e, KIND != 200);
-      double cf[15], t[6];
-      if constexpr (KIND == 200) {
-        double zz[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) zz[k] = nd[g[k]];
-        const double coef = G.s * (xmul / 360.0);
-        const double h1 = fmax(zz[3] - zz[0], 1e-3) * coef;
-        const double h2 = fmax(zz[4] - zz[1], 1e-3) * coef;
-        const double h3 = fmax(zz[5] - zz[2], 1e-3) * coef;
-        const double ht = h1 + h2 + h3;
-        cf[3] = cf[6] = h1 + h2 + ht;
-        cf[4] = cf[9] = h1 + h3 + ht;
-        cf[8] = cf[10] = h2 + h3 + ht;
-        cf[2] = 4 * h1 + ht + ht;
-        cf[7] = 4 * h2 + ht + ht;
-        cf[11] = 4 * h3 + ht + ht;
-        cf[0] = cf[12] = cf[3] + cf[3];
-        cf[1] = cf[13] = cf[4] + cf[4];
-        cf[5] = cf[14] = cf[8] + cf[8];
-        t[0] = t[3] = cf[2] + cf[2];
-        t[1] = t[4] = cf[7] + cf[7];
-        t[2] = t[5] = cf[11] + cf[11];
-      } else {
-        // (Z, F, G, H + H_up) of the six nodes, two 16-byte loads each
-        double zz[6], fv[6], gv[6], hs[3];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          const double2 a0 = *reinterpret_cast<const double2*>(nd + 4 * (size_t)g[k]);
-          const double2 a1 = *reinterpret_cast<const double2*>(nd + 4 * (size_t)g[k] + 2);
-          zz[k] = a0.x;
-          fv[k] = a0.y;
-          gv[k] = a1.x;
-          if (k < 3) hs[k] = a1.y;
-        }
-        const double hh[3] = {zz[3] - zz[0], zz[4] - zz[1], zz[5] - zz[2]};
-        bool dry = hh[0] < 1e-4 || hh[1] < 1e-4 || hh[2] < 1e-4;
-        if (inchyd)
-          dry = dry || fmax(fmax(zz[0], zz[1]), zz[2]) > fmin(fmin(zz[3], zz[4]), zz[5]);
-        double nf[6], ng[6], nh[3];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          nf[k] = dry ? 0.0 : fv[k] / hh[k % 3];
-          ng[k] = dry ? 0.0 : gv[k] / hh[k % 3];
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) nh[k] = dry ? 0.0 : hs[k] / hh[k];
-        const double dd = (xmul / 2880.0) / G.s;
-        const double x2 = G.x2, x3 = G.x3;
-        const double y2 = G.y2, y3 = G.y3;
-        mt02pp_dir(y2 - y3, y3, -y2, nf, hh, zz, dd, cf, false);
-        mt02pp_dir(x3 - x2, -x3, x2, ng, hh, zz, dd, cf, true);
-        const double dz = h_zero ? 0.0 : G.s * (xmul / 6.0);
-        cf[2] = cf[2] - dz * nh[0];
-        cf[7] = cf[7] - dz * nh[1];
-        cf[11] = cf[11] - dz * nh[2];
-        if (mon) {
-#pragma unroll
-          for (int q = 0; q < 15; ++q) cf[q] = xmul > 0.0 ? fmin(cf[q], 0.0) : fmax(cf[q], 0.0);
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          double sum = -cf[kPrismRowPairs[k][0]];
-#pragma unroll
-          for (int j = 1; j < 5; ++j) sum = sum - cf[kPrismRowPairs[k][j]];
-          t[k] = sum;
-        }
-      }
-      if (accumulate) {
-#pragma unroll
-        for (int q = 0; q < 15; ++q) {
-          double* p = xa + (size_t)q * nslot_pad + s;
-          *p = *p + cf[q];
-        }
-        // a symmetric element matrix into a non-symmetric ('Q') target goes to
-        // both halves (OM 'M=M+N' of an 'S' N into a 'Q' M)
-        if (target_q) {
-#pragma unroll
-          for (int q = 0; q < 15; ++q) {
-            double* p = xa + (size_t)(15 + q) * nslot_pad + s;
-            *p = *p + cf[q];
-          }
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 15; ++q) xa[(size_t)q * nslot_pad + s] = cf[q];
-      }
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-        if (l[k] < nown) acc[l[k]] += t[k];
+    PrismIn cur;
+    if constexpr (PD == 2) {
+      cur = p0;
+      p0 = p1;
+      load_slot(p1, first_slot(c + 2));
+    } else if constexpr (PD == 1) {
+      cur = p0;
+      load_slot(p0, first_slot(c + 1));
+    } else {
+      load_slot(cur, first_slot(c));
+    }
+    if (cur.s >= 0) compute_slot(cur);
+    // slots beyond the first kThreads of a large colour
+    for (int s = s0 + threadIdx.x + kThreads; s < s1; s += kThreads) {
+      PrismIn in;
+      load_slot(in, s);
+      compute_slot(in);
     }
     __syncthreads();
   }
   for (int l = threadIdx.x; l < nown; l += blockDim.x) {
-    if (accumulate)
+    if constexpr (ACC > 0)
       da[n0 + l] += acc[l];
     else
       da[n0 + l] = acc[l];
@@ -3319,18 +3422,41 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
   // node fields in the context's order: Z alone (MT01PP) or MT02PP's record
   const double* nd = kind == 200 ? node_in(c, 0, z) : node_in_prism4(c, z, f, g, h, npou0);
   if (!nd) return BHIP_EDEVICE;
-  const size_t lds = (size_t)L.max_owned * sizeof(double);
-#define BHIP_PRISM(KIND)                                                                       \
-  hipLaunchKernelGGL(k_elem_prism<KIND>, dim3(L.nblocks), dim3(kThreads), lds, c->stream, xmul, \
-                     dS.p, dX.p, dY.p, ne, nd, h_zero, inchyd, mon, c->slot_elem.p, c->lid.p,   \
-                     L.nslot_pad, c->node_start.p, c->halo_start.p, c->halo.p,                  \
-                     c->color_off.p, L.nblocks, c->xa.p, c->da.p, accumulate, target_q,         \
-                     use_geo ? c->geo.p : nullptr)
-  if (kind == 200)
-    BHIP_PRISM(200);
-  else
-    BHIP_PRISM(201);
-#undef BHIP_PRISM
+  // LDS: the diagonal accumulator, then (when the block's records fit the
+  // device's per-workgroup LDS) the records of its local nodes
+  const int rec_off = (L.max_owned + 1) & ~1;
+  const size_t lds_acc = (size_t)L.max_owned * sizeof(double);
+  const size_t lds_rec =
+      ((size_t)rec_off + (size_t)(kind == 200 ? 1 : 4) * L.max_local) * sizeof(double);
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+      hipSuccess)
+    return BHIP_EDEVICE;
+  const bool staged = lds_rec <= (size_t)lds_max;
+  auto launch = [&](auto kern, size_t lds) -> int {
+    if (lds > 65536)
+      HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(L.nblocks), dim3(kThreads), lds, c->stream, xmul, dS.p, dX.p,
+                       dY.p, ne, nd, h_zero, inchyd, mon, c->slot_elem.p, c->lid.p, L.nslot_pad,
+                       c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks,
+                       c->xa.p, c->da.p, use_geo ? c->geo.p : nullptr, rec_off, (int)L.col_ids);
+    return BHIP_OK;
+  };
+  auto pick = [&](auto k200, auto k201, size_t lds) {
+    return kind == 200 ? launch(k200, lds) : launch(k201, lds);
+  };
+  const int acc_kind = !accumulate ? 0 : target_q ? 2 : 1;
+#define BHIP_PRISM_PICK(ST, ACC) \
+  pick(k_elem_prism<200, ST, ACC>, k_elem_prism<201, ST, ACC>, ST ? lds_rec : lds_acc)
+  const int ls = staged ? (acc_kind == 0   ? BHIP_PRISM_PICK(1, 0)
+                           : acc_kind == 1 ? BHIP_PRISM_PICK(1, 1)
+                                           : BHIP_PRISM_PICK(1, 2))
+                        : (acc_kind == 0   ? BHIP_PRISM_PICK(0, 0)
+                           : acc_kind == 1 ? BHIP_PRISM_PICK(0, 1)
+                                           : BHIP_PRISM_PICK(0, 2));
+#undef BHIP_PRISM_PICK
+  if (ls != BHIP_OK) return ls;
   HIP_OK(hipGetLastError());
   if (!use_geo) HIP_OK(hipStreamSynchronize(c->stream));
   if (input_done(c)) return BHIP_EDEVICE;

@@ -273,3 +273,44 @@ def test_prism_device_geometry_matches_caller_arrays(gpu):
     assert np.array_equal(da_a, da_b) and np.array_equal(xa_a, xa_b)
     with pytest.raises(bief.BiefError):
         a.matrix_mt01pp(0.5, None, p.z)  # no geometry kept on this context
+
+
+@pytest.mark.parametrize("bn", [500, 1000])
+def test_prism_element_matrices_records_staged_and_gathered(gpu, bn):
+    """The prism element kernels copy each block's node records (32 B) into LDS
+    after the diagonal accumulator (8 B per owned node) when they fit the
+    160 KB a gfx950 workgroup holds, and gather them from global memory
+    otherwise. A 500-plane mesh cut into one-column blocks needs 132 KB (staged,
+    above the 64 KB default), into two-column blocks 184 KB (gathered); either
+    way MT01PP and MT02PP agree with the oracle, and the off-diagonal terms,
+    computed per element from the same values, give the same bits."""
+    base = M.random_small_mesh(5, 4, seed=6)
+    rng = np.random.default_rng(3)
+    bot = -rng.uniform(1, 5, base.npoin)
+    p = M.extrude_prisms(base, 500, bot, rng.uniform(-0.2, 0.2, base.npoin))
+    st = bief.layout_check(p, bn)
+    need = (((st["max_owned"] + 1) // 2) * 2 + 4 * st["max_local"]) * 8
+    assert (64 * 1024 < need <= 160 * 1024) if bn == 500 else need > 160 * 1024, st
+    eb = np.arange(p.nelem) % base.nelem
+    xe, ye = base.xel_yel()
+    xel, yel, s = np.ascontiguousarray(xe[:, eb]), np.ascontiguousarray(ye[:, eb]), \
+        base.surfac()[eb]
+    f, g = rng.uniform(0.5, 2, p.npoin), rng.uniform(0.5, 2, p.npoin)
+    h = 10 ** rng.uniform(-4, -2, p.npoin)
+    ctx = bief.Context(p, block_nodes=bn)
+    ctx.matrix_mt01pp(0.5, s, p.z)
+    _, da, xa = ctx.get_matrix()
+    t1, x1 = O.mt01pp(s, p.z, p.ikle, 0.5)
+    assert _close(xa, x1, 1e-14) and _close(da, O.assemble_diag(t1, p.ikle, p.npoin), 1e-13)
+    out = {}
+    for mon in (0, 1):
+        ctx.matrix_mt02pp(-0.7, xel, yel, s, p.z, f, g, h, base.npoin, False, True, bool(mon))
+        _, da, xa = ctx.get_matrix()
+        t, xm = O.mt02pp(f, g, h, xel, yel, p.z, s, p.ikle, base.npoin, -0.7, False, True,
+                         bool(mon))
+        assert _close(xa, xm, 1e-12) and _close(da, O.assemble_diag(t, p.ikle, p.npoin), 1e-12)
+        out[mon] = xa
+    other = bief.Context(p, block_nodes=1500 - bn)
+    for mon in (0, 1):
+        other.matrix_mt02pp(-0.7, xel, yel, s, p.z, f, g, h, base.npoin, False, True, bool(mon))
+        assert np.array_equal(other.get_matrix()[2], out[mon])
