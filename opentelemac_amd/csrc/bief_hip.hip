@@ -1373,9 +1373,9 @@ static int ctx_setup(std::unique_ptr<bhip_ctx>& c, bhip_ctx** out) {
   }
   HIP_OK(hipGetDevice(&c->device));
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  // the element kernels need more than the default dynamic LDS limit only
-  // for large blocks; raise it for every instantiation lazily is not
-  // possible in HIP, so blocks are sized to stay within 64 KiB above.
+  // the EBE product's many instantiations keep to the default 64 KiB of
+  // dynamic LDS (checked above); only the prism element kernels raise their
+  // limit, per launch, for their node records
   const int n = L.npoin;
   c->grid_nodes = grid_for(n);
   HIP_OK(c->node_start.alloc(L.node_start.size()));
