@@ -464,7 +464,8 @@ class Context:
             z = _host_n(z, self.npoin, "z")
             _check(lib().bhip_matrix_mt01pp(self._h, float(xmul), _ptr(s), z.ctypes.data,
                                             int(accumulate)), "bhip_matrix_mt01pp")
-        self.typdia, self.typext = "Q", "S"
+        # 'M=M+N' keeps a non-symmetric target non-symmetric (both halves)
+        self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     def matrix_mt02pp(self, xmul, xel, yel, surfac, z, f, g, h, npou0, h_zero=False,
                       inchyd=False, mon=False, accumulate=False):
@@ -490,7 +491,7 @@ class Context:
             _check(lib().bhip_matrix_mt02pp(self._h, float(xmul), *[_ptr(a) for a in arrs],
                                             int(h_zero), int(npou0), int(inchyd), int(mon),
                                             int(accumulate)), "bhip_matrix_mt02pp")
-        self.typdia, self.typext = "Q", "S"
+        self.typdia, self.typext = "Q", ("Q" if accumulate and self.typext == "Q" else "S")
 
     # ---- MATVEC ---------------------------------------------------------
     def _on_own_stream(self):

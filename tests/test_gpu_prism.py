@@ -314,3 +314,20 @@ def test_prism_element_matrices_records_staged_and_gathered(gpu, bn):
     for mon in (0, 1):
         other.matrix_mt02pp(-0.7, xel, yel, s, p.z, f, g, h, base.npoin, False, True, bool(mon))
         assert np.array_equal(other.get_matrix()[2], out[mon])
+    # 'M=M+N' into a symmetric and a non-symmetric target (both halves),
+    # MT01PP and MT02PP, in the same layout
+    da0 = rng.uniform(1, 2, p.npoin)
+    for typext, nc in (("S", 15), ("Q", 30)):
+        xa0 = rng.normal(size=(nc, p.nelem))
+        ctx.set_matrix("Q", typext, da0, xa0)
+        ctx.matrix_mt01pp(0.5, s, p.z, accumulate=True)
+        ctx.matrix_mt02pp(-0.7, xel, yel, s, p.z, f, g, h, base.npoin, False, True, False,
+                          accumulate=True)
+        td, da, xa = ctx.get_matrix()
+        t2, x2 = O.mt02pp(f, g, h, xel, yel, p.z, s, p.ikle, base.npoin, -0.7, False, True,
+                          False)
+        want = xa0 + np.vstack([x1 + x2] * (nc // 15))
+        assert td == "Q" and xa.shape == want.shape
+        assert _close(xa, want, 1e-12), typext
+        assert _close(da, da0 + O.assemble_diag(t1, p.ikle, p.npoin)
+                      + O.assemble_diag(t2, p.ikle, p.npoin), 1e-12), typext
