@@ -157,8 +157,14 @@ int bhip_matrix_set_dev(bhip_ctx* ctx, char typdia, char typext,
 int bhip_matrix_save(bhip_ctx* ctx);
 int bhip_matrix_restore(bhip_ctx* ctx);
 /* Reads the context's (possibly preconditioned) matrix back in BIEF layout;
- * *typdia receives the current TYPDIA. */
+ * *typdia receives the current TYPDIA. xa must hold ncoef*nelem values, ncoef
+ * as bhip_matrix_info reports it. */
 int bhip_matrix_get(bhip_ctx* ctx, char* typdia, double* da, double* xa);
+/* The context matrix's current TYPDIA, TYPEXT and number of off-diagonal
+ * columns (the A%TYPDIA/A%TYPEXT a BIEF caller reads after MATRIX or OM; the
+ * element-matrix and OM entry points change them). Any pointer may be NULL.
+ * No single counterpart in the reference: BIEF keeps these in BIEF_OBJ. */
+int bhip_matrix_info(const bhip_ctx* ctx, char* typdia, char* typext, int* ncoef);
 
 /* ---- MATVEC ------------------------------------------------------------ */
 

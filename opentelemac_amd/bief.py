@@ -90,6 +90,8 @@ def lib():
         L.bhip_matrix_set.argtypes = [_P, _C, _C, _P, _P]
         L.bhip_matrix_set_dev.argtypes = [_P, _C, _C, _P, _P]
         L.bhip_matrix_get.argtypes = [_P, ctypes.c_char_p, _P, _P]
+        L.bhip_matrix_info.argtypes = [_P, ctypes.c_char_p, ctypes.c_char_p,
+                                       ctypes.POINTER(ctypes.c_int)]
         L.bhip_matvec.argtypes = [_P, _I, _D, _P, _P]
         L.bhip_matvec_dev.argtypes = [_P, _I, _D, _P, _P]
         L.bhip_solve.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P, ctypes.POINTER(SolveInfo)]
@@ -298,9 +300,16 @@ class Context:
         """OM 'M=N' back from the saved copy (queued on the context stream)."""
         _check(lib().bhip_matrix_restore(self._h), "bhip_matrix_restore")
 
+    def matrix_info(self):
+        """(TYPDIA, TYPEXT, number of off-diagonal columns) of the context's
+        matrix, as the library holds them."""
+        td, te, nc = ctypes.create_string_buffer(1), ctypes.create_string_buffer(1), ctypes.c_int()
+        _check(lib().bhip_matrix_info(self._h, td, te, ctypes.byref(nc)), "bhip_matrix_info")
+        return td.raw[:1].decode(), te.raw[:1].decode(), nc.value
+
     def get_matrix(self):
-        npair = self.ndp * (self.ndp - 1) // 2
-        ncoef = {"Q": 2 * npair, "S": npair, "0": 0}[self.typext]
+        # sized by what the library holds
+        _, self.typext, ncoef = self.matrix_info()
         da = np.empty(self.npoin)
         xa = np.empty((max(ncoef, 1), self.nelem))
         td = ctypes.create_string_buffer(1)

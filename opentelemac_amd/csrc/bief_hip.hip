@@ -1658,6 +1658,15 @@ int bhip_matrix_restore(bhip_ctx* c) {
   return BHIP_OK;
 }
 
+int bhip_matrix_info(const bhip_ctx* c, char* typdia, char* typext, int* ncoef) {
+  if (!c) return BHIP_EBADARG;
+  const bool m = c->have_matrix;
+  if (typdia) *typdia = m ? c->typdia : '0';
+  if (typext) *typext = m ? c->typext : '0';
+  if (ncoef) *ncoef = m ? c->ncoef : 0;
+  return BHIP_OK;
+}
+
 int bhip_matrix_get(bhip_ctx* c, char* typdia, double* da, double* xa) {
   if (!c || !c->have_matrix) return BHIP_EBADARG;
   const Layout& L = c->L;

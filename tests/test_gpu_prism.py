@@ -323,6 +323,7 @@ def test_prism_element_matrices_records_staged_and_gathered(gpu, bn):
         ctx.matrix_mt01pp(0.5, s, p.z, accumulate=True)
         ctx.matrix_mt02pp(-0.7, xel, yel, s, p.z, f, g, h, base.npoin, False, True, False,
                           accumulate=True)
+        assert ctx.matrix_info() == ("Q", typext, nc)
         td, da, xa = ctx.get_matrix()
         t2, x2 = O.mt02pp(f, g, h, xel, yel, p.z, s, p.ikle, base.npoin, -0.7, False, True,
                           False)
