@@ -11,7 +11,8 @@ relative residual of the preconditioned system (GRACJG's stop rule).
 
 A step = restore the assembled EBE matrix from its device copy (SOLVE
 consumes it, as BIEF's does; telemac2d keeps a copy, propag.F:1643) + one
-SOLVE, data resident in HBM.
+SOLVE, data resident in HBM. The restore is fused into SOLVE's diagonal
+preconditioning (bhip_solve_dev_restored), which reads the saved copy.
 `value` is mixed-precision solves/s (FP32 EBE CG inside FP64 iterative
 refinement); the FP64 path is timed the same way and reported beside it.
 
@@ -239,14 +240,14 @@ def bench_estuary(args, dev, m, sysd):
     ts = ctx.torch_stream()
 
     def step(precision):
-        # SOLVE consumes A (D A D in place): restore the assembled copy first,
-        # as telemac2d copies AM1 before its SOLVE (propag.F:1643)
+        # SOLVE consumes A (D A D in place): it starts from the assembled copy,
+        # as telemac2d copies AM1 before its SOLVE (propag.F:1643); the copy is
+        # made by the preconditioning itself (restore=True)
         with torch.cuda.stream(ts):
-            ctx.restore_matrix()
             x.copy_(x00)
             b.copy_(b0)
             _, info = ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000,
-                                precision=precision, inner_eps=args.inner_eps)
+                                precision=precision, inner_eps=args.inner_eps, restore=True)
         return info
 
     results = {}
@@ -551,11 +552,10 @@ def bench_prisms(args):
         res = {}
         for pname, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
             def step():
-                ctx.restore_matrix()
                 x.copy_(x0)
                 b.copy_(bd)
                 return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=20000, precision=prec,
-                                 inner_eps=args.inner_eps)[1]
+                                 inner_eps=args.inner_eps, restore=True)[1]
             for _ in range(args.warmup):
                 step()
             torch.cuda.synchronize()
@@ -783,11 +783,10 @@ def bench_supg(args, dt=10.0, nu=0.1, mesh=None):
     sols = {}
     for name, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
         def step():
-            ctx.restore_matrix()
             xx.copy_(x0)
             bb.copy_(b0)
             return ctx.solve(xx, bb, slv=7, precon=2, krylov=5, eps=1e-8, nitmax=2000,
-                             precision=prec)[1]
+                             precision=prec, restore=True)[1]
 
         for _ in range(args.warmup):
             step()
@@ -874,10 +873,10 @@ def bench_ensemble(args, nsys=256):
            "workgroup_threads": 256, "eps": 1e-8}
     for name, prec in (("mixed", bief.PREC_MIXED), ("fp64", bief.PREC_FP64)):
         def step():
-            ens.restore_matrix()
             x.copy_(x0)
             b.copy_(b0)
-            return ens.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec)[1]
+            return ens.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec,
+                             restore=True)[1]
 
         for _ in range(args.warmup):
             step()
@@ -960,11 +959,10 @@ def bench_segments(args, m, sysd):
     for name, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
         def step():
             with torch.cuda.stream(ts):
-                ctx.restore_matrix()
                 x.copy_(x00)
                 b.copy_(b0)
                 return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec,
-                                 inner_eps=args.inner_eps)[1]
+                                 inner_eps=args.inner_eps, restore=True)[1]
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
@@ -1009,10 +1007,10 @@ def bench_channel(args):
     sols = {}
     for name, prec in (("fp64", bief.PREC_FP64), ("mixed", bief.PREC_MIXED)):
         def step():
-            ctx.restore_matrix()
             x.copy_(x0)
             b.copy_(b0)
-            return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec)[1]
+            return ctx.solve(x, b, slv=1, precon=2, eps=1e-8, nitmax=5000, precision=prec,
+                             restore=True)[1]
         for _ in range(3):
             step()
         torch.cuda.synchronize()

@@ -191,6 +191,13 @@ int bhip_solve_dev(bhip_ctx* ctx, const bhip_solver_cfg* cfg, double* x_dev,
  * permutation at the boundary. */
 int bhip_solve_dev_internal(bhip_ctx* ctx, const bhip_solver_cfg* cfg,
                             double* x_dev, double* b_dev, bhip_solve_info* info);
+/* bhip_matrix_restore followed by bhip_solve_dev, as telemac2d copies the
+ * assembled matrix before each SOLVE (OM 'M=N', propag.F:1643) since SOLVE
+ * leaves it preconditioned. With diagonal preconditioning the copy is not
+ * made: PRECD1 and D*A*D read the saved matrix and write the context's. The
+ * results, and the matrix left behind, are those of the two calls. */
+int bhip_solve_dev_restored(bhip_ctx* ctx, const bhip_solver_cfg* cfg,
+                            double* x_dev, double* b_dev, bhip_solve_info* info);
 /* bhip_solve_dev for callers that cannot allocate device memory (code under
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc). No
  * counterpart in the reference. It calls no hipMalloc/hipFree: the partial

@@ -114,6 +114,8 @@ def lib():
         L.bhip_matrix_mt02pp.argtypes = [_P, _D, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I]
         L.bhip_solve_dev_internal.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
                                               ctypes.POINTER(SolveInfo)]
+        L.bhip_solve_dev_restored.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
+                                              ctypes.POINTER(SolveInfo)]
         L.bhip_solve_dev_noalloc.argtypes = [_P, ctypes.POINTER(SolverCfg), _P, _P,
                                              ctypes.POINTER(SolveInfo)]
         L.bhip_ctx_perm.argtypes = [_P, _P]
@@ -299,6 +301,7 @@ class Context:
     def restore_matrix(self):
         """OM 'M=N' back from the saved copy (queued on the context stream)."""
         _check(lib().bhip_matrix_restore(self._h), "bhip_matrix_restore")
+        self.typext = getattr(self, "_saved_typext", self.typext)
 
     def matrix_info(self):
         """(TYPDIA, TYPEXT, number of off-diagonal columns) of the context's
@@ -609,9 +612,12 @@ class Context:
 
     # ---- SOLVE ----------------------------------------------------------
     def solve(self, x, b, slv=1, precon=2, krylov=5, nitmax=1000, eps=1e-8,
-              precision=PREC_FP64, inner_eps=1e-4, inner_nitmax=0):
+              precision=PREC_FP64, inner_eps=1e-4, inner_nitmax=0, restore=False):
         """SOLVE: x (initial guess, overwritten), b (overwritten by D*b as in
         the reference); the context's matrix is left preconditioned.
+        restore=True solves with the saved matrix (restore_matrix() then
+        SOLVE; with device vectors and diagonal preconditioning the copy is
+        fused into the preconditioning, bhip_solve_dev_restored).
         Returns (x, info dict, status)."""
         cfg = SolverCfg(slv, precon, krylov, nitmax, eps, precision, inner_eps, inner_nitmax)
         info = SolveInfo()
@@ -619,11 +625,15 @@ class Context:
             _dev_n(x, self.npoin, "solve: x")
             _dev_n(b, self.npoin, "solve: b")
             self._sync_torch()
-            rc = lib().bhip_solve_dev(self._h, ctypes.byref(cfg), x.data_ptr(), b.data_ptr(),
-                                      ctypes.byref(info))
-            _check(rc, "bhip_solve_dev")
+            fn = lib().bhip_solve_dev_restored if restore else lib().bhip_solve_dev
+            rc = fn(self._h, ctypes.byref(cfg), x.data_ptr(), b.data_ptr(), ctypes.byref(info))
+            _check(rc, "bhip_solve_dev_restored" if restore else "bhip_solve_dev")
             self._join_torch()
+            if restore:
+                self.typext = getattr(self, "_saved_typext", self.typext)
         else:
+            if restore:
+                self.restore_matrix()
             xh = _host_n(x, self.npoin, "solve: x")
             bh = _host_n(b, self.npoin, "solve: b")
             if not xh.flags.writeable or xh is x:

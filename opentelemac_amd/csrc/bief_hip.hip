@@ -961,24 +961,29 @@ __global__ void k_to_f32(float* __restrict__ out, const double* __restrict__ in,
 // X and B are read from the caller's vectors (xs, bs; through perm when it
 // is given, i.e. in the caller's node order) and written to the context's
 // internal-order copies x and b, so SOLVE needs no separate gather.
+// da_in and da_out may be the same array (the context's diagonal in place),
+// or the saved copy and the context's diagonal (OM 'M=N' fused into SOLVE)
 __global__ void k_precd1_nodes(double* x, double* b, const double* xs, const double* bs,
-                               const int32_t* __restrict__ perm,
-                               double* __restrict__ da, double* __restrict__ dout, int n,
-                               int precon, int typdia_q, int typdia_i) {
+                               const int32_t* __restrict__ perm, const double* da_in,
+                               double* da_out, double* __restrict__ dout, int n, int precon,
+                               int typdia_q, int typdia_i) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int j = perm ? perm[i] : i;
     const double xv = xs[j], bv = bs[j];
-    double d = da[i];
+    const double a = da_in[i];
+    double d = a;
     if (precon == 5) d = fabs(d);
     d = sqrt(d);
     x[i] = xv * d;
     d = (fabs(d) > 1e-10) ? 1.0 / d : 1.0;
     dout[i] = d;
     if (typdia_q) {
-      double v = da[i] * d;
-      da[i] = v * d;
+      const double v = a * d;
+      da_out[i] = v * d;
     } else if (typdia_i) {
-      da[i] = d * d;
+      da_out[i] = d * d;
+    } else if (da_out != da_in) {
+      da_out[i] = a;
     }
     b[i] = bv * d;
   }
@@ -988,10 +993,12 @@ __global__ void k_precd1_nodes(double* x, double* b, const double* xs, const dou
 // for the pair (i<j) of column c (om1111.f:583-600, om4141.f), per slot. One
 // workgroup per node block: D of its owned and halo nodes is staged in LDS,
 // so the per-slot gathers are LDS reads. xa32 (optional) receives an FP32
-// copy of the result.
+// copy of the result. The coefficients are read from src, which is xa itself
+// or the saved copy (OM 'M=N' fused into SOLVE); each slot's are all loaded
+// before any is stored.
 template <int NDP>
 __global__ void __launch_bounds__(kThreads)
-k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, int ncoef, int nslot_pad,
+k_dmd_slots(const double* src, double* xa, float* __restrict__ xa32, int ncoef, int nslot_pad,
             const uint16_t* __restrict__ lid, const double* __restrict__ d,
             const int32_t* __restrict__ node_start, const int32_t* __restrict__ halo_start,
             const int32_t* __restrict__ halo, const int32_t* __restrict__ color_off,
@@ -1014,25 +1021,29 @@ k_dmd_slots(double* __restrict__ xa, float* __restrict__ xa32, int ncoef, int ns
   const int s0 = color_off[(size_t)b * (kMaxColors + 1)];
   const int s1 = color_off[(size_t)b * (kMaxColors + 1) + kMaxColors];
   constexpr int npair = NDP * (NDP - 1) / 2;
+  const bool two = ncoef > npair;
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    double dv[NDP];
+    double dv[NDP], c1[npair], c2[npair];
 #pragma unroll
     for (int k = 0; k < NDP; ++k) dv[k] = ds[lid[(size_t)k * nslot_pad + s]];
+#pragma unroll
+    for (int q = 0; q < npair; ++q) {
+      c1[q] = src[(size_t)q * nslot_pad + s];
+      c2[q] = two ? src[(size_t)(q + npair) * nslot_pad + s] : 0.0;
+    }
     int q = 0;
 #pragma unroll
     for (int i = 0; i < NDP; ++i)
 #pragma unroll
       for (int j = i + 1; j < NDP; ++j) {
-        double* p = xa + (size_t)q * nslot_pad + s;
         // elements: XM * D(j) * D(i) (om1111.f:583-600); segments (NDP 2):
         // XM * D(GLOSEG1) * D(GLOSEG2) (omseg.f:524-536)
-        const double v1 = NDP == 2 ? (*p) * dv[i] * dv[j] : (*p) * dv[j] * dv[i];
-        *p = v1;
+        const double v1 = NDP == 2 ? c1[q] * dv[i] * dv[j] : c1[q] * dv[j] * dv[i];
+        xa[(size_t)q * nslot_pad + s] = v1;
         if (xa32) xa32[(size_t)q * nslot_pad + s] = (float)v1;
-        if (ncoef > npair) {
-          double* p2 = xa + (size_t)(q + npair) * nslot_pad + s;
-          const double v2 = NDP == 2 ? (*p2) * dv[i] * dv[j] : (*p2) * dv[j] * dv[i];
-          *p2 = v2;
+        if (two) {
+          const double v2 = NDP == 2 ? c2[q] * dv[i] * dv[j] : c2[q] * dv[j] * dv[i];
+          xa[(size_t)(q + npair) * nslot_pad + s] = v2;
           if (xa32) xa32[(size_t)(q + npair) * nslot_pad + s] = (float)v2;
         }
         ++q;
@@ -1204,12 +1215,15 @@ static size_t dmd_lds(const Layout& L) { return ((size_t)L.max_local + 1) * size
 static int grid_for(int n);
 
 // OM 'M=DMD' on the context's coefficients with the node vector d (and the
-// FP32 copy of the result when xa32 is given)
-static hipError_t launch_dmd(bhip_ctx* c, float* xa32, const double* d) {
+// FP32 copy of the result when xa32 is given); the coefficients come from src
+// (default: the context's own)
+static hipError_t launch_dmd(bhip_ctx* c, float* xa32, const double* d,
+                             const double* src = nullptr) {
   const Layout& L = c->L;
+  if (!src) src = c->xa.p;
 #define BHIP_DMD(NDP)                                                                          \
   hipLaunchKernelGGL(k_dmd_slots<NDP>, dim3(L.nblocks), dim3(kThreads), dmd_lds(L), c->stream, \
-                     c->xa.p, xa32, c->ncoef, L.nslot_pad, c->lid.p, d, c->node_start.p,      \
+                     src, c->xa.p, xa32, c->ncoef, L.nslot_pad, c->lid.p, d, c->node_start.p, \
                      c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks)
   if (L.ndp == 2)
     BHIP_DMD(2);
@@ -2288,8 +2302,9 @@ __global__ void k_copy_scaled(double* __restrict__ out, const double* __restrict
 // SOLVE on device vectors, in the caller's node order or (internal) in the
 // context's own order, which skips the permutations.
 static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, double* b_dev,
-                     bhip_solve_info* info, bool internal) {
-  if (!c || !cfg || !x_dev || !b_dev || !c->have_matrix) return BHIP_EBADARG;
+                     bhip_solve_info* info, bool internal, bool from_saved = false) {
+  if (!c || !cfg || !x_dev || !b_dev) return BHIP_EBADARG;
+  if (from_saved ? !c->have_saved : !c->have_matrix) return BHIP_EBADARG;
   bhip_solve_info dummy;
   if (!info) info = &dummy;
   std::memset(info, 0, sizeof(*info));
@@ -2307,6 +2322,22 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
   const int n = L.npoin;
   const int g = c->grid_nodes;
   const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
+  // OM 'M=N' from the saved copy first: with diagonal preconditioning PRECD1
+  // and D A D read the saved copy and write the context's matrix, which
+  // takes the place of the copy; otherwise the copy is made as it is
+  const double* da_src = c->da.p;
+  const double* xa_src = c->xa.p;
+  if (from_saved && diag) {
+    c->typdia = c->typdia_saved;
+    c->typext = c->typext_saved;
+    c->ncoef = c->ncoef_saved;
+    c->have_matrix = true;
+    da_src = c->da_saved.p;
+    xa_src = c->xa_saved.p;
+  } else if (from_saved) {
+    const int rr = bhip_matrix_restore(c);
+    if (rr) return rr;
+  }
   // caller order -> internal order (with diagonal preconditioning the
   // PRECD1 kernel below reads the caller's vectors itself)
   if (!diag && internal) {
@@ -2327,14 +2358,14 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
   if (diag) {
     // PRECD1 (precd1.f:90-149), X and B taken from the caller's vectors
     hipLaunchKernelGGL(k_precd1_nodes, dim3(g), dim3(kThreads), 0, c->stream, c->vx.p, c->vb.p,
-                       x_dev, b_dev, internal ? nullptr : c->perm.p, c->da.p, c->vdinv.p, n,
-                       precon, c->typdia == 'Q', c->typdia == 'I');
+                       x_dev, b_dev, internal ? nullptr : c->perm.p, da_src, c->da.p, c->vdinv.p,
+                       n, precon, c->typdia == 'Q', c->typdia == 'I');
     HIP_OK(hipGetLastError());
     if (c->typdia == 'I') c->typdia = 'Q';
     if (c->ncoef > 0) {
       // the mixed path takes its FP32 copy of D A D in the same pass
       float* x32 = mixed ? c->xa32.p : nullptr;
-      HIP_OK(launch_dmd(c, x32, c->vdinv.p));
+      HIP_OK(launch_dmd(c, x32, c->vdinv.p, xa_src));
       xa32_ready = mixed;
     }
     if (precon % 2 == 0 || precon % 3 == 0) c->typdia = 'I';
@@ -2399,6 +2430,11 @@ int bhip_solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, doubl
 int bhip_solve_dev_internal(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev,
                             double* b_dev, bhip_solve_info* info) {
   return solve_dev(c, cfg, x_dev, b_dev, info, true);
+}
+
+int bhip_solve_dev_restored(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev,
+                            double* b_dev, bhip_solve_info* info) {
+  return solve_dev(c, cfg, x_dev, b_dev, info, false, true);
 }
 
 int bhip_solve_dev_noalloc(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev,

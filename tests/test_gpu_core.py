@@ -343,3 +343,95 @@ def test_cgstab_gmres_nitmax(gpu, slv):
     assert info["status"] == "maxiter" and info["niter"] == info_ref.niter == 3
     rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
     assert rel <= 1e-11
+
+
+def _restored_pair(ctxs, x0, b0, **kw):
+    """Solve on two contexts holding the same saved matrix (and another one
+    current): restore_matrix() then SOLVE on the first, the fused
+    solve(restore=True) on the second. Returns both (x, b, info, matrix)."""
+    import torch
+
+    out = []
+    for k, ctx in enumerate(ctxs):
+        x = torch.from_numpy(x0.copy()).to("cuda:0")
+        b = torch.from_numpy(b0.copy()).to("cuda:0")
+        if k == 0:
+            ctx.restore_matrix()
+            x, info = ctx.solve(x, b, **kw)
+        else:
+            x, info = ctx.solve(x, b, restore=True, **kw)
+        torch.cuda.synchronize()
+        out.append((x.cpu().numpy(), b.cpu().numpy(), info, ctx.get_matrix()))
+    return out
+
+
+@pytest.mark.parametrize("typext", ["S", "Q"])
+@pytest.mark.parametrize("case", [(1, 2, bief.PREC_FP64), (1, 2, bief.PREC_MIXED),
+                                  (1, 0, bief.PREC_FP64), (1, 5, bief.PREC_MIXED),
+                                  (6, 2, bief.PREC_FP64), (7, 2, bief.PREC_FP64),
+                                  (7, 2, bief.PREC_MIXED)])
+def test_solve_restored_equals_restore_then_solve(gpu, typext, case):
+    """bhip_solve_dev_restored (the saved matrix copied back by PRECD1 and
+    D A D themselves) gives the bits of bhip_matrix_restore followed by
+    bhip_solve_dev: X, D*B, the iteration count and the preconditioned
+    matrix left behind; twice in a row, the second time over the matrix the
+    first solve left."""
+    slv, precon, prec = case
+    m = M.random_small_mesh(13, 10, seed=61)
+    if typext == "S":
+        sysd = S.wave_system(m, 1.0 + m.x / 4.0, dt=2.0)
+        da, xa = sysd["da"], sysd["xa"]
+    else:
+        da, xa = _nonsym_system(m, 62)
+    if slv == 1 and typext == "Q":
+        pytest.skip("CG takes symmetric matrices")
+    rng = np.random.default_rng(63)
+    ctxs = [bief.Context(m, block_nodes=64) for _ in range(2)]
+    for ctx in ctxs:
+        ctx.set_matrix("Q", typext, da, xa)
+        ctx.save_matrix()
+        ctx.set_matrix("Q", typext, da * 2.0, xa * 0.5)  # not the saved one
+    for rep in range(2):
+        x0, b0 = rng.normal(size=m.npoin), rng.normal(size=m.npoin)
+        (xa_, ba, ia, ma), (xb, bb, ib, mb) = _restored_pair(
+            ctxs, x0, b0, slv=slv, precon=precon, eps=1e-8, nitmax=500, precision=prec)
+        assert ia["converged"] and ia["niter"] == ib["niter"], (rep, ia, ib)
+        assert np.array_equal(xa_, xb) and np.array_equal(ba, bb), rep
+        assert ma[0] == mb[0] and np.array_equal(ma[1], mb[1]) and np.array_equal(ma[2], mb[2])
+
+
+def test_solve_restored_prisms_and_ensemble(gpu):
+    """The fused restore on prisms (15 coefficients, FP64 and mixed CG) and
+    on an ensemble of three meshes (mixed CG, per-mesh states)."""
+    base = M.random_small_mesh(9, 7, seed=64)
+    rng = np.random.default_rng(65)
+    p = M.extrude_prisms(base, 5, -rng.uniform(2, 6, base.npoin), np.zeros(base.npoin))
+    eb = np.arange(p.nelem) % base.nelem
+    s = base.surfac()[eb]
+    ctxs = [bief.Context(p, block_nodes=60) for _ in range(2)]
+    for ctx in ctxs:
+        ctx.matrix_mt01pp(1.0, s, p.z)
+        ctx.save_matrix()
+        ctx.matrix_mt01pp(3.0, s, p.z)
+    for prec in (bief.PREC_FP64, bief.PREC_MIXED):
+        x0, b0 = np.zeros(p.npoin), rng.normal(size=p.npoin)
+        (xa_, ba, ia, ma), (xb, bb, ib, mb) = _restored_pair(
+            ctxs, x0, b0, slv=1, precon=2, eps=1e-10, nitmax=500, precision=prec)
+        assert ia["converged"] and ia["niter"] == ib["niter"]
+        assert np.array_equal(xa_, xb) and np.array_equal(ba, bb)
+        assert np.array_equal(ma[1], mb[1]) and np.array_equal(ma[2], mb[2])
+    meshes = [M.random_small_mesh(8 + k, 7, seed=70 + k) for k in range(3)]
+    ens = [bief.Ensemble(meshes) for _ in range(2)]
+    sysds = [S.wave_system(mm, 1.0 + mm.x / 4.0, dt=2.0) for mm in meshes]
+    da = np.concatenate([sd["da"] for sd in sysds])
+    xa = np.concatenate([sd["xa"] for sd in sysds], axis=1)
+    for e in ens:
+        e.set_matrix("Q", "S", da, xa)
+        e.save_matrix()
+        e.set_matrix("Q", "S", da * 1.5, xa)
+    x0, b0 = np.zeros(da.size), rng.normal(size=da.size)
+    (xa_, ba, ia, ma), (xb, bb, ib, mb) = _restored_pair(
+        ens, x0, b0, slv=1, precon=2, eps=1e-10, nitmax=500, precision=bief.PREC_MIXED)
+    assert ia["converged"] and ia["niter"] == ib["niter"]
+    assert np.array_equal(xa_, xb) and np.array_equal(ba, bb)
+    assert np.array_equal(ma[1], mb[1]) and np.array_equal(ma[2], mb[2])
