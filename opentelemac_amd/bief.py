@@ -636,18 +636,22 @@ class Context:
                 self.restore_matrix()
             xh = _host_n(x, self.npoin, "solve: x")
             bh = _host_n(b, self.npoin, "solve: b")
-            if not xh.flags.writeable or xh is x:
+            # SOLVE overwrites X and B (B with D*B): a writeable float64
+            # array is passed as it is; anything else (a read-only view, a
+            # list, another dtype) through a copy, then written back where the
+            # caller's object can take it
+            if not xh.flags.writeable:
                 xh = xh.copy()
-            # SOLVE overwrites B with D*B: never write into the caller's
-            # array through the C ABI (a read-only view would be clobbered)
-            if not bh.flags.writeable or bh is b:
+            if not bh.flags.writeable:
                 bh = bh.copy()
             rc = lib().bhip_solve(self._h, ctypes.byref(cfg), xh.ctypes.data, bh.ctypes.data,
                                   ctypes.byref(info))
             _check(rc, "bhip_solve")
-            if isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.writeable:
+            if xh is not x and isinstance(x, np.ndarray) and x.dtype == np.float64 \
+                    and x.flags.writeable:
                 x[...] = xh
-            if isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.writeable:
+            if bh is not b and isinstance(b, np.ndarray) and b.dtype == np.float64 \
+                    and b.flags.writeable:
                 b[...] = bh
             x = xh
         self.typdia = "I" if precon in (2,) else self.typdia

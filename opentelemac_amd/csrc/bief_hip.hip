@@ -1197,7 +1197,11 @@ struct bhip_ctx {
   bhip::DevBuf<double> wv_geo, wv_w, wv_vol, wv_h, wv_zb, wv_f, wv_vs, wv_c, wv_usx, wv_usy,
       wv_fb, wv_da, wv_sbx, wv_sby, wv_svs, wv_seb;
   bool wv_ready = false;
+  // device copies of a host entry point's caller arrays (kept between calls)
+  bhip::DevBuf<double> h_xa, h_x, h_b;
 };
+
+#include "hostcopy.inl"
 
 namespace bhip {
 
@@ -1588,7 +1592,10 @@ static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double
   if (da) {
     // DA in internal order
     double* tmp = c->vio0.p;
-    HIP_OK(hipMemcpyAsync(tmp, da, sizeof(double) * L.npoin, kind, c->stream));
+    if (kind == hipMemcpyHostToDevice)
+      HIP_OK(to_device(c, tmp, da, sizeof(double) * L.npoin));
+    else
+      HIP_OK(hipMemcpyAsync(tmp, da, sizeof(double) * L.npoin, kind, c->stream));
     hipLaunchKernelGGL(k_gather<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
                        c->da.p, tmp, c->perm.p, L.npoin);
   } else {
@@ -1598,11 +1605,10 @@ static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double
   HIP_OK(hipGetLastError());
   if (ncoef > 0) {
     double* src = nullptr;
-    bhip::DevBuf<double> staging;
     if (kind == hipMemcpyHostToDevice) {
-      HIP_OK(staging.alloc((size_t)ncoef * L.nelem));
-      HIP_OK(hipMemcpyAsync(staging.p, xa, sizeof(double) * ncoef * L.nelem, kind, c->stream));
-      src = staging.p;
+      HIP_OK(c->h_xa.alloc((size_t)ncoef * L.nelem));
+      HIP_OK(to_device(c, c->h_xa.p, xa, sizeof(double) * ncoef * L.nelem));
+      src = c->h_xa.p;
     } else {
       src = const_cast<double*>(xa);
     }
@@ -1610,10 +1616,6 @@ static int matrix_set_common(bhip_ctx* c, char typdia, char typext, const double
                        c->stream, c->xa.p, src, c->slot_elem.p, L.nslot, L.nslot_pad, L.nelem,
                        ncoef);
     HIP_OK(hipGetLastError());
-    if (kind == hipMemcpyHostToDevice) {
-      HIP_OK(hipStreamSynchronize(c->stream));
-      staging.release();
-    }
   } else {
     hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(L.nslot_pad)), dim3(kThreads), 0,
                        c->stream, c->xa.p, 0.0, (long)npair * L.nslot_pad);
@@ -1689,20 +1691,15 @@ int bhip_matrix_get(bhip_ctx* c, char* typdia, double* da, double* xa) {
     hipLaunchKernelGGL(k_scatter<double>, dim3(c->grid_nodes), dim3(kThreads), 0, c->stream,
                        c->vio0.p, c->da.p, c->perm.p, L.npoin);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(da, c->vio0.p, sizeof(double) * L.npoin, hipMemcpyDeviceToHost,
-                          c->stream));
+    HIP_OK(to_host(c, da, c->vio0.p, sizeof(double) * L.npoin));
   }
   if (xa && c->ncoef > 0) {
-    bhip::DevBuf<double> tmp;
-    HIP_OK(tmp.alloc((size_t)c->ncoef * L.nelem));
+    HIP_OK(c->h_xa.alloc((size_t)c->ncoef * L.nelem));
     hipLaunchKernelGGL(k_matrix_from_slots, dim3(grid_for(L.nelem * c->ncoef)), dim3(kThreads),
-                       0, c->stream, tmp.p, c->xa.p, c->first_slot.p, L.nelem, L.nslot_pad,
+                       0, c->stream, c->h_xa.p, c->xa.p, c->first_slot.p, L.nelem, L.nslot_pad,
                        c->ncoef);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(xa, tmp.p, sizeof(double) * c->ncoef * L.nelem,
-                          hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    tmp.release();
+    HIP_OK(to_host(c, xa, c->h_xa.p, sizeof(double) * c->ncoef * L.nelem));
   }
   HIP_OK(hipStreamSynchronize(c->stream));
   return BHIP_OK;
@@ -1744,19 +1741,14 @@ int bhip_matvec_dev(bhip_ctx* c, int op, double cst, const double* y, double* x)
 int bhip_matvec(bhip_ctx* c, int op, double cst, const double* y, double* x) {
   if (!c || !c->have_matrix || !y || !x) return BHIP_EBADARG;
   const Layout& L = c->L;
-  DevBuf<double> dy, dx;
-  HIP_OK(dy.alloc(L.npoin));
-  HIP_OK(dx.alloc(L.npoin));
-  HIP_OK(hipMemcpyAsync(dy.p, y, sizeof(double) * L.npoin, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(dx.p, x, sizeof(double) * L.npoin, hipMemcpyHostToDevice, c->stream));
-  int rc = bhip_matvec_dev(c, op, cst, dy.p, dx.p);
-  if (rc == BHIP_OK) {
-    HIP_OK(hipMemcpyAsync(x, dx.p, sizeof(double) * L.npoin, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-  }
+  const size_t bytes = sizeof(double) * L.npoin;
+  HIP_OK(c->h_b.alloc(L.npoin));
+  HIP_OK(c->h_x.alloc(L.npoin));
+  HIP_OK(to_device(c, c->h_b.p, y, bytes));
+  HIP_OK(to_device(c, c->h_x.p, x, bytes));
+  int rc = bhip_matvec_dev(c, op, cst, c->h_b.p, c->h_x.p);
+  if (rc == BHIP_OK) HIP_OK(to_host(c, x, c->h_x.p, bytes));
   (void)hipStreamSynchronize(c->stream);
-  dy.release();
-  dx.release();
   return rc;
 }
 
@@ -2651,19 +2643,17 @@ int bhip_solve(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x, double* b,
                bhip_solve_info* info) {
   if (!c || !x || !b) return BHIP_EBADARG;
   const int n = c->L.npoin;
-  DevBuf<double> dx, db;
-  HIP_OK(dx.alloc(n));
-  HIP_OK(db.alloc(n));
-  HIP_OK(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  int rc = bhip_solve_dev(c, cfg, dx.p, db.p, info);
+  const size_t bytes = sizeof(double) * n;
+  HIP_OK(c->h_x.alloc(n));
+  HIP_OK(c->h_b.alloc(n));
+  HIP_OK(to_device(c, c->h_x.p, x, bytes));
+  HIP_OK(to_device(c, c->h_b.p, b, bytes));
+  int rc = bhip_solve_dev(c, cfg, c->h_x.p, c->h_b.p, info);
   if (rc == BHIP_OK || rc == BHIP_MAXITER || rc == BHIP_ZERO_RHS) {
-    HIP_OK(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipMemcpyAsync(b, db.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(to_host(c, x, c->h_x.p, bytes));
+    HIP_OK(to_host(c, b, c->h_b.p, bytes));
   }
   (void)hipStreamSynchronize(c->stream);
-  dx.release();
-  db.release();
   return rc;
 }
 
@@ -3190,7 +3180,7 @@ static double* node_in(bhip_ctx* c, int k, const double* src, int shift = 0) {
   const double* from = src;
   if (!c->in_dev) {
     double* stage = c->nsc.p + (size_t)6 * n;
-    if (hipMemcpyAsync(stage, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream) !=
+    if (to_device(c, stage, src, sizeof(double) * n) !=
         hipSuccess)
       return nullptr;
     from = stage;
@@ -3211,7 +3201,7 @@ static const double* elem_in(bhip_ctx* c, int k, const double* src) {
   const int ne = c->L.nelem;
   if (c->esc.alloc((size_t)4 * ne) != hipSuccess) return nullptr;
   double* dst = c->esc.p + (size_t)k * ne;
-  if (hipMemcpyAsync(dst, src, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream) !=
+  if (to_device(c, dst, src, sizeof(double) * ne) !=
       hipSuccess)
     return nullptr;
   return dst;
@@ -3243,7 +3233,7 @@ static double* node_in_prism4(bhip_ctx* c, const double* z, const double* f, con
   if (!c->in_dev) {
     for (int k = 0; k < 4; ++k) {
       double* stage = c->nsc.p + (size_t)(4 + k) * n;
-      if (hipMemcpyAsync(stage, src[k], sizeof(double) * n, hipMemcpyHostToDevice, c->stream) !=
+      if (to_device(c, stage, src[k], sizeof(double) * n) !=
           hipSuccess)
         return nullptr;
       src[k] = stage;
@@ -3277,8 +3267,7 @@ static int dirich_impl(bhip_ctx* c, double* f, double* sm, const double* fbor,
   int32_t* il = c->isc.p;
   const int32_t* lsrc = limdir;
   if (!c->in_dev) {
-    HIP_OK(hipMemcpyAsync(c->isc.p + n, limdir, sizeof(int32_t) * n, hipMemcpyHostToDevice,
-                          c->stream));
+    HIP_OK(to_device(c, c->isc.p + n, limdir, sizeof(int32_t) * n));
     lsrc = c->isc.p + n;
   }
   // internal order: vio0 = FBOR, vd0 = SM, vd1 = F, il = LIMDIR
@@ -3309,12 +3298,12 @@ static int dirich_impl(bhip_ctx* c, double* f, double* sm, const double* fbor,
   hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p,
                      c->vd0.p, c->perm.p, n);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(sm, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(to_host(c, sm, c->vio1.p, sizeof(double) * n));
   HIP_OK(hipStreamSynchronize(c->stream));
   hipLaunchKernelGGL(k_scatter<double>, dim3(g), dim3(kThreads), 0, c->stream, c->vio1.p,
                      c->vd1.p, c->perm.p, n);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(f, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(to_host(c, f, c->vio1.p, sizeof(double) * n));
   HIP_OK(hipStreamSynchronize(c->stream));
   return BHIP_OK;
 }
@@ -3333,7 +3322,7 @@ static int lump_impl(bhip_ctx* c, double xmul, double* diag) {
                      c->perm.p, n);
   HIP_OK(hipGetLastError());
   if (!c->in_dev) {
-    HIP_OK(hipMemcpyAsync(diag, c->vio1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(to_host(c, diag, c->vio1.p, sizeof(double) * n));
     HIP_OK(hipStreamSynchronize(c->stream));
   }
   return BHIP_OK;
@@ -3394,14 +3383,12 @@ static int elem_common(bhip_ctx* c, int kind, double xmul, const double* xel, co
   DevBuf<double> dS, dX, dY;
   if (!use_geo) {
     HIP_OK(dS.alloc(ne));
-    HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(to_device(c, dS.p, surfac, sizeof(double) * ne));
     if (kind != 100) {
       HIP_OK(dX.alloc((size_t)3 * ne));
       HIP_OK(dY.alloc((size_t)3 * ne));
-      HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
-                            c->stream));
-      HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
-                            c->stream));
+      HIP_OK(to_device(c, dX.p, xel, sizeof(double) * 3 * ne));
+      HIP_OK(to_device(c, dY.p, yel, sizeof(double) * 3 * ne));
     }
   }
   const double* dU = nullptr;
@@ -3454,14 +3441,12 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
   DevBuf<double> dS, dX, dY;
   if (!use_geo) {
     HIP_OK(dS.alloc(ne));
-    HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(to_device(c, dS.p, surfac, sizeof(double) * ne));
     if (kind == 201) {
       HIP_OK(dX.alloc((size_t)3 * ne));
       HIP_OK(dY.alloc((size_t)3 * ne));
-      HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
-                            c->stream));
-      HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice,
-                            c->stream));
+      HIP_OK(to_device(c, dX.p, xel, sizeof(double) * 3 * ne));
+      HIP_OK(to_device(c, dY.p, yel, sizeof(double) * 3 * ne));
     }
   }
   // node fields in the context's order: Z alone (MT01PP) or MT02PP's record
@@ -3526,9 +3511,9 @@ int bhip_ctx_set_geometry(bhip_ctx* c, const double* xel, const double* yel,
   HIP_OK(dX.alloc((size_t)3 * ne));
   HIP_OK(dY.alloc((size_t)3 * ne));
   HIP_OK(c->geo.alloc((size_t)5 * L.nslot_pad));
-  HIP_OK(hipMemcpyAsync(dS.p, surfac, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(dX.p, xel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(dY.p, yel, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(to_device(c, dS.p, surfac, sizeof(double) * ne));
+  HIP_OK(to_device(c, dX.p, xel, sizeof(double) * 3 * ne));
+  HIP_OK(to_device(c, dY.p, yel, sizeof(double) * 3 * ne));
   hipLaunchKernelGGL(k_geo_slots, dim3(grid_for(L.nslot)), dim3(kThreads), 0, c->stream,
                      c->geo.p, dX.p, dY.p, dS.p, c->slot_elem.p, L.nslot, L.nslot_pad, ne);
   HIP_OK(hipGetLastError());
