@@ -2314,12 +2314,14 @@ static int solve_dev(bhip_ctx* c, const bhip_solver_cfg* cfg, double* x_dev, dou
   const int n = L.npoin;
   const int g = c->grid_nodes;
   const bool diag = (precon % 2 == 0) || (precon % 5 == 0);
-  // OM 'M=N' from the saved copy first: with diagonal preconditioning PRECD1
-  // and D A D read the saved copy and write the context's matrix, which
-  // takes the place of the copy; otherwise the copy is made as it is
+  // OM 'M=N' from the saved copy first: with diagonal preconditioning and
+  // off-diagonal terms, PRECD1 and D A D read the saved copy and write the
+  // context's matrix, which takes the place of the copy; otherwise the copy
+  // is made as it is (a TYPEXT '0' matrix is stored as zero coefficients,
+  // which D A D does not write)
   const double* da_src = c->da.p;
   const double* xa_src = c->xa.p;
-  if (from_saved && diag) {
+  if (from_saved && diag && c->ncoef_saved > 0) {
     c->typdia = c->typdia_saved;
     c->typext = c->typext_saved;
     c->ncoef = c->ncoef_saved;

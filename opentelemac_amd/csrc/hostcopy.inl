@@ -29,11 +29,14 @@ class HostCopier {
     return *inst;
   }
 
-  hipError_t to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  // dev: the device of stream s, current on the calling thread
+  hipError_t to_device(void* dst, const void* src, size_t bytes, hipStream_t s, int dev) {
     if (bytes < kMinStaged) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
     std::lock_guard<std::mutex> use(use_mu_);
-    hipError_t e = init();
+    hipError_t e = init(dev);
     if (e != hipSuccess) return e;
+    char** pin = ring_[dev].pin;
+    hipEvent_t* ev = ring_[dev].ev;
     const char* in = static_cast<const char*>(src);
     char* out = static_cast<char*>(dst);
     const size_t nch = (bytes + kChunk - 1) / kChunk;
@@ -41,33 +44,36 @@ class HostCopier {
       const int slot = (int)(k % kSlots);
       const size_t off = k * kChunk, len = std::min(kChunk, bytes - off);
       // the slot's previous DMA (of this call or an earlier one) has read it
-      if ((e = hipEventSynchronize(ev_[slot])) != hipSuccess) return e;
-      pcopy(pin_[slot], in + off, len);
-      if ((e = hipMemcpyAsync(out + off, pin_[slot], len, hipMemcpyHostToDevice, s)) !=
+      if ((e = hipEventSynchronize(ev[slot])) != hipSuccess) return e;
+      pcopy(pin[slot], in + off, len);
+      if ((e = hipMemcpyAsync(out + off, pin[slot], len, hipMemcpyHostToDevice, s)) !=
           hipSuccess)
         return e;
-      if ((e = hipEventRecord(ev_[slot], s)) != hipSuccess) return e;
+      if ((e = hipEventRecord(ev[slot], s)) != hipSuccess) return e;
     }
     return hipSuccess;
   }
 
-  hipError_t to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  hipError_t to_host(void* dst, const void* src, size_t bytes, hipStream_t s, int dev) {
     if (bytes < kMinStaged) {
       hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
       return e != hipSuccess ? e : hipStreamSynchronize(s);
     }
     std::lock_guard<std::mutex> use(use_mu_);
-    hipError_t e = init();
+    hipError_t e = init(dev);
     if (e != hipSuccess) return e;
+    char** pin = ring_[dev].pin;
+    hipEvent_t* ev = ring_[dev].ev;
     const char* in = static_cast<const char*>(src);
     char* out = static_cast<char*>(dst);
     const size_t nch = (bytes + kChunk - 1) / kChunk;
     auto issue = [&](size_t k) {
       const int slot = (int)(k % kSlots);
       const size_t off = k * kChunk, len = std::min(kChunk, bytes - off);
-      hipError_t r = hipEventSynchronize(ev_[slot]);
-      if (r == hipSuccess) r = hipMemcpyAsync(pin_[slot], in + off, len, hipMemcpyDeviceToHost, s);
-      if (r == hipSuccess) r = hipEventRecord(ev_[slot], s);
+      hipError_t r = hipEventSynchronize(ev[slot]);
+      if (r == hipSuccess)
+        r = hipMemcpyAsync(pin[slot], in + off, len, hipMemcpyDeviceToHost, s);
+      if (r == hipSuccess) r = hipEventRecord(ev[slot], s);
       return r;
     };
     for (size_t k = 0; k < std::min(nch, (size_t)kSlots); ++k)
@@ -75,8 +81,8 @@ class HostCopier {
     for (size_t k = 0; k < nch; ++k) {
       const int slot = (int)(k % kSlots);
       const size_t off = k * kChunk, len = std::min(kChunk, bytes - off);
-      if ((e = hipEventSynchronize(ev_[slot])) != hipSuccess) return e;
-      pcopy(out + off, pin_[slot], len);
+      if ((e = hipEventSynchronize(ev[slot])) != hipSuccess) return e;
+      pcopy(out + off, pin[slot], len);
       if (k + kSlots < nch && (e = issue(k + kSlots)) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -85,20 +91,29 @@ class HostCopier {
  private:
   HostCopier() = default;
 
-  hipError_t init() {
-    if (pin_[0]) return hipSuccess;
-    void* p = nullptr;
-    hipError_t e = hipHostMalloc(&p, kChunk * kSlots, hipHostMallocDefault);
-    if (e != hipSuccess) return e;
-    for (int k = 0; k < kSlots; ++k) {
-      if ((e = hipEventCreateWithFlags(&ev_[k], hipEventDisableTiming)) != hipSuccess) return e;
+  // the ring of device dev (pinned slots and the events of their last DMA,
+  // created with dev current), and the thread pool on first use
+  hipError_t init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    Ring& r = ring_[dev];
+    if (!r.pin[0]) {
+      void* p = nullptr;
+      hipError_t e = hipHostMalloc(&p, kChunk * kSlots, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+      for (int k = 0; k < kSlots; ++k) {
+        if ((e = hipEventCreateWithFlags(&r.ev[k], hipEventDisableTiming)) != hipSuccess)
+          return e;
+      }
+      for (int k = 0; k < kSlots; ++k) r.pin[k] = static_cast<char*>(p) + k * kChunk;
     }
-    for (int k = 0; k < kSlots; ++k) pin_[k] = static_cast<char*>(p) + k * kChunk;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    parts_ = (int)std::min(8u, hw);
-    for (int k = 1; k < parts_; ++k) {
-      std::thread t([this, k] { worker(k); });
-      t.detach();
+    if (!pool_started_) {
+      const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+      parts_ = (int)std::min(8u, hw);
+      for (int k = 1; k < parts_; ++k) {
+        std::thread t([this, k] { worker(k); });
+        t.detach();
+      }
+      pool_started_ = true;
     }
     return hipSuccess;
   }
@@ -145,9 +160,14 @@ class HostCopier {
     }
   }
 
-  std::mutex use_mu_;  // one transfer at a time through the ring
-  char* pin_[kSlots] = {nullptr, nullptr, nullptr};
-  hipEvent_t ev_[kSlots] = {nullptr, nullptr, nullptr};
+  static constexpr int kMaxDevices = 64;
+  struct Ring {
+    char* pin[kSlots] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[kSlots] = {nullptr, nullptr, nullptr};
+  };
+  std::mutex use_mu_;  // one transfer at a time (the thread pool is shared)
+  Ring ring_[kMaxDevices];
+  bool pool_started_ = false;
   int parts_ = 1;
   // the pool's current copy
   std::mutex mu_;
@@ -161,10 +181,10 @@ class HostCopier {
 
 // copies of caller (host) arrays, to and from the context's stream
 static inline hipError_t to_device(bhip_ctx* c, void* dst, const void* src, size_t bytes) {
-  return HostCopier::get().to_device(dst, src, bytes, c->stream);
+  return HostCopier::get().to_device(dst, src, bytes, c->stream, c->device);
 }
 static inline hipError_t to_host(bhip_ctx* c, void* dst, const void* src, size_t bytes) {
-  return HostCopier::get().to_host(dst, src, bytes, c->stream);
+  return HostCopier::get().to_host(dst, src, bytes, c->stream, c->device);
 }
 
 }  // namespace bhip

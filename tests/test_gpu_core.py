@@ -365,7 +365,7 @@ def _restored_pair(ctxs, x0, b0, **kw):
     return out
 
 
-@pytest.mark.parametrize("typext", ["S", "Q"])
+@pytest.mark.parametrize("typext", ["S", "Q", "0"])
 @pytest.mark.parametrize("case", [(1, 2, bief.PREC_FP64), (1, 2, bief.PREC_MIXED),
                                   (1, 0, bief.PREC_FP64), (1, 5, bief.PREC_MIXED),
                                   (6, 2, bief.PREC_FP64), (7, 2, bief.PREC_FP64),
@@ -378,19 +378,24 @@ def test_solve_restored_equals_restore_then_solve(gpu, typext, case):
     first solve left."""
     slv, precon, prec = case
     m = M.random_small_mesh(13, 10, seed=61)
-    if typext == "S":
+    if typext == "Q":
+        da, xa = _nonsym_system(m, 62)
+    else:
         sysd = S.wave_system(m, 1.0 + m.x / 4.0, dt=2.0)
         da, xa = sysd["da"], sysd["xa"]
-    else:
-        da, xa = _nonsym_system(m, 62)
     if slv == 1 and typext == "Q":
         pytest.skip("CG takes symmetric matrices")
+    if typext == "0" and (slv != 1 or prec != bief.PREC_FP64):
+        pytest.skip("a diagonal matrix: FP64 CG is enough")
+    cur = (da * 2.0, xa * 0.5)  # the current matrix, not the saved one
     rng = np.random.default_rng(63)
     ctxs = [bief.Context(m, block_nodes=64) for _ in range(2)]
     for ctx in ctxs:
-        ctx.set_matrix("Q", typext, da, xa)
+        ctx.set_matrix("Q", typext, da, None if typext == "0" else xa)
         ctx.save_matrix()
-        ctx.set_matrix("Q", typext, da * 2.0, xa * 0.5)  # not the saved one
+        # a current matrix with off-diagonal terms, which the restore of a
+        # TYPEXT '0' copy must clear
+        ctx.set_matrix("Q", "S" if typext == "0" else typext, *cur)
     for rep in range(2):
         x0, b0 = rng.normal(size=m.npoin), rng.normal(size=m.npoin)
         (xa_, ba, ia, ma), (xb, bb, ib, mb) = _restored_pair(
@@ -398,6 +403,10 @@ def test_solve_restored_equals_restore_then_solve(gpu, typext, case):
         assert ia["converged"] and ia["niter"] == ib["niter"], (rep, ia, ib)
         assert np.array_equal(xa_, xb) and np.array_equal(ba, bb), rep
         assert ma[0] == mb[0] and np.array_equal(ma[1], mb[1]) and np.array_equal(ma[2], mb[2])
+        if typext == "0":
+            # the saved matrix is diagonal: no trace of the current one's
+            # terms (which would move X by O(1)); the solve stops at eps 1e-8
+            assert np.abs(xb - b0 / da).max() <= 1e-6 * np.abs(b0 / da).max()
 
 
 def test_solve_restored_prisms_and_ensemble(gpu):
