@@ -365,11 +365,17 @@ def _restored_pair(ctxs, x0, b0, **kw):
     return out
 
 
-@pytest.mark.parametrize("typext", ["S", "Q", "0"])
-@pytest.mark.parametrize("case", [(1, 2, bief.PREC_FP64), (1, 2, bief.PREC_MIXED),
-                                  (1, 0, bief.PREC_FP64), (1, 5, bief.PREC_MIXED),
-                                  (6, 2, bief.PREC_FP64), (7, 2, bief.PREC_FP64),
-                                  (7, 2, bief.PREC_MIXED)])
+_RESTORED_CASES = [(1, 2, bief.PREC_FP64), (1, 2, bief.PREC_MIXED), (1, 0, bief.PREC_FP64),
+                   (1, 5, bief.PREC_MIXED), (6, 2, bief.PREC_FP64), (7, 2, bief.PREC_FP64),
+                   (7, 2, bief.PREC_MIXED)]
+
+
+# CG takes symmetric matrices; a diagonal ('0') matrix needs FP64 CG only
+@pytest.mark.parametrize("typext,case",
+                         [("S", c) for c in _RESTORED_CASES]
+                         + [("Q", c) for c in _RESTORED_CASES if c[0] != 1]
+                         + [("0", c) for c in _RESTORED_CASES
+                            if c[0] == 1 and c[2] == bief.PREC_FP64])
 def test_solve_restored_equals_restore_then_solve(gpu, typext, case):
     """bhip_solve_dev_restored (the saved matrix copied back by PRECD1 and
     D A D themselves) gives the bits of bhip_matrix_restore followed by
@@ -383,10 +389,6 @@ def test_solve_restored_equals_restore_then_solve(gpu, typext, case):
     else:
         sysd = S.wave_system(m, 1.0 + m.x / 4.0, dt=2.0)
         da, xa = sysd["da"], sysd["xa"]
-    if slv == 1 and typext == "Q":
-        pytest.skip("CG takes symmetric matrices")
-    if typext == "0" and (slv != 1 or prec != bief.PREC_FP64):
-        pytest.skip("a diagonal matrix: FP64 CG is enough")
     cur = (da * 2.0, xa * 0.5)  # the current matrix, not the saved one
     rng = np.random.default_rng(63)
     ctxs = [bief.Context(m, block_nodes=64) for _ in range(2)]
