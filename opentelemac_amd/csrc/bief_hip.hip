@@ -2928,7 +2928,7 @@ struct PrismIn {
 // matrices, 1 adds them to a symmetric target ('M=M+N'), 2 to both halves of
 // a non-symmetric one.
 template <int KIND, int ST, int ACC>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(ST == 2 ? 2 * kThreads : kThreads)
 k_elem_prism(double xmul, const double* __restrict__ surfac,
              const double* __restrict__ xel, const double* __restrict__ yel, int nelem,
              const double* __restrict__ nd, int h_zero, int inchyd, int mon,
@@ -2949,6 +2949,10 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
   const int32_t* co = color_off + (size_t)b * (kMaxColors + 1);
   const int send = co[kMaxColors];
   const size_t np = nslot_pad;
+  // ST 2: two groups of kThreads threads, each on its own colour
+  constexpr int NT = ST == 2 ? 2 * kThreads : kThreads;
+  const int tg = (int)threadIdx.x % kThreads;
+  const int grp = (int)threadIdx.x / kThreads;
 
   auto load_slot = [&](PrismIn& in, int s) {
     in.s = s;
@@ -2966,11 +2970,12 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
   // the thread's first slot of colour c, or -1
   auto first_slot = [&](int c) {
     if (c >= kMaxColors || co[c] >= send) return -1;
-    const int s = co[c] + (int)threadIdx.x;
+    const int s = co[c] + tg;
     return s < co[c + 1] ? s : -1;
   };
 
-  auto compute_slot = [&](const PrismIn& in) {
+  // the element matrix of a slot into XA, its diagonal terms into t
+  auto compute_slot = [&](const PrismIn& in, double (&t)[6]) {
     const int s = in.s;
     const SlotGeo& G = in.G;
     int g[6];
@@ -2979,7 +2984,7 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
       for (int k = 0; k < 6; ++k)
         g[k] = in.l[k] < nown ? n0 + in.l[k] : halo[h0 + in.l[k] - nown];
     }
-    double cf[15], t[6];
+    double cf[15];
     if constexpr (KIND == 200) {
       double zz[6];
 #pragma unroll
@@ -3068,13 +3073,21 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
 #pragma unroll
       for (int q = 0; q < 15; ++q) xa[(size_t)q * np + s] = cf[q];
     }
+  };
+  auto add_slot = [&](const PrismIn& in, const double (&t)[6]) {
 #pragma unroll
     for (int k = 0; k < 6; ++k)
       if (in.l[k] < nown) acc[in.l[k]] += t[k];
   };
+  auto do_slot = [&](const PrismIn& in) {
+    double t[6];
+    compute_slot(in, t);
+    add_slot(in, t);
+  };
 
-  // prefetch depth in colours (MT01PP's smaller blocks share their CU instead)
-  constexpr int PD = ST && KIND == 201 ? 1 : 0;
+  // prefetch depth in colours (MT01PP's smaller blocks share their CU; the
+  // two-group form has a second colour in flight instead)
+  constexpr int PD = ST == 1 && KIND == 201 ? 1 : 0;
   PrismIn p0, p1;
   if constexpr (PD >= 1) load_slot(p0, first_slot(0));
   if constexpr (PD >= 2) load_slot(p1, first_slot(1));
@@ -3083,11 +3096,11 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
     // records of the local nodes, kStage per thread in flight at once
     constexpr int kStage = 4;
     const int nloc = nown + halo_start[b + 1] - h0;
-    for (int l0 = threadIdx.x; l0 < nloc; l0 += kStage * kThreads) {
+    for (int l0 = threadIdx.x; l0 < nloc; l0 += kStage * NT) {
       double2 va[kStage], vb[kStage];
 #pragma unroll
       for (int u = 0; u < kStage; ++u) {
-        const int l = l0 + u * kThreads;
+        const int l = l0 + u * NT;
         const int lc = l < nloc ? l : nloc - 1;
         const size_t gl = lc < nown ? n0 + lc : halo[h0 + lc - nown];
         if constexpr (W == 1) {
@@ -3100,7 +3113,7 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
       }
 #pragma unroll
       for (int u = 0; u < kStage; ++u) {
-        const int l = l0 + u * kThreads;
+        const int l = l0 + u * NT;
         if (l < nloc) {
           if constexpr (W == 1) {
             rec[l] = va[u].x;
@@ -3114,29 +3127,56 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
     }
   }
   __syncthreads();
-  for (int c = 0; c < kMaxColors; ++c) {
-    const int s0 = co[c];
-    if (s0 >= send) break;
-    const int s1 = co[c + 1];
-    PrismIn cur;
-    if constexpr (PD == 2) {
-      cur = p0;
-      p0 = p1;
-      load_slot(p1, first_slot(c + 2));
-    } else if constexpr (PD == 1) {
-      cur = p0;
-      load_slot(p0, first_slot(c + 1));
-    } else {
+  if constexpr (ST == 2) {
+    // colours c0 and c0 + 1 computed at once by the two groups; their
+    // diagonal terms go into the accumulator in colour order, one group
+    // after the other, so the sums are those of the one-group form
+    for (int c0 = 0; c0 < kMaxColors; c0 += 2) {
+      if (co[c0] >= send) break;
+      const int c = c0 + grp;
+      const bool has = c < kMaxColors && co[c] < send;
+      PrismIn cur;
+      double t[6];
       load_slot(cur, first_slot(c));
+      if (cur.s >= 0) compute_slot(cur, t);
+      for (int k = 0; k < 2; ++k) {
+        if (grp == k && has) {
+          if (cur.s >= 0) add_slot(cur, t);
+          // slots beyond the first kThreads of a large colour
+          for (int s = co[c] + tg + kThreads; s < co[c + 1]; s += kThreads) {
+            PrismIn in;
+            load_slot(in, s);
+            do_slot(in);
+          }
+        }
+        __syncthreads();
+      }
     }
-    if (cur.s >= 0) compute_slot(cur);
-    // slots beyond the first kThreads of a large colour
-    for (int s = s0 + threadIdx.x + kThreads; s < s1; s += kThreads) {
-      PrismIn in;
-      load_slot(in, s);
-      compute_slot(in);
+  } else {
+    for (int c = 0; c < kMaxColors; ++c) {
+      const int s0 = co[c];
+      if (s0 >= send) break;
+      const int s1 = co[c + 1];
+      PrismIn cur;
+      if constexpr (PD == 2) {
+        cur = p0;
+        p0 = p1;
+        load_slot(p1, first_slot(c + 2));
+      } else if constexpr (PD == 1) {
+        cur = p0;
+        load_slot(p0, first_slot(c + 1));
+      } else {
+        load_slot(cur, first_slot(c));
+      }
+      if (cur.s >= 0) do_slot(cur);
+      // slots beyond the first kThreads of a large colour
+      for (int s = s0 + tg + kThreads; s < s1; s += kThreads) {
+        PrismIn in;
+        load_slot(in, s);
+        do_slot(in);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   for (int l = threadIdx.x; l < nown; l += blockDim.x) {
     if constexpr (ACC > 0)
@@ -3465,22 +3505,25 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
       hipSuccess)
     return BHIP_EDEVICE;
   const bool staged = lds_rec <= (size_t)lds_max;
-  auto launch = [&](auto kern, size_t lds) -> int {
+  auto launch = [&](auto kern, size_t lds, int threads) -> int {
     if (lds > 65536)
       HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(L.nblocks), dim3(kThreads), lds, c->stream, xmul, dS.p, dX.p,
+    hipLaunchKernelGGL(kern, dim3(L.nblocks), dim3(threads), lds, c->stream, xmul, dS.p, dX.p,
                        dY.p, ne, nd, h_zero, inchyd, mon, c->slot_elem.p, c->lid.p, L.nslot_pad,
                        c->node_start.p, c->halo_start.p, c->halo.p, c->color_off.p, L.nblocks,
                        c->xa.p, c->da.p, use_geo ? c->geo.p : nullptr, rec_off, (int)L.col_ids);
     return BHIP_OK;
   };
-  auto pick = [&](auto k200, auto k201, size_t lds) {
-    return kind == 200 ? launch(k200, lds) : launch(k201, lds);
+  // staged: MT01PP one group of kThreads per workgroup (its records leave
+  // room for several workgroups per CU), MT02PP two groups (one per CU)
+  auto pick = [&](auto k200, auto k201, size_t lds, int t201) {
+    return kind == 200 ? launch(k200, lds, kThreads) : launch(k201, lds, t201);
   };
   const int acc_kind = !accumulate ? 0 : target_q ? 2 : 1;
-#define BHIP_PRISM_PICK(ST, ACC) \
-  pick(k_elem_prism<200, ST, ACC>, k_elem_prism<201, ST, ACC>, ST ? lds_rec : lds_acc)
+#define BHIP_PRISM_PICK(ST, ACC)                                                             \
+  (ST ? pick(k_elem_prism<200, 1, ACC>, k_elem_prism<201, 2, ACC>, lds_rec, 2 * kThreads) \
+      : pick(k_elem_prism<200, 0, ACC>, k_elem_prism<201, 0, ACC>, lds_acc, kThreads))
   const int ls = staged ? (acc_kind == 0   ? BHIP_PRISM_PICK(1, 0)
                            : acc_kind == 1 ? BHIP_PRISM_PICK(1, 1)
                                            : BHIP_PRISM_PICK(1, 2))
