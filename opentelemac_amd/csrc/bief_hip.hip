@@ -2920,13 +2920,15 @@ struct PrismIn {
 // ST 1 copies the records of the block's local nodes (owned, then halo) into
 // LDS once, after the diagonal accumulator (rec_off doubles in), and the
 // slots read them there; ST 0 gathers each slot's six records from global
-// memory. Both give the same bits. A workgroup of the staged MT02PP holds up
-// to 160 KB of LDS, so it may be alone on its CU: each thread loads the inputs
-// of its slot in the next colour (ids, geometry, and the target's current
-// coefficients for 'M=M+N') before it computes the current one, so that their
-// latency overlaps the computation and the barriers. ACC 0 writes the element
-// matrices, 1 adds them to a symmetric target ('M=M+N'), 2 to both halves of
-// a non-symmetric one.
+// memory. All forms give the same bits. A workgroup of the staged MT02PP
+// holds up to 160 KB of LDS, so it may be alone on its CU. ST 2 (2 * kThreads
+// threads) therefore runs two groups, each computing the slots of its own
+// colour, and adds their diagonal terms in colour order; ST 1 (MT01PP's
+// smaller records) has one group, which for MT02PP would load the inputs of
+// its slot in the next colour (ids, geometry, and the target's current
+// coefficients for 'M=M+N') before computing the current one. ACC 0 writes
+// the element matrices, 1 adds them to a symmetric target ('M=M+N'), 2 to
+// both halves of a non-symmetric one.
 template <int KIND, int ST, int ACC>
 __global__ void __launch_bounds__(ST == 2 ? 2 * kThreads : kThreads)
 k_elem_prism(double xmul, const double* __restrict__ surfac,
