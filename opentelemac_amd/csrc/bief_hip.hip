@@ -2901,7 +2901,7 @@ __constant__ int kPrismRowPairs[6][5] = {{0, 1, 2, 3, 4},   {0, 5, 6, 7, 8},
                                         {1, 5, 9, 10, 11},  {2, 6, 9, 12, 13},
                                         {3, 7, 10, 12, 14}, {4, 8, 11, 13, 14}};
 
-// inputs of one prism slot, loaded ahead of its colour phase
+// inputs of one prism slot: ids, geometry, current coefficients for M=M+N
 struct PrismIn {
   int s;           // slot; -1 when the thread has none in the phase
   int l[6];
@@ -2917,18 +2917,15 @@ struct PrismIn {
 //             above (the caller's II4 = I1 + NPOU0); h_zero cancels the
 //             vertical term, inchyd the hydrostatic inconsistency test, mon
 //             the 'MON' clipping
-// ST 1 copies the records of the block's local nodes (owned, then halo) into
+// ST 2 copies the records of the block's local nodes (owned, then halo) into
 // LDS once, after the diagonal accumulator (rec_off doubles in), and the
 // slots read them there; ST 0 gathers each slot's six records from global
-// memory. All forms give the same bits. A workgroup of the staged MT02PP
-// holds up to 160 KB of LDS, so it may be alone on its CU. ST 2 (2 * kThreads
-// threads) therefore runs two groups, each computing the slots of its own
-// colour, and adds their diagonal terms in colour order; ST 1 (MT01PP's
-// smaller records) has one group, which for MT02PP would load the inputs of
-// its slot in the next colour (ids, geometry, and the target's current
-// coefficients for 'M=M+N') before computing the current one. ACC 0 writes
-// the element matrices, 1 adds them to a symmetric target ('M=M+N'), 2 to
-// both halves of a non-symmetric one.
+// memory. Both give the same bits. A workgroup of the staged MT02PP holds up
+// to 160 KB of LDS, so it may be alone on its CU: the staged form therefore
+// runs 2 * kThreads threads in two groups, each computing the slots of its
+// own colour, and adds their diagonal terms in colour order. ACC 0 writes the
+// element matrices, 1 adds them to a symmetric target ('M=M+N'), 2 to both
+// halves of a non-symmetric one.
 template <int KIND, int ST, int ACC>
 __global__ void __launch_bounds__(ST == 2 ? 2 * kThreads : kThreads)
 k_elem_prism(double xmul, const double* __restrict__ surfac,
@@ -3087,12 +3084,6 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
     add_slot(in, t);
   };
 
-  // prefetch depth in colours (MT01PP's smaller blocks share their CU; the
-  // two-group form has a second colour in flight instead)
-  constexpr int PD = ST == 1 && KIND == 201 ? 1 : 0;
-  PrismIn p0, p1;
-  if constexpr (PD >= 1) load_slot(p0, first_slot(0));
-  if constexpr (PD >= 2) load_slot(p1, first_slot(1));
   for (int l = threadIdx.x; l < nown; l += blockDim.x) acc[l] = 0.0;
   if constexpr (ST) {
     // records of the local nodes, kStage per thread in flight at once
@@ -3160,16 +3151,7 @@ k_elem_prism(double xmul, const double* __restrict__ surfac,
       if (s0 >= send) break;
       const int s1 = co[c + 1];
       PrismIn cur;
-      if constexpr (PD == 2) {
-        cur = p0;
-        p0 = p1;
-        load_slot(p1, first_slot(c + 2));
-      } else if constexpr (PD == 1) {
-        cur = p0;
-        load_slot(p0, first_slot(c + 1));
-      } else {
-        load_slot(cur, first_slot(c));
-      }
+      load_slot(cur, first_slot(c));
       if (cur.s >= 0) do_slot(cur);
       // slots beyond the first kThreads of a large colour
       for (int s = s0 + tg + kThreads; s < s1; s += kThreads) {
@@ -3517,14 +3499,13 @@ static int elem_prism(bhip_ctx* c, int kind, double xmul, const double* xel, con
                        c->xa.p, c->da.p, use_geo ? c->geo.p : nullptr, rec_off, (int)L.col_ids);
     return BHIP_OK;
   };
-  // staged: MT01PP one group of kThreads per workgroup (its records leave
-  // room for several workgroups per CU), MT02PP two groups (one per CU)
-  auto pick = [&](auto k200, auto k201, size_t lds, int t201) {
-    return kind == 200 ? launch(k200, lds, kThreads) : launch(k201, lds, t201);
+  // staged: two groups of kThreads per workgroup
+  auto pick = [&](auto k200, auto k201, size_t lds, int threads) {
+    return kind == 200 ? launch(k200, lds, threads) : launch(k201, lds, threads);
   };
   const int acc_kind = !accumulate ? 0 : target_q ? 2 : 1;
 #define BHIP_PRISM_PICK(ST, ACC)                                                             \
-  (ST ? pick(k_elem_prism<200, 1, ACC>, k_elem_prism<201, 2, ACC>, lds_rec, 2 * kThreads) \
+  (ST ? pick(k_elem_prism<200, 2, ACC>, k_elem_prism<201, 2, ACC>, lds_rec, 2 * kThreads) \
       : pick(k_elem_prism<200, 0, ACC>, k_elem_prism<201, 0, ACC>, lds_acc, kThreads))
   const int ls = staged ? (acc_kind == 0   ? BHIP_PRISM_PICK(1, 0)
                            : acc_kind == 1 ? BHIP_PRISM_PICK(1, 1)
